@@ -1,0 +1,305 @@
+"""CPU oracle for the embedding-optimizer path (test infrastructure only).
+
+This package is the checker.  Only ``tests/``, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py`` may import it; the product package
+``fastembedr_amd`` never does.
+
+It wraps ``liboracle.so`` built from:
+
+* ``oracle.c`` -- FP64 restatement of the t-SNE affinity, force, FFT-grid and
+  update functions of ``src/tsne_neighbors.cpp`` (each C function cites the
+  lines it follows) plus the exact rank metrics of
+  ``src/embedding_utils.cpp:1582-1752``;
+* ``oracle_umap.c`` -- the UMAP edge-sampled SGD epoch and its schedule/hash
+  (``src/embedding_cuda_kernels.cpp:253-274,364-407,3044-3154``);
+* ``oracle_transform.c`` -- the fixed-reference transform
+  (``src/tsne_neighbors.cpp:2124-2373``);
+* ``cpu_port.c`` -- the float32, multi-threaded restatement of the reference
+  CPU optimiser used as the timed CPU baseline.
+
+Every array crossing this boundary is C-contiguous; neighbour indices are
+0-based and row-major (n x k).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+_ip = ctypes.POINTER(ctypes.c_int)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+def build() -> None:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L):
+    c_int, c_long, c_double = ctypes.c_int, ctypes.c_long, ctypes.c_double
+    L.or_row_probabilities.argtypes = [_dp, c_int, c_double, _dp]
+    L.or_build_p.argtypes = [c_int, c_int, _ip, _dp, c_double, _ip, _ip, _dp, _dp]
+    L.or_build_p.restype = c_long
+    L.or_symmetrize.argtypes = [c_int, c_int, _ip, _dp, _ip, _ip, _dp]
+    L.or_symmetrize.restype = c_long
+    L.or_attractive.argtypes = [c_int, _ip, _ip, _dp, _dp, c_double, _dp]
+    L.or_exact_repulsive.argtypes = [c_int, _dp, _dp]
+    L.or_exact_repulsive.restype = c_double
+    L.or_sum_q.argtypes = [c_int, _dp]
+    L.or_sum_q.restype = c_double
+    L.or_kl.argtypes = [c_int, _ip, _ip, _dp, _dp]
+    L.or_kl.restype = c_double
+    L.or_fft_repulsive.argtypes = [c_int, _dp, c_int, _dp, _dp, _dp, _dp]
+    L.or_fft2.argtypes = [_dp, _dp, c_int, c_int]
+    L.or_grid_geometry.argtypes = [c_int, _dp, c_int, _dp]
+    L.or_direct_convolution.argtypes = [c_int, c_double, _dp, c_int, _dp]
+    L.or_update.argtypes = [c_int, _dp, _dp, _dp, _dp, c_double, c_double, c_double, c_double]
+    L.or_zero_mean.argtypes = [c_int, _dp]
+    L.or_grid_size_cpu.argtypes = [c_int]
+    L.or_grid_size_gpu.argtypes = [c_int]
+    L.or_tsne_optimize.argtypes = [c_int, _ip, _ip, _dp, _dp, c_int, c_int, c_int, c_double,
+                                   c_double, c_double, c_double, c_double, c_double, c_double,
+                                   c_double]
+    L.or_structure_metrics.argtypes = [c_int, c_int, _dp, c_int, _dp, c_int, _dp]
+    L.port_tsne_run.argtypes = [c_int, c_int, _ip, _fp, _fp, c_double, c_int, c_int, c_double,
+                                c_double, c_double, c_double, c_double, c_double, c_double,
+                                c_int, c_int, _fp, _dp]
+    if hasattr(L, "or_umap_epoch"):
+        _declare_umap(L)
+    if hasattr(L, "or_transform"):
+        _declare_transform(L)
+
+
+def _declare_umap(L):
+    c_int, c_float, c_uint = ctypes.c_int, ctypes.c_float, ctypes.c_uint
+    L.or_umap_mix_uint.argtypes = [c_uint]
+    L.or_umap_mix_uint.restype = c_uint
+    L.or_umap_negative_vertex.argtypes = [c_int, c_uint, c_int, c_int, c_int, c_int]
+    L.or_umap_negative_vertex.restype = c_int
+    L.or_umap_n_positive.argtypes = [c_int, c_float]
+    L.or_umap_n_positive.restype = c_int
+    L.or_umap_n_negative.argtypes = [c_int, c_float, c_int]
+    L.or_umap_n_negative.restype = c_int
+    L.or_umap_epoch.argtypes = [_ip, _ip, _fp, c_int, _dp, c_int, c_int, c_int, c_int,
+                                ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                ctypes.c_double, c_uint]
+    L.or_umap_schedule_digest.argtypes = [_ip, _ip, _fp, c_int, c_int, c_int, c_int, c_int,
+                                          c_uint, _u32p]
+
+
+def _declare_transform(L):
+    c_int, c_double = ctypes.c_int, ctypes.c_double
+    L.or_transform.argtypes = [c_int, _dp, c_int, c_int, _ip, _dp, _dp, c_int, c_double, c_int,
+                               c_int, c_double, c_double, c_double, c_double, c_double,
+                               c_double, c_double, c_int, c_int, c_int, _dp]
+    L.or_transform.restype = c_int
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+class CSR:
+    """Symmetric affinity matrix in CSR form (0-based, ascending columns)."""
+
+    def __init__(self, row_ptr, col, val):
+        self.row_ptr, self.col, self.val = row_ptr, col, val
+
+    @property
+    def n(self):
+        return len(self.row_ptr) - 1
+
+    def dense(self):
+        n = self.n
+        out = np.zeros((n, n))
+        for i in range(n):
+            s, e = self.row_ptr[i], self.row_ptr[i + 1]
+            out[i, self.col[s:e]] = self.val[s:e]
+        return out
+
+
+_ERRORS = {1: "KNN indices are out of range.",
+           2: "KNN distances must be finite and non-negative.",
+           3: "KNN graph produced no non-self t-SNE edges.",
+           4: "t-SNE probability normalization failed.",
+           5: "invalid argument", 6: "allocation failed"}
+
+
+def row_probabilities(dist_row, perplexity):
+    d = _d(dist_row)
+    out = np.zeros_like(d)
+    lib().or_row_probabilities(_ptr(d, _dp), len(d), float(perplexity), _ptr(out, _dp))
+    return out
+
+
+def build_p(indices, distances, perplexity, return_directed=False):
+    """FP64 P: perplexity rows then symmetrised CSR (0-based indices)."""
+    idx, dist = _i(indices), _d(distances)
+    n, k = idx.shape
+    row_ptr = np.zeros(n + 1, np.int32)
+    col = np.zeros(2 * n * k, np.int32)
+    val = np.zeros(2 * n * k)
+    p_dir = np.zeros((n, k))
+    nnz = lib().or_build_p(n, k, _ptr(idx, _ip), _ptr(dist, _dp), float(perplexity),
+                           _ptr(row_ptr, _ip), _ptr(col, _ip), _ptr(val, _dp), _ptr(p_dir, _dp))
+    if nnz < 0:
+        raise ValueError(_ERRORS.get(-nnz, "oracle error"))
+    csr = CSR(row_ptr, col[:nnz].copy(), val[:nnz].copy())
+    return (csr, p_dir) if return_directed else csr
+
+
+def attractive(csr, y, exaggeration=1.0):
+    y = _d(y)
+    out = np.zeros_like(y)
+    lib().or_attractive(csr.n, _ptr(_i(csr.row_ptr), _ip), _ptr(_i(csr.col), _ip),
+                        _ptr(_d(csr.val), _dp), _ptr(y, _dp), float(exaggeration), _ptr(out, _dp))
+    return out
+
+
+def exact_repulsive(y):
+    y = _d(y)
+    out = np.zeros_like(y)
+    sq = lib().or_exact_repulsive(len(y), _ptr(y, _dp), _ptr(out, _dp))
+    return out, sq
+
+
+def sum_q(y):
+    y = _d(y)
+    return lib().or_sum_q(len(y), _ptr(y, _dp))
+
+
+def kl(csr, y):
+    y = _d(y)
+    return lib().or_kl(csr.n, _ptr(_i(csr.row_ptr), _ip), _ptr(_i(csr.col), _ip),
+                       _ptr(_d(csr.val), _dp), _ptr(y, _dp))
+
+
+def fft_repulsive(y, grid_size, return_grids=False):
+    """FP64 FFT-grid repulsion; returns (rep, Z[, grids(4,g,g), splat(3,g,g)])."""
+    y = _d(y)
+    n = len(y)
+    g = int(grid_size)
+    rep = np.zeros_like(y)
+    z = np.zeros(1)
+    grids = np.zeros((4, g, g))
+    splat = np.zeros((3, g, g))
+    rc = lib().or_fft_repulsive(n, _ptr(y, _dp), g, _ptr(rep, _dp), _ptr(z, _dp),
+                                _ptr(grids, _dp), _ptr(splat, _dp))
+    if rc != 0:
+        raise ValueError("grid_size must be a power of two >= 4")
+    if return_grids:
+        return rep, float(z[0]), grids, splat
+    return rep, float(z[0])
+
+
+def grid_geometry(y, grid_size):
+    y = _d(y)
+    out = np.zeros(3)
+    lib().or_grid_geometry(len(y), _ptr(y, _dp), int(grid_size), _ptr(out, _dp))
+    return out
+
+
+def fft2(re, im, inverse=False):
+    re, im = _d(re).copy(), _d(im).copy()
+    lib().or_fft2(_ptr(re, _dp), _ptr(im, _dp), re.shape[0], int(bool(inverse)))
+    return re, im
+
+
+def direct_convolution(src, spacing, power):
+    src = _d(src)
+    g = src.shape[0]
+    out = np.zeros_like(src)
+    lib().or_direct_convolution(g, float(spacing), _ptr(src, _dp), int(power), _ptr(out, _dp))
+    return out
+
+
+def update(y, upd, gains, grad, lr, momentum, min_gain, max_step):
+    """In-place FP64 gains/momentum/clip update (arrays float64, n x 2)."""
+    lib().or_update(len(y), _ptr(y, _dp), _ptr(upd, _dp), _ptr(gains, _dp), _ptr(_d(grad), _dp),
+                    float(lr), float(momentum), float(min_gain), float(max_step))
+
+
+def zero_mean(y):
+    lib().or_zero_mean(len(y), _ptr(y, _dp))
+
+
+def grid_size_cpu(n):
+    return lib().or_grid_size_cpu(int(n))
+
+
+def grid_size_gpu(n):
+    return lib().or_grid_size_gpu(int(n))
+
+
+def tsne_optimize(csr, y_init, grid_size, early_iter, n_iter, early_exag=12.0, exag=1.0,
+                  lr_early=None, lr_normal=None, mom0=0.8, mom1=0.8, min_gain=0.01,
+                  max_step=5.0):
+    """FP64 optimisation from a fixed P; grid_size 0 selects exact repulsion."""
+    y = _d(y_init).copy()
+    n = len(y)
+    lr_early = n / early_exag if lr_early is None else lr_early
+    lr_normal = n / exag if lr_normal is None else lr_normal
+    lib().or_tsne_optimize(n, _ptr(_i(csr.row_ptr), _ip), _ptr(_i(csr.col), _ip),
+                           _ptr(_d(csr.val), _dp), _ptr(y, _dp), int(grid_size), int(early_iter),
+                           int(n_iter), float(early_exag), float(exag), float(lr_early),
+                           float(lr_normal), float(mom0), float(mom1), float(min_gain),
+                           float(max_step))
+    return y
+
+
+def structure_metrics(high, low, k):
+    """(trustworthiness, continuity, knn_preservation, rank_error) at k."""
+    h, l_ = _d(high), _d(low)
+    out = np.zeros(4)
+    rc = lib().or_structure_metrics(len(h), h.shape[1], _ptr(h, _dp), l_.shape[1], _ptr(l_, _dp),
+                                    int(k), _ptr(out, _dp))
+    if rc != 0:
+        raise ValueError("each k must satisfy 1 <= k < (2 * n - 1) / 3")
+    return out
+
+
+def port_tsne_run(indices, distances, y_init, perplexity=30.0, early_iter=250, n_iter=500,
+                  early_exag=12.0, exag=1.0, learning_rate=None, mom0=0.8, mom1=0.8,
+                  min_gain=0.01, max_step=5.0, grid=0, n_threads=None):
+    """float32 reference-algorithm CPU run; returns (Y, affinity_s, optimise_s)."""
+    idx = _i(indices)
+    dist = np.ascontiguousarray(distances, dtype=np.float32)
+    yi = np.ascontiguousarray(y_init, dtype=np.float32)
+    n, k = idx.shape
+    out = np.zeros((n, 2), np.float32)
+    t = np.zeros(2)
+    threads = n_threads or os.cpu_count() or 1
+    rc = lib().port_tsne_run(n, k, _ptr(idx, _ip), _ptr(dist, _fp), _ptr(yi, _fp),
+                             float(perplexity), int(early_iter), int(n_iter), float(early_exag),
+                             float(exag), float(learning_rate or 0.0), float(mom0), float(mom1),
+                             float(min_gain), float(max_step), int(grid), int(threads),
+                             _ptr(out, _fp), _ptr(t, _dp))
+    if rc != 0:
+        raise ValueError(_ERRORS.get(rc, "port error %d" % rc))
+    return out, float(t[0]), float(t[1])
