@@ -1,0 +1,787 @@
+// Host engine and C-ABI for the t-SNE FFT-grid optimizer on gfx950.
+//
+// The driver follows fastembedr_cuda_opentsne_fft_from_knn_impl
+// (src/embedding_cuda_kernels.cpp:5629-6176): validate, build P on the device,
+// initialise the layout, run the early-exaggeration and normal phases
+// (captured into graphs), copy the layout back.  The per-iteration work is
+// the five kernels of tsne_device.hpp.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/fastembedr_hip.h"
+#include "fe_common.hpp"
+#include "tsne_device.hpp"
+#include "tsne_affinity.hpp"
+
+namespace fe {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string& message) { g_last_error = message; }
+const std::string& last_error() { return g_last_error; }
+void clear_error() { g_last_error.clear(); }
+
+int resolve_grid_size(int n, int requested) {
+  if (requested > 0) return requested;
+  int g = 512;
+  if (n < 20000) g = 256;
+  if (n >= 100000) g = 1024;
+  const char* env = std::getenv("FASTEMBEDR_TSNE_FFT_GRID");
+  if (env && env[0]) {
+    const int r = std::atoi(env);
+    if (r >= 16 && r <= 1024 && (r & (r - 1)) == 0) g = r;
+  }
+  return g;
+}
+
+static bool graph_capture_enabled() {
+  const char* env = std::getenv("FASTEMBEDR_HIP_GRAPH_CAPTURE");
+  return !(env && env[0] == '0');
+}
+
+// ---------------------------------------------------------------------------
+// P on the device
+// ---------------------------------------------------------------------------
+struct DeviceCsr {
+  int* row_ptr = nullptr;
+  int* col = nullptr;
+  float* val = nullptr;
+  long long nnz = 0;
+};
+
+static int exclusive_scan(const int* in, int n, int* out, int* sums, int* total, hipStream_t s) {
+  const int nb = (n + kScanBlock - 1) / kScanBlock;
+  scan_blocks<<<nb, 256, 0, s>>>(in, n, out, sums);
+  FE_LAUNCH_CHECK("scan_blocks");
+  scan_sums<<<1, 256, 0, s>>>(sums, nb, total);
+  FE_LAUNCH_CHECK("scan_sums");
+  scan_add<<<(n + 255) / 256, 256, 0, s>>>(out, n, sums);
+  FE_LAUNCH_CHECK("scan_add");
+  return 0;
+}
+
+template <typename DistT>
+static int launch_perplexity(const int* idx, const DistT* dist, int n, int k, int offset,
+                             double perplexity, int* nbr, float* pdir, unsigned* flags,
+                             hipStream_t s) {
+  constexpr int rpb = aff_rows_per_block<DistT>();
+  const size_t lds = static_cast<size_t>(rpb) * k * (sizeof(DistT) + sizeof(int));
+  if (lds > 65536) {
+    FE_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&perplexity_rows<DistT>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 static_cast<int>(lds)));
+  }
+  perplexity_rows<DistT><<<(n + rpb - 1) / rpb, 256, lds, s>>>(idx, dist, n, k, offset, perplexity,
+                                                                nbr, pdir, flags);
+  FE_LAUNCH_CHECK("perplexity_rows");
+  return 0;
+}
+
+// Builds P into `state` (col/val allocated there after nnz is known through
+// `alloc_csr`).  idx/dist are device pointers in the column-major layout.
+template <typename DistT, typename AllocCsr>
+static int build_affinities(const int* idx, const DistT* dist, int n, int k, int offset,
+                            double perplexity, hipStream_t s, DeviceCsr* csr, AllocCsr alloc_csr) {
+  const long long total = static_cast<long long>(n) * k;
+  const int count_blocks = 1024;
+  const int nb_scan = (n + 1 + kScanBlock - 1) / kScanBlock;
+  ArenaPlan plan;
+  plan.add<int>(total);                  // nbr
+  plan.add<float>(total);                // pdir
+  plan.add<int>(n + 1);                  // deg
+  plan.add<int>(n + 1);                  // raw_ptr
+  plan.add<int>(n + 1);                  // cursor
+  plan.add<int>(n + 1);                  // merged_len
+  plan.add<unsigned long long>(2 * total);  // raw entries
+  plan.add<int>(nb_scan + 1);            // scan sums
+  plan.add<int>(4);                      // scan total, flags
+  plan.add<double>(count_blocks);        // mass partials
+  Arena tmp;
+  if (tmp.init(plan.bytes, "t-SNE affinity workspace")) return 1;
+  int* nbr = tmp.take<int>(total);
+  float* pdir = tmp.take<float>(total);
+  int* deg = tmp.take<int>(n + 1);
+  int* raw_ptr = tmp.take<int>(n + 1);
+  int* cursor = tmp.take<int>(n + 1);
+  int* merged_len = tmp.take<int>(n + 1);
+  unsigned long long* raw = tmp.take<unsigned long long>(2 * total);
+  int* sums = tmp.take<int>(nb_scan + 1);
+  int* misc = tmp.take<int>(4);
+  double* mass_part = tmp.take<double>(count_blocks);
+  unsigned* flags = reinterpret_cast<unsigned*>(misc + 1);
+
+  FE_CHECK(hipMemsetAsync(misc, 0, 4 * sizeof(int), s));
+  if (launch_perplexity<DistT>(idx, dist, n, k, offset, perplexity, nbr, pdir, flags, s)) return 1;
+  unsigned h_flags = 0;
+  FE_CHECK(hipMemcpyAsync(&h_flags, flags, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  FE_CHECK(hipStreamSynchronize(s));
+  if (h_flags & kErrIndexRange) return fail("KNN indices are out of range.");
+  if (h_flags & kErrDistance) return fail("KNN distances must be finite and non-negative.");
+
+  FE_CHECK(hipMemsetAsync(deg, 0, (n + 1) * sizeof(int), s));
+  sym_count<<<count_blocks, 256, 0, s>>>(nbr, pdir, total, k, deg, mass_part);
+  FE_LAUNCH_CHECK("sym_count");
+  if (exclusive_scan(deg, n + 1, raw_ptr, sums, misc, s)) return 1;
+  FE_CHECK(hipMemsetAsync(cursor, 0, (n + 1) * sizeof(int), s));
+  sym_fill<<<count_blocks, 256, 0, s>>>(nbr, pdir, total, k, raw_ptr, cursor, raw);
+  FE_LAUNCH_CHECK("sym_fill");
+  sym_sort_merge<<<n, 256, 0, s>>>(raw, raw_ptr, merged_len, flags);
+  FE_LAUNCH_CHECK("sym_sort_merge");
+  FE_CHECK(hipMemsetAsync(merged_len + n, 0, sizeof(int), s));
+  std::vector<double> mp(count_blocks);
+  FE_CHECK(hipMemcpyAsync(mp.data(), mass_part, count_blocks * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+  FE_CHECK(hipMemcpyAsync(&h_flags, flags, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  FE_CHECK(hipStreamSynchronize(s));
+  if (h_flags & kErrRowTooLong)
+    return fail("symmetric affinity row longer than " + std::to_string(kSortCap) +
+                " entries (a point is the neighbour of too many others)");
+  double mass = 0.0;
+  for (double v : mp) mass += v;
+  if (!std::isfinite(mass) || mass <= 0.0) {
+    bool any = false;
+    for (double v : mp) any = any || v != 0.0;
+    return fail(any ? "t-SNE probability normalization failed."
+                    : "KNN graph produced no non-self t-SNE edges.");
+  }
+  // final row pointers (scanned into `deg`, which is free again)
+  if (exclusive_scan(merged_len, n + 1, deg, sums, misc, s)) return 1;
+  int h_nnz = 0;
+  FE_CHECK(hipMemcpyAsync(&h_nnz, deg + n, sizeof(int), hipMemcpyDeviceToHost, s));
+  FE_CHECK(hipStreamSynchronize(s));
+  int* row_ptr = nullptr;
+  int* col = nullptr;
+  float* val = nullptr;
+  if (alloc_csr(h_nnz, &row_ptr, &col, &val)) return 1;
+  FE_CHECK(hipMemcpyAsync(row_ptr, deg, (n + 1) * sizeof(int), hipMemcpyDeviceToDevice, s));
+  const float norm = static_cast<float>(0.5 / mass);
+  sym_compact<<<(n + 3) / 4, 256, 0, s>>>(raw, raw_ptr, row_ptr, n, norm, col, val);
+  FE_LAUNCH_CHECK("sym_compact");
+  FE_CHECK(hipStreamSynchronize(s));
+  csr->row_ptr = row_ptr;
+  csr->col = col;
+  csr->val = val;
+  csr->nnz = h_nnz;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// The resident optimizer state
+// ---------------------------------------------------------------------------
+class TsnePlan {
+ public:
+  int n = 0, k = 0, g = 0, f = 0, tpr = 32;
+  long long iters_run = 0;
+  double setup_us = 0.0;
+  DeviceCsr csr;
+  Arena state;
+  Arena csr_store;
+  float2* y[2] = {nullptr, nullptr};
+  float2* gains = nullptr;
+  float2* upd = nullptr;
+  unsigned long long* grids = nullptr;
+  float2 *z1 = nullptr, *z2 = nullptr, *kt = nullptr, *o1 = nullptr, *o2 = nullptr;
+  float4* field = nullptr;
+  float2* tw = nullptr;
+  TsneStats* st = nullptr;
+  LayoutPartial* lpart = nullptr;
+  double* zpart = nullptr;
+  float2* diag = nullptr;  // 2n scratch for diagnostics / output
+  int cur = 0;
+  int upd_blocks = 0, splat_blocks = 0, stat_blocks = 0;
+  Stream stream;
+  struct CachedGraph {
+    StepParams sp;
+    int iters;
+    int parity;
+    std::unique_ptr<GraphExec> exec;
+  };
+  std::vector<CachedGraph> graphs;
+
+  template <typename DistT>
+  int create(const int* idx, const DistT* dist, bool device_inputs, int n_, int k_, int offset,
+             float perplexity, int grid_size, const float* init, unsigned seed, bool centre);
+  int enqueue_iteration(const StepParams& sp);
+  int enqueue_diag(const StepParams& sp);
+  int run(int iters, const StepParams& sp);
+  int layout(float* out_host);
+  int profile(int iters, const StepParams& sp, double* ms);
+};
+
+static int choose_tpr(double avg_row) {
+  if (avg_row < 12) return 4;
+  if (avg_row < 24) return 8;
+  if (avg_row < 48) return 16;
+  if (avg_row < 160) return 32;
+  return 64;
+}
+
+template <typename DistT>
+int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int n_, int k_,
+                     int offset, float perplexity, int grid_size, const float* init, unsigned seed,
+                     bool centre) {
+  n = n_;
+  k = k_;
+  g = resolve_grid_size(n, grid_size);
+  f = 2 * g;
+  if (g < 16 || g > 1024 || (g & (g - 1)) != 0)
+    return fail("grid_size must be a power of two between 16 and 1024");
+  if (stream.init()) return 1;
+  hipStream_t s = stream.get();
+  const auto t0 = std::chrono::steady_clock::now();
+
+  // inputs on the device
+  Arena upload;
+  const int* d_idx = idx;
+  const DistT* d_dist = dist;
+  const size_t total = static_cast<size_t>(n) * k;
+  if (!device_inputs) {
+    ArenaPlan up;
+    up.add<int>(total);
+    up.add<DistT>(total);
+    if (upload.init(up.bytes, "t-SNE KNN upload")) return 1;
+    int* ui = upload.take<int>(total);
+    DistT* ud = upload.take<DistT>(total);
+    FE_CHECK(hipMemcpyAsync(ui, idx, total * sizeof(int), hipMemcpyHostToDevice, s));
+    FE_CHECK(hipMemcpyAsync(ud, dist, total * sizeof(DistT), hipMemcpyHostToDevice, s));
+    d_idx = ui;
+    d_dist = ud;
+  }
+  auto alloc_csr = [&](int nnz, int** row_ptr, int** col, float** val) -> int {
+    const size_t m = nnz > 0 ? static_cast<size_t>(nnz) : 1;
+    ArenaPlan cp;
+    cp.add<int>(n + 1);
+    cp.add<int>(m);
+    cp.add<float>(m);
+    if (csr_store.init(cp.bytes, "t-SNE affinity CSR")) return 1;
+    *row_ptr = csr_store.take<int>(n + 1);
+    *col = csr_store.take<int>(m);
+    *val = csr_store.take<float>(m);
+    return 0;
+  };
+  if (build_affinities<DistT>(d_idx, d_dist, n, k, offset, perplexity, s, &csr, alloc_csr)) return 1;
+  upload.release();
+  tpr = choose_tpr(static_cast<double>(csr.nnz) / n);
+
+  // optimizer state
+  const size_t gg = static_cast<size_t>(g) * g;
+  const size_t gf = static_cast<size_t>(g) * f;
+  upd_blocks = static_cast<int>(std::min<long long>(
+      (static_cast<long long>(n) * tpr + kThreads - 1) / kThreads, 2048));
+  splat_blocks = static_cast<int>(std::min<long long>((n + kThreads - 1) / kThreads, 2048));
+  stat_blocks = splat_blocks;
+  ArenaPlan sp;
+  sp.add<float2>(n);  // y0
+  sp.add<float2>(n);  // y1
+  sp.add<float2>(n);  // gains
+  sp.add<float2>(n);  // upd
+  sp.add<float2>(2 * static_cast<size_t>(n));  // diag
+  sp.add<unsigned long long>(3 * gg);
+  sp.add<float2>(gf);  // z1
+  sp.add<float2>(gf);  // z2
+  sp.add<float2>(static_cast<size_t>(g + 1) * g);  // kt
+  sp.add<float2>(gf);  // o1
+  sp.add<float2>(gf);  // o2
+  sp.add<float4>(gg);  // field
+  sp.add<float2>(f);   // twiddles
+  sp.add<TsneStats>(1);
+  sp.add<LayoutPartial>(std::max(upd_blocks, stat_blocks));
+  sp.add<double>(g + 1);
+  if (state.init(sp.bytes, "t-SNE optimizer state")) return 1;
+  y[0] = state.take<float2>(n);
+  y[1] = state.take<float2>(n);
+  gains = state.take<float2>(n);
+  upd = state.take<float2>(n);
+  diag = state.take<float2>(2 * static_cast<size_t>(n));
+  grids = state.take<unsigned long long>(3 * gg);
+  z1 = state.take<float2>(gf);
+  z2 = state.take<float2>(gf);
+  kt = state.take<float2>(static_cast<size_t>(g + 1) * g);
+  o1 = state.take<float2>(gf);
+  o2 = state.take<float2>(gf);
+  field = state.take<float4>(gg);
+  tw = state.take<float2>(f);
+  st = state.take<TsneStats>(1);
+  lpart = state.take<LayoutPartial>(std::max(upd_blocks, stat_blocks));
+  zpart = state.take<double>(g + 1);
+
+  std::vector<float2> htw(f);
+  for (int t = 0; t < f; ++t) {
+    const double a = -2.0 * M_PI * static_cast<double>(t) / static_cast<double>(f);
+    htw[t] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+  }
+  FE_CHECK(hipMemcpyAsync(tw, htw.data(), f * sizeof(float2), hipMemcpyHostToDevice, s));
+  std::vector<float> y0(2 * static_cast<size_t>(n));
+  if (init) {
+    std::memcpy(y0.data(), init, y0.size() * sizeof(float));
+  } else {
+    // tsne_neighbors.cpp:1944-1950
+    std::mt19937 rng(seed);
+    std::normal_distribution<float> normal(0.0f, 1.0e-4f);
+    for (float& v : y0) v = normal(rng);
+  }
+  for (float v : y0)
+    if (!std::isfinite(v)) return fail("`Y_init` must contain only finite coordinates.");
+  FE_CHECK(hipMemcpyAsync(y[0], y0.data(), y0.size() * sizeof(float), hipMemcpyHostToDevice, s));
+  std::vector<float2> ones(n, make_float2(1.0f, 1.0f));
+  FE_CHECK(hipMemcpyAsync(gains, ones.data(), n * sizeof(float2), hipMemcpyHostToDevice, s));
+  FE_CHECK(hipMemsetAsync(upd, 0, n * sizeof(float2), s));
+  FE_CHECK(hipMemsetAsync(grids, 0, 3 * gg * sizeof(unsigned long long), s));
+  FE_CHECK(hipMemsetAsync(st, 0, sizeof(TsneStats), s));
+  layout_stats_kernel<<<stat_blocks, kThreads, 0, s>>>(y[0], lpart, st, n, g, centre ? 1 : 0);
+  FE_LAUNCH_CHECK("layout_stats_kernel");
+  if (f > 1024) {
+    // conv_cols uses 5*F*8 bytes of dynamic LDS
+    const int bytes = 5 * f * static_cast<int>(sizeof(float2));
+    hipError_t rc = hipSuccess;
+    if (f == 2048)
+      rc = hipFuncSetAttribute(reinterpret_cast<const void*>(&conv_cols<2048>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (check(rc, "hipFuncSetAttribute(conv_cols)")) return 1;
+  }
+  FE_CHECK(hipStreamSynchronize(s));
+  setup_us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+  cur = 0;
+  return 0;
+}
+
+template <int F>
+static int launch_conv(TsnePlan& p, hipStream_t s) {
+  constexpr int G = F / 2;
+  conv_rows_fwd<F><<<G + G / 2, kThreads, 0, s>>>(p.grids, p.st, p.tw, p.z1, p.z2, p.kt);
+  FE_LAUNCH_CHECK("conv_rows_fwd");
+  conv_cols<F><<<G + 1, kThreads, 5 * F * sizeof(float2), s>>>(p.z1, p.z2, p.kt, p.tw, p.o1, p.o2,
+                                                               p.zpart, p.st);
+  FE_LAUNCH_CHECK("conv_cols");
+  conv_rows_inv<F><<<G, kThreads, 0, s>>>(p.o1, p.o2, p.tw, p.field);
+  FE_LAUNCH_CHECK("conv_rows_inv");
+  return 0;
+}
+
+static int launch_conv_any(TsnePlan& p, hipStream_t s) {
+  switch (p.f) {
+    case 32: return launch_conv<32>(p, s);
+    case 64: return launch_conv<64>(p, s);
+    case 128: return launch_conv<128>(p, s);
+    case 256: return launch_conv<256>(p, s);
+    case 512: return launch_conv<512>(p, s);
+    case 1024: return launch_conv<1024>(p, s);
+    case 2048: return launch_conv<2048>(p, s);
+    default: return fail("unsupported FFT size");
+  }
+}
+
+template <int MODE>
+static int launch_gather(TsnePlan& p, const StepParams& sp, hipStream_t s) {
+  const float2* yin = p.y[p.cur];
+  float2* yout = p.y[p.cur ^ 1];
+  float2* da = p.diag;
+  float2* dr = p.diag + p.n;
+#define FE_GATHER(T)                                                                          \
+  gather_attract_update<T, MODE><<<p.upd_blocks, kThreads, 0, s>>>(                           \
+      yin, yout, p.gains, p.upd, p.csr.row_ptr, p.csr.col, p.csr.val, p.field, p.st, p.lpart, \
+      p.n, p.g, sp, da, dr)
+  switch (p.tpr) {
+    case 4: FE_GATHER(4); break;
+    case 8: FE_GATHER(8); break;
+    case 16: FE_GATHER(16); break;
+    case 32: FE_GATHER(32); break;
+    default: FE_GATHER(64); break;
+  }
+#undef FE_GATHER
+  FE_LAUNCH_CHECK("gather_attract_update");
+  return 0;
+}
+
+int TsnePlan::enqueue_iteration(const StepParams& sp) {
+  hipStream_t s = stream.get();
+  splat_kernel<<<splat_blocks, kThreads, 0, s>>>(y[cur], st, grids, n, g);
+  FE_LAUNCH_CHECK("splat_kernel");
+  if (launch_conv_any(*this, s)) return 1;
+  if (launch_gather<0>(*this, sp, s)) return 1;
+  cur ^= 1;
+  return 0;
+}
+
+int TsnePlan::enqueue_diag(const StepParams& sp) {
+  hipStream_t s = stream.get();
+  splat_kernel<<<splat_blocks, kThreads, 0, s>>>(y[cur], st, grids, n, g);
+  FE_LAUNCH_CHECK("splat_kernel");
+  if (launch_conv_any(*this, s)) return 1;
+  return launch_gather<1>(*this, sp, s);
+}
+
+static bool same_params(const StepParams& a, const StepParams& b) {
+  return std::memcmp(&a, &b, sizeof(StepParams)) == 0;
+}
+
+int TsnePlan::run(int iters, const StepParams& sp) {
+  constexpr int kChunk = 50;  // even, so a captured chunk preserves buffer parity
+  hipStream_t s = stream.get();
+  int left = iters;
+  if (graph_capture_enabled() && iters >= kChunk) {
+    GraphExec* exec = nullptr;
+    for (auto& c : graphs)
+      if (c.iters == kChunk && c.parity == cur && same_params(c.sp, sp)) exec = c.exec.get();
+    if (!exec) {
+      hipGraph_t graph = nullptr;
+      bool ok = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+      const int cur0 = cur;
+      for (int it = 0; ok && it < kChunk; ++it) ok = enqueue_iteration(sp) == 0;
+      const hipError_t end = hipStreamEndCapture(s, &graph);
+      cur = cur0;
+      if (ok && end == hipSuccess && graph) {
+        CachedGraph c{sp, kChunk, cur0, std::make_unique<GraphExec>()};
+        if (c.exec->instantiate(graph) == 0) {
+          exec = c.exec.get();
+          graphs.push_back(std::move(c));
+        }
+      } else if (graph) {
+        (void)hipGraphDestroy(graph);
+      }
+      (void)hipGetLastError();
+      clear_error();
+    }
+    if (exec) {
+      while (left >= kChunk) {
+        FE_CHECK(hipGraphLaunch(exec->get(), s));
+        left -= kChunk;
+      }
+    }
+  }
+  for (; left > 0; --left)
+    if (enqueue_iteration(sp)) return 1;
+  iters_run += iters;
+  return 0;
+}
+
+int TsnePlan::layout(float* out_host) {
+  hipStream_t s = stream.get();
+  centred_copy_kernel<<<(n + kThreads - 1) / kThreads, kThreads, 0, s>>>(y[cur], st, diag, n);
+  FE_LAUNCH_CHECK("centred_copy_kernel");
+  FE_CHECK(hipMemcpyAsync(out_host, diag, 2 * static_cast<size_t>(n) * sizeof(float),
+                          hipMemcpyDeviceToHost, s));
+  FE_CHECK(hipStreamSynchronize(s));
+  return 0;
+}
+
+int TsnePlan::profile(int iters, const StepParams& sp, double* ms) {
+  hipStream_t s = stream.get();
+  std::vector<hipEvent_t> ev(6 * iters);
+  for (auto& e : ev) FE_CHECK(hipEventCreate(&e));
+  for (int it = 0; it < iters; ++it) {
+    hipEvent_t* e = &ev[6 * it];
+    FE_CHECK(hipEventRecord(e[0], s));
+    splat_kernel<<<splat_blocks, kThreads, 0, s>>>(y[cur], st, grids, n, g);
+    FE_CHECK(hipEventRecord(e[1], s));
+    const int F = f;
+    int rc = 0;
+#define FE_PROF_CONV(FF)                                                                      \
+  case FF: {                                                                                  \
+    constexpr int G = FF / 2;                                                                 \
+    conv_rows_fwd<FF><<<G + G / 2, kThreads, 0, s>>>(grids, st, tw, z1, z2, kt);              \
+    (void)hipEventRecord(e[2], s);                                                            \
+    conv_cols<FF><<<G + 1, kThreads, 5 * FF * sizeof(float2), s>>>(z1, z2, kt, tw, o1, o2,   \
+                                                                   zpart, st);                \
+    (void)hipEventRecord(e[3], s);                                                            \
+    conv_rows_inv<FF><<<G, kThreads, 0, s>>>(o1, o2, tw, field);                              \
+    break;                                                                                    \
+  }
+    switch (F) {
+      FE_PROF_CONV(32)
+      FE_PROF_CONV(64)
+      FE_PROF_CONV(128)
+      FE_PROF_CONV(256)
+      FE_PROF_CONV(512)
+      FE_PROF_CONV(1024)
+      FE_PROF_CONV(2048)
+      default: rc = 1;
+    }
+#undef FE_PROF_CONV
+    if (rc) return fail("unsupported FFT size");
+    FE_CHECK(hipEventRecord(e[4], s));
+    if (launch_gather<0>(*this, sp, s)) return 1;
+    FE_CHECK(hipEventRecord(e[5], s));
+    cur ^= 1;
+  }
+  FE_CHECK(hipStreamSynchronize(s));
+  for (int q = 0; q < 6; ++q) ms[q] = 0.0;
+  for (int it = 0; it < iters; ++it) {
+    hipEvent_t* e = &ev[6 * it];
+    for (int q = 0; q < 5; ++q) {
+      float t = 0.f;
+      FE_CHECK(hipEventElapsedTime(&t, e[q], e[q + 1]));
+      ms[q] += t;
+    }
+    float t = 0.f;
+    FE_CHECK(hipEventElapsedTime(&t, e[0], e[5]));
+    ms[5] += t;
+  }
+  for (int q = 0; q < 6; ++q) ms[q] /= iters;
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  iters_run += iters;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// shared validation (embedding_cuda_kernels.cpp:5656-5672,
+// embedding_cuda_impl.cpp:2265-2301)
+// ---------------------------------------------------------------------------
+static int validate_tsne(const void* idx, const void* dist, int n, int k, int n_components,
+                         float perplexity, int early_iter, int n_iter, float early_exag,
+                         float exag, float lr, int lr_auto, float mom0, float mom1,
+                         float min_gain, int index_offset) {
+  if (!idx || !dist) return fail("null HIP openTSNE input pointer");
+  if (n < 2 || k < 1) return fail("KNN input must have at least two rows and one neighbor column.");
+  if (k > kMaxK) return fail("HIP openTSNE FFT-grid currently supports at most 256 neighbors.");
+  if (n_components != 2)
+    return fail("HIP openTSNE FFT-grid currently supports exactly two output components.");
+  if (static_cast<long long>(n) * k * 2 > 2147483647ll)
+    return fail("n * k is too large for 32-bit CSR offsets.");
+  if (!(perplexity > 0.0f)) return fail("perplexity must be positive");
+  if (early_iter < 0 || n_iter < 0 || early_iter + n_iter < 1)
+    return fail("HIP openTSNE FFT-grid requires at least one optimization iteration.");
+  if (!(early_exag > 0.0f) || !(exag > 0.0f))
+    return fail("HIP openTSNE FFT-grid exaggeration values must be positive.");
+  if (!lr_auto && !(lr > 0.0f)) return fail("HIP openTSNE FFT-grid learning rate must be positive.");
+  if (mom0 < 0.0f || mom1 < 0.0f)
+    return fail("HIP openTSNE FFT-grid momentum values must be non-negative.");
+  if (!(min_gain > 0.0f)) return fail("HIP openTSNE FFT-grid min_gain must be positive.");
+  if (index_offset != 0 && index_offset != 1) return fail("index_offset must be 0 or 1");
+  return 0;
+}
+
+template <typename DistT>
+static int tsne_one_shot(const int* indices, const DistT* distances, bool device_inputs,
+                         const float* init, int has_init, int n, int k, int n_components,
+                         float perplexity, int early_iter, int n_iter, float early_exag,
+                         float exag, float lr, int lr_auto, float mom0, float mom1, float min_gain,
+                         float max_step, unsigned seed, int index_offset, int grid_size,
+                         float* out) {
+  clear_error();
+  if (validate_tsne(indices, distances, n, k, n_components, perplexity, early_iter, n_iter,
+                    early_exag, exag, lr, lr_auto, mom0, mom1, min_gain, index_offset))
+    return 1;
+  if (!out || (has_init && !init)) return fail("null HIP openTSNE input pointer");
+  TsnePlan plan;
+  if (plan.create<DistT>(indices, distances, device_inputs, n, k, index_offset, perplexity,
+                         grid_size, has_init ? init : nullptr, seed, true))
+    return 1;
+  const float lr_early = lr_auto ? static_cast<float>(n) / early_exag : lr;
+  const float lr_normal = lr_auto ? static_cast<float>(n) / exag : lr;
+  if (early_iter > 0 &&
+      plan.run(early_iter, StepParams{early_exag, lr_early, mom0, min_gain, max_step}))
+    return 1;
+  if (n_iter > 0 && plan.run(n_iter, StepParams{exag, lr_normal, mom1, min_gain, max_step}))
+    return 1;
+  return plan.layout(out);
+}
+
+}  // namespace fe
+
+using fe::fail;
+
+struct fastembedr_hip_tsne_plan {
+  fe::TsnePlan impl;
+};
+
+extern "C" {
+
+int fastembedr_hip_available(void) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1) return 0;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return 0;
+  return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
+}
+
+const char* fastembedr_hip_last_error(void) { return fe::last_error().c_str(); }
+
+int fastembedr_hip_opentsne_fft_grid_size(int n) { return fe::resolve_grid_size(n, 0); }
+
+int fastembedr_hip_opentsne_fft_from_knn(const int* indices, const double* distances,
+                                         const float* init, int has_init, int n, int k,
+                                         int n_components, float perplexity,
+                                         int early_exaggeration_iter, int n_iter,
+                                         float early_exaggeration, float exaggeration,
+                                         float learning_rate, int learning_rate_auto,
+                                         float initial_momentum, float final_momentum,
+                                         float min_gain, float max_step_norm, unsigned int seed,
+                                         int index_offset, int grid_size, float* out) {
+  return fe::tsne_one_shot<double>(indices, distances, false, init, has_init, n, k, n_components,
+                                   perplexity, early_exaggeration_iter, n_iter,
+                                   early_exaggeration, exaggeration, learning_rate,
+                                   learning_rate_auto, initial_momentum, final_momentum, min_gain,
+                                   max_step_norm, seed, index_offset, grid_size, out);
+}
+
+int fastembedr_hip_opentsne_fft_from_knn_float(const int* indices, const float* distances,
+                                               const float* init, int has_init, int n, int k,
+                                               int n_components, float perplexity,
+                                               int early_exaggeration_iter, int n_iter,
+                                               float early_exaggeration, float exaggeration,
+                                               float learning_rate, int learning_rate_auto,
+                                               float initial_momentum, float final_momentum,
+                                               float min_gain, float max_step_norm,
+                                               unsigned int seed, int index_offset,
+                                               int grid_size, float* out) {
+  return fe::tsne_one_shot<float>(indices, distances, false, init, has_init, n, k, n_components,
+                                  perplexity, early_exaggeration_iter, n_iter, early_exaggeration,
+                                  exaggeration, learning_rate, learning_rate_auto,
+                                  initial_momentum, final_momentum, min_gain, max_step_norm, seed,
+                                  index_offset, grid_size, out);
+}
+
+int fastembedr_hip_opentsne_fft_from_device_knn_float(
+    const int* indices, const float* distances, const float* init, int has_init, int n, int k,
+    int n_components, float perplexity, int early_exaggeration_iter, int n_iter,
+    float early_exaggeration, float exaggeration, float learning_rate, int learning_rate_auto,
+    float initial_momentum, float final_momentum, float min_gain, float max_step_norm,
+    unsigned int seed, int index_offset, int grid_size, float* out) {
+  return fe::tsne_one_shot<float>(indices, distances, true, init, has_init, n, k, n_components,
+                                  perplexity, early_exaggeration_iter, n_iter, early_exaggeration,
+                                  exaggeration, learning_rate, learning_rate_auto,
+                                  initial_momentum, final_momentum, min_gain, max_step_norm, seed,
+                                  index_offset, grid_size, out);
+}
+
+int fastembedr_hip_tsne_gradient_diagnostic(const int* indices, const float* distances, int n,
+                                            int k, int index_offset, float perplexity,
+                                            float exaggeration, int grid_size,
+                                            const float* layout, float* attractive,
+                                            float* repulsive, float* gradient, double* sum_q,
+                                            int* row_ptr, int* col, float* val,
+                                            long long* nnz) {
+  fe::clear_error();
+  if (fe::validate_tsne(indices, distances, n, k, 2, perplexity, 0, 1, 1.0f, 1.0f, 1.0f, 1, 0.0f,
+                        0.0f, 1.0f, index_offset))
+    return 1;
+  if (!layout) return fail("`layout` must have one row per KNN row and exactly two columns.");
+  if (!(exaggeration > 0.0f) || !std::isfinite(exaggeration))
+    return fail("`exaggeration` must be positive and finite.");
+  if (grid_size < 16 || grid_size > 1024 || (grid_size & (grid_size - 1)) != 0)
+    return fail("`grid_size` must be a power of two between 16 and 1024.");
+  fe::TsnePlan p;
+  if (p.create<float>(indices, distances, false, n, k, index_offset, perplexity, grid_size, layout,
+                      0u, false))
+    return 1;
+  if (p.enqueue_diag(fe::StepParams{exaggeration, 0.f, 0.f, 0.01f, 0.f})) return 1;
+  hipStream_t s = p.stream.get();
+  std::vector<float> a(2 * static_cast<size_t>(n)), r(2 * static_cast<size_t>(n));
+  FE_CHECK(hipMemcpyAsync(a.data(), p.diag, a.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+  FE_CHECK(hipMemcpyAsync(r.data(), p.diag + n, r.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+  fe::TsneStats hst;
+  FE_CHECK(hipMemcpyAsync(&hst, p.st, sizeof(hst), hipMemcpyDeviceToHost, s));
+  FE_CHECK(hipStreamSynchronize(s));
+  if (attractive) std::memcpy(attractive, a.data(), a.size() * sizeof(float));
+  if (repulsive) std::memcpy(repulsive, r.data(), r.size() * sizeof(float));
+  if (gradient)
+    for (size_t t = 0; t < a.size(); ++t) gradient[t] = a[t] + r[t];
+  if (sum_q) *sum_q = std::fmax(hst.sum_q_total - static_cast<double>(n), static_cast<double>(FLT_MIN));
+  if (nnz) *nnz = p.csr.nnz;
+  if (row_ptr)
+    FE_CHECK(hipMemcpy(row_ptr, p.csr.row_ptr, (n + 1) * sizeof(int), hipMemcpyDeviceToHost));
+  if (col) FE_CHECK(hipMemcpy(col, p.csr.col, p.csr.nnz * sizeof(int), hipMemcpyDeviceToHost));
+  if (val) FE_CHECK(hipMemcpy(val, p.csr.val, p.csr.nnz * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int fastembedr_hip_tsne_plan_create(const int* indices, const float* distances,
+                                    int device_inputs, int n, int k, int index_offset,
+                                    float perplexity, int grid_size, const float* init,
+                                    unsigned int seed, fastembedr_hip_tsne_plan** plan) {
+  fe::clear_error();
+  if (!plan) return fail("null plan pointer");
+  *plan = nullptr;
+  if (fe::validate_tsne(indices, distances, n, k, 2, perplexity, 0, 1, 1.0f, 1.0f, 1.0f, 1, 0.0f,
+                        0.0f, 1.0f, index_offset))
+    return 1;
+  auto* p = new (std::nothrow) fastembedr_hip_tsne_plan;
+  if (!p) return fail("out of host memory");
+  if (p->impl.create<float>(indices, distances, device_inputs != 0, n, k, index_offset, perplexity,
+                            grid_size, init, seed, true)) {
+    delete p;
+    return 1;
+  }
+  *plan = p;
+  return 0;
+}
+
+int fastembedr_hip_tsne_plan_run(fastembedr_hip_tsne_plan* plan, int n_iter, float exaggeration,
+                                 float momentum, float learning_rate, float min_gain,
+                                 float max_step_norm) {
+  fe::clear_error();
+  if (!plan) return fail("null plan");
+  if (n_iter < 0) return fail("iteration counts must be non-negative.");
+  if (!(exaggeration > 0.0f) || !(learning_rate > 0.0f) || momentum < 0.0f || !(min_gain > 0.0f))
+    return fail("invalid t-SNE phase parameters");
+  return plan->impl.run(n_iter,
+                        fe::StepParams{exaggeration, learning_rate, momentum, min_gain, max_step_norm});
+}
+
+int fastembedr_hip_tsne_plan_sync(fastembedr_hip_tsne_plan* plan) {
+  if (!plan) return fail("null plan");
+  return fe::check(hipStreamSynchronize(plan->impl.stream.get()), "hipStreamSynchronize");
+}
+
+int fastembedr_hip_tsne_plan_layout(fastembedr_hip_tsne_plan* plan, float* out) {
+  fe::clear_error();
+  if (!plan || !out) return fail("null plan or output");
+  return plan->impl.layout(out);
+}
+
+int fastembedr_hip_tsne_plan_info(fastembedr_hip_tsne_plan* plan, long long* info) {
+  if (!plan || !info) return fail("null plan or output");
+  const fe::TsnePlan& p = plan->impl;
+  info[0] = p.n;
+  info[1] = p.k;
+  info[2] = p.g;
+  info[3] = p.csr.nnz;
+  info[4] = p.tpr;
+  info[5] = static_cast<long long>(p.state.capacity() + p.csr_store.capacity());
+  info[6] = p.iters_run;
+  info[7] = static_cast<long long>(p.setup_us);
+  return 0;
+}
+
+int fastembedr_hip_tsne_plan_affinities(fastembedr_hip_tsne_plan* plan, int* row_ptr, int* col,
+                                        float* val) {
+  fe::clear_error();
+  if (!plan) return fail("null plan");
+  const fe::TsnePlan& p = plan->impl;
+  FE_CHECK(hipStreamSynchronize(p.stream.get()));
+  if (row_ptr)
+    FE_CHECK(hipMemcpy(row_ptr, p.csr.row_ptr, (p.n + 1) * sizeof(int), hipMemcpyDeviceToHost));
+  if (col) FE_CHECK(hipMemcpy(col, p.csr.col, p.csr.nnz * sizeof(int), hipMemcpyDeviceToHost));
+  if (val) FE_CHECK(hipMemcpy(val, p.csr.val, p.csr.nnz * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int fastembedr_hip_tsne_plan_profile(fastembedr_hip_tsne_plan* plan, int n_iter,
+                                     float exaggeration, float momentum, float learning_rate,
+                                     float min_gain, float max_step_norm, double* kernel_ms) {
+  fe::clear_error();
+  if (!plan || !kernel_ms || n_iter < 1) return fail("invalid profile arguments");
+  return plan->impl.profile(
+      n_iter, fe::StepParams{exaggeration, learning_rate, momentum, min_gain, max_step_norm},
+      kernel_ms);
+}
+
+void fastembedr_hip_tsne_plan_destroy(fastembedr_hip_tsne_plan* plan) {
+  if (plan) {
+    (void)hipStreamSynchronize(plan->impl.stream.get());
+    delete plan;
+  }
+}
+
+}  // extern "C"

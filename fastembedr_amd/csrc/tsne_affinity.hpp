@@ -1,0 +1,319 @@
+// Device construction of the symmetric t-SNE affinity matrix P (setup path,
+// run once per job).
+//
+//   perplexity_rows   one wavefront per KNN row: FP64 bisection on beta to the
+//                     target entropy log(perplexity), with the min-d^2 shift,
+//                     tolerance 1e-5 and 200 iterations of
+//                     compute_row_probabilities_flat (tsne_neighbors.cpp:
+//                     452-530).  Reads the reference's column-major k x n
+//                     KNN layout through an LDS transpose; writes row-major
+//                     0-based neighbour ids and float p_{j|i}.
+//   sym_count / sym_fill / sym_sort_merge / sym_compact
+//                     the symmetrisation of build_tsne_probabilities_float
+//                     (tsne_neighbors.cpp:616-724): every directed entry is
+//                     placed in both endpoint rows, each row is sorted by
+//                     (column, value bits) so the result does not depend on
+//                     atomic arrival order, equal columns are summed in FP64,
+//                     and P_ij = float(sum) * float(0.5 / total_mass).  Rows
+//                     come out with ascending columns, exactly as the
+//                     reference's key-sorted fill lays them out.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+namespace fe {
+
+constexpr int kAffRowsPerBlock = 64;  // rows staged per 256-thread block
+constexpr int kMaxK = 256;            // embedding_cuda_kernels.cpp:5661
+constexpr int kSortCap = 8192;        // longest symmetric row sorted in LDS
+
+enum AffinityError : unsigned { kErrIndexRange = 1u, kErrDistance = 2u, kErrRowTooLong = 4u };
+
+template <typename DistT>
+constexpr int aff_rows_per_block() {
+  return sizeof(DistT) == 8 ? kAffRowsPerBlock / 2 : kAffRowsPerBlock;
+}
+
+template <typename DistT>
+__global__ __launch_bounds__(256) void perplexity_rows(const int* __restrict__ indices,
+                                                       const DistT* __restrict__ distances,
+                                                       int n, int k, int index_offset,
+                                                       double perplexity,
+                                                       int* __restrict__ nbr,
+                                                       float* __restrict__ pdir,
+                                                       unsigned* error_flags) {
+  constexpr int RPB = aff_rows_per_block<DistT>();
+  extern __shared__ unsigned char smem_raw[];
+  DistT* s_d = reinterpret_cast<DistT*>(smem_raw);
+  int* s_idx = reinterpret_cast<int*>(s_d + RPB * k);
+  const int row0 = blockIdx.x * RPB;
+  const int rows = min(RPB, n - row0);
+  // coalesced column-major loads: for each neighbour column j, rows are
+  // contiguous in memory
+  for (int t = threadIdx.x; t < rows * k; t += blockDim.x) {
+    const int j = t / rows, r = t - j * rows;
+    const size_t src = static_cast<size_t>(j) * n + row0 + r;
+    const int nb = indices[src] - index_offset;
+    const DistT d = distances[src];
+    if (nb < 0 || nb >= n) atomicOr(error_flags, kErrIndexRange);
+    if (!isfinite(static_cast<double>(d)) || d < DistT(0)) atomicOr(error_flags, kErrDistance);
+    s_idx[r * k + j] = nb;
+    s_d[r * k + j] = d;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  constexpr int PER = kMaxK / 64;
+  const double target = log(perplexity);
+  for (int r = wid; r < rows; r += blockDim.x / 64) {
+    const int row = row0 + r;
+    double d2[PER];
+    double mn = DBL_MAX, mx = 0.0;
+#pragma unroll
+    for (int s = 0; s < PER; ++s) {
+      const int j = lane + 64 * s;
+      d2[s] = 0.0;
+      if (j < k) {
+        const double d = static_cast<double>(s_d[r * k + j]);
+        d2[s] = d * d;
+        mn = fmin(mn, d2[s]);
+        mx = fmax(mx, d2[s]);
+      }
+    }
+    mn = -wave_max(-mn);
+    mx = wave_max(mx);
+    double p[PER];
+    const double spread = mx - mn;
+    if (spread <= DBL_EPSILON * fmax(1.0, mx)) {
+#pragma unroll
+      for (int s = 0; s < PER; ++s) p[s] = 1.0 / static_cast<double>(k);
+    } else {
+      bool found = false;
+      double beta = 1.0, min_beta = -DBL_MAX, max_beta = DBL_MAX, sum_p = DBL_MIN;
+      for (int iter = 0; !found && iter < 200; ++iter) {
+        double sp = 0.0, sh = 0.0;
+#pragma unroll
+        for (int s = 0; s < PER; ++s) {
+          const int j = lane + 64 * s;
+          p[s] = 0.0;
+          if (j < k) {
+            const double sh2 = d2[s] - mn;
+            p[s] = exp(-beta * sh2);
+            sp += p[s];
+            sh += sh2 * p[s];
+          }
+        }
+        sum_p = DBL_MIN + wave_sum(sp);
+        sh = wave_sum(sh);
+        const double entropy = beta * sh / sum_p + log(sum_p);
+        const double diff = entropy - target;
+        if (fabs(diff) < 1e-5) {
+          found = true;
+        } else if (diff > 0.0) {
+          min_beta = beta;
+          beta = (max_beta == DBL_MAX || max_beta == -DBL_MAX) ? beta * 2.0 : (beta + max_beta) / 2.0;
+        } else {
+          max_beta = beta;
+          beta = (min_beta == -DBL_MAX || min_beta == DBL_MAX) ? beta / 2.0 : (beta + min_beta) / 2.0;
+        }
+        if (!isfinite(beta)) break;
+      }
+      if (!isfinite(sum_p) || sum_p <= DBL_MIN) {
+        double tied = 0.0;
+#pragma unroll
+        for (int s = 0; s < PER; ++s) {
+          const int j = lane + 64 * s;
+          if (j < k && fabs(d2[s] - mn) <= DBL_EPSILON * fmax(1.0, mn)) tied += 1.0;
+        }
+        tied = wave_sum(tied);
+        const double mass = 1.0 / fmax(tied, 1.0);
+#pragma unroll
+        for (int s = 0; s < PER; ++s) p[s] = fabs(d2[s] - mn) <= DBL_EPSILON * fmax(1.0, mn) ? mass : 0.0;
+      } else {
+        const double inv = 1.0 / sum_p;
+#pragma unroll
+        for (int s = 0; s < PER; ++s) p[s] *= inv;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < PER; ++s) {
+      const int j = lane + 64 * s;
+      if (j < k) {
+        const size_t o = static_cast<size_t>(row) * k + j;
+        nbr[o] = s_idx[r * k + j];
+        pdir[o] = static_cast<float>(p[s]);
+      }
+    }
+  }
+}
+
+// Count both endpoints of every non-self directed entry; block partial sums of
+// the directed mass (FP64) for the global normaliser.
+__global__ __launch_bounds__(256) void sym_count(const int* __restrict__ nbr,
+                                                 const float* __restrict__ pdir, long long total,
+                                                 int k, int* __restrict__ deg,
+                                                 double* __restrict__ mass_partials) {
+  double local = 0.0;
+  for (long long e = blockIdx.x * 256ll + threadIdx.x; e < total; e += gridDim.x * 256ll) {
+    const int i = static_cast<int>(e / k);
+    const int j = nbr[e];
+    if (j == i) continue;
+    atomicAdd(deg + i, 1);
+    atomicAdd(deg + j, 1);
+    local += static_cast<double>(pdir[e]);
+  }
+  __shared__ double red[4];
+  local = wave_sum(local);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = local;
+  __syncthreads();
+  if (threadIdx.x == 0) mass_partials[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ __launch_bounds__(256) void sym_fill(const int* __restrict__ nbr,
+                                                const float* __restrict__ pdir, long long total,
+                                                int k, const int* __restrict__ raw_ptr,
+                                                int* __restrict__ cursor,
+                                                unsigned long long* __restrict__ raw) {
+  for (long long e = blockIdx.x * 256ll + threadIdx.x; e < total; e += gridDim.x * 256ll) {
+    const int i = static_cast<int>(e / k);
+    const int j = nbr[e];
+    if (j == i) continue;
+    const unsigned long long bits = __float_as_uint(pdir[e]);
+    int pos = raw_ptr[i] + atomicAdd(cursor + i, 1);
+    raw[pos] = (static_cast<unsigned long long>(static_cast<unsigned>(j)) << 32) | bits;
+    pos = raw_ptr[j] + atomicAdd(cursor + j, 1);
+    raw[pos] = (static_cast<unsigned long long>(static_cast<unsigned>(i)) << 32) | bits;
+  }
+}
+
+// One block per row: LDS bitonic sort of (col << 32 | value bits), then a
+// merge of equal columns with FP64 sums.  Merged entries are written back to
+// the front of the row's raw range as (col << 32 | float(sum) bits).
+__global__ __launch_bounds__(256) void sym_sort_merge(unsigned long long* __restrict__ raw,
+                                                      const int* __restrict__ raw_ptr,
+                                                      int* __restrict__ merged_len,
+                                                      unsigned* error_flags) {
+  __shared__ unsigned long long keys[kSortCap];
+  __shared__ int flag_scan[256];
+  const int row = blockIdx.x;
+  const int beg = raw_ptr[row], len = raw_ptr[row + 1] - beg;
+  if (len > kSortCap) {
+    if (threadIdx.x == 0) {
+      atomicOr(error_flags, kErrRowTooLong);
+      merged_len[row] = 0;
+    }
+    return;
+  }
+  int pw = 1;
+  while (pw < len) pw <<= 1;
+  for (int t = threadIdx.x; t < pw; t += 256) keys[t] = t < len ? raw[beg + t] : ~0ull;
+  __syncthreads();
+  for (int size = 2; size <= pw; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = threadIdx.x; t < pw / 2; t += 256) {
+        const int lo = 2 * stride * (t / stride) + (t % stride);
+        const int hi = lo + stride;
+        const bool up = ((lo & size) == 0);
+        const unsigned long long a = keys[lo], b = keys[hi];
+        if ((a > b) == up) {
+          keys[lo] = b;
+          keys[hi] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // merge: a thread owns a chunk of consecutive sorted entries
+  constexpr int CH = kSortCap / 256;
+  const int c0 = threadIdx.x * CH;
+  int heads = 0;
+  for (int t = c0; t < min(c0 + CH, len); ++t)
+    if (t == 0 || (keys[t] >> 32) != (keys[t - 1] >> 32)) ++heads;
+  flag_scan[threadIdx.x] = heads;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int w = 0; w < 256; ++w) {
+      const int v = flag_scan[w];
+      flag_scan[w] = run;
+      run += v;
+    }
+    merged_len[row] = run;
+  }
+  __syncthreads();
+  int out = flag_scan[threadIdx.x];
+  for (int t = c0; t < min(c0 + CH, len); ++t) {
+    if (t == 0 || (keys[t] >> 32) != (keys[t - 1] >> 32)) {
+      const unsigned long long cidx = keys[t] >> 32;
+      double s = 0.0;
+      for (int u = t; u < len && (keys[u] >> 32) == cidx; ++u)
+        s += static_cast<double>(__uint_as_float(static_cast<unsigned>(keys[u] & 0xffffffffu)));
+      raw[beg + out] = (cidx << 32) | __float_as_uint(static_cast<float>(s));
+      ++out;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void sym_compact(const unsigned long long* __restrict__ raw,
+                                                   const int* __restrict__ raw_ptr,
+                                                   const int* __restrict__ row_ptr, int n,
+                                                   float norm, int* __restrict__ col,
+                                                   float* __restrict__ val) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= n) return;
+  const int lane = threadIdx.x & 63;
+  const int src = raw_ptr[row], dst = row_ptr[row], len = row_ptr[row + 1] - dst;
+  for (int t = lane; t < len; t += 64) {
+    const unsigned long long e = raw[src + t];
+    col[dst + t] = static_cast<int>(e >> 32);
+    val[dst + t] = __uint_as_float(static_cast<unsigned>(e & 0xffffffffu)) * norm;
+  }
+}
+
+// Exclusive scan of n ints into out[0..n] (out[n] = total), three launches:
+// per-block scans, a single-block scan of the block totals, offsets.
+constexpr int kScanBlock = 1024;
+
+__global__ __launch_bounds__(256) void scan_blocks(const int* __restrict__ in, int n,
+                                                   int* __restrict__ out, int* __restrict__ sums) {
+  __shared__ int s[kScanBlock];
+  const int base = blockIdx.x * kScanBlock;
+  for (int t = threadIdx.x; t < kScanBlock; t += 256) s[t] = base + t < n ? in[base + t] : 0;
+  __syncthreads();
+  for (int off = 1; off < kScanBlock; off <<= 1) {
+    int v[4];
+    for (int q = 0; q < 4; ++q) {
+      const int t = threadIdx.x + q * 256;
+      v[q] = t >= off ? s[t - off] : 0;
+    }
+    __syncthreads();
+    for (int q = 0; q < 4; ++q) s[threadIdx.x + q * 256] += v[q];
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < kScanBlock; t += 256) {
+    if (base + t < n) out[base + t] = t == 0 ? 0 : s[t - 1];
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = s[kScanBlock - 1];
+}
+
+__global__ __launch_bounds__(256) void scan_sums(int* sums, int nb, int* total_out) {
+  if (threadIdx.x == 0) {
+    long long run = 0;
+    for (int b = 0; b < nb; ++b) {
+      const int v = sums[b];
+      sums[b] = static_cast<int>(run);
+      run += v;
+    }
+    *total_out = static_cast<int>(run);
+  }
+}
+
+__global__ __launch_bounds__(256) void scan_add(int* __restrict__ out, int n,
+                                                const int* __restrict__ sums) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i] += sums[i / kScanBlock];
+}
+
+}  // namespace fe

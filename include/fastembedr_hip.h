@@ -1,0 +1,143 @@
+/*
+ * fastembedr_hip.h -- C-ABI of the MI355X (gfx950) embedding-optimizer
+ * backend.  Plain pointers and sizes only; every function returns 0 on
+ * success and 1 on failure (message from fastembedr_hip_last_error()), and
+ * nothing throws across this boundary.
+ *
+ * Layout conventions are the reference CUDA ABI's
+ * (src/embedding_cuda_impl.cpp:789-797, 933-945, 2435-2523):
+ *   indices / distances  R column-major n x k: element (i, j) at j*n + i,
+ *                        int32 ids with `index_offset` (0 or 1), Euclidean
+ *                        (not squared), self excluded;
+ *   init / out           row-major interleaved n x 2 float32.
+ * Host-pointer entry points copy their inputs to the device and free every
+ * device allocation before returning; plan handles own resident state until
+ * destroyed.  Calls are synchronous and run on the current device.
+ */
+#ifndef FASTEMBEDR_HIP_H
+#define FASTEMBEDR_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- availability / errors / policy ---------------------------------- */
+
+/* Replaces fastembedr_cuda_available (src/embedding_cuda_kernels.cpp:3833-3841). */
+int fastembedr_hip_available(void);
+
+/* Replaces fastembedr_cuda_embedding_last_error (embedding_cuda_kernels.cpp:3829-3831). */
+const char* fastembedr_hip_last_error(void);
+
+/* Replaces fastembedr_cuda_opentsne_fft_grid_size (embedding_cuda_kernels.cpp:5625-5627):
+ * 256 for n < 20000, 1024 for n >= 100000, else 512; FASTEMBEDR_TSNE_FFT_GRID
+ * in {16, 32, ..., 1024} overrides (resolve_cuda_tsne_fft_grid_size :5610-5623). */
+int fastembedr_hip_opentsne_fft_grid_size(int n);
+
+/* ---- one-shot t-SNE (reference: embedding_cuda_kernels.cpp:6178-6264) ---
+ * Same argument order as the CUDA functions plus `grid_size` (0 = the rule
+ * above).  n_components must be 2; k <= 256.  learning_rate_auto != 0 uses
+ * n / exaggeration of each phase (:6077-6082).  Without init the layout is
+ * N(0, 1e-4^2) from std::mt19937(seed) (tsne_neighbors.cpp:1944-1950).      */
+
+/* Replaces fastembedr_cuda_opentsne_fft_from_knn (double distances, :6178-6205). */
+int fastembedr_hip_opentsne_fft_from_knn(const int* indices, const double* distances,
+                                         const float* init, int has_init, int n, int k,
+                                         int n_components, float perplexity,
+                                         int early_exaggeration_iter, int n_iter,
+                                         float early_exaggeration, float exaggeration,
+                                         float learning_rate, int learning_rate_auto,
+                                         float initial_momentum, float final_momentum,
+                                         float min_gain, float max_step_norm, unsigned int seed,
+                                         int index_offset, int grid_size, float* out);
+
+/* Replaces fastembedr_cuda_opentsne_fft_from_knn_float (:6207-6234). */
+int fastembedr_hip_opentsne_fft_from_knn_float(const int* indices, const float* distances,
+                                               const float* init, int has_init, int n, int k,
+                                               int n_components, float perplexity,
+                                               int early_exaggeration_iter, int n_iter,
+                                               float early_exaggeration, float exaggeration,
+                                               float learning_rate, int learning_rate_auto,
+                                               float initial_momentum, float final_momentum,
+                                               float min_gain, float max_step_norm,
+                                               unsigned int seed, int index_offset,
+                                               int grid_size, float* out);
+
+/* Replaces fastembedr_cuda_opentsne_fft_from_device_knn_float (:6236-6264):
+ * indices/distances are device pointers (borrowed); init/out are host. */
+int fastembedr_hip_opentsne_fft_from_device_knn_float(
+    const int* indices, const float* distances, const float* init, int has_init, int n, int k,
+    int n_components, float perplexity, int early_exaggeration_iter, int n_iter,
+    float early_exaggeration, float exaggeration, float learning_rate, int learning_rate_auto,
+    float initial_momentum, float final_momentum, float min_gain, float max_step_norm,
+    unsigned int seed, int index_offset, int grid_size, float* out);
+
+/* ---- force diagnostic ------------------------------------------------
+ * HIP counterpart of opentsne_force_diagnostic_cpp (tsne_neighbors.cpp:
+ * 1698-1817): from a fixed row-major n x 2 layout, the attractive force
+ * (exaggeration included), the FFT-grid repulsive force and their sum (the
+ * optimizer gradient, = objective gradient / 4), the grid estimate of
+ * Z = sum_{i != j} q_ij, and optionally the symmetric P as 0-based CSR
+ * (row_ptr n+1, col/val capacity 2*n*k; *nnz receives the count).  Host
+ * pointers; any output may be NULL. */
+int fastembedr_hip_tsne_gradient_diagnostic(const int* indices, const float* distances, int n,
+                                            int k, int index_offset, float perplexity,
+                                            float exaggeration, int grid_size,
+                                            const float* layout, float* attractive,
+                                            float* repulsive, float* gradient, double* sum_q,
+                                            int* row_ptr, int* col, float* val,
+                                            long long* nnz);
+
+/* ---- resident plans ----------------------------------------------------
+ * No counterpart in the reference (which frees everything per call,
+ * embedding_cuda_kernels.cpp:88-127): a plan keeps P, the layout, the
+ * optimizer state, the FFT work arrays and captured iteration graphs on the
+ * device so a caller can run phases, read the layout, time kernels, and
+ * serve repeated requests without re-uploading.  Handles are opaque. */
+typedef struct fastembedr_hip_tsne_plan fastembedr_hip_tsne_plan;
+
+/* indices/distances host (device_inputs = 0) or device (1) pointers in the
+ * column-major layout above; init is host row-major n x 2 or NULL. */
+int fastembedr_hip_tsne_plan_create(const int* indices, const float* distances,
+                                    int device_inputs, int n, int k, int index_offset,
+                                    float perplexity, int grid_size, const float* init,
+                                    unsigned int seed, fastembedr_hip_tsne_plan** plan);
+
+/* Enqueue n_iter optimizer iterations with one phase's constants (captured
+ * into hipGraphs of up to 50 iterations unless FASTEMBEDR_HIP_GRAPH_CAPTURE=0).
+ * Asynchronous: call _plan_sync or _plan_layout to wait. */
+int fastembedr_hip_tsne_plan_run(fastembedr_hip_tsne_plan* plan, int n_iter, float exaggeration,
+                                 float momentum, float learning_rate, float min_gain,
+                                 float max_step_norm);
+
+int fastembedr_hip_tsne_plan_sync(fastembedr_hip_tsne_plan* plan);
+
+/* Centred layout, host row-major n x 2. */
+int fastembedr_hip_tsne_plan_layout(fastembedr_hip_tsne_plan* plan, float* out);
+
+/* info[0..7] = n, k, grid size G, nnz of P, lanes per CSR row, device bytes,
+ * iterations run, CSR setup microseconds. */
+int fastembedr_hip_tsne_plan_info(fastembedr_hip_tsne_plan* plan, long long* info);
+
+/* Copies the plan's symmetric P to host: row_ptr (n+1), col (nnz, 0-based,
+ * ascending per row), val (nnz) -- the affinity_row_ptr0 / affinity_col1 /
+ * affinity_weight fields of opentsne_force_diagnostic_cpp
+ * (tsne_neighbors.cpp:1793-1808), with 0-based columns. */
+int fastembedr_hip_tsne_plan_affinities(fastembedr_hip_tsne_plan* plan, int* row_ptr, int* col,
+                                        float* val);
+
+/* Runs n_iter eager iterations with hipEvents between the five kernels on
+ * the plan's stream; kernel_ms[0..4] = mean ms of splat, conv_rows_fwd,
+ * conv_cols, conv_rows_inv, gather_attract_update; kernel_ms[5] = mean ms
+ * per iteration. */
+int fastembedr_hip_tsne_plan_profile(fastembedr_hip_tsne_plan* plan, int n_iter,
+                                     float exaggeration, float momentum, float learning_rate,
+                                     float min_gain, float max_step_norm, double* kernel_ms);
+
+void fastembedr_hip_tsne_plan_destroy(fastembedr_hip_tsne_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FASTEMBEDR_HIP_H */

@@ -82,6 +82,8 @@ def _declare(L):
     L.port_tsne_run.argtypes = [c_int, c_int, _ip, _fp, _fp, c_double, c_int, c_int, c_double,
                                 c_double, c_double, c_double, c_double, c_double, c_double,
                                 c_int, c_int, _fp, _dp]
+    L.port_tsne_iterate.argtypes = [c_int, _ip, _ip, _fp, _fp, c_int, c_double, c_double,
+                                    c_double, c_double, c_double, c_int, c_int, _dp]
     if hasattr(L, "or_umap_epoch"):
         _declare_umap(L)
     if hasattr(L, "or_transform"):
@@ -282,6 +284,25 @@ def structure_metrics(high, low, k):
     if rc != 0:
         raise ValueError("each k must satisfy 1 <= k < (2 * n - 1) / 3")
     return out
+
+
+def port_tsne_iterate(row_ptr, col, val, y, iters, exag=12.0, learning_rate=None, momentum=0.8,
+                      min_gain=0.01, max_step=5.0, grid=0, n_threads=None):
+    """float32 reference-algorithm iterations from a given CSR P; returns
+    (Y, seconds).  y is copied."""
+    rp, c = _i(row_ptr), _i(col)
+    v = np.ascontiguousarray(val, dtype=np.float32)
+    yy = np.ascontiguousarray(y, dtype=np.float32).copy()
+    n = len(rp) - 1
+    lr = n / exag if learning_rate is None else learning_rate
+    sec = np.zeros(1)
+    rc = lib().port_tsne_iterate(n, _ptr(rp, _ip), _ptr(c, _ip), _ptr(v, _fp), _ptr(yy, _fp),
+                                 int(iters), float(exag), float(lr), float(momentum),
+                                 float(min_gain), float(max_step), int(grid),
+                                 int(n_threads or os.cpu_count() or 1), _ptr(sec, _dp))
+    if rc != 0:
+        raise ValueError("port error %d" % rc)
+    return yy, float(sec[0])
 
 
 def port_tsne_run(indices, distances, y_init, perplexity=30.0, early_iter=250, n_iter=500,

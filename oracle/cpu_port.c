@@ -739,6 +739,44 @@ static void zero_mean_f(float* y, int n) {
 }
 
 /*
+ * Optimisation only, from a given symmetric P (0-based CSR): `iters`
+ * iterations of one phase (gradient, update, zero mean) on y (n x 2, in
+ * place).  Used to time a bounded sample of the reference CPU iteration on a
+ * large workload without repeating its affinity stage.  *seconds receives
+ * the wall time of the loop.
+ */
+int port_tsne_iterate(int n, const int* row_ptr, const int* col, const float* val, float* y,
+                      int iters, double exag, double lr, double mom, double min_gain,
+                      double max_step, int grid, int n_threads, double* seconds) {
+  if (n < 2 || n_threads < 1 || iters < 0) return 5;
+  pool_t* pool = pool_create(n_threads);
+  csr_f p = {n, (int*)row_ptr, (int*)col, (float*)val};
+  float* grad = (float*)calloc(2 * (size_t)n, sizeof(float));
+  float* upd = (float*)calloc(2 * (size_t)n, sizeof(float));
+  float* gains = (float*)malloc(sizeof(float) * 2 * (size_t)n);
+  for (size_t t = 0; t < 2 * (size_t)n; ++t) gains[t] = 1.0f;
+  const int g = grid == 0 ? (n >= 50000 ? 256 : 128) : grid;
+  grid_ws ws;
+  memset(&ws, 0, sizeof(ws));
+  grid_ws_init(&ws, g, n, pool->n_threads);
+  upd_ctx u = {y, upd, gains, grad, (float)lr, (float)mom, (float)min_gain, (float)max_step, 0};
+  u.clip = isfinite(max_step) && max_step > 0.0;
+  const double t0 = now_sec();
+  for (int it = 0; it < iters; ++it) {
+    gradient_fft_f(pool, &p, y, n, (float)exag, &ws, grad);
+    parallel_for(pool, n, update_rows, &u);
+    zero_mean_f(y, n);
+  }
+  if (seconds) *seconds = now_sec() - t0;
+  grid_ws_free(&ws);
+  free(grad);
+  free(upd);
+  free(gains);
+  pool_destroy(pool);
+  return 0;
+}
+
+/*
  * Full reference-algorithm run.  idx/dist are row-major n x k (0-based).
  * y_init (n x 2, row-major) is required (seeded explicit init, as in the
  * measurement plan).  grid: 0 -> reference CPU rule (128 / 256), < 0 ->

@@ -181,7 +181,7 @@ long or_build_p(int n, int k, const int* idx, const double* dist, double perplex
     if (idx[e] < 0 || idx[e] >= n) return -OR_ERR_RANGE;
     if (!isfinite(dist[e]) || dist[e] < 0.0) return -OR_ERR_DIST;
   }
-  double* p_dir = p_dir_out ? p_dir_out : (double*)malloc(sizeof(double) * (size_t)n * k);
+  double* p_dir = p_dir_out ? p_dir_out : (double*)calloc((size_t)n * k, sizeof(double));
   if (!p_dir) return -OR_ERR_ALLOC;
   for (int i = 0; i < n; ++i) {
     or_row_probabilities(dist + (size_t)i * k, k, perplexity, p_dir + (size_t)i * k);
