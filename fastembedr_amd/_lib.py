@@ -51,6 +51,24 @@ SIGNATURES = {
     "fastembedr_hip_tsne_plan_profile": (
         c_int, [_vp, c_int, c_float, c_float, c_float, c_float, c_float, _dp]),
     "fastembedr_hip_tsne_plan_destroy": (None, [_vp]),
+    "fastembedr_hip_umap_optimize_coo": (
+        c_int, [_ip, _ip, _fp, _fp, _fp, c_int, c_int, c_int, c_int, c_float, c_float, c_float,
+                c_float, c_uint, c_int, _fp]),
+    "fastembedr_hip_umap_optimize_coo_device": (
+        c_int, [_vp, _vp, _vp, _vp, _fp, c_int, c_int, c_int, c_int, c_float, c_float, c_float,
+                c_float, c_uint, c_int, _fp]),
+    "fastembedr_hip_umap_schedule": (
+        c_int, [_ip, _ip, _fp, c_int, c_int, c_int, c_int, c_uint, c_int, _ip, _ip, _ip]),
+    "fastembedr_hip_tsne_transform": (
+        c_int, [_fp, c_int, _ip, _dp, c_int, c_int, c_int, _fp, c_int, c_double, c_int, c_int,
+                c_double, c_double, c_double, c_double, c_double, c_double, c_double, c_int,
+                c_int, c_int, _fp]),
+    "fastembedr_hip_transform_reference_create": (c_int, [_fp, c_int, _vpp]),
+    "fastembedr_hip_transform_reference_run": (
+        c_int, [_vp, _ip, _fp, c_int, c_int, c_int, _fp, c_int, c_double, c_int, c_int, c_double,
+                c_double, c_double, c_double, c_double, c_double, c_double, c_int, c_int, c_int,
+                _fp]),
+    "fastembedr_hip_transform_reference_destroy": (None, [_vp]),
 }
 
 _lib = None

@@ -81,8 +81,8 @@ static int launch_perplexity(const int* idx, const DistT* dist, int n, int k, in
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                  static_cast<int>(lds)));
   }
-  perplexity_rows<DistT><<<(n + rpb - 1) / rpb, 256, lds, s>>>(idx, dist, n, k, offset, perplexity,
-                                                                nbr, pdir, flags);
+  perplexity_rows<DistT><<<(n + rpb - 1) / rpb, 256, lds, s>>>(idx, dist, n, n, k, offset,
+                                                                perplexity, nbr, pdir, flags);
   FE_LAUNCH_CHECK("perplexity_rows");
   return 0;
 }

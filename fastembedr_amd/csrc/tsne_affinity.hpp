@@ -25,6 +25,9 @@
 #include <cstdint>
 
 namespace fe {
+// kernels have internal linkage: every translation unit that includes this
+// header gets its own copy of the device code
+namespace {
 
 constexpr int kAffRowsPerBlock = 64;  // rows staged per 256-thread block
 constexpr int kMaxK = 256;            // embedding_cuda_kernels.cpp:5661
@@ -40,11 +43,13 @@ constexpr int aff_rows_per_block() {
 template <typename DistT>
 __global__ __launch_bounds__(256) void perplexity_rows(const int* __restrict__ indices,
                                                        const DistT* __restrict__ distances,
-                                                       int n, int k, int index_offset,
-                                                       double perplexity,
+                                                       int n, int n_targets, int k,
+                                                       int index_offset, double perplexity,
                                                        int* __restrict__ nbr,
                                                        float* __restrict__ pdir,
                                                        unsigned* error_flags) {
+  // n rows of k neighbours whose ids must lie in [0, n_targets) after the
+  // offset (n_targets = n for t-SNE, the reference size for the transform)
   constexpr int RPB = aff_rows_per_block<DistT>();
   extern __shared__ unsigned char smem_raw[];
   DistT* s_d = reinterpret_cast<DistT*>(smem_raw);
@@ -58,7 +63,7 @@ __global__ __launch_bounds__(256) void perplexity_rows(const int* __restrict__ i
     const size_t src = static_cast<size_t>(j) * n + row0 + r;
     const int nb = indices[src] - index_offset;
     const DistT d = distances[src];
-    if (nb < 0 || nb >= n) atomicOr(error_flags, kErrIndexRange);
+    if (nb < 0 || nb >= n_targets) atomicOr(error_flags, kErrIndexRange);
     if (!isfinite(static_cast<double>(d)) || d < DistT(0)) atomicOr(error_flags, kErrDistance);
     s_idx[r * k + j] = nb;
     s_d[r * k + j] = d;
@@ -316,4 +321,5 @@ __global__ __launch_bounds__(256) void scan_add(int* __restrict__ out, int n,
   if (i < n) out[i] += sums[i / kScanBlock];
 }
 
+}  // namespace
 }  // namespace fe

@@ -31,6 +31,9 @@
 #include "fe_fft.hpp"
 
 namespace fe {
+// kernels have internal linkage: every translation unit that includes this
+// header gets its own copy of the device code
+namespace {
 
 constexpr int kThreads = 256;
 
@@ -545,4 +548,5 @@ __global__ void centred_copy_kernel(const float2* __restrict__ y, const TsneStat
   }
 }
 
+}  // namespace
 }  // namespace fe

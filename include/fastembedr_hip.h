@@ -136,6 +136,83 @@ int fastembedr_hip_tsne_plan_profile(fastembedr_hip_tsne_plan* plan, int n_iter,
 
 void fastembedr_hip_tsne_plan_destroy(fastembedr_hip_tsne_plan* plan);
 
+/* ---- UMAP --------------------------------------------------------------- */
+
+/* Replaces fastembedr_cuda_umap_optimize_coo (embedding_cuda_kernels.cpp:
+ * 5483-5608): COO heads/tails (0-based), weights, epochs_per_sample
+ * (= max_w / max(w, 1e-6)), init/out row-major n x 2, n_epochs epochs with
+ * negative_sample_rate negatives, a/b curve, repulsion gamma, hash seed.
+ * optimizer_mode 0 = in-place float atomics (the reference's hogwild form),
+ * 1 = epoch-start reads with a 64-bit fixed-point delta (deterministic). */
+int fastembedr_hip_umap_optimize_coo(const int* heads, const int* tails, const float* weights,
+                                     const float* epochs_per_sample, const float* init, int n,
+                                     int n_edges, int n_epochs, int negative_sample_rate,
+                                     float learning_rate, float a, float b,
+                                     float repulsion_strength, unsigned int seed,
+                                     int optimizer_mode, float* out);
+
+/* Same with heads/tails/weights/epochs_per_sample as device pointers
+ * (borrowed); init/out host.  No reference counterpart (its UMAP entry takes
+ * host buffers only); for callers whose fuzzy graph already lives on the GPU. */
+int fastembedr_hip_umap_optimize_coo_device(const int* heads, const int* tails,
+                                            const float* weights, const float* epochs_per_sample,
+                                            const float* init, int n, int n_edges, int n_epochs,
+                                            int negative_sample_rate, float learning_rate, float a,
+                                            float b, float repulsion_strength, unsigned int seed,
+                                            int optimizer_mode, float* out);
+
+/* The sampling schedule of one epoch as the epoch kernel computes it
+ * (positive_samples_this_epoch_umap_schedule, negative_samples_..._schedule,
+ * deterministic_vertex; embedding_cuda_kernels.cpp:262-274, 3044-3084):
+ * per edge n_pos, n_neg and the first max_neg negative ids (-1 = skipped as
+ * head or tail).  No reference counterpart; exposes the schedule for exact
+ * parity checks.  Host pointers. */
+int fastembedr_hip_umap_schedule(const int* heads, const int* tails,
+                                 const float* epochs_per_sample, int n_edges, int n, int epoch,
+                                 int negative_sample_rate, unsigned int seed, int max_neg,
+                                 int* n_pos, int* n_neg, int* neg_ids);
+
+/* ---- fixed-reference t-SNE transform ------------------------------------
+ * The reference binds transform_tsne_cuda_cpp (R/transform_tsne.R:275) but
+ * defines it nowhere; the argument list follows transform_tsne_metal_cpp
+ * (src/embedding_metal_bridge.cpp:323-342) and the semantics
+ * transform_tsne_cpp (src/tsne_neighbors.cpp:2124-2373), two dimensions.
+ * reference_layout: row-major n_reference x 2; indices/distances:
+ * column-major n_query x k into the reference; init: row-major n_query x 2
+ * or NULL; initialization 0 = median, 1 = weighted, 2 = random;
+ * max_grad_norm / max_step_norm <= 0 or non-finite mean no clip. */
+int fastembedr_hip_tsne_transform(const float* reference_layout, int n_reference,
+                                  const int* indices, const double* distances, int n_query, int k,
+                                  int index_offset, const float* init, int initialization,
+                                  double perplexity, int n_iter, int early_exaggeration_iter,
+                                  double learning_rate, double early_exaggeration,
+                                  double exaggeration, double initial_momentum,
+                                  double final_momentum, double max_grad_norm,
+                                  double max_step_norm, int n_negatives,
+                                  int exact_repulsion_threshold, int seed, float* out);
+
+/* A reference layout kept resident for repeated transform requests (no
+ * reference counterpart: serving without re-uploading the reference). */
+typedef struct fastembedr_hip_transform_reference fastembedr_hip_transform_reference;
+
+int fastembedr_hip_transform_reference_create(const float* reference_layout, int n_reference,
+                                              fastembedr_hip_transform_reference** ref);
+
+/* As fastembedr_hip_tsne_transform against the resident reference, with
+ * float32 distances. */
+int fastembedr_hip_transform_reference_run(fastembedr_hip_transform_reference* ref,
+                                           const int* indices, const float* distances,
+                                           int n_query, int k, int index_offset, const float* init,
+                                           int initialization, double perplexity, int n_iter,
+                                           int early_exaggeration_iter, double learning_rate,
+                                           double early_exaggeration, double exaggeration,
+                                           double initial_momentum, double final_momentum,
+                                           double max_grad_norm, double max_step_norm,
+                                           int n_negatives, int exact_repulsion_threshold,
+                                           int seed, float* out);
+
+void fastembedr_hip_transform_reference_destroy(fastembedr_hip_transform_reference* ref);
+
 #ifdef __cplusplus
 }
 #endif

@@ -91,7 +91,7 @@ def _declare(L):
 
 
 def _declare_umap(L):
-    c_int, c_float, c_uint = ctypes.c_int, ctypes.c_float, ctypes.c_uint
+    c_int, c_float, c_uint, c_double = ctypes.c_int, ctypes.c_float, ctypes.c_uint, ctypes.c_double
     L.or_umap_mix_uint.argtypes = [c_uint]
     L.or_umap_mix_uint.restype = c_uint
     L.or_umap_negative_vertex.argtypes = [c_int, c_uint, c_int, c_int, c_int, c_int]
@@ -100,11 +100,14 @@ def _declare_umap(L):
     L.or_umap_n_positive.restype = c_int
     L.or_umap_n_negative.argtypes = [c_int, c_float, c_int]
     L.or_umap_n_negative.restype = c_int
-    L.or_umap_epoch.argtypes = [_ip, _ip, _fp, c_int, _dp, c_int, c_int, c_int, c_int,
-                                ctypes.c_double, ctypes.c_double, ctypes.c_double,
-                                ctypes.c_double, c_uint]
-    L.or_umap_schedule_digest.argtypes = [_ip, _ip, _fp, c_int, c_int, c_int, c_int, c_int,
-                                          c_uint, _u32p]
+    L.or_umap_fast_pow.argtypes = [c_float, c_float]
+    L.or_umap_fast_pow.restype = c_float
+    L.or_umap_epoch.argtypes = [_ip, _ip, _fp, c_int, _dp, c_int, c_int, c_int, c_int, c_double,
+                                c_double, c_double, c_double, c_uint, c_int, _fp]
+    L.or_umap_optimize.argtypes = [_ip, _ip, _fp, c_int, c_int, c_int, c_int, c_double, c_double,
+                                   c_double, c_double, c_uint, c_int, _fp]
+    L.or_umap_schedule.argtypes = [_ip, _ip, _fp, c_int, c_int, c_int, c_int, c_uint, c_int, _ip,
+                                   _ip, _ip]
 
 
 def _declare_transform(L):
@@ -113,6 +116,10 @@ def _declare_transform(L):
                                c_int, c_double, c_double, c_double, c_double, c_double,
                                c_double, c_double, c_int, c_int, c_int, _dp]
     L.or_transform.restype = c_int
+    L.or_mix_uint32.argtypes = [ctypes.c_uint32]
+    L.or_mix_uint32.restype = ctypes.c_uint32
+    L.or_uniform_index.argtypes = [_u32p, c_int]
+    L.or_uniform_index.restype = c_int
 
 
 def _d(a):
@@ -324,3 +331,73 @@ def port_tsne_run(indices, distances, y_init, perplexity=30.0, early_iter=250, n
     if rc != 0:
         raise ValueError(_ERRORS.get(rc, "port error %d" % rc))
     return out, float(t[0]), float(t[1])
+
+
+# ------------------------------- UMAP -----------------------------------
+# min_dist 0.01, spread 1 (find_ab_params, src/fast_knn_umap.cpp:437-497)
+UMAP_A = 1.895605865596314
+UMAP_B = 0.8006377738365004
+
+
+def umap_schedule(heads, tails, period, n, epoch, nsr, seed, max_neg):
+    h, t = _i(heads), _i(tails)
+    p = np.ascontiguousarray(period, dtype=np.float32)
+    m = len(h)
+    npos = np.zeros(m, np.int32)
+    nneg = np.zeros(m, np.int32)
+    ids = np.zeros((m, max(max_neg, 1)), np.int32)
+    lib().or_umap_schedule(_ptr(h, _ip), _ptr(t, _ip), _ptr(p, _fp), m, int(n), int(epoch),
+                           int(nsr), int(seed) & 0xFFFFFFFF, int(max_neg), _ptr(npos, _ip),
+                           _ptr(nneg, _ip), _ptr(ids, _ip))
+    return npos, nneg, ids[:, :max_neg]
+
+
+def umap_optimize(heads, tails, period, y_init, n_epochs, nsr=5, lr=1.0, a=UMAP_A, b=UMAP_B,
+                  gamma=1.0, seed=0, mode=1):
+    h, t = _i(heads), _i(tails)
+    p = np.ascontiguousarray(period, dtype=np.float32)
+    y = np.ascontiguousarray(y_init, dtype=np.float32).copy()
+    lib().or_umap_optimize(_ptr(h, _ip), _ptr(t, _ip), _ptr(p, _fp), len(h), len(y),
+                           int(n_epochs), int(nsr), float(lr), float(a), float(b), float(gamma),
+                           int(seed) & 0xFFFFFFFF, int(mode), _ptr(y, _fp))
+    return y
+
+
+def umap_epoch(heads, tails, period, y, epoch, n_epochs, nsr=5, lr=1.0, a=UMAP_A, b=UMAP_B,
+               gamma=1.0, seed=0, mode=1):
+    h, t = _i(heads), _i(tails)
+    p = np.ascontiguousarray(period, dtype=np.float32)
+    yy = np.ascontiguousarray(y, dtype=np.float32).copy()
+    scratch = np.zeros(2 * len(yy))
+    lib().or_umap_epoch(_ptr(h, _ip), _ptr(t, _ip), _ptr(p, _fp), len(h), _ptr(scratch, _dp),
+                        len(yy), int(epoch), int(n_epochs), int(nsr), float(lr), float(a),
+                        float(b), float(gamma), int(seed) & 0xFFFFFFFF, int(mode), _ptr(yy, _fp))
+    return yy
+
+
+# ----------------------------- transform ---------------------------------
+def transform(reference_layout, indices, distances, Y_init=None, initialization="median",
+              perplexity=5.0, n_iter=250, early_exaggeration_iter=0, learning_rate=0.1,
+              early_exaggeration=4.0, exaggeration=1.5, initial_momentum=0.8, final_momentum=0.8,
+              max_grad_norm=0.25, max_step_norm=np.inf, n_negatives=None,
+              exact_repulsion_threshold=4096, seed=4):
+    """FP64 fixed-reference transform (defaults of R/transform_tsne.R:58-67)."""
+    ref = _d(reference_layout)
+    idx, dist = _i(indices), _d(distances)
+    nq, k = idx.shape
+    nref = len(ref)
+    if n_negatives is None:
+        n_negatives = nref if nref <= exact_repulsion_threshold else min(256, nref)
+    init = None if Y_init is None else _d(Y_init)
+    out = np.zeros((nq, 2))
+    mode = {"median": 0, "weighted": 1}[initialization] if init is None else 0
+    rc = lib().or_transform(nref, _ptr(ref, _dp), nq, k, _ptr(idx, _ip), _ptr(dist, _dp),
+                            _ptr(init, _dp) if init is not None else None, mode,
+                            float(perplexity), int(n_iter), int(early_exaggeration_iter),
+                            float(learning_rate), float(early_exaggeration), float(exaggeration),
+                            float(initial_momentum), float(final_momentum), float(max_grad_norm),
+                            float(max_step_norm), int(n_negatives), int(exact_repulsion_threshold),
+                            int(seed), _ptr(out, _dp))
+    if rc != 0:
+        raise ValueError(_ERRORS.get(rc, "oracle error"))
+    return out
