@@ -51,6 +51,14 @@ SIGNATURES = {
     "fastembedr_hip_tsne_plan_profile": (
         c_int, [_vp, c_int, c_float, c_float, c_float, c_float, c_float, _dp]),
     "fastembedr_hip_tsne_plan_destroy": (None, [_vp]),
+    "fastembedr_hip_tsne_batch_create": (c_int, [c_int, c_ll, _vpp]),
+    "fastembedr_hip_tsne_job_bytes": (c_ll, [c_int, c_int, c_int, c_int]),
+    "fastembedr_hip_tsne_batch_add": (c_int, [_vp, _vp, c_int, _ip]),
+    "fastembedr_hip_tsne_batch_step": (c_int, [_vp, c_int]),
+    "fastembedr_hip_tsne_batch_status": (c_int, [_vp, _ip, _ip]),
+    "fastembedr_hip_tsne_batch_layout": (c_int, [_vp, c_int, _fp]),
+    "fastembedr_hip_tsne_batch_remove": (c_int, [_vp, c_int]),
+    "fastembedr_hip_tsne_batch_destroy": (None, [_vp]),
     "fastembedr_hip_umap_optimize_coo": (
         c_int, [_ip, _ip, _fp, _fp, _fp, c_int, c_int, c_int, c_int, c_float, c_float, c_float,
                 c_float, c_uint, c_int, _fp]),

@@ -20,6 +20,7 @@
 #include "fe_common.hpp"
 #include "tsne_device.hpp"
 #include "tsne_affinity.hpp"
+#include "tsne_batch.hpp"
 
 namespace fe {
 
@@ -587,6 +588,280 @@ static int tsne_one_shot(const int* indices, const DistT* distances, bool device
   return plan.layout(out);
 }
 
+// ---------------------------------------------------------------------------
+// Batched jobs (job-table kernels of tsne_batch.hpp) with HBM-budget admission
+// ---------------------------------------------------------------------------
+
+// Device bytes a job holds while resident (P at its worst case of 2*n*k
+// entries) and the transient peak while its P is built.
+static long long job_resident_bytes(int n, int k, int g) {
+  const long long f = 2ll * g, gg = 1ll * g * g, nk = 1ll * n * k;
+  return 8ll * 2 * nk + 4ll * (n + 1) + 8ll * 6 * n + 8ll * 3 * gg + 8ll * 4 * g * f +
+         8ll * (g + 1) * g + 16ll * gg + 8ll * f + 32ll * 2048 + 8ll * (g + 1) + 4096;
+}
+
+static long long job_setup_bytes(int n, int k) {
+  const long long nk = 1ll * n * k;
+  return 8ll * nk /* upload */ + 8ll * nk /* nbr, pdir */ + 16ll * nk /* raw */ + 16ll * n;
+}
+
+class TsneBatch {
+ public:
+  int max_jobs = 0;
+  long long budget = 0, used = 0;
+  Stream stream;
+  std::vector<std::unique_ptr<TsnePlan>> slots;
+  std::vector<long long> slot_bytes;
+  std::vector<JobDesc> desc;
+  Arena dev;
+  JobDesc* d_desc = nullptr;
+  int* d_tab = nullptr;
+  int tab_cap = 0;
+  bool dirty = true;
+  struct Launch {
+    int kind;   // 0 splat, 1 rows_fwd, 2 cols, 3 rows_inv, 4 gather
+    int param;  // F or TPR
+    int off;    // offset of slot[] in d_tab; first[] follows it
+    int m;      // entries
+    int blocks;
+  };
+  std::vector<Launch> launches;
+
+  int init(int max_jobs_, long long budget_) {
+    max_jobs = max_jobs_;
+    budget = budget_;
+    if (stream.init()) return 1;
+    slots.resize(max_jobs);
+    slot_bytes.assign(max_jobs, 0);
+    desc.resize(max_jobs);
+    std::memset(desc.data(), 0, desc.size() * sizeof(JobDesc));
+    tab_cap = 2 * (5 * max_jobs + 16) + 64;
+    ArenaPlan p;
+    p.add<JobDesc>(max_jobs);
+    p.add<int>(tab_cap);
+    if (dev.init(p.bytes, "t-SNE batch tables")) return 1;
+    d_desc = dev.take<JobDesc>(max_jobs);
+    d_tab = dev.take<int>(tab_cap);
+    FE_CHECK(hipMemsetAsync(d_desc, 0, max_jobs * sizeof(JobDesc), stream.get()));
+    return check(hipStreamSynchronize(stream.get()), "batch init");
+  }
+
+  // returns 0 admitted, 1 error, 2 not admitted (budget or no free slot)
+  int add(const fastembedr_hip_tsne_job& j, bool device_inputs, int* slot_out) {
+    int slot = -1;
+    for (int s = 0; s < max_jobs; ++s)
+      if (!slots[s]) {
+        slot = s;
+        break;
+      }
+    if (slot < 0) {
+      set_error("no free batch slot");
+      return 2;
+    }
+    const int g = resolve_grid_size(j.n, j.grid_size);
+    const long long need = job_resident_bytes(j.n, j.k, g);
+    if (budget > 0 && used + need + job_setup_bytes(j.n, j.k) > budget) {
+      set_error("HBM budget does not admit this job yet");
+      return 2;
+    }
+    if (validate_tsne(j.indices, j.distances, j.n, j.k, 2, j.perplexity, j.early_exaggeration_iter,
+                      j.n_iter, j.early_exaggeration, j.exaggeration, j.learning_rate_early, 0,
+                      j.initial_momentum, j.final_momentum, j.min_gain, j.index_offset))
+      return 1;
+    if (!(j.learning_rate_normal > 0.0f)) return fail("HIP openTSNE FFT-grid learning rate must be positive.");
+    auto plan = std::make_unique<TsnePlan>();
+    if (plan->create<float>(j.indices, j.distances, device_inputs, j.n, j.k, j.index_offset,
+                            j.perplexity, j.grid_size, j.init, j.seed, true))
+      return 1;
+    JobDesc d;
+    std::memset(&d, 0, sizeof(d));
+    TsnePlan& p = *plan;
+    d.y[0] = p.y[0];
+    d.y[1] = p.y[1];
+    d.gains = p.gains;
+    d.upd = p.upd;
+    d.row_ptr = p.csr.row_ptr;
+    d.col = p.csr.col;
+    d.val = p.csr.val;
+    d.grids = p.grids;
+    d.z1 = p.z1;
+    d.z2 = p.z2;
+    d.kt = p.kt;
+    d.o1 = p.o1;
+    d.o2 = p.o2;
+    d.field = p.field;
+    d.tw = p.tw;
+    d.st = p.st;
+    d.lpart = p.lpart;
+    d.zpart = p.zpart;
+    d.n = p.n;
+    d.g = p.g;
+    d.cur = p.cur;
+    d.iter = 0;
+    d.early_iter = j.early_exaggeration_iter;
+    d.total_iter = j.early_exaggeration_iter + j.n_iter;
+    d.active = 1;
+    d.early_exag = j.early_exaggeration;
+    d.exag = j.exaggeration;
+    d.lr_early = j.learning_rate_early;
+    d.lr_normal = j.learning_rate_normal;
+    d.mom0 = j.initial_momentum;
+    d.mom1 = j.final_momentum;
+    d.min_gain = j.min_gain;
+    d.max_step = j.max_step_norm;
+    desc[slot] = d;
+    FE_CHECK(hipMemcpyAsync(d_desc + slot, &desc[slot], sizeof(JobDesc), hipMemcpyHostToDevice,
+                            stream.get()));
+    FE_CHECK(hipStreamSynchronize(stream.get()));
+    slot_bytes[slot] = static_cast<long long>(p.state.capacity() + p.csr_store.capacity());
+    used += slot_bytes[slot];
+    slots[slot] = std::move(plan);
+    dirty = true;
+    *slot_out = slot;
+    return 0;
+  }
+
+  int remove(int slot) {
+    if (slot < 0 || slot >= max_jobs || !slots[slot]) return fail("invalid batch slot");
+    FE_CHECK(hipStreamSynchronize(stream.get()));
+    JobDesc z;
+    std::memset(&z, 0, sizeof(z));
+    desc[slot] = z;
+    FE_CHECK(hipMemcpy(d_desc + slot, &z, sizeof(JobDesc), hipMemcpyHostToDevice));
+    slots[slot].reset();
+    used -= slot_bytes[slot];
+    slot_bytes[slot] = 0;
+    dirty = true;
+    return 0;
+  }
+
+  int rebuild() {
+    launches.clear();
+    std::vector<int> tab;
+    auto add_launch = [&](int kind, int param, const std::vector<std::pair<int, int>>& ent) {
+      if (ent.empty()) return;
+      Launch L{kind, param, static_cast<int>(tab.size()), static_cast<int>(ent.size()), 0};
+      for (auto& e : ent) tab.push_back(e.first);
+      int acc = 0;
+      for (auto& e : ent) {
+        tab.push_back(acc);
+        acc += e.second;
+      }
+      tab.push_back(acc);
+      L.blocks = acc;
+      launches.push_back(L);
+    };
+    std::vector<std::pair<int, int>> splat;
+    for (int s = 0; s < max_jobs; ++s)
+      if (slots[s]) splat.push_back({s, slots[s]->splat_blocks});
+    add_launch(0, 0, splat);
+    for (int kind = 1; kind <= 3; ++kind) {
+      for (int F = 32; F <= 2048; F *= 2) {
+        std::vector<std::pair<int, int>> ent;
+        for (int s = 0; s < max_jobs; ++s) {
+          if (!slots[s] || slots[s]->f != F) continue;
+          const int G = F / 2;
+          ent.push_back({s, kind == 1 ? G + G / 2 : (kind == 2 ? G + 1 : G)});
+        }
+        add_launch(kind, F, ent);
+      }
+    }
+    for (int tpr = 4; tpr <= 64; tpr *= 2) {
+      std::vector<std::pair<int, int>> ent;
+      for (int s = 0; s < max_jobs; ++s)
+        if (slots[s] && slots[s]->tpr == tpr) ent.push_back({s, slots[s]->upd_blocks});
+      add_launch(4, tpr, ent);
+    }
+    if (static_cast<int>(tab.size()) > tab_cap) return fail("batch table overflow");
+    if (!tab.empty())
+      FE_CHECK(hipMemcpy(d_tab, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice));
+    dirty = false;
+    return 0;
+  }
+
+  int enqueue_iteration() {
+    hipStream_t s = stream.get();
+    for (const Launch& L : launches) {
+      const int* sl = d_tab + L.off;
+      const int* fi = sl + L.m;
+      switch (L.kind) {
+        case 0:
+          batch_splat<<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
+          break;
+#define FE_B_CONV(FF)                                                                          \
+  case FF:                                                                                     \
+    if (L.kind == 1)                                                                           \
+      batch_rows_fwd<FF><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);                   \
+    else if (L.kind == 2)                                                                      \
+      batch_cols<FF><<<L.blocks, kThreads, 5 * FF * sizeof(float2), s>>>(d_desc, sl, fi, L.m); \
+    else                                                                                       \
+      batch_rows_inv<FF><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);                   \
+    break;
+        case 1:
+        case 2:
+        case 3:
+          switch (L.param) {
+            FE_B_CONV(32)
+            FE_B_CONV(64)
+            FE_B_CONV(128)
+            FE_B_CONV(256)
+            FE_B_CONV(512)
+            FE_B_CONV(1024)
+            FE_B_CONV(2048)
+            default:
+              return fail("unsupported FFT size");
+          }
+          break;
+#undef FE_B_CONV
+        default:
+          switch (L.param) {
+            case 4: batch_gather<4><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
+            case 8: batch_gather<8><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
+            case 16: batch_gather<16><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
+            case 32: batch_gather<32><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
+            default: batch_gather<64><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
+          }
+      }
+      FE_LAUNCH_CHECK("batch kernel");
+    }
+    return 0;
+  }
+
+  int step(int iters) {
+    if (dirty && rebuild()) return 1;
+    if (launches.empty()) return 0;
+    static bool attr_done = false;
+    if (!attr_done) {
+      FE_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&batch_cols<2048>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   5 * 2048 * static_cast<int>(sizeof(float2))));
+      attr_done = true;
+    }
+    for (int it = 0; it < iters; ++it)
+      if (enqueue_iteration()) return 1;
+    return 0;
+  }
+
+  int status(int* iters, int* active) {
+    std::vector<JobDesc> h(max_jobs);
+    FE_CHECK(hipMemcpyAsync(h.data(), d_desc, max_jobs * sizeof(JobDesc), hipMemcpyDeviceToHost,
+                            stream.get()));
+    FE_CHECK(hipStreamSynchronize(stream.get()));
+    for (int s = 0; s < max_jobs; ++s) {
+      if (iters) iters[s] = slots[s] ? h[s].iter : -1;
+      if (active) active[s] = slots[s] ? h[s].active : 0;
+      if (slots[s]) slots[s]->cur = h[s].cur;
+    }
+    return 0;
+  }
+
+  int layout(int slot, float* out) {
+    if (slot < 0 || slot >= max_jobs || !slots[slot]) return fail("invalid batch slot");
+    if (status(nullptr, nullptr)) return 1;
+    return slots[slot]->layout(out);
+  }
+};
+
 }  // namespace fe
 
 using fe::fail;
@@ -781,6 +1056,75 @@ void fastembedr_hip_tsne_plan_destroy(fastembedr_hip_tsne_plan* plan) {
   if (plan) {
     (void)hipStreamSynchronize(plan->impl.stream.get());
     delete plan;
+  }
+}
+
+}  // extern "C"
+
+struct fastembedr_hip_tsne_batch {
+  fe::TsneBatch impl;
+};
+
+extern "C" {
+
+int fastembedr_hip_tsne_batch_create(int max_jobs, long long hbm_budget_bytes,
+                                     fastembedr_hip_tsne_batch** batch) {
+  fe::clear_error();
+  if (!batch) return fail("null batch pointer");
+  *batch = nullptr;
+  if (max_jobs < 1 || max_jobs > 4096) return fail("max_jobs must lie in [1, 4096]");
+  auto* b = new (std::nothrow) fastembedr_hip_tsne_batch;
+  if (!b) return fail("out of host memory");
+  if (b->impl.init(max_jobs, hbm_budget_bytes)) {
+    delete b;
+    return 1;
+  }
+  *batch = b;
+  return 0;
+}
+
+long long fastembedr_hip_tsne_job_bytes(int n, int k, int grid_size, int setup_peak) {
+  const int g = fe::resolve_grid_size(n, grid_size);
+  return fe::job_resident_bytes(n, k, g) + (setup_peak ? fe::job_setup_bytes(n, k) : 0);
+}
+
+int fastembedr_hip_tsne_batch_add(fastembedr_hip_tsne_batch* batch,
+                                  const fastembedr_hip_tsne_job* job, int device_inputs,
+                                  int* slot) {
+  fe::clear_error();
+  if (!batch || !job || !slot) return fail("null batch argument");
+  return batch->impl.add(*job, device_inputs != 0, slot);
+}
+
+int fastembedr_hip_tsne_batch_step(fastembedr_hip_tsne_batch* batch, int n_iter) {
+  fe::clear_error();
+  if (!batch || n_iter < 0) return fail("invalid batch step");
+  return batch->impl.step(n_iter);
+}
+
+int fastembedr_hip_tsne_batch_status(fastembedr_hip_tsne_batch* batch, int* iterations,
+                                     int* active) {
+  fe::clear_error();
+  if (!batch) return fail("null batch");
+  return batch->impl.status(iterations, active);
+}
+
+int fastembedr_hip_tsne_batch_layout(fastembedr_hip_tsne_batch* batch, int slot, float* out) {
+  fe::clear_error();
+  if (!batch || !out) return fail("null batch argument");
+  return batch->impl.layout(slot, out);
+}
+
+int fastembedr_hip_tsne_batch_remove(fastembedr_hip_tsne_batch* batch, int slot) {
+  fe::clear_error();
+  if (!batch) return fail("null batch");
+  return batch->impl.remove(slot);
+}
+
+void fastembedr_hip_tsne_batch_destroy(fastembedr_hip_tsne_batch* batch) {
+  if (batch) {
+    (void)hipStreamSynchronize(batch->impl.stream.get());
+    delete batch;
   }
 }
 

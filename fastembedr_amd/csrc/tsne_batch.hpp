@@ -1,0 +1,115 @@
+// Job-table kernels: many t-SNE jobs of uneven size advance together.
+//
+// A launch covers several jobs.  Each job owns a contiguous range of blocks
+// (`first[e] .. first[e+1]`, the same block count its single-job launch
+// would use), finds its descriptor by a binary search of blockIdx.x over the
+// prefix table, and runs the single-job stage body with its job-local block
+// index.  Jobs therefore give bit-identical results alone and in a batch,
+// and a batch of S jobs costs 1 + 3 * (#grid sizes) + (#lane widths) launches
+// per iteration instead of 5 * S.  The descriptor's iteration counter, the
+// phase constants and the layout-buffer parity live on the device: the block
+// that finalises a job's layout statistics advances them, and a job whose
+// schedule is complete goes inactive, so captured sequences of iterations
+// can run jobs with different schedules.
+#pragma once
+
+#include "tsne_device.hpp"
+
+namespace fe {
+namespace {
+
+struct JobDesc {
+  float2* y[2];
+  float2* gains;
+  float2* upd;
+  const int* row_ptr;
+  const int* col;
+  const float* val;
+  unsigned long long* grids;
+  float2* z1;
+  float2* z2;
+  float2* kt;
+  float2* o1;
+  float2* o2;
+  float4* field;
+  const float2* tw;
+  TsneStats* st;
+  LayoutPartial* lpart;
+  double* zpart;
+  int n, g;
+  int cur, iter, total_iter, early_iter, active, pad;
+  float early_exag, exag, lr_early, lr_normal, mom0, mom1, min_gain, max_step;
+};
+
+// Entry e of a launch: descriptor jobs[slot[e]], blocks first[e] .. first[e+1].
+__device__ __forceinline__ int find_entry(const int* __restrict__ first, int m) {
+  const int bid = static_cast<int>(blockIdx.x);
+  int lo = 0, hi = m - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first[mid] <= bid) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kThreads) void batch_splat(JobDesc* jobs, const int* __restrict__ slot,
+                                                        const int* __restrict__ first, int m) {
+  const int e = find_entry(first, m);
+  JobDesc* J = jobs + slot[e];
+  if (!J->active) return;
+  splat_body(J->y[J->cur], J->st, J->grids, J->n, J->g, blockIdx.x - first[e],
+             first[e + 1] - first[e]);
+}
+
+template <int F>
+__global__ __launch_bounds__(kThreads) void batch_rows_fwd(JobDesc* jobs, const int* __restrict__ slot,
+                                                           const int* __restrict__ first, int m) {
+  const int e = find_entry(first, m);
+  JobDesc* J = jobs + slot[e];
+  if (!J->active) return;
+  conv_rows_fwd_body<F>(J->grids, J->st, J->tw, J->z1, J->z2, J->kt, blockIdx.x - first[e]);
+}
+
+template <int F>
+__global__ __launch_bounds__(kThreads) void batch_cols(JobDesc* jobs, const int* __restrict__ slot,
+                                                       const int* __restrict__ first, int m) {
+  extern __shared__ float2 lds[];
+  const int e = find_entry(first, m);
+  JobDesc* J = jobs + slot[e];
+  if (!J->active) return;
+  conv_cols_body<F>(J->z1, J->z2, J->kt, J->tw, J->o1, J->o2, J->zpart, J->st, lds,
+                    blockIdx.x - first[e], first[e + 1] - first[e]);
+}
+
+template <int F>
+__global__ __launch_bounds__(kThreads) void batch_rows_inv(JobDesc* jobs, const int* __restrict__ slot,
+                                                           const int* __restrict__ first, int m) {
+  const int e = find_entry(first, m);
+  JobDesc* J = jobs + slot[e];
+  if (!J->active) return;
+  conv_rows_inv_body<F>(J->o1, J->o2, J->tw, J->field, blockIdx.x - first[e]);
+}
+
+template <int TPR>
+__global__ __launch_bounds__(kThreads) void batch_gather(JobDesc* jobs, const int* __restrict__ slot,
+                                                         const int* __restrict__ first, int m) {
+  const int e = find_entry(first, m);
+  JobDesc* J = jobs + slot[e];
+  if (!J->active) return;
+  const int it = J->iter, cur = J->cur;
+  const bool early = it < J->early_iter;
+  const StepParams sp{early ? J->early_exag : J->exag, early ? J->lr_early : J->lr_normal,
+                      early ? J->mom0 : J->mom1, J->min_gain, J->max_step};
+  const bool last = gather_attract_update_body<TPR, 0>(
+      J->y[cur], J->y[cur ^ 1], J->gains, J->upd, J->row_ptr, J->col, J->val, J->field, J->st,
+      J->lpart, J->n, J->g, sp, nullptr, nullptr, blockIdx.x - first[e], first[e + 1] - first[e]);
+  if (last) {
+    J->cur = cur ^ 1;
+    J->iter = it + 1;
+    if (it + 1 >= J->total_iter) J->active = 0;
+  }
+}
+
+}  // namespace
+}  // namespace fe

@@ -137,10 +137,12 @@ __device__ inline void finalize_geometry(TsneStats* st, double sx, double sy, fl
   st->fx_xy = ldexp(1.0, min(61 - exy, 1000));
 }
 
-// Block reduction of LayoutPartial then last-block finalize.  `partials` has
-// gridDim.x entries.
-__device__ inline void reduce_layout_stats(LayoutPartial mine, LayoutPartial* partials,
-                                           TsneStats* st, int n, int g, bool centre = true) {
+// Block reduction of LayoutPartial then last-block finalize over the nblk
+// blocks that work on this layout (`partials` has nblk entries, this block is
+// blk).  Returns true in thread 0 of the block that finalised.
+__device__ inline bool reduce_layout_stats(LayoutPartial mine, LayoutPartial* partials,
+                                           TsneStats* st, int n, int g, bool centre, int blk,
+                                           int nblk) {
   __shared__ LayoutPartial red[kThreads / 64];
   __shared__ bool last;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -162,19 +164,19 @@ __device__ inline void reduce_layout_stats(LayoutPartial mine, LayoutPartial* pa
       b.mny = fminf(b.mny, red[w].mny);
       b.mxy = fmaxf(b.mxy, red[w].mxy);
     }
-    partials[blockIdx.x] = b;
+    partials[blk] = b;
     __threadfence();
     const unsigned prev = atomicAdd(&st->done_stats, 1u);
-    last = (prev == gridDim.x - 1);
+    last = (prev == static_cast<unsigned>(nblk - 1));
   }
   __syncthreads();
-  if (!last) return;
+  if (!last) return false;
   __threadfence();
   if (threadIdx.x == 0) {
     const volatile LayoutPartial* vp = partials;
     double sx = 0.0, sy = 0.0;
     float mnx = FLT_MAX, mxx = -FLT_MAX, mny = FLT_MAX, mxy = -FLT_MAX;
-    for (unsigned b = 0; b < gridDim.x; ++b) {
+    for (int b = 0; b < nblk; ++b) {
       sx += vp[b].sx;
       sy += vp[b].sy;
       mnx = fminf(mnx, vp[b].mnx);
@@ -184,7 +186,9 @@ __device__ inline void reduce_layout_stats(LayoutPartial mine, LayoutPartial* pa
     }
     finalize_geometry(st, sx, sy, mnx, mxx, mny, mxy, n, g, centre);
     st->done_stats = 0u;
+    return true;
   }
+  return false;
 }
 
 // ----------------------------------------------------------------------------
@@ -205,22 +209,28 @@ __global__ __launch_bounds__(kThreads) void layout_stats_kernel(const float2* __
     m.mny = fminf(m.mny, p.y);
     m.mxy = fmaxf(m.mxy, p.y);
   }
-  reduce_layout_stats(m, partials, st, n, g, centre != 0);
+  reduce_layout_stats(m, partials, st, n, g, centre != 0, blockIdx.x, gridDim.x);
 }
 
 // ----------------------------------------------------------------------------
+// Every stage below is a __device__ body taking its block index within the
+// job (blk) and the job's block count (nblk); the single-job kernels pass
+// blockIdx.x / gridDim.x and the job-table kernels of tsne_batch.hpp pass the
+// job-local values, so a job gives the same bits alone and inside a batch.
+//
 // 1. splat: embedding_cuda_kernels.cpp:2622-2667 / tsne_neighbors.cpp:
 //    1186-1226, accumulated in 64-bit fixed point (deterministic).
 // grids: 3 channels of g*g int64, channel-major, row-major nodes.
 // ----------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void splat_kernel(const float2* __restrict__ y,
-                                                         const TsneStats* __restrict__ st,
-                                                         unsigned long long* grids, int n, int g) {
+__device__ __forceinline__ void splat_body(const float2* __restrict__ y,
+                                           const TsneStats* __restrict__ st,
+                                           unsigned long long* grids, int n, int g, int blk,
+                                           int nblk) {
   const float mx = st->mean_xf, my = st->mean_yf;
   const float lx = st->lower_x, ly = st->lower_y, inv = st->inv_spacing;
   const double sm = st->fx_mass, sxy = st->fx_xy;
   const size_t gg = static_cast<size_t>(g) * g;
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+  for (int i = blk * kThreads + threadIdx.x; i < n; i += nblk * kThreads) {
     const float2 p = y[i];
     const float xc = p.x - mx, yc = p.y - my;
     const Cell c = grid_cell(xc, yc, lx, ly, inv, g);
@@ -240,21 +250,26 @@ __global__ __launch_bounds__(kThreads) void splat_kernel(const float2* __restric
   }
 }
 
+__global__ __launch_bounds__(kThreads) void splat_kernel(const float2* __restrict__ y,
+                                                         const TsneStats* __restrict__ st,
+                                                         unsigned long long* grids, int n, int g) {
+  splat_body(y, st, grids, n, g, blockIdx.x, gridDim.x);
+}
+
 // ----------------------------------------------------------------------------
 // 2. row FFTs.  Blocks [0, g): data row y.  Blocks [g, g + g/2): kernel rows
 //    dy and dy + g/2.  Kernel row spectra are stored transposed,
 //    kt[k2 * g + dy] for k2 in [0, g], so conv_cols reads them contiguously.
 // ----------------------------------------------------------------------------
 template <int F>
-__global__ __launch_bounds__(kThreads) void conv_rows_fwd(unsigned long long* grids,
-                                                          const TsneStats* __restrict__ st,
-                                                          const float2* __restrict__ tw,
-                                                          float2* __restrict__ z1,
-                                                          float2* __restrict__ z2,
-                                                          float2* __restrict__ kt) {
+__device__ __forceinline__ void conv_rows_fwd_body(unsigned long long* grids,
+                                                   const TsneStats* __restrict__ st,
+                                                   const float2* __restrict__ tw,
+                                                   float2* __restrict__ z1,
+                                                   float2* __restrict__ z2,
+                                                   float2* __restrict__ kt, int b) {
   constexpr int G = F / 2;
   __shared__ float2 buf[2 * F];
-  const int b = blockIdx.x;
   if (b < G) {
     const int row = b;
     const double inv_m = 1.0 / st->fx_mass, inv_xy = 1.0 / st->fx_xy;
@@ -310,29 +325,38 @@ __global__ __launch_bounds__(kThreads) void conv_rows_fwd(unsigned long long* gr
   }
 }
 
+template <int F>
+__global__ __launch_bounds__(kThreads) void conv_rows_fwd(unsigned long long* grids,
+                                                          const TsneStats* __restrict__ st,
+                                                          const float2* __restrict__ tw,
+                                                          float2* __restrict__ z1,
+                                                          float2* __restrict__ z2,
+                                                          float2* __restrict__ kt) {
+  conv_rows_fwd_body<F>(grids, st, tw, z1, z2, kt, blockIdx.x);
+}
+
 // ----------------------------------------------------------------------------
 // 3. column pass for the pair (c, F - c), c in [0, G].  LDS arrays:
 //    0: z1 col c, 1: z2 col c, 2: z1 col F-c, 3: z2 col F-c, 4: kernel col.
 // Output O1 = A*Kq + i*X*Kq2 (-> q, xq2) and O2 = A*Kq2 + i*Y*Kq2 (-> q2,
 // yq2) for rows y < G, where A = FFT(mass), X = FFT(mass_x), Y = FFT(mass_y).
 // The 1/F^2 of the inverse transform is folded into the kernel spectra.
+// `lds` holds 5*F float2 (dynamic shared memory of the launching kernel).
 // ----------------------------------------------------------------------------
 template <int F>
-__global__ __launch_bounds__(kThreads) void conv_cols(const float2* __restrict__ z1,
-                                                      const float2* __restrict__ z2,
-                                                      const float2* __restrict__ kt,
-                                                      const float2* __restrict__ tw,
-                                                      float2* __restrict__ o1,
-                                                      float2* __restrict__ o2,
-                                                      double* partials, TsneStats* st) {
+__device__ __forceinline__ void conv_cols_body(const float2* __restrict__ z1,
+                                               const float2* __restrict__ z2,
+                                               const float2* __restrict__ kt,
+                                               const float2* __restrict__ tw,
+                                               float2* __restrict__ o1, float2* __restrict__ o2,
+                                               double* partials, TsneStats* st, float2* lds,
+                                               int c, int nblk) {
   constexpr int G = F / 2;
-  extern __shared__ float2 lds[];
   float2* b0 = lds;
   float2* b1 = lds + F;
   float2* b2 = lds + 2 * F;
   float2* b3 = lds + 3 * F;
   float2* bk = lds + 4 * F;
-  const int c = blockIdx.x;
   const int cp = (F - c) & (F - 1);
   const bool pair = (cp != c);
   for (int y = threadIdx.x; y < F; y += kThreads) {
@@ -400,14 +424,14 @@ __global__ __launch_bounds__(kThreads) void conv_cols(const float2* __restrict__
     for (int w = 0; w < kThreads / 64; ++w) s += red[w];
     partials[c] = s;
     __threadfence();
-    last = (atomicAdd(&st->done_cols, 1u) == gridDim.x - 1);
+    last = (atomicAdd(&st->done_cols, 1u) == static_cast<unsigned>(nblk - 1));
   }
   __syncthreads();
   if (last && threadIdx.x == 0) {
     __threadfence();
     const volatile double* vp = partials;
     double s = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b) s += vp[b];
+    for (int b = 0; b < nblk; ++b) s += vp[b];
     // Parseval: sum_nodes mass*q = (1/F^2) sum_k |A(k)|^2 Kq_hat(k); kq above
     // already carries the 1/F^2.
     st->sum_q_total = s;
@@ -415,17 +439,28 @@ __global__ __launch_bounds__(kThreads) void conv_cols(const float2* __restrict__
   }
 }
 
+template <int F>
+__global__ __launch_bounds__(kThreads) void conv_cols(const float2* __restrict__ z1,
+                                                      const float2* __restrict__ z2,
+                                                      const float2* __restrict__ kt,
+                                                      const float2* __restrict__ tw,
+                                                      float2* __restrict__ o1,
+                                                      float2* __restrict__ o2,
+                                                      double* partials, TsneStats* st) {
+  extern __shared__ float2 lds[];
+  conv_cols_body<F>(z1, z2, kt, tw, o1, o2, partials, st, lds, blockIdx.x, gridDim.x);
+}
+
 // ----------------------------------------------------------------------------
 // 4. inverse row FFTs for rows y < G; field(y, x) = (q2, xq2, yq2, q).
 // ----------------------------------------------------------------------------
 template <int F>
-__global__ __launch_bounds__(kThreads) void conv_rows_inv(const float2* __restrict__ o1,
-                                                          const float2* __restrict__ o2,
-                                                          const float2* __restrict__ tw,
-                                                          float4* __restrict__ field) {
+__device__ __forceinline__ void conv_rows_inv_body(const float2* __restrict__ o1,
+                                                   const float2* __restrict__ o2,
+                                                   const float2* __restrict__ tw,
+                                                   float4* __restrict__ field, int row) {
   constexpr int G = F / 2;
   __shared__ float2 buf[2 * F];
-  const int row = blockIdx.x;
   const float2* s1 = o1 + static_cast<size_t>(row) * F;
   const float2* s2 = o2 + static_cast<size_t>(row) * F;
   for (int x = threadIdx.x; x < F; x += kThreads) {
@@ -439,6 +474,14 @@ __global__ __launch_bounds__(kThreads) void conv_rows_inv(const float2* __restri
     const float2 a = buf[x], b = buf[F + x];
     out[x] = make_float4(b.x, a.y, b.y, a.x);
   }
+}
+
+template <int F>
+__global__ __launch_bounds__(kThreads) void conv_rows_inv(const float2* __restrict__ o1,
+                                                          const float2* __restrict__ o2,
+                                                          const float2* __restrict__ tw,
+                                                          float4* __restrict__ field) {
+  conv_rows_inv_body<F>(o1, o2, tw, field, blockIdx.x);
 }
 
 // ----------------------------------------------------------------------------
@@ -455,16 +498,18 @@ struct StepParams {
 
 __device__ __forceinline__ float sign_tsne(float x) { return x == 0.0f ? 0.0f : (x < 0.0f ? -1.0f : 1.0f); }
 
+// Returns true in thread 0 of the block that finalised the layout statistics
+// (MODE 0 only).
 template <int TPR, int MODE>
-__global__ __launch_bounds__(kThreads) void gather_attract_update(
+__device__ __forceinline__ bool gather_attract_update_body(
     const float2* __restrict__ y_in, float2* __restrict__ y_out, float2* __restrict__ gains,
     float2* __restrict__ upd, const int* __restrict__ row_ptr, const int* __restrict__ col,
     const float* __restrict__ val, const float4* __restrict__ field, TsneStats* st,
     LayoutPartial* partials, int n, int g, StepParams sp, float2* __restrict__ diag_attr,
-    float2* __restrict__ diag_rep) {
+    float2* __restrict__ diag_rep, int blk, int nblk) {
   const int lane = threadIdx.x % TPR;
-  const int group = (blockIdx.x * kThreads + threadIdx.x) / TPR;
-  const int n_groups = gridDim.x * (kThreads / TPR);
+  const int group = (blk * kThreads + static_cast<int>(threadIdx.x)) / TPR;
+  const int n_groups = nblk * (kThreads / TPR);
   const float mxf = st->mean_xf, myf = st->mean_yf;
   const float lx = st->lower_x, ly = st->lower_y, inv = st->inv_spacing;
   const float inv_z = static_cast<float>(
@@ -535,7 +580,20 @@ __global__ __launch_bounds__(kThreads) void gather_attract_update(
       }
     }
   }
-  if constexpr (MODE == 0) reduce_layout_stats(m, partials, st, n, g);
+  if constexpr (MODE == 0) return reduce_layout_stats(m, partials, st, n, g, true, blk, nblk);
+  return false;
+}
+
+template <int TPR, int MODE>
+__global__ __launch_bounds__(kThreads) void gather_attract_update(
+    const float2* __restrict__ y_in, float2* __restrict__ y_out, float2* __restrict__ gains,
+    float2* __restrict__ upd, const int* __restrict__ row_ptr, const int* __restrict__ col,
+    const float* __restrict__ val, const float4* __restrict__ field, TsneStats* st,
+    LayoutPartial* partials, int n, int g, StepParams sp, float2* __restrict__ diag_attr,
+    float2* __restrict__ diag_rep) {
+  (void)gather_attract_update_body<TPR, MODE>(y_in, y_out, gains, upd, row_ptr, col, val, field,
+                                              st, partials, n, g, sp, diag_attr, diag_rep,
+                                              blockIdx.x, gridDim.x);
 }
 
 // Write the centred layout (final output): out_i = fl(y_i - mean).

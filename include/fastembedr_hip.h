@@ -136,6 +136,59 @@ int fastembedr_hip_tsne_plan_profile(fastembedr_hip_tsne_plan* plan, int n_iter,
 
 void fastembedr_hip_tsne_plan_destroy(fastembedr_hip_tsne_plan* plan);
 
+/* ---- batched jobs ------------------------------------------------------
+ * No reference counterpart (the reference runs one job per call).  Many
+ * t-SNE jobs of uneven size advance together through job-table kernels:
+ * every iteration of every resident job costs 1 + 3 * (#distinct grid sizes)
+ * + (#distinct lane widths) launches, and each job gives the same bits as
+ * its single-job plan.  Jobs are admitted while the device-memory budget
+ * allows, run their own schedules, and are removed by the caller; _add and
+ * _remove may be called between _step calls (continuous batching). */
+typedef struct fastembedr_hip_tsne_job {
+  const int* indices;     /* column-major n x k */
+  const float* distances; /* column-major n x k */
+  int n, k, index_offset, grid_size;
+  float perplexity;
+  const float* init; /* row-major n x 2 or NULL (mt19937(seed) N(0, 1e-4^2)) */
+  unsigned int seed;
+  int early_exaggeration_iter, n_iter;
+  float early_exaggeration, exaggeration, learning_rate_early, learning_rate_normal;
+  float initial_momentum, final_momentum, min_gain, max_step_norm;
+} fastembedr_hip_tsne_job;
+
+typedef struct fastembedr_hip_tsne_batch fastembedr_hip_tsne_batch;
+
+/* hbm_budget_bytes <= 0: no budget. */
+int fastembedr_hip_tsne_batch_create(int max_jobs, long long hbm_budget_bytes,
+                                     fastembedr_hip_tsne_batch** batch);
+
+/* Device bytes a job of this shape holds while resident, and at its peak
+ * during setup (for admission decisions). */
+long long fastembedr_hip_tsne_job_bytes(int n, int k, int grid_size, int setup_peak);
+
+/* Builds the job's P and state and makes it resident.  Returns 0 and the
+ * slot, 2 when the budget or the slot table does not admit it now (retry
+ * after removing jobs), 1 on error.  indices/distances are host pointers
+ * (device_inputs = 0) or device pointers (1). */
+int fastembedr_hip_tsne_batch_add(fastembedr_hip_tsne_batch* batch,
+                                  const fastembedr_hip_tsne_job* job, int device_inputs,
+                                  int* slot);
+
+/* Enqueue n_iter iterations of every resident, unfinished job (async). */
+int fastembedr_hip_tsne_batch_step(fastembedr_hip_tsne_batch* batch, int n_iter);
+
+/* Waits for the batch and reports per slot (max_jobs entries) the iterations
+ * done (-1 = empty slot) and whether the job still runs. */
+int fastembedr_hip_tsne_batch_status(fastembedr_hip_tsne_batch* batch, int* iterations,
+                                     int* active);
+
+/* Centred layout of a slot, host row-major n x 2. */
+int fastembedr_hip_tsne_batch_layout(fastembedr_hip_tsne_batch* batch, int slot, float* out);
+
+int fastembedr_hip_tsne_batch_remove(fastembedr_hip_tsne_batch* batch, int slot);
+
+void fastembedr_hip_tsne_batch_destroy(fastembedr_hip_tsne_batch* batch);
+
 /* ---- UMAP --------------------------------------------------------------- */
 
 /* Replaces fastembedr_cuda_umap_optimize_coo (embedding_cuda_kernels.cpp:
