@@ -45,6 +45,9 @@ def parse():
                     help="rocprofv3-derived HBM bytes per launch of the dominant kernel")
     ap.add_argument("--quality", action="store_true",
                     help="also report trustworthiness / kNN preservation on a 2000-point sample")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="run a plain device copy of this many MiB (3 times) so hardware byte "
+                         "counters can be scaled against a known transfer")
     return ap.parse_args()
 
 
@@ -67,6 +70,14 @@ def main():
 
     if not fe.available():
         raise SystemExit("no gfx950 device visible to the HIP backend")
+    if args.copy_calibration_mb:
+        a = torch.ones(args.copy_calibration_mb * (1 << 18), dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+        log(f"[bench] copy calibration: 3 x {args.copy_calibration_mb} MiB read + write")
     n, k = args.n, args.k
     t0 = time.perf_counter()
     x, labels, centres = workloads.gaussian_mixture(n, 50, 100, seed=201, dirichlet=True)
@@ -118,7 +129,8 @@ def main():
         "conv_rows_fwd": 12 * g * g + 2 * 8 * g * f,
         "conv_cols": 4 * 8 * g * f,
         "conv_rows_inv": 2 * 8 * g * f + 16 * g * g,
-        "gather_attract_update": 8 * nnz_dir + 100 * n + 16 * g * g,
+        "attract": 8 * nnz_dir + 28 * n,
+        "update": 72 * n + 16 * g * g,
     }
     dominant = max(algo, key=lambda kk: prof[kk])
     achieved = algo[dominant] / (prof[dominant] * 1e-3) / 1e9

@@ -199,6 +199,8 @@ class TsnePlan {
   double* zpart = nullptr;
   float2* diag = nullptr;  // 2n scratch for diagnostics / output
   int cur = 0;
+  // blocks per launch: attraction (TPR lanes per point), per-point kernels,
+  // splat chunks
   int upd_blocks = 0, splat_blocks = 0, stat_blocks = 0;
   Stream stream;
   struct CachedGraph {
@@ -279,8 +281,8 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
   const size_t gf = static_cast<size_t>(g) * f;
   upd_blocks = static_cast<int>(std::min<long long>(
       (static_cast<long long>(n) * tpr + kThreads - 1) / kThreads, 2048));
-  splat_blocks = static_cast<int>(std::min<long long>((n + kThreads - 1) / kThreads, 2048));
-  stat_blocks = splat_blocks;
+  splat_blocks = static_cast<int>(std::min<long long>((n + kSplatChunk - 1) / kSplatChunk, 2048));
+  stat_blocks = static_cast<int>(std::min<long long>((n + kThreads - 1) / kThreads, 2048));
   ArenaPlan sp;
   sp.add<float2>(n);  // y0
   sp.add<float2>(n);  // y1
@@ -296,7 +298,7 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
   sp.add<float4>(gg);  // field
   sp.add<float2>(f);   // twiddles
   sp.add<TsneStats>(1);
-  sp.add<LayoutPartial>(std::max(upd_blocks, stat_blocks));
+  sp.add<LayoutPartial>(stat_blocks);
   sp.add<double>(g + 1);
   if (state.init(sp.bytes, "t-SNE optimizer state")) return 1;
   y[0] = state.take<float2>(n);
@@ -313,7 +315,7 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
   field = state.take<float4>(gg);
   tw = state.take<float2>(f);
   st = state.take<TsneStats>(1);
-  lpart = state.take<LayoutPartial>(std::max(upd_blocks, stat_blocks));
+  lpart = state.take<LayoutPartial>(stat_blocks);
   zpart = state.take<double>(g + 1);
 
   std::vector<float2> htw(f);
@@ -361,10 +363,11 @@ static int launch_conv(TsnePlan& p, hipStream_t s) {
   constexpr int G = F / 2;
   conv_rows_fwd<F><<<G + G / 2, kThreads, 0, s>>>(p.grids, p.st, p.tw, p.z1, p.z2, p.kt);
   FE_LAUNCH_CHECK("conv_rows_fwd");
-  conv_cols<F><<<G + 1, kThreads, 5 * F * sizeof(float2), s>>>(p.z1, p.z2, p.kt, p.tw, p.o1, p.o2,
+  conv_cols<F><<<conv_cols_blocks(G), kThreads, 5 * F * sizeof(float2), s>>>(p.z1, p.z2, p.kt, p.tw,
+                                                                             p.o1, p.o2,
                                                                p.zpart, p.st);
   FE_LAUNCH_CHECK("conv_cols");
-  conv_rows_inv<F><<<G, kThreads, 0, s>>>(p.o1, p.o2, p.tw, p.field);
+  conv_rows_inv<F><<<G, kThreads, 0, s>>>(p.o1, p.o2, p.tw, p.st, p.field);
   FE_LAUNCH_CHECK("conv_rows_inv");
   return 0;
 }
@@ -382,25 +385,32 @@ static int launch_conv_any(TsnePlan& p, hipStream_t s) {
   }
 }
 
-template <int MODE>
-static int launch_gather(TsnePlan& p, const StepParams& sp, hipStream_t s) {
+// attraction into diag[0, n), then the per-point gather/update (MODE 0) or
+// the repulsive force into diag[n, 2n) (MODE 1)
+static int launch_attract(TsnePlan& p, float exaggeration, hipStream_t s) {
   const float2* yin = p.y[p.cur];
-  float2* yout = p.y[p.cur ^ 1];
-  float2* da = p.diag;
-  float2* dr = p.diag + p.n;
-#define FE_GATHER(T)                                                                          \
-  gather_attract_update<T, MODE><<<p.upd_blocks, kThreads, 0, s>>>(                           \
-      yin, yout, p.gains, p.upd, p.csr.row_ptr, p.csr.col, p.csr.val, p.field, p.st, p.lpart, \
-      p.n, p.g, sp, da, dr)
+#define FE_ATTR(T)                                                                     \
+  attract_kernel<T><<<p.upd_blocks, kThreads, 0, s>>>(yin, p.csr.row_ptr, p.csr.col,   \
+                                                      p.csr.val, p.st, exaggeration,   \
+                                                      p.diag, p.n)
   switch (p.tpr) {
-    case 4: FE_GATHER(4); break;
-    case 8: FE_GATHER(8); break;
-    case 16: FE_GATHER(16); break;
-    case 32: FE_GATHER(32); break;
-    default: FE_GATHER(64); break;
+    case 4: FE_ATTR(4); break;
+    case 8: FE_ATTR(8); break;
+    case 16: FE_ATTR(16); break;
+    case 32: FE_ATTR(32); break;
+    default: FE_ATTR(64); break;
   }
-#undef FE_GATHER
-  FE_LAUNCH_CHECK("gather_attract_update");
+#undef FE_ATTR
+  FE_LAUNCH_CHECK("attract_kernel");
+  return 0;
+}
+
+template <int MODE>
+static int launch_update(TsnePlan& p, const StepParams& sp, hipStream_t s) {
+  update_kernel<MODE><<<p.stat_blocks, kThreads, 0, s>>>(p.y[p.cur], p.y[p.cur ^ 1], p.gains,
+                                                         p.upd, p.diag, p.field, p.st, p.lpart,
+                                                         p.n, p.g, sp, p.diag + p.n);
+  FE_LAUNCH_CHECK("update_kernel");
   return 0;
 }
 
@@ -409,7 +419,8 @@ int TsnePlan::enqueue_iteration(const StepParams& sp) {
   splat_kernel<<<splat_blocks, kThreads, 0, s>>>(y[cur], st, grids, n, g);
   FE_LAUNCH_CHECK("splat_kernel");
   if (launch_conv_any(*this, s)) return 1;
-  if (launch_gather<0>(*this, sp, s)) return 1;
+  if (launch_attract(*this, sp.exaggeration, s)) return 1;
+  if (launch_update<0>(*this, sp, s)) return 1;
   cur ^= 1;
   return 0;
 }
@@ -419,7 +430,8 @@ int TsnePlan::enqueue_diag(const StepParams& sp) {
   splat_kernel<<<splat_blocks, kThreads, 0, s>>>(y[cur], st, grids, n, g);
   FE_LAUNCH_CHECK("splat_kernel");
   if (launch_conv_any(*this, s)) return 1;
-  return launch_gather<1>(*this, sp, s);
+  if (launch_attract(*this, sp.exaggeration, s)) return 1;
+  return launch_update<1>(*this, sp, s);
 }
 
 static bool same_params(const StepParams& a, const StepParams& b) {
@@ -476,29 +488,31 @@ int TsnePlan::layout(float* out_host) {
   return 0;
 }
 
+// kernel_ms[0..5]: splat, conv_rows_fwd, conv_cols, conv_rows_inv,
+// attract, update; kernel_ms[6]: whole iteration
 int TsnePlan::profile(int iters, const StepParams& sp, double* ms) {
+  constexpr int K = 6;
   hipStream_t s = stream.get();
-  std::vector<hipEvent_t> ev(6 * iters);
+  std::vector<hipEvent_t> ev((K + 1) * iters);
   for (auto& e : ev) FE_CHECK(hipEventCreate(&e));
   for (int it = 0; it < iters; ++it) {
-    hipEvent_t* e = &ev[6 * it];
+    hipEvent_t* e = &ev[(K + 1) * it];
     FE_CHECK(hipEventRecord(e[0], s));
     splat_kernel<<<splat_blocks, kThreads, 0, s>>>(y[cur], st, grids, n, g);
     FE_CHECK(hipEventRecord(e[1], s));
-    const int F = f;
     int rc = 0;
 #define FE_PROF_CONV(FF)                                                                      \
   case FF: {                                                                                  \
     constexpr int G = FF / 2;                                                                 \
     conv_rows_fwd<FF><<<G + G / 2, kThreads, 0, s>>>(grids, st, tw, z1, z2, kt);              \
     (void)hipEventRecord(e[2], s);                                                            \
-    conv_cols<FF><<<G + 1, kThreads, 5 * FF * sizeof(float2), s>>>(z1, z2, kt, tw, o1, o2,   \
-                                                                   zpart, st);                \
+    conv_cols<FF><<<conv_cols_blocks(G), kThreads, 5 * FF * sizeof(float2), s>>>(            \
+        z1, z2, kt, tw, o1, o2, zpart, st);                                                   \
     (void)hipEventRecord(e[3], s);                                                            \
-    conv_rows_inv<FF><<<G, kThreads, 0, s>>>(o1, o2, tw, field);                              \
+    conv_rows_inv<FF><<<G, kThreads, 0, s>>>(o1, o2, tw, st, field);                          \
     break;                                                                                    \
   }
-    switch (F) {
+    switch (f) {
       FE_PROF_CONV(32)
       FE_PROF_CONV(64)
       FE_PROF_CONV(128)
@@ -511,24 +525,26 @@ int TsnePlan::profile(int iters, const StepParams& sp, double* ms) {
 #undef FE_PROF_CONV
     if (rc) return fail("unsupported FFT size");
     FE_CHECK(hipEventRecord(e[4], s));
-    if (launch_gather<0>(*this, sp, s)) return 1;
+    if (launch_attract(*this, sp.exaggeration, s)) return 1;
     FE_CHECK(hipEventRecord(e[5], s));
+    if (launch_update<0>(*this, sp, s)) return 1;
+    FE_CHECK(hipEventRecord(e[6], s));
     cur ^= 1;
   }
   FE_CHECK(hipStreamSynchronize(s));
-  for (int q = 0; q < 6; ++q) ms[q] = 0.0;
+  for (int q = 0; q <= K; ++q) ms[q] = 0.0;
   for (int it = 0; it < iters; ++it) {
-    hipEvent_t* e = &ev[6 * it];
-    for (int q = 0; q < 5; ++q) {
+    hipEvent_t* e = &ev[(K + 1) * it];
+    for (int q = 0; q < K; ++q) {
       float t = 0.f;
       FE_CHECK(hipEventElapsedTime(&t, e[q], e[q + 1]));
       ms[q] += t;
     }
     float t = 0.f;
-    FE_CHECK(hipEventElapsedTime(&t, e[0], e[5]));
-    ms[5] += t;
+    FE_CHECK(hipEventElapsedTime(&t, e[0], e[K]));
+    ms[K] += t;
   }
-  for (int q = 0; q < 6; ++q) ms[q] /= iters;
+  for (int q = 0; q <= K; ++q) ms[q] /= iters;
   for (auto& e : ev) (void)hipEventDestroy(e);
   iters_run += iters;
   return 0;
@@ -619,7 +635,7 @@ class TsneBatch {
   int tab_cap = 0;
   bool dirty = true;
   struct Launch {
-    int kind;   // 0 splat, 1 rows_fwd, 2 cols, 3 rows_inv, 4 gather
+    int kind;   // 0 splat, 1 rows_fwd, 2 cols, 3 rows_inv, 4 attract, 5 update
     int param;  // F or TPR
     int off;    // offset of slot[] in d_tab; first[] follows it
     int m;      // entries
@@ -694,6 +710,7 @@ class TsneBatch {
     d.st = p.st;
     d.lpart = p.lpart;
     d.zpart = p.zpart;
+    d.attr = p.diag;
     d.n = p.n;
     d.g = p.g;
     d.cur = p.cur;
@@ -761,7 +778,7 @@ class TsneBatch {
         for (int s = 0; s < max_jobs; ++s) {
           if (!slots[s] || slots[s]->f != F) continue;
           const int G = F / 2;
-          ent.push_back({s, kind == 1 ? G + G / 2 : (kind == 2 ? G + 1 : G)});
+          ent.push_back({s, kind == 1 ? G + G / 2 : (kind == 2 ? conv_cols_blocks(G) : G)});
         }
         add_launch(kind, F, ent);
       }
@@ -772,6 +789,10 @@ class TsneBatch {
         if (slots[s] && slots[s]->tpr == tpr) ent.push_back({s, slots[s]->upd_blocks});
       add_launch(4, tpr, ent);
     }
+    std::vector<std::pair<int, int>> upd;
+    for (int s = 0; s < max_jobs; ++s)
+      if (slots[s]) upd.push_back({s, slots[s]->stat_blocks});
+    add_launch(5, 0, upd);
     if (static_cast<int>(tab.size()) > tab_cap) return fail("batch table overflow");
     if (!tab.empty())
       FE_CHECK(hipMemcpy(d_tab, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -813,14 +834,17 @@ class TsneBatch {
           }
           break;
 #undef FE_B_CONV
-        default:
+        case 4:
           switch (L.param) {
-            case 4: batch_gather<4><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
-            case 8: batch_gather<8><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
-            case 16: batch_gather<16><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
-            case 32: batch_gather<32><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
-            default: batch_gather<64><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
+            case 4: batch_attract<4><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
+            case 8: batch_attract<8><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
+            case 16: batch_attract<16><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
+            case 32: batch_attract<32><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
+            default: batch_attract<64><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
           }
+          break;
+        default:
+          batch_update<<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
       }
       FE_LAUNCH_CHECK("batch kernel");
     }

@@ -36,6 +36,7 @@ struct JobDesc {
   TsneStats* st;
   LayoutPartial* lpart;
   double* zpart;
+  float2* attr;
   int n, g;
   int cur, iter, total_iter, early_iter, active, pad;
   float early_exag, exag, lr_early, lr_normal, mom0, mom1, min_gain, max_step;
@@ -71,15 +72,20 @@ __global__ __launch_bounds__(kThreads) void batch_rows_fwd(JobDesc* jobs, const 
   conv_rows_fwd_body<F>(J->grids, J->st, J->tw, J->z1, J->z2, J->kt, blockIdx.x - first[e]);
 }
 
+// blocks of a job's range follow the XCD-aware column order of conv_cols;
+// every entry of this launch has a multiple of 8 blocks, so job-local block
+// ids keep the hardware's XCD assignment
 template <int F>
 __global__ __launch_bounds__(kThreads) void batch_cols(JobDesc* jobs, const int* __restrict__ slot,
                                                        const int* __restrict__ first, int m) {
   extern __shared__ float2 lds[];
+  constexpr int G = F / 2;
   const int e = find_entry(first, m);
   JobDesc* J = jobs + slot[e];
   if (!J->active) return;
-  conv_cols_body<F>(J->z1, J->z2, J->kt, J->tw, J->o1, J->o2, J->zpart, J->st, lds,
-                    blockIdx.x - first[e], first[e + 1] - first[e]);
+  const int c = xcd_column(blockIdx.x - first[e], G);
+  if (c > G) return;
+  conv_cols_body<F>(J->z1, J->z2, J->kt, J->tw, J->o1, J->o2, J->zpart, J->st, lds, c, G + 1);
 }
 
 template <int F>
@@ -88,11 +94,21 @@ __global__ __launch_bounds__(kThreads) void batch_rows_inv(JobDesc* jobs, const 
   const int e = find_entry(first, m);
   JobDesc* J = jobs + slot[e];
   if (!J->active) return;
-  conv_rows_inv_body<F>(J->o1, J->o2, J->tw, J->field, blockIdx.x - first[e]);
+  conv_rows_inv_body<F>(J->o1, J->o2, J->tw, J->st, J->field, blockIdx.x - first[e]);
 }
 
 template <int TPR>
-__global__ __launch_bounds__(kThreads) void batch_gather(JobDesc* jobs, const int* __restrict__ slot,
+__global__ __launch_bounds__(kThreads) void batch_attract(JobDesc* jobs, const int* __restrict__ slot,
+                                                          const int* __restrict__ first, int m) {
+  const int e = find_entry(first, m);
+  JobDesc* J = jobs + slot[e];
+  if (!J->active) return;
+  const float exag = J->iter < J->early_iter ? J->early_exag : J->exag;
+  attract_body<TPR>(J->y[J->cur], J->row_ptr, J->col, J->val, J->st, exag, J->attr, J->n,
+                    blockIdx.x - first[e], first[e + 1] - first[e]);
+}
+
+__global__ __launch_bounds__(kThreads) void batch_update(JobDesc* jobs, const int* __restrict__ slot,
                                                          const int* __restrict__ first, int m) {
   const int e = find_entry(first, m);
   JobDesc* J = jobs + slot[e];
@@ -101,9 +117,9 @@ __global__ __launch_bounds__(kThreads) void batch_gather(JobDesc* jobs, const in
   const bool early = it < J->early_iter;
   const StepParams sp{early ? J->early_exag : J->exag, early ? J->lr_early : J->lr_normal,
                       early ? J->mom0 : J->mom1, J->min_gain, J->max_step};
-  const bool last = gather_attract_update_body<TPR, 0>(
-      J->y[cur], J->y[cur ^ 1], J->gains, J->upd, J->row_ptr, J->col, J->val, J->field, J->st,
-      J->lpart, J->n, J->g, sp, nullptr, nullptr, blockIdx.x - first[e], first[e + 1] - first[e]);
+  const bool last = update_body<0>(J->y[cur], J->y[cur ^ 1], J->gains, J->upd, J->attr, J->field,
+                                   J->st, J->lpart, J->n, J->g, sp, nullptr,
+                                   blockIdx.x - first[e], first[e + 1] - first[e]);
   if (last) {
     J->cur = cur ^ 1;
     J->iter = it + 1;

@@ -1,21 +1,21 @@
 // t-SNE device kernels for gfx950.
 //
-// Per-iteration pipeline (five launches, all state resident in HBM):
+// Per-iteration pipeline (six launches, all state resident in HBM):
 //   1. splat            bilinear spread of every point onto the G x G grid
-//                       (mass, mass*x, mass*y) with 64-bit fixed-point atomics,
+//                       (mass, mass*x, mass*y) with 64-bit fixed-point atomics
+//                       (LDS-aggregated per chunk when the chunk is compact),
 //                       so the grid sums do not depend on arrival order;
-//   2. conv_rows_fwd    row FFTs of z1 = mass + i*mass_x and z2 = mass_y
-//                       (zero padded to F = 2G), and of the G kernel rows
-//                       Kq + i*Kq^2 (both real and even, so the spectra come
-//                       out as real + i*real);
+//   2. conv_rows_fwd    row FFTs of z1 = mass + i*mass_x/x_scale and
+//                       z2 = mass_y (zero padded to F = 2G), and of the G
+//                       kernel rows Kq + i*Kq^2 (both real and even, so the
+//                       spectra come out as real + i*real);
 //   3. conv_cols        per column pair (c, F-c): column FFTs, Hermitian
 //                       split of z1 into FFT(mass) and FFT(mass_x), the
 //                       pointwise products with the kernel spectra, inverse
 //                       column FFTs, and Z by Parseval in FP64;
 //   4. conv_rows_inv    inverse row FFTs; writes the (q2, xq2, yq2, q) field;
-//   5. gather_attract_update
-//                       per point: bilinear gather of the field, CSR
-//                       attraction over the symmetric P (no atomics), the
+//   5. attract          CSR attraction over the symmetric P (no atomics);
+//   6. update           per point: bilinear gather of the field, the
 //                       gains/momentum/clip update, and the FP64 layout
 //                       statistics (mean, bounds) for the next iteration.
 // Centring is applied lazily: the layout buffer holds uncentred coordinates
@@ -45,6 +45,10 @@ struct TsneStats {
   float lower_x, lower_y;  // grid corner, centred coordinates
   float spacing, inv_spacing;
   float min_x, max_x, min_y, max_y;  // bounds of the centred layout
+  // power of two >= max |x|: the mass*x channel shares a complex FFT with
+  // the mass channel, and is carried as mass*x / x_scale so both parts have
+  // the magnitude of the mass (the Hermitian split then loses no digits)
+  double x_scale;
   unsigned int done_cols;  // last-block counters (reset by the last block)
   unsigned int done_stats;
 };
@@ -135,6 +139,8 @@ __device__ inline void finalize_geometry(TsneStats* st, double sx, double sy, fl
   const int exy = ilogb(nr) + 1;
   st->fx_mass = ldexp(1.0, 61 - en);
   st->fx_xy = ldexp(1.0, min(61 - exy, 1000));
+  const double rx = fmax(fabs(static_cast<double>(cmnx)), fabs(static_cast<double>(cmxx)));
+  st->x_scale = rx > 0.0 && isfinite(rx) ? ldexp(1.0, ilogb(rx) + 1) : 1.0;
 }
 
 // Block reduction of LayoutPartial then last-block finalize over the nblk
@@ -222,31 +228,123 @@ __global__ __launch_bounds__(kThreads) void layout_stats_kernel(const float2* __
 //    1186-1226, accumulated in 64-bit fixed point (deterministic).
 // grids: 3 channels of g*g int64, channel-major, row-major nodes.
 // ----------------------------------------------------------------------------
+// A block takes chunks of kSplatChunk consecutive points.  When the chunk's
+// cells fit a kTileSide x kTileSide node window (neighbouring point ids are
+// usually close in the layout, and early-exaggeration clusters are compact),
+// the chunk accumulates in LDS and flushes each touched node once with global
+// atomics; otherwise its points go straight to global atomics.  The sums are
+// integers, so both routes give identical grids.
+constexpr int kSplatPerThread = 4;
+constexpr int kSplatChunk = kThreads * kSplatPerThread;
+constexpr int kTileSide = 32;
+constexpr int kTileNodes = kTileSide * kTileSide;
+
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
 __device__ __forceinline__ void splat_body(const float2* __restrict__ y,
                                            const TsneStats* __restrict__ st,
                                            unsigned long long* grids, int n, int g, int blk,
                                            int nblk) {
+  __shared__ unsigned long long tile[3 * kTileNodes];
+  __shared__ int box[4];
   const float mx = st->mean_xf, my = st->mean_yf;
   const float lx = st->lower_x, ly = st->lower_y, inv = st->inv_spacing;
   const double sm = st->fx_mass, sxy = st->fx_xy;
   const size_t gg = static_cast<size_t>(g) * g;
-  for (int i = blk * kThreads + threadIdx.x; i < n; i += nblk * kThreads) {
-    const float2 p = y[i];
-    const float xc = p.x - mx, yc = p.y - my;
-    const Cell c = grid_cell(xc, yc, lx, ly, inv, g);
-    const float w00 = (1.0f - c.tx) * (1.0f - c.ty);
-    const float w10 = c.tx * (1.0f - c.ty);
-    const float w01 = (1.0f - c.tx) * c.ty;
-    const float w11 = c.tx * c.ty;
-    const size_t p00 = static_cast<size_t>(c.y0) * g + c.x0;
-    const size_t pos[4] = {p00, p00 + 1, p00 + g, p00 + g + 1};
-    const float w[4] = {w00, w10, w01, w11};
+  const int n_chunks = (n + kSplatChunk - 1) / kSplatChunk;
+  for (int ch = blk; ch < n_chunks; ch += nblk) {
+    Cell cl[kSplatPerThread];
+    float xc[kSplatPerThread], yc[kSplatPerThread];
+    bool ok[kSplatPerThread];
+    int bx0 = 1 << 30, by0 = 1 << 30, bx1 = -1, by1 = -1;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      atomicAdd(grids + pos[q], to_fixed(w[q], sm));
-      atomicAdd(grids + gg + pos[q], to_fixed(w[q] * xc, sxy));
-      atomicAdd(grids + 2 * gg + pos[q], to_fixed(w[q] * yc, sxy));
+    for (int u = 0; u < kSplatPerThread; ++u) {
+      const int i = ch * kSplatChunk + u * kThreads + threadIdx.x;
+      ok[u] = i < n;
+      if (ok[u]) {
+        const float2 p = y[i];
+        xc[u] = p.x - mx;
+        yc[u] = p.y - my;
+        cl[u] = grid_cell(xc[u], yc[u], lx, ly, inv, g);
+        bx0 = min(bx0, cl[u].x0);
+        by0 = min(by0, cl[u].y0);
+        bx1 = max(bx1, cl[u].x0);
+        by1 = max(by1, cl[u].y0);
+      }
     }
+    if (threadIdx.x == 0) {
+      box[0] = 1 << 30;
+      box[1] = 1 << 30;
+      box[2] = -1;
+      box[3] = -1;
+    }
+    __syncthreads();
+    bx0 = wave_min_i(bx0);
+    by0 = wave_min_i(by0);
+    bx1 = wave_max_i(bx1);
+    by1 = wave_max_i(by1);
+    if ((threadIdx.x & 63) == 0) {
+      atomicMin(&box[0], bx0);
+      atomicMin(&box[1], by0);
+      atomicMax(&box[2], bx1);
+      atomicMax(&box[3], by1);
+    }
+    __syncthreads();
+    const int ox = box[0], oy = box[1];
+    const int bw = box[2] - ox + 2, bh = box[3] - oy + 2;  // node window
+    const bool tiled = bw <= kTileSide && bh <= kTileSide;
+    if (tiled) {
+      for (int t = threadIdx.x; t < 3 * kTileNodes; t += kThreads) tile[t] = 0ull;
+      __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < kSplatPerThread; ++u) {
+      if (!ok[u]) continue;
+      const Cell c = cl[u];
+      const float w[4] = {(1.0f - c.tx) * (1.0f - c.ty), c.tx * (1.0f - c.ty),
+                          (1.0f - c.tx) * c.ty, c.tx * c.ty};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int nx = c.x0 + (q & 1), ny = c.y0 + (q >> 1);
+        const unsigned long long fm = to_fixed(w[q], sm);
+        const unsigned long long fx = to_fixed(w[q] * xc[u], sxy);
+        const unsigned long long fy = to_fixed(w[q] * yc[u], sxy);
+        if (tiled) {
+          const int t = (ny - oy) * kTileSide + (nx - ox);
+          atomicAdd(tile + t, fm);
+          atomicAdd(tile + kTileNodes + t, fx);
+          atomicAdd(tile + 2 * kTileNodes + t, fy);
+        } else {
+          const size_t node = static_cast<size_t>(ny) * g + nx;
+          atomicAdd(grids + node, fm);
+          atomicAdd(grids + gg + node, fx);
+          atomicAdd(grids + 2 * gg + node, fy);
+        }
+      }
+    }
+    if (tiled) {
+      __syncthreads();
+      for (int t = threadIdx.x; t < bw * bh; t += kThreads) {
+        const int ty = t / bw, tx = t - ty * bw;
+        const int lt = ty * kTileSide + tx;
+        const size_t node = static_cast<size_t>(oy + ty) * g + (ox + tx);
+#pragma unroll
+        for (int c3 = 0; c3 < 3; ++c3) {
+          const unsigned long long v = tile[c3 * kTileNodes + lt];
+          if (v != 0ull) atomicAdd(grids + c3 * gg + node, v);
+        }
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -273,6 +371,7 @@ __device__ __forceinline__ void conv_rows_fwd_body(unsigned long long* grids,
   if (b < G) {
     const int row = b;
     const double inv_m = 1.0 / st->fx_mass, inv_xy = 1.0 / st->fx_xy;
+    const double inv_xs = inv_xy / st->x_scale;
     constexpr size_t gg = static_cast<size_t>(G) * G;
     for (int x = threadIdx.x; x < F; x += kThreads) {
       float2 a = make_float2(0.f, 0.f), c = make_float2(0.f, 0.f);
@@ -285,7 +384,7 @@ __device__ __forceinline__ void conv_rows_fwd_body(unsigned long long* grids,
         grids[gg + node] = 0ull;
         grids[2 * gg + node] = 0ull;
         a = make_float2(static_cast<float>(static_cast<double>(m) * inv_m),
-                        static_cast<float>(static_cast<double>(mxv) * inv_xy));
+                        static_cast<float>(static_cast<double>(mxv) * inv_xs));
         c = make_float2(static_cast<float>(static_cast<double>(myv) * inv_xy), 0.f);
       }
       buf[x] = a;
@@ -439,6 +538,18 @@ __device__ __forceinline__ void conv_cols_body(const float2* __restrict__ z1,
   }
 }
 
+// XCD-aware column order: workgroups are dealt round-robin to the eight XCDs,
+// so block b runs on XCD b % 8.  Giving each XCD a contiguous range of
+// columns makes the blocks that share 128-byte lines of the strided column
+// reads and writes run on one XCD, where they hit the same L2.  A launch has
+// conv_cols_blocks(G) blocks; the surplus blocks of the last range exit.
+__host__ __device__ constexpr int conv_cols_per_xcd(int g) { return (g + 1 + 7) / 8; }
+__host__ __device__ constexpr int conv_cols_blocks(int g) { return 8 * conv_cols_per_xcd(g); }
+
+__device__ __forceinline__ int xcd_column(int b, int g) {
+  return (b & 7) * conv_cols_per_xcd(g) + (b >> 3);
+}
+
 template <int F>
 __global__ __launch_bounds__(kThreads) void conv_cols(const float2* __restrict__ z1,
                                                       const float2* __restrict__ z2,
@@ -448,7 +559,10 @@ __global__ __launch_bounds__(kThreads) void conv_cols(const float2* __restrict__
                                                       float2* __restrict__ o2,
                                                       double* partials, TsneStats* st) {
   extern __shared__ float2 lds[];
-  conv_cols_body<F>(z1, z2, kt, tw, o1, o2, partials, st, lds, blockIdx.x, gridDim.x);
+  constexpr int G = F / 2;
+  const int c = xcd_column(blockIdx.x, G);
+  if (c > G) return;
+  conv_cols_body<F>(z1, z2, kt, tw, o1, o2, partials, st, lds, c, G + 1);
 }
 
 // ----------------------------------------------------------------------------
@@ -458,6 +572,7 @@ template <int F>
 __device__ __forceinline__ void conv_rows_inv_body(const float2* __restrict__ o1,
                                                    const float2* __restrict__ o2,
                                                    const float2* __restrict__ tw,
+                                                   const TsneStats* __restrict__ st,
                                                    float4* __restrict__ field, int row) {
   constexpr int G = F / 2;
   __shared__ float2 buf[2 * F];
@@ -470,9 +585,10 @@ __device__ __forceinline__ void conv_rows_inv_body(const float2* __restrict__ o1
   __syncthreads();
   fft_lds<F, 2, kThreads, true>(buf, tw);
   float4* out = field + static_cast<size_t>(row) * G;
+  const double xs = st->x_scale;
   for (int x = threadIdx.x; x < G; x += kThreads) {
     const float2 a = buf[x], b = buf[F + x];
-    out[x] = make_float4(b.x, a.y, b.y, a.x);
+    out[x] = make_float4(b.x, static_cast<float>(static_cast<double>(a.y) * xs), b.y, a.x);
   }
 }
 
@@ -480,17 +596,19 @@ template <int F>
 __global__ __launch_bounds__(kThreads) void conv_rows_inv(const float2* __restrict__ o1,
                                                           const float2* __restrict__ o2,
                                                           const float2* __restrict__ tw,
+                                                          const TsneStats* __restrict__ st,
                                                           float4* __restrict__ field) {
-  conv_rows_inv_body<F>(o1, o2, tw, field, blockIdx.x);
+  conv_rows_inv_body<F>(o1, o2, tw, st, field, blockIdx.x);
 }
 
 // ----------------------------------------------------------------------------
-// 5. gather + attraction + update.  TPR lanes cooperate on one point's CSR
-//    row (deterministic lane-strided sums and a fixed butterfly reduction).
-// MODE 0: optimiser step (apply_open_tsne_update_f, tsne_neighbors.cpp:
-//         1297-1336; CUDA twin embedding_cuda_kernels.cpp:2819-2886), writes
-//         the new raw layout into y_out and the layout statistics.
-// MODE 1: diagnostic; writes the attractive and repulsive forces.
+// 5a. attraction (add_sparse_attractive_gradient_f, tsne_neighbors.cpp:
+//     1061-1088; the CUDA directed-edge twin is embedding_cuda_kernels.cpp:
+//     2788-2817).  TPR lanes own one point's CSR row; each lane keeps U
+//     entries in flight (U independent col/val loads, then U independent
+//     neighbour loads) so the row costs two dependent memory round trips per
+//     U*TPR entries.  Lane-strided sums and a fixed butterfly reduction keep
+//     it deterministic.  attr[i] = exaggeration * sum_j P_ij q_ij (y_i - y_j).
 // ----------------------------------------------------------------------------
 struct StepParams {
   float exaggeration, learning_rate, momentum, min_gain, max_step_norm;
@@ -498,102 +616,149 @@ struct StepParams {
 
 __device__ __forceinline__ float sign_tsne(float x) { return x == 0.0f ? 0.0f : (x < 0.0f ? -1.0f : 1.0f); }
 
-// Returns true in thread 0 of the block that finalised the layout statistics
-// (MODE 0 only).
-template <int TPR, int MODE>
-__device__ __forceinline__ bool gather_attract_update_body(
-    const float2* __restrict__ y_in, float2* __restrict__ y_out, float2* __restrict__ gains,
-    float2* __restrict__ upd, const int* __restrict__ row_ptr, const int* __restrict__ col,
-    const float* __restrict__ val, const float4* __restrict__ field, TsneStats* st,
-    LayoutPartial* partials, int n, int g, StepParams sp, float2* __restrict__ diag_attr,
-    float2* __restrict__ diag_rep, int blk, int nblk) {
+constexpr int kAttrUnroll = 4;
+
+template <int TPR>
+__device__ __forceinline__ void attract_body(const float2* __restrict__ y_in,
+                                             const int* __restrict__ row_ptr,
+                                             const int* __restrict__ col,
+                                             const float* __restrict__ val,
+                                             const TsneStats* __restrict__ st, float exaggeration,
+                                             float2* __restrict__ attr, int n, int blk, int nblk) {
+  constexpr int U = kAttrUnroll;
   const int lane = threadIdx.x % TPR;
   const int group = (blk * kThreads + static_cast<int>(threadIdx.x)) / TPR;
   const int n_groups = nblk * (kThreads / TPR);
   const float mxf = st->mean_xf, myf = st->mean_yf;
-  const float lx = st->lower_x, ly = st->lower_y, inv = st->inv_spacing;
-  const float inv_z = static_cast<float>(
-      1.0 / fmax(st->sum_q_total - static_cast<double>(n), static_cast<double>(FLT_MIN)));
-  LayoutPartial m{0.0, 0.0, FLT_MAX, -FLT_MAX, FLT_MAX, -FLT_MAX};
   for (int i = group; i < n; i += n_groups) {
     const float2 pi = y_in[i];
     const float xi = pi.x - mxf, yi = pi.y - myf;
     float ax = 0.0f, ay = 0.0f;
     const int beg = row_ptr[i], end = row_ptr[i + 1];
-    for (int p = beg + lane; p < end; p += TPR) {
-      const int j = col[p];
-      const float2 pj = y_in[j];
-      const float dx = xi - (pj.x - mxf), dy = yi - (pj.y - myf);
-      const float q = 1.0f / (1.0f + dx * dx + dy * dy);
-      const float cf = sp.exaggeration * val[p] * q;
-      ax += cf * dx;
-      ay += cf * dy;
+    for (int base = beg + lane; base < end; base += U * TPR) {
+      int j[U];
+      float v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int p = base + u * TPR;
+        const bool ok = p < end;
+        j[u] = ok ? col[p] : i;
+        v[u] = ok ? val[p] : 0.0f;
+      }
+      float2 pj[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) pj[u] = y_in[j[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const float dx = xi - (pj[u].x - mxf), dy = yi - (pj[u].y - myf);
+        const float q = 1.0f / (1.0f + dx * dx + dy * dy);
+        const float cf = exaggeration * v[u] * q;
+        ax += cf * dx;
+        ay += cf * dy;
+      }
     }
 #pragma unroll
     for (int o = TPR / 2; o > 0; o >>= 1) {
       ax += __shfl_xor(ax, o, TPR);
       ay += __shfl_xor(ay, o, TPR);
     }
-    if (lane == 0) {
-      const Cell c = grid_cell(xi, yi, lx, ly, inv, g);
-      const size_t p00 = static_cast<size_t>(c.y0) * g + c.x0;
-      const float4 f00 = field[p00], f10 = field[p00 + 1];
-      const float4 f01 = field[p00 + g], f11 = field[p00 + g + 1];
-      const float w00 = (1.0f - c.tx) * (1.0f - c.ty), w10 = c.tx * (1.0f - c.ty);
-      const float w01 = (1.0f - c.tx) * c.ty, w11 = c.tx * c.ty;
-      const float q2 = w00 * f00.x + w10 * f10.x + w01 * f01.x + w11 * f11.x;
-      const float xq2 = w00 * f00.y + w10 * f10.y + w01 * f01.y + w11 * f11.y;
-      const float yq2 = w00 * f00.z + w10 * f10.z + w01 * f01.z + w11 * f11.z;
-      const float rx = -(xi * q2 - xq2) * inv_z;
-      const float ry = -(yi * q2 - yq2) * inv_z;
-      if constexpr (MODE == 1) {
-        diag_attr[i] = make_float2(ax, ay);
-        diag_rep[i] = make_float2(rx, ry);
-      } else {
-        const float gx = rx + ax, gy = ry + ay;
-        float2 gn = gains[i];
-        const float2 old = upd[i];
-        gn.x = sign_tsne(old.x) != sign_tsne(gx) ? gn.x + 0.2f : gn.x * 0.8f + sp.min_gain;
-        gn.y = sign_tsne(old.y) != sign_tsne(gy) ? gn.y + 0.2f : gn.y * 0.8f + sp.min_gain;
-        gn.x = fmaxf(gn.x, sp.min_gain);
-        gn.y = fmaxf(gn.y, sp.min_gain);
-        float sx = sp.momentum * old.x - sp.learning_rate * gn.x * gx;
-        float sy = sp.momentum * old.y - sp.learning_rate * gn.y * gy;
-        if (isfinite(sp.max_step_norm) && sp.max_step_norm > 0.0f) {
-          const float norm2 = sx * sx + sy * sy;
-          if (norm2 > sp.max_step_norm * sp.max_step_norm) {
-            const float scale = sp.max_step_norm / sqrtf(fmaxf(norm2, FLT_MIN));
-            sx *= scale;
-            sy *= scale;
-          }
+    if (lane == 0) attr[i] = make_float2(ax, ay);
+  }
+}
+
+template <int TPR>
+__global__ __launch_bounds__(kThreads) void attract_kernel(const float2* __restrict__ y_in,
+                                                           const int* __restrict__ row_ptr,
+                                                           const int* __restrict__ col,
+                                                           const float* __restrict__ val,
+                                                           const TsneStats* __restrict__ st,
+                                                           float exaggeration,
+                                                           float2* __restrict__ attr, int n) {
+  attract_body<TPR>(y_in, row_ptr, col, val, st, exaggeration, attr, n, blockIdx.x, gridDim.x);
+}
+
+// ----------------------------------------------------------------------------
+// 5b. gather + update, one thread per point (coalesced).  Back-interpolation
+//     of (q2, xq2, yq2) at the point's cell (tsne_neighbors.cpp:1247-1258),
+//     rep = -(y q2 - xq2, y q2 - yq2) / Z, grad = rep + attr.
+// MODE 0: optimiser step (apply_open_tsne_update_f, tsne_neighbors.cpp:
+//         1297-1336; CUDA twin embedding_cuda_kernels.cpp:2819-2886) into
+//         y_out plus the layout statistics for the next iteration; returns
+//         true in thread 0 of the block that finalised them.
+// MODE 1: diagnostic; writes the repulsive force to diag_rep.
+// ----------------------------------------------------------------------------
+template <int MODE>
+__device__ __forceinline__ bool update_body(const float2* __restrict__ y_in,
+                                            float2* __restrict__ y_out,
+                                            float2* __restrict__ gains, float2* __restrict__ upd,
+                                            const float2* __restrict__ attr,
+                                            const float4* __restrict__ field, TsneStats* st,
+                                            LayoutPartial* partials, int n, int g, StepParams sp,
+                                            float2* __restrict__ diag_rep, int blk, int nblk) {
+  const float mxf = st->mean_xf, myf = st->mean_yf;
+  const float lx = st->lower_x, ly = st->lower_y, inv = st->inv_spacing;
+  const float inv_z = static_cast<float>(
+      1.0 / fmax(st->sum_q_total - static_cast<double>(n), static_cast<double>(FLT_MIN)));
+  LayoutPartial m{0.0, 0.0, FLT_MAX, -FLT_MAX, FLT_MAX, -FLT_MAX};
+  for (int i = blk * kThreads + threadIdx.x; i < n; i += nblk * kThreads) {
+    const float2 pi = y_in[i];
+    const float xi = pi.x - mxf, yi = pi.y - myf;
+    const Cell c = grid_cell(xi, yi, lx, ly, inv, g);
+    const size_t p00 = static_cast<size_t>(c.y0) * g + c.x0;
+    const float4 f00 = field[p00], f10 = field[p00 + 1];
+    const float4 f01 = field[p00 + g], f11 = field[p00 + g + 1];
+    const float w00 = (1.0f - c.tx) * (1.0f - c.ty), w10 = c.tx * (1.0f - c.ty);
+    const float w01 = (1.0f - c.tx) * c.ty, w11 = c.tx * c.ty;
+    const float q2 = w00 * f00.x + w10 * f10.x + w01 * f01.x + w11 * f11.x;
+    const float xq2 = w00 * f00.y + w10 * f10.y + w01 * f01.y + w11 * f11.y;
+    const float yq2 = w00 * f00.z + w10 * f10.z + w01 * f01.z + w11 * f11.z;
+    const float rx = -(xi * q2 - xq2) * inv_z;
+    const float ry = -(yi * q2 - yq2) * inv_z;
+    if constexpr (MODE == 1) {
+      diag_rep[i] = make_float2(rx, ry);
+    } else {
+      const float2 a = attr[i];
+      const float gx = rx + a.x, gy = ry + a.y;
+      float2 gn = gains[i];
+      const float2 old = upd[i];
+      gn.x = sign_tsne(old.x) != sign_tsne(gx) ? gn.x + 0.2f : gn.x * 0.8f + sp.min_gain;
+      gn.y = sign_tsne(old.y) != sign_tsne(gy) ? gn.y + 0.2f : gn.y * 0.8f + sp.min_gain;
+      gn.x = fmaxf(gn.x, sp.min_gain);
+      gn.y = fmaxf(gn.y, sp.min_gain);
+      float sx = sp.momentum * old.x - sp.learning_rate * gn.x * gx;
+      float sy = sp.momentum * old.y - sp.learning_rate * gn.y * gy;
+      if (isfinite(sp.max_step_norm) && sp.max_step_norm > 0.0f) {
+        const float norm2 = sx * sx + sy * sy;
+        if (norm2 > sp.max_step_norm * sp.max_step_norm) {
+          const float scale = sp.max_step_norm / sqrtf(fmaxf(norm2, FLT_MIN));
+          sx *= scale;
+          sy *= scale;
         }
-        const float nx = xi + sx, ny = yi + sy;
-        gains[i] = gn;
-        upd[i] = make_float2(sx, sy);
-        y_out[i] = make_float2(nx, ny);
-        m.sx += nx;
-        m.sy += ny;
-        m.mnx = fminf(m.mnx, nx);
-        m.mxx = fmaxf(m.mxx, nx);
-        m.mny = fminf(m.mny, ny);
-        m.mxy = fmaxf(m.mxy, ny);
       }
+      const float nx = xi + sx, ny = yi + sy;
+      gains[i] = gn;
+      upd[i] = make_float2(sx, sy);
+      y_out[i] = make_float2(nx, ny);
+      m.sx += nx;
+      m.sy += ny;
+      m.mnx = fminf(m.mnx, nx);
+      m.mxx = fmaxf(m.mxx, nx);
+      m.mny = fminf(m.mny, ny);
+      m.mxy = fmaxf(m.mxy, ny);
     }
   }
   if constexpr (MODE == 0) return reduce_layout_stats(m, partials, st, n, g, true, blk, nblk);
   return false;
 }
 
-template <int TPR, int MODE>
-__global__ __launch_bounds__(kThreads) void gather_attract_update(
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void update_kernel(
     const float2* __restrict__ y_in, float2* __restrict__ y_out, float2* __restrict__ gains,
-    float2* __restrict__ upd, const int* __restrict__ row_ptr, const int* __restrict__ col,
-    const float* __restrict__ val, const float4* __restrict__ field, TsneStats* st,
-    LayoutPartial* partials, int n, int g, StepParams sp, float2* __restrict__ diag_attr,
+    float2* __restrict__ upd, const float2* __restrict__ attr, const float4* __restrict__ field,
+    TsneStats* st, LayoutPartial* partials, int n, int g, StepParams sp,
     float2* __restrict__ diag_rep) {
-  (void)gather_attract_update_body<TPR, MODE>(y_in, y_out, gains, upd, row_ptr, col, val, field,
-                                              st, partials, n, g, sp, diag_attr, diag_rep,
-                                              blockIdx.x, gridDim.x);
+  (void)update_body<MODE>(y_in, y_out, gains, upd, attr, field, st, partials, n, g, sp, diag_rep,
+                          blockIdx.x, gridDim.x);
 }
 
 // Write the centred layout (final output): out_i = fl(y_i - mean).

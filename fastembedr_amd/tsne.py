@@ -281,11 +281,11 @@ class TsnePlan:
 
     def profile(self, n_iter, exaggeration, momentum, learning_rate, min_gain=0.01,
                 max_step_norm=5.0):
-        ms = np.zeros(6)
+        ms = np.zeros(7)
         check(_lib.lib().fastembedr_hip_tsne_plan_profile(
             self._h, int(n_iter), float(exaggeration), float(momentum), float(learning_rate),
             float(min_gain), float(max_step_norm), ptr(ms, _dp)), "plan profile")
-        keys = ("splat", "conv_rows_fwd", "conv_cols", "conv_rows_inv", "gather_attract_update",
+        keys = ("splat", "conv_rows_fwd", "conv_cols", "conv_rows_inv", "attract", "update",
                 "iteration")
         return dict(zip(keys, ms.tolist()))
 

@@ -126,10 +126,10 @@ int fastembedr_hip_tsne_plan_info(fastembedr_hip_tsne_plan* plan, long long* inf
 int fastembedr_hip_tsne_plan_affinities(fastembedr_hip_tsne_plan* plan, int* row_ptr, int* col,
                                         float* val);
 
-/* Runs n_iter eager iterations with hipEvents between the five kernels on
- * the plan's stream; kernel_ms[0..4] = mean ms of splat, conv_rows_fwd,
- * conv_cols, conv_rows_inv, gather_attract_update; kernel_ms[5] = mean ms
- * per iteration. */
+/* Runs n_iter eager iterations with hipEvents between the six kernels on
+ * the plan's stream; kernel_ms[0..5] = mean ms of splat, conv_rows_fwd,
+ * conv_cols, conv_rows_inv, attract, update; kernel_ms[6] = mean ms per
+ * iteration. */
 int fastembedr_hip_tsne_plan_profile(fastembedr_hip_tsne_plan* plan, int n_iter,
                                      float exaggeration, float momentum, float learning_rate,
                                      float min_gain, float max_step_norm, double* kernel_ms);
