@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--k", type=int, default=90)
     ap.add_argument("--profile-iters", type=int, default=20)
-    ap.add_argument("--cpu-iters", type=int, default=6,
+    ap.add_argument("--cpu-iters", type=int, default=160,
                     help="iterations of the CPU reference algorithm in the baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu_count)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -125,8 +125,8 @@ def main():
     nnz_dir = n * k
     # algorithmic bytes per launch of each kernel
     algo = {
-        "splat": 8 * n + 12 * g * g,
-        "conv_rows_fwd": 12 * g * g + 2 * 8 * g * f,
+        "splat": 32 * n + 16 * g * g,
+        "conv_rows_fwd": 16 * g * g + 2 * 8 * g * f + 8 * (g + 1) * g,
         "conv_cols": 4 * 8 * g * f,
         "conv_rows_inv": 2 * 8 * g * f + 16 * g * g,
         "attract": 8 * nnz_dir + 28 * n,

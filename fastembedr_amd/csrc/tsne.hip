@@ -187,10 +187,19 @@ class TsnePlan {
   DeviceCsr csr;
   Arena state;
   Arena csr_store;
+  // entry-balanced attraction: row-start bits per 64 entries, first row of
+  // every wave's range (used when every row of P has an entry)
+  Arena attr_store;
+  unsigned long long* row_start = nullptr;
+  int* wave_row = nullptr;
+  int attr_waves = 0;
+  bool flat_attr = false;
   float2* y[2] = {nullptr, nullptr};
   float2* gains = nullptr;
   float2* upd = nullptr;
-  unsigned long long* grids = nullptr;
+  // splat: bins of points by grid tile, work items, the float grid of the
+  // row FFTs
+  SplatBuffers splat{};
   float2 *z1 = nullptr, *z2 = nullptr, *kt = nullptr, *o1 = nullptr, *o2 = nullptr;
   float4* field = nullptr;
   float2* tw = nullptr;
@@ -200,8 +209,8 @@ class TsnePlan {
   float2* diag = nullptr;  // 2n scratch for diagnostics / output
   int cur = 0;
   // blocks per launch: attraction (TPR lanes per point), per-point kernels,
-  // splat chunks
-  int upd_blocks = 0, splat_blocks = 0, stat_blocks = 0;
+  // splat binning (point ranges) and splat tiles
+  int upd_blocks = 0, splat_blocks = 0, stat_blocks = 0, item_blocks = 0;
   Stream stream;
   struct CachedGraph {
     StepParams sp;
@@ -214,6 +223,7 @@ class TsnePlan {
   template <typename DistT>
   int create(const int* idx, const DistT* dist, bool device_inputs, int n_, int k_, int offset,
              float perplexity, int grid_size, const float* init, unsigned seed, bool centre);
+  int setup_attraction(hipStream_t s);
   int enqueue_iteration(const StepParams& sp);
   int enqueue_diag(const StepParams& sp);
   int run(int iters, const StepParams& sp);
@@ -227,6 +237,50 @@ static int choose_tpr(double avg_row) {
   if (avg_row < 48) return 16;
   if (avg_row < 160) return 32;
   return 64;
+}
+
+// FASTEMBEDR_HIP_ATTRACTION=rows keeps the row-per-lane-group kernel (both
+// kernels compute the same per-entry terms; only the summation order differs)
+static bool entry_balanced_requested() {
+  const char* v = std::getenv("FASTEMBEDR_HIP_ATTRACTION");
+  return !(v && std::strcmp(v, "rows") == 0);
+}
+
+int TsnePlan::setup_attraction(hipStream_t s) {
+  const long long nnz = csr.nnz;
+  flat_attr = false;
+  if (nnz <= 0 || !entry_balanced_requested()) return 0;
+  int waves = static_cast<int>(std::min<long long>(std::max<long long>(nnz / 2048, 4), 32768));
+  waves = (waves + 3) / 4 * 4;
+  const size_t words = static_cast<size_t>((nnz + 63) / 64) + kFlatWords;
+  ArenaPlan ap;
+  ap.add<unsigned long long>(words);
+  ap.add<int>(waves + 1);
+  ap.add<unsigned int>(1);
+  if (attr_store.init(ap.bytes, "t-SNE attraction tables")) return 1;
+  row_start = attr_store.take<unsigned long long>(words);
+  wave_row = attr_store.take<int>(waves + 1);
+  unsigned int* d_empty = attr_store.take<unsigned int>(1);
+  FE_CHECK(hipMemsetAsync(row_start, 0, words * sizeof(unsigned long long), s));
+  FE_CHECK(hipMemsetAsync(d_empty, 0, sizeof(unsigned int), s));
+  attract_row_starts<<<(n + kThreads - 1) / kThreads, kThreads, 0, s>>>(csr.row_ptr, n, row_start,
+                                                                      d_empty);
+  FE_LAUNCH_CHECK("attract_row_starts");
+  attract_wave_rows<<<(waves + kThreads) / kThreads, kThreads, 0, s>>>(csr.row_ptr, n, nnz, waves,
+                                                                      wave_row);
+  FE_LAUNCH_CHECK("attract_wave_rows");
+  unsigned int empty = 0;
+  FE_CHECK(hipMemcpyAsync(&empty, d_empty, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+  FE_CHECK(hipStreamSynchronize(s));
+  if (empty != 0) {
+    attr_store.release();
+    row_start = nullptr;
+    wave_row = nullptr;
+    return 0;
+  }
+  attr_waves = waves;
+  flat_attr = true;
+  return 0;
 }
 
 template <typename DistT>
@@ -261,7 +315,8 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
     d_dist = ud;
   }
   auto alloc_csr = [&](int nnz, int** row_ptr, int** col, float** val) -> int {
-    const size_t m = nnz > 0 ? static_cast<size_t>(nnz) : 1;
+    // padded so the entry-balanced attraction can load whole 512-entry steps
+    const size_t m = static_cast<size_t>(nnz > 0 ? nnz : 0) + kFlatStep;
     ArenaPlan cp;
     cp.add<int>(n + 1);
     cp.add<int>(m);
@@ -275,13 +330,16 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
   if (build_affinities<DistT>(d_idx, d_dist, n, k, offset, perplexity, s, &csr, alloc_csr)) return 1;
   upload.release();
   tpr = choose_tpr(static_cast<double>(csr.nnz) / n);
+  if (setup_attraction(s)) return 1;
 
   // optimizer state
   const size_t gg = static_cast<size_t>(g) * g;
   const size_t gf = static_cast<size_t>(g) * f;
   upd_blocks = static_cast<int>(std::min<long long>(
       (static_cast<long long>(n) * tpr + kThreads - 1) / kThreads, 2048));
-  splat_blocks = static_cast<int>(std::min<long long>((n + kSplatChunk - 1) / kSplatChunk, 2048));
+  splat_blocks = static_cast<int>(std::max<long long>(1, std::min<long long>((n + 2047) / 2048, 1024)));
+  item_blocks = splat_item_blocks(n, g);
+  const int nbins = splat_bins(g), ntiles = nbins / 4;
   stat_blocks = static_cast<int>(std::min<long long>((n + kThreads - 1) / kThreads, 2048));
   ArenaPlan sp;
   sp.add<float2>(n);  // y0
@@ -289,7 +347,14 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
   sp.add<float2>(n);  // gains
   sp.add<float2>(n);  // upd
   sp.add<float2>(2 * static_cast<size_t>(n));  // diag
-  sp.add<unsigned long long>(3 * gg);
+  sp.add<int>(nbins);                     // bin_count
+  sp.add<int>(nbins + 1);                 // bin_start
+  sp.add<int>(nbins);                     // bin_cursor
+  sp.add<int>(ntiles + 1);                // tile_items
+  sp.add<unsigned int>(ntiles);           // tile_done
+  sp.add<float2>(n);                      // binned
+  sp.add<unsigned long long>(3 * gg);     // wide
+  sp.add<float4>(gg);                     // sgrid
   sp.add<float2>(gf);  // z1
   sp.add<float2>(gf);  // z2
   sp.add<float2>(static_cast<size_t>(g + 1) * g);  // kt
@@ -306,7 +371,14 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
   gains = state.take<float2>(n);
   upd = state.take<float2>(n);
   diag = state.take<float2>(2 * static_cast<size_t>(n));
-  grids = state.take<unsigned long long>(3 * gg);
+  splat.bin_count = state.take<int>(nbins);
+  splat.bin_start = state.take<int>(nbins + 1);
+  splat.bin_cursor = state.take<int>(nbins);
+  splat.tile_items = state.take<int>(ntiles + 1);
+  splat.tile_done = state.take<unsigned int>(ntiles);
+  splat.binned = state.take<float2>(n);
+  splat.wide = state.take<unsigned long long>(3 * gg);
+  splat.sgrid = state.take<float4>(gg);
   z1 = state.take<float2>(gf);
   z2 = state.take<float2>(gf);
   kt = state.take<float2>(static_cast<size_t>(g + 1) * g);
@@ -339,7 +411,9 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
   std::vector<float2> ones(n, make_float2(1.0f, 1.0f));
   FE_CHECK(hipMemcpyAsync(gains, ones.data(), n * sizeof(float2), hipMemcpyHostToDevice, s));
   FE_CHECK(hipMemsetAsync(upd, 0, n * sizeof(float2), s));
-  FE_CHECK(hipMemsetAsync(grids, 0, 3 * gg * sizeof(unsigned long long), s));
+  FE_CHECK(hipMemsetAsync(splat.bin_count, 0, nbins * sizeof(int), s));
+  FE_CHECK(hipMemsetAsync(splat.tile_done, 0, ntiles * sizeof(unsigned int), s));
+  FE_CHECK(hipMemsetAsync(splat.wide, 0, 3 * gg * sizeof(unsigned long long), s));
   FE_CHECK(hipMemsetAsync(st, 0, sizeof(TsneStats), s));
   layout_stats_kernel<<<stat_blocks, kThreads, 0, s>>>(y[0], lpart, st, n, g, centre ? 1 : 0);
   FE_LAUNCH_CHECK("layout_stats_kernel");
@@ -361,7 +435,7 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
 template <int F>
 static int launch_conv(TsnePlan& p, hipStream_t s) {
   constexpr int G = F / 2;
-  conv_rows_fwd<F><<<G + G / 2, kThreads, 0, s>>>(p.grids, p.st, p.tw, p.z1, p.z2, p.kt);
+  conv_rows_fwd<F><<<G + G / 2, kThreads, 0, s>>>(p.splat.sgrid, p.st, p.tw, p.z1, p.z2, p.kt);
   FE_LAUNCH_CHECK("conv_rows_fwd");
   conv_cols<F><<<conv_cols_blocks(G), kThreads, 5 * F * sizeof(float2), s>>>(p.z1, p.z2, p.kt, p.tw,
                                                                              p.o1, p.o2,
@@ -369,6 +443,17 @@ static int launch_conv(TsnePlan& p, hipStream_t s) {
   FE_LAUNCH_CHECK("conv_cols");
   conv_rows_inv<F><<<G, kThreads, 0, s>>>(p.o1, p.o2, p.tw, p.st, p.field);
   FE_LAUNCH_CHECK("conv_rows_inv");
+  return 0;
+}
+
+static int launch_splat(TsnePlan& p, hipStream_t s) {
+  const float2* yin = p.y[p.cur];
+  splat_count<<<p.splat_blocks, kThreads, 0, s>>>(yin, p.st, p.splat, p.n, p.g);
+  FE_LAUNCH_CHECK("splat_count");
+  splat_scatter<<<p.splat_blocks, kThreads, 0, s>>>(yin, p.st, p.splat, p.n, p.g);
+  FE_LAUNCH_CHECK("splat_scatter");
+  splat_tiles<<<p.item_blocks, kThreads, 0, s>>>(p.st, p.splat, p.g);
+  FE_LAUNCH_CHECK("splat_tiles");
   return 0;
 }
 
@@ -389,6 +474,13 @@ static int launch_conv_any(TsnePlan& p, hipStream_t s) {
 // the repulsive force into diag[n, 2n) (MODE 1)
 static int launch_attract(TsnePlan& p, float exaggeration, hipStream_t s) {
   const float2* yin = p.y[p.cur];
+  if (p.flat_attr) {
+    attract_flat<<<p.attr_waves / (kThreads / 64), kThreads, 0, s>>>(
+        yin, p.csr.row_ptr, p.csr.col, p.csr.val, p.row_start, p.wave_row, p.st, exaggeration,
+        p.diag, p.attr_waves);
+    FE_LAUNCH_CHECK("attract_flat");
+    return 0;
+  }
 #define FE_ATTR(T)                                                                     \
   attract_kernel<T><<<p.upd_blocks, kThreads, 0, s>>>(yin, p.csr.row_ptr, p.csr.col,   \
                                                       p.csr.val, p.st, exaggeration,   \
@@ -416,8 +508,7 @@ static int launch_update(TsnePlan& p, const StepParams& sp, hipStream_t s) {
 
 int TsnePlan::enqueue_iteration(const StepParams& sp) {
   hipStream_t s = stream.get();
-  splat_kernel<<<splat_blocks, kThreads, 0, s>>>(y[cur], st, grids, n, g);
-  FE_LAUNCH_CHECK("splat_kernel");
+  if (launch_splat(*this, s)) return 1;
   if (launch_conv_any(*this, s)) return 1;
   if (launch_attract(*this, sp.exaggeration, s)) return 1;
   if (launch_update<0>(*this, sp, s)) return 1;
@@ -427,8 +518,7 @@ int TsnePlan::enqueue_iteration(const StepParams& sp) {
 
 int TsnePlan::enqueue_diag(const StepParams& sp) {
   hipStream_t s = stream.get();
-  splat_kernel<<<splat_blocks, kThreads, 0, s>>>(y[cur], st, grids, n, g);
-  FE_LAUNCH_CHECK("splat_kernel");
+  if (launch_splat(*this, s)) return 1;
   if (launch_conv_any(*this, s)) return 1;
   if (launch_attract(*this, sp.exaggeration, s)) return 1;
   return launch_update<1>(*this, sp, s);
@@ -498,13 +588,13 @@ int TsnePlan::profile(int iters, const StepParams& sp, double* ms) {
   for (int it = 0; it < iters; ++it) {
     hipEvent_t* e = &ev[(K + 1) * it];
     FE_CHECK(hipEventRecord(e[0], s));
-    splat_kernel<<<splat_blocks, kThreads, 0, s>>>(y[cur], st, grids, n, g);
+    if (launch_splat(*this, s)) return 1;
     FE_CHECK(hipEventRecord(e[1], s));
     int rc = 0;
 #define FE_PROF_CONV(FF)                                                                      \
   case FF: {                                                                                  \
     constexpr int G = FF / 2;                                                                 \
-    conv_rows_fwd<FF><<<G + G / 2, kThreads, 0, s>>>(grids, st, tw, z1, z2, kt);              \
+    conv_rows_fwd<FF><<<G + G / 2, kThreads, 0, s>>>(splat.sgrid, st, tw, z1, z2, kt);        \
     (void)hipEventRecord(e[2], s);                                                            \
     conv_cols<FF><<<conv_cols_blocks(G), kThreads, 5 * FF * sizeof(float2), s>>>(            \
         z1, z2, kt, tw, o1, o2, zpart, st);                                                   \
@@ -612,8 +702,9 @@ static int tsne_one_shot(const int* indices, const DistT* distances, bool device
 // entries) and the transient peak while its P is built.
 static long long job_resident_bytes(int n, int k, int g) {
   const long long f = 2ll * g, gg = 1ll * g * g, nk = 1ll * n * k;
-  return 8ll * 2 * nk + 4ll * (n + 1) + 8ll * 6 * n + 8ll * 3 * gg + 8ll * 4 * g * f +
-         8ll * (g + 1) * g + 16ll * gg + 8ll * f + 32ll * 2048 + 8ll * (g + 1) + 4096;
+  return 8ll * 2 * nk + 4ll * (n + 1) + 8ll * 7 * n + 16ll * (splat_bins(g) + 1) + 40ll * gg +
+         8ll * 4 * g * f + 8ll * (g + 1) * g + 16ll * gg + 8ll * f + 32ll * 2048 + 8ll * (g + 1) +
+         4096;
 }
 
 static long long job_setup_bytes(int n, int k) {
@@ -635,7 +726,8 @@ class TsneBatch {
   int tab_cap = 0;
   bool dirty = true;
   struct Launch {
-    int kind;   // 0 splat, 1 rows_fwd, 2 cols, 3 rows_inv, 4 attract, 5 update
+    int kind;   // 0 splat_count, 6 splat_scatter, 7 splat_tiles, 1 rows_fwd, 2 cols,
+                // 3 rows_inv, 4 attract, 5 update
     int param;  // F or TPR
     int off;    // offset of slot[] in d_tab; first[] follows it
     int m;      // entries
@@ -651,7 +743,7 @@ class TsneBatch {
     slot_bytes.assign(max_jobs, 0);
     desc.resize(max_jobs);
     std::memset(desc.data(), 0, desc.size() * sizeof(JobDesc));
-    tab_cap = 2 * (5 * max_jobs + 16) + 64;
+    tab_cap = 2 * (8 * max_jobs + 40) + 64;  // every job is in 8 launches
     ArenaPlan p;
     p.add<JobDesc>(max_jobs);
     p.add<int>(tab_cap);
@@ -699,7 +791,9 @@ class TsneBatch {
     d.row_ptr = p.csr.row_ptr;
     d.col = p.csr.col;
     d.val = p.csr.val;
-    d.grids = p.grids;
+    d.row_start = p.row_start;
+    d.wave_row = p.wave_row;
+    d.splat = p.splat;
     d.z1 = p.z1;
     d.z2 = p.z2;
     d.kt = p.kt;
@@ -768,10 +862,15 @@ class TsneBatch {
       L.blocks = acc;
       launches.push_back(L);
     };
-    std::vector<std::pair<int, int>> splat;
+    std::vector<std::pair<int, int>> splat, tiles;
     for (int s = 0; s < max_jobs; ++s)
-      if (slots[s]) splat.push_back({s, slots[s]->splat_blocks});
+      if (slots[s]) {
+        splat.push_back({s, slots[s]->splat_blocks});
+        tiles.push_back({s, slots[s]->item_blocks});
+      }
     add_launch(0, 0, splat);
+    add_launch(6, 0, splat);
+    add_launch(7, 0, tiles);
     for (int kind = 1; kind <= 3; ++kind) {
       for (int F = 32; F <= 2048; F *= 2) {
         std::vector<std::pair<int, int>> ent;
@@ -783,10 +882,18 @@ class TsneBatch {
         add_launch(kind, F, ent);
       }
     }
+    {
+      std::vector<std::pair<int, int>> ent;
+      for (int s = 0; s < max_jobs; ++s)
+        if (slots[s] && slots[s]->flat_attr)
+          ent.push_back({s, slots[s]->attr_waves / (kThreads / 64)});
+      add_launch(4, 0, ent);
+    }
     for (int tpr = 4; tpr <= 64; tpr *= 2) {
       std::vector<std::pair<int, int>> ent;
       for (int s = 0; s < max_jobs; ++s)
-        if (slots[s] && slots[s]->tpr == tpr) ent.push_back({s, slots[s]->upd_blocks});
+        if (slots[s] && !slots[s]->flat_attr && slots[s]->tpr == tpr)
+          ent.push_back({s, slots[s]->upd_blocks});
       add_launch(4, tpr, ent);
     }
     std::vector<std::pair<int, int>> upd;
@@ -807,7 +914,13 @@ class TsneBatch {
       const int* fi = sl + L.m;
       switch (L.kind) {
         case 0:
-          batch_splat<<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
+          batch_splat_count<<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
+          break;
+        case 6:
+          batch_splat_scatter<<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
+          break;
+        case 7:
+          batch_splat_tiles<<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
           break;
 #define FE_B_CONV(FF)                                                                          \
   case FF:                                                                                     \
@@ -836,6 +949,7 @@ class TsneBatch {
 #undef FE_B_CONV
         case 4:
           switch (L.param) {
+            case 0: batch_attract_flat<<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
             case 4: batch_attract<4><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
             case 8: batch_attract<8><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
             case 16: batch_attract<16><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
@@ -843,8 +957,11 @@ class TsneBatch {
             default: batch_attract<64><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
           }
           break;
-        default:
+        case 5:
           batch_update<<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
+          break;
+        default:
+          return fail("unknown batch stage");
       }
       FE_LAUNCH_CHECK("batch kernel");
     }
@@ -1046,7 +1163,7 @@ int fastembedr_hip_tsne_plan_info(fastembedr_hip_tsne_plan* plan, long long* inf
   info[1] = p.k;
   info[2] = p.g;
   info[3] = p.csr.nnz;
-  info[4] = p.tpr;
+  info[4] = p.flat_attr ? 0 : p.tpr;
   info[5] = static_cast<long long>(p.state.capacity() + p.csr_store.capacity());
   info[6] = p.iters_run;
   info[7] = static_cast<long long>(p.setup_us);

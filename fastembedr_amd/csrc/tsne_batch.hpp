@@ -5,8 +5,8 @@
 // would use), finds its descriptor by a binary search of blockIdx.x over the
 // prefix table, and runs the single-job stage body with its job-local block
 // index.  Jobs therefore give bit-identical results alone and in a batch,
-// and a batch of S jobs costs 1 + 3 * (#grid sizes) + (#lane widths) launches
-// per iteration instead of 5 * S.  The descriptor's iteration counter, the
+// and a batch of S jobs costs 4 + 3 * (#grid sizes) + (#lane widths)
+// launches per iteration instead of 8 * S.  The descriptor's iteration counter, the
 // phase constants and the layout-buffer parity live on the device: the block
 // that finalises a job's layout statistics advances them, and a job whose
 // schedule is complete goes inactive, so captured sequences of iterations
@@ -25,7 +25,9 @@ struct JobDesc {
   const int* row_ptr;
   const int* col;
   const float* val;
-  unsigned long long* grids;
+  const unsigned long long* row_start;  // entry-balanced attraction tables
+  const int* wave_row;
+  SplatBuffers splat;
   float2* z1;
   float2* z2;
   float2* kt;
@@ -54,13 +56,34 @@ __device__ __forceinline__ int find_entry(const int* __restrict__ first, int m) 
   return lo;
 }
 
-__global__ __launch_bounds__(kThreads) void batch_splat(JobDesc* jobs, const int* __restrict__ slot,
-                                                        const int* __restrict__ first, int m) {
+__global__ __launch_bounds__(kThreads) void batch_splat_count(JobDesc* jobs,
+                                                              const int* __restrict__ slot,
+                                                              const int* __restrict__ first, int m) {
   const int e = find_entry(first, m);
   JobDesc* J = jobs + slot[e];
   if (!J->active) return;
-  splat_body(J->y[J->cur], J->st, J->grids, J->n, J->g, blockIdx.x - first[e],
-             first[e + 1] - first[e]);
+  splat_count_body(J->y[J->cur], J->st, J->splat, J->n, J->g, blockIdx.x - first[e],
+                   first[e + 1] - first[e]);
+}
+
+__global__ __launch_bounds__(kThreads) void batch_splat_scatter(JobDesc* jobs,
+                                                                const int* __restrict__ slot,
+                                                                const int* __restrict__ first,
+                                                                int m) {
+  const int e = find_entry(first, m);
+  JobDesc* J = jobs + slot[e];
+  if (!J->active) return;
+  splat_scatter_body(J->y[J->cur], J->st, J->splat, J->n, J->g, blockIdx.x - first[e],
+                     first[e + 1] - first[e]);
+}
+
+__global__ __launch_bounds__(kThreads) void batch_splat_tiles(JobDesc* jobs,
+                                                              const int* __restrict__ slot,
+                                                              const int* __restrict__ first, int m) {
+  const int e = find_entry(first, m);
+  JobDesc* J = jobs + slot[e];
+  if (!J->active) return;
+  splat_tiles_body(J->st, J->splat, J->g, blockIdx.x - first[e]);
 }
 
 template <int F>
@@ -69,7 +92,7 @@ __global__ __launch_bounds__(kThreads) void batch_rows_fwd(JobDesc* jobs, const 
   const int e = find_entry(first, m);
   JobDesc* J = jobs + slot[e];
   if (!J->active) return;
-  conv_rows_fwd_body<F>(J->grids, J->st, J->tw, J->z1, J->z2, J->kt, blockIdx.x - first[e]);
+  conv_rows_fwd_body<F>(J->splat.sgrid, J->st, J->tw, J->z1, J->z2, J->kt, blockIdx.x - first[e]);
 }
 
 // blocks of a job's range follow the XCD-aware column order of conv_cols;
@@ -95,6 +118,20 @@ __global__ __launch_bounds__(kThreads) void batch_rows_inv(JobDesc* jobs, const 
   JobDesc* J = jobs + slot[e];
   if (!J->active) return;
   conv_rows_inv_body<F>(J->o1, J->o2, J->tw, J->st, J->field, blockIdx.x - first[e]);
+}
+
+__global__ __launch_bounds__(kThreads) void batch_attract_flat(JobDesc* jobs,
+                                                               const int* __restrict__ slot,
+                                                               const int* __restrict__ first,
+                                                               int m) {
+  const int e = find_entry(first, m);
+  JobDesc* J = jobs + slot[e];
+  if (!J->active) return;
+  const float exag = J->iter < J->early_iter ? J->early_exag : J->exag;
+  const int wave = (static_cast<int>(blockIdx.x) - first[e]) * (kThreads / 64) +
+                   static_cast<int>(threadIdx.x >> 6);
+  attract_flat_body(J->y[J->cur], J->row_ptr, J->col, J->val, J->row_start, J->wave_row, J->st,
+                    exag, J->attr, wave);
 }
 
 template <int TPR>

@@ -1,10 +1,12 @@
 // t-SNE device kernels for gfx950.
 //
-// Per-iteration pipeline (six launches, all state resident in HBM):
+// Per-iteration pipeline (eight launches, all state resident in HBM):
 //   1. splat            bilinear spread of every point onto the G x G grid
-//                       (mass, mass*x, mass*y) with 64-bit fixed-point atomics
-//                       (LDS-aggregated per chunk when the chunk is compact),
-//                       so the grid sums do not depend on arrival order;
+//                       (mass, mass*x, mass*y): points are binned by grid
+//                       tile (count, scatter), then one block per tile sums
+//                       its points in 64-bit fixed point in LDS and writes its
+//                       nodes once, so the grid does not depend on arrival
+//                       order and no global atomics touch it;
 //   2. conv_rows_fwd    row FFTs of z1 = mass + i*mass_x/x_scale and
 //                       z2 = mass_y (zero padded to F = 2G), and of the G
 //                       kernel rows Kq + i*Kq^2 (both real and even, so the
@@ -51,6 +53,7 @@ struct TsneStats {
   double x_scale;
   unsigned int done_cols;  // last-block counters (reset by the last block)
   unsigned int done_stats;
+  unsigned int done_bins;
 };
 
 struct LayoutPartial {
@@ -75,6 +78,16 @@ __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
   return v;
+}
+
+// Workgroups are dealt round-robin to the eight XCDs, so block b runs on XCD
+// b % 8.  xcd_remap gives the blocks of one XCD one contiguous range of
+// logical block ids [x*per + min(x, rem), ...), so work on neighbouring
+// points or rows (which share neighbours, grid cells and cache lines) stays
+// in one XCD's L2.  It is a permutation of [0, nblk).
+__host__ __device__ __forceinline__ int xcd_remap(int b, int nblk) {
+  const int per = nblk / 8, rem = nblk % 8, x = b % 8;
+  return x * per + (x < rem ? x : rem) + b / 8;
 }
 
 // Bilinear cell of a centred point: tsne_neighbors.cpp:1190-1206 (the CUDA
@@ -178,19 +191,40 @@ __device__ inline bool reduce_layout_stats(LayoutPartial mine, LayoutPartial* pa
   __syncthreads();
   if (!last) return false;
   __threadfence();
-  if (threadIdx.x == 0) {
+  // the whole last block folds the partials: thread t takes entries t,
+  // t + kThreads, ... in order, then the fixed wave/LDS tree above
+  {
     const volatile LayoutPartial* vp = partials;
-    double sx = 0.0, sy = 0.0;
-    float mnx = FLT_MAX, mxx = -FLT_MAX, mny = FLT_MAX, mxy = -FLT_MAX;
-    for (int b = 0; b < nblk; ++b) {
-      sx += vp[b].sx;
-      sy += vp[b].sy;
-      mnx = fminf(mnx, vp[b].mnx);
-      mxx = fmaxf(mxx, vp[b].mxx);
-      mny = fminf(mny, vp[b].mny);
-      mxy = fmaxf(mxy, vp[b].mxy);
+    LayoutPartial a{0.0, 0.0, FLT_MAX, -FLT_MAX, FLT_MAX, -FLT_MAX};
+    for (int b = threadIdx.x; b < nblk; b += kThreads) {
+      a.sx += vp[b].sx;
+      a.sy += vp[b].sy;
+      a.mnx = fminf(a.mnx, vp[b].mnx);
+      a.mxx = fmaxf(a.mxx, vp[b].mxx);
+      a.mny = fminf(a.mny, vp[b].mny);
+      a.mxy = fmaxf(a.mxy, vp[b].mxy);
     }
-    finalize_geometry(st, sx, sy, mnx, mxx, mny, mxy, n, g, centre);
+    a.sx = wave_sum(a.sx);
+    a.sy = wave_sum(a.sy);
+    a.mnx = static_cast<float>(wave_min(a.mnx));
+    a.mxx = static_cast<float>(wave_max(a.mxx));
+    a.mny = static_cast<float>(wave_min(a.mny));
+    a.mxy = static_cast<float>(wave_max(a.mxy));
+    __syncthreads();  // red[] of the block partial above is consumed
+    if (lane == 0) red[wid] = a;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    LayoutPartial b = red[0];
+    for (int w = 1; w < kThreads / 64; ++w) {
+      b.sx += red[w].sx;
+      b.sy += red[w].sy;
+      b.mnx = fminf(b.mnx, red[w].mnx);
+      b.mxx = fmaxf(b.mxx, red[w].mxx);
+      b.mny = fminf(b.mny, red[w].mny);
+      b.mxy = fmaxf(b.mxy, red[w].mxy);
+    }
+    finalize_geometry(st, b.sx, b.sy, b.mnx, b.mxx, b.mny, b.mxy, n, g, centre);
     st->done_stats = 0u;
     return true;
   }
@@ -226,132 +260,385 @@ __global__ __launch_bounds__(kThreads) void layout_stats_kernel(const float2* __
 //
 // 1. splat: embedding_cuda_kernels.cpp:2622-2667 / tsne_neighbors.cpp:
 //    1186-1226, accumulated in 64-bit fixed point (deterministic).
-// grids: 3 channels of g*g int64, channel-major, row-major nodes.
+//
+// The node grid is cut into kTile x kTile tiles; tile (tx, ty) owns nodes
+// [kTile*tx, kTile*tx + kTile) x [kTile*ty, kTile*ty + kTile).  A point in
+// bilinear cell (x0, y0) touches nodes x0..x0+1 x y0..y0+1: its cell's tile,
+// and also the tile to the right when x0 is that tile's last cell column, the
+// tile below when y0 is its last cell row, or the diagonal one when both.
+// Points are binned by (cell tile, edge class) with the classes ordered
+//     0 interior, 1 right edge, 2 right and bottom edge, 3 bottom edge,
+// so that what a tile owner needs from each neighbour is one contiguous range
+// of bins: own tile [0, 4), left neighbour [1, 3), upper neighbour [2, 4),
+// upper-left neighbour [2, 3).
+//   a. splat_count    per-block LDS histogram of the bins, added to the
+//                     global counts; the last block turns the counts into bin
+//                     starts and cursors (clearing the counts) and cuts every
+//                     tile's points into work items of at most kSplatItem;
+//   b. splat_scatter  each point takes a slot in its bin (one global atomic
+//                     per block and non-empty bin) and stores its centred
+//                     coordinates there;
+//   c. splat_tiles    one block per work item accumulates its points in an
+//                     LDS window of the tile (a lane sums consecutive points of
+//                     one cell in registers before touching LDS, which keeps
+//                     collapsed clusters cheap).  A tile of one item writes its
+//                     nodes once with plain stores; the items of a larger tile
+//                     add their windows to the 64-bit `wide` grid and the last
+//                     of them converts and clears it.  The output is the input
+//                     of the row FFTs:
+//                       sgrid(node) = (mass, mass*x / x_scale, mass*y, 0).
+// Slots inside a bin are taken in atomic order, but all sums are integer
+// sums, so the grid is the same on every run and matches a direct splat.
 // ----------------------------------------------------------------------------
-// A block takes chunks of kSplatChunk consecutive points.  When the chunk's
-// cells fit a kTileSide x kTileSide node window (neighbouring point ids are
-// usually close in the layout, and early-exaggeration clusters are compact),
-// the chunk accumulates in LDS and flushes each touched node once with global
-// atomics; otherwise its points go straight to global atomics.  The sums are
-// integers, so both routes give identical grids.
-constexpr int kSplatPerThread = 4;
-constexpr int kSplatChunk = kThreads * kSplatPerThread;
-constexpr int kTileSide = 32;
-constexpr int kTileNodes = kTileSide * kTileSide;
+constexpr int kTile = 32;
+constexpr int kTileNodes = kTile * kTile;
+constexpr int kMaxTilesSide = 32;  // grids up to 1024 x 1024 nodes
+constexpr int kMaxTiles = kMaxTilesSide * kMaxTilesSide;
+constexpr int kMaxBins = 4 * kMaxTiles;
+constexpr int kSplatItem = 4096;  // points per tile work item
 
-__device__ __forceinline__ int wave_min_i(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-  return v;
-}
-__device__ __forceinline__ int wave_max_i(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-  return v;
+struct SplatBuffers {
+  int* bin_count;            // bins; zero between passes
+  int* bin_start;            // bins + 1
+  int* bin_cursor;           // bins
+  int* tile_items;           // tiles + 1: first work item of every tile
+  unsigned int* tile_done;   // tiles; zero between passes
+  float2* binned;            // n centred points in bin order
+  unsigned long long* wide;  // 3 x g*g sums of multi-item tiles; zero between passes
+  float4* sgrid;             // g*g input of the row FFTs
+};
+
+__host__ __device__ __forceinline__ int tiles_side(int g) { return (g + kTile - 1) / kTile; }
+__host__ __device__ __forceinline__ int splat_bins(int g) { return 4 * tiles_side(g) * tiles_side(g); }
+// launch size of splat_tiles: every tile has at most 1 + points/kSplatItem
+// items and a point is counted by at most four tiles
+__host__ __device__ __forceinline__ int splat_item_blocks(int n, int g) {
+  return tiles_side(g) * tiles_side(g) +
+         static_cast<int>((4ll * n + kSplatItem - 1) / kSplatItem);
 }
 
-__device__ __forceinline__ void splat_body(const float2* __restrict__ y,
-                                           const TsneStats* __restrict__ st,
-                                           unsigned long long* grids, int n, int g, int blk,
-                                           int nblk) {
-  __shared__ unsigned long long tile[3 * kTileNodes];
-  __shared__ int box[4];
-  const float mx = st->mean_xf, my = st->mean_yf;
-  const float lx = st->lower_x, ly = st->lower_y, inv = st->inv_spacing;
-  const double sm = st->fx_mass, sxy = st->fx_xy;
-  const size_t gg = static_cast<size_t>(g) * g;
-  const int n_chunks = (n + kSplatChunk - 1) / kSplatChunk;
-  for (int ch = blk; ch < n_chunks; ch += nblk) {
-    Cell cl[kSplatPerThread];
-    float xc[kSplatPerThread], yc[kSplatPerThread];
-    bool ok[kSplatPerThread];
-    int bx0 = 1 << 30, by0 = 1 << 30, bx1 = -1, by1 = -1;
+__device__ __forceinline__ int point_bin(const Cell& c, int ts) {
+  const int tx = c.x0 / kTile, ty = c.y0 / kTile;
+  const bool xe = (c.x0 % kTile) == kTile - 1, ye = (c.y0 % kTile) == kTile - 1;
+  const int cls = xe ? (ye ? 2 : 1) : (ye ? 3 : 0);
+  return 4 * (ty * ts + tx) + cls;
+}
+
+// The four bin ranges that feed tile t (own, left, upper, upper-left);
+// missing neighbours give empty ranges.
+__device__ __forceinline__ void tile_ranges(const int* bs, int t, int ts, int* b0, int* b1) {
+  const int tx = t % ts, ty = t / ts;
 #pragma unroll
-    for (int u = 0; u < kSplatPerThread; ++u) {
-      const int i = ch * kSplatChunk + u * kThreads + threadIdx.x;
-      ok[u] = i < n;
-      if (ok[u]) {
-        const float2 p = y[i];
-        xc[u] = p.x - mx;
-        yc[u] = p.y - my;
-        cl[u] = grid_cell(xc[u], yc[u], lx, ly, inv, g);
-        bx0 = min(bx0, cl[u].x0);
-        by0 = min(by0, cl[u].y0);
-        bx1 = max(bx1, cl[u].x0);
-        by1 = max(by1, cl[u].y0);
-      }
+  for (int r = 0; r < 4; ++r) {
+    const int dx = (r & 1) ? 1 : 0, dy = (r & 2) ? 1 : 0;
+    const int lo = r == 0 ? 0 : (r == 1 ? 1 : 2), hi = r == 0 ? 4 : (r == 2 ? 4 : 3);
+    if (tx < dx || ty < dy) {
+      b0[r] = b1[r] = 0;
+    } else {
+      const int sb = 4 * ((ty - dy) * ts + (tx - dx));
+      b0[r] = bs[sb + lo];
+      b1[r] = bs[sb + hi];
     }
-    if (threadIdx.x == 0) {
-      box[0] = 1 << 30;
-      box[1] = 1 << 30;
-      box[2] = -1;
-      box[3] = -1;
-    }
-    __syncthreads();
-    bx0 = wave_min_i(bx0);
-    by0 = wave_min_i(by0);
-    bx1 = wave_max_i(bx1);
-    by1 = wave_max_i(by1);
-    if ((threadIdx.x & 63) == 0) {
-      atomicMin(&box[0], bx0);
-      atomicMin(&box[1], by0);
-      atomicMax(&box[2], bx1);
-      atomicMax(&box[3], by1);
-    }
-    __syncthreads();
-    const int ox = box[0], oy = box[1];
-    const int bw = box[2] - ox + 2, bh = box[3] - oy + 2;  // node window
-    const bool tiled = bw <= kTileSide && bh <= kTileSide;
-    if (tiled) {
-      for (int t = threadIdx.x; t < 3 * kTileNodes; t += kThreads) tile[t] = 0ull;
-      __syncthreads();
-    }
-#pragma unroll
-    for (int u = 0; u < kSplatPerThread; ++u) {
-      if (!ok[u]) continue;
-      const Cell c = cl[u];
-      const float w[4] = {(1.0f - c.tx) * (1.0f - c.ty), c.tx * (1.0f - c.ty),
-                          (1.0f - c.tx) * c.ty, c.tx * c.ty};
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int nx = c.x0 + (q & 1), ny = c.y0 + (q >> 1);
-        const unsigned long long fm = to_fixed(w[q], sm);
-        const unsigned long long fx = to_fixed(w[q] * xc[u], sxy);
-        const unsigned long long fy = to_fixed(w[q] * yc[u], sxy);
-        if (tiled) {
-          const int t = (ny - oy) * kTileSide + (nx - ox);
-          atomicAdd(tile + t, fm);
-          atomicAdd(tile + kTileNodes + t, fx);
-          atomicAdd(tile + 2 * kTileNodes + t, fy);
-        } else {
-          const size_t node = static_cast<size_t>(ny) * g + nx;
-          atomicAdd(grids + node, fm);
-          atomicAdd(grids + gg + node, fx);
-          atomicAdd(grids + 2 * gg + node, fy);
-        }
-      }
-    }
-    if (tiled) {
-      __syncthreads();
-      for (int t = threadIdx.x; t < bw * bh; t += kThreads) {
-        const int ty = t / bw, tx = t - ty * bw;
-        const int lt = ty * kTileSide + tx;
-        const size_t node = static_cast<size_t>(oy + ty) * g + (ox + tx);
-#pragma unroll
-        for (int c3 = 0; c3 < 3; ++c3) {
-          const unsigned long long v = tile[c3 * kTileNodes + lt];
-          if (v != 0ull) atomicAdd(grids + c3 * gg + node, v);
-        }
-      }
-    }
-    __syncthreads();
   }
 }
 
-__global__ __launch_bounds__(kThreads) void splat_kernel(const float2* __restrict__ y,
-                                                         const TsneStats* __restrict__ st,
-                                                         unsigned long long* grids, int n, int g) {
-  splat_body(y, st, grids, n, g, blockIdx.x, gridDim.x);
+// points [beg, end) of block blk of nblk
+__device__ __forceinline__ void point_range(int n, int blk, int nblk, int* beg, int* end) {
+  const int chunk = (n + nblk - 1) / nblk;
+  *beg = min(n, blk * chunk);
+  *end = min(n, *beg + chunk);
+}
+
+// exclusive prefix sum of v over the block in thread order; *total = block sum
+__device__ __forceinline__ int block_exclusive_scan(int v, int* total) {
+  __shared__ int wsum[kThreads / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int incl = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += u;
+  }
+  if (lane == 63) wsum[wid] = incl;
+  __syncthreads();
+  int off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kThreads / 64; ++w) {
+    const int s = wsum[w];
+    off += w < wid ? s : 0;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + incl - v;
+}
+
+__device__ __forceinline__ void splat_count_body(const float2* __restrict__ y, TsneStats* st,
+                                                 SplatBuffers sb, int n, int g, int blk,
+                                                 int nblk) {
+  __shared__ int hist[kMaxBins + 1];
+  __shared__ bool last;
+  const int ts = tiles_side(g), tiles = ts * ts, nbins = 4 * tiles;
+  const float mx = st->mean_xf, my = st->mean_yf;
+  const float lx = st->lower_x, ly = st->lower_y, inv = st->inv_spacing;
+  for (int t = threadIdx.x; t < nbins; t += kThreads) hist[t] = 0;
+  __syncthreads();
+  int beg, end;
+  point_range(n, xcd_remap(blk, nblk), nblk, &beg, &end);
+  for (int i = beg + threadIdx.x; i < end; i += kThreads) {
+    const float2 p = y[i];
+    const Cell c = grid_cell(p.x - mx, p.y - my, lx, ly, inv, g);
+    atomicAdd(&hist[point_bin(c, ts)], 1);
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < nbins; t += kThreads)
+    if (hist[t]) atomicAdd(sb.bin_count + t, hist[t]);
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(&st->done_bins, 1u) == static_cast<unsigned>(nblk - 1);
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  // bin starts: thread t scans bins [t*per, t*per + per) after the block scan
+  // of the per-thread sums; the starts are also kept in hist (LDS)
+  {
+    const volatile int* vc = sb.bin_count;
+    const int per = (nbins + kThreads - 1) / kThreads;
+    const int b0 = threadIdx.x * per;
+    int mine = 0;
+    for (int q = 0; q < per; ++q)
+      if (b0 + q < nbins) mine += vc[b0 + q];
+    int total;
+    int run = block_exclusive_scan(mine, &total);
+    for (int q = 0; q < per; ++q) {
+      const int b = b0 + q;
+      if (b < nbins) {
+        const int cnt = vc[b];
+        sb.bin_start[b] = run;
+        sb.bin_cursor[b] = run;
+        sb.bin_count[b] = 0;
+        hist[b] = run;
+        run += cnt;
+      }
+    }
+    if (threadIdx.x == 0) {
+      sb.bin_start[nbins] = total;
+      hist[nbins] = total;
+      st->done_bins = 0u;
+    }
+    __syncthreads();
+  }
+  // work items per tile
+  {
+    const int per = (tiles + kThreads - 1) / kThreads;
+    const int t0 = threadIdx.x * per;
+    int items[(kMaxTiles + kThreads - 1) / kThreads];
+    int mine = 0;
+#pragma unroll
+    for (int q = 0; q < (kMaxTiles + kThreads - 1) / kThreads; ++q) {
+      items[q] = 0;
+      const int t = t0 + q;
+      if (q < per && t < tiles) {
+        int b0[4], b1[4];
+        tile_ranges(hist, t, ts, b0, b1);
+        const int pts = (b1[0] - b0[0]) + (b1[1] - b0[1]) + (b1[2] - b0[2]) + (b1[3] - b0[3]);
+        items[q] = max(1, (pts + kSplatItem - 1) / kSplatItem);
+        mine += items[q];
+      }
+    }
+    int total;
+    int run = block_exclusive_scan(mine, &total);
+#pragma unroll
+    for (int q = 0; q < (kMaxTiles + kThreads - 1) / kThreads; ++q) {
+      const int t = t0 + q;
+      if (q < per && t < tiles) {
+        sb.tile_items[t] = run;
+        run += items[q];
+      }
+    }
+    if (threadIdx.x == 0) sb.tile_items[tiles] = total;
+  }
+}
+
+__device__ __forceinline__ void splat_scatter_body(const float2* __restrict__ y,
+                                                   const TsneStats* __restrict__ st,
+                                                   SplatBuffers sb, int n, int g, int blk,
+                                                   int nblk) {
+  __shared__ int hist[kMaxBins];
+  __shared__ int base[kMaxBins];
+  const int ts = tiles_side(g), nbins = 4 * ts * ts;
+  const float mx = st->mean_xf, my = st->mean_yf;
+  const float lx = st->lower_x, ly = st->lower_y, inv = st->inv_spacing;
+  for (int t = threadIdx.x; t < nbins; t += kThreads) hist[t] = 0;
+  __syncthreads();
+  int beg, end;
+  point_range(n, xcd_remap(blk, nblk), nblk, &beg, &end);
+  for (int i = beg + threadIdx.x; i < end; i += kThreads) {
+    const float2 p = y[i];
+    const Cell c = grid_cell(p.x - mx, p.y - my, lx, ly, inv, g);
+    atomicAdd(&hist[point_bin(c, ts)], 1);
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < nbins; t += kThreads) {
+    const int c = hist[t];
+    if (c) {
+      base[t] = atomicAdd(sb.bin_cursor + t, c);
+      hist[t] = 0;
+    }
+  }
+  __syncthreads();
+  for (int i = beg + threadIdx.x; i < end; i += kThreads) {
+    const float2 p = y[i];
+    const float xc = p.x - mx, yc = p.y - my;
+    const Cell c = grid_cell(xc, yc, lx, ly, inv, g);
+    const int b = point_bin(c, ts);
+    const int r = atomicAdd(&hist[b], 1);
+    sb.binned[base[b] + r] = make_float2(xc, yc);
+  }
+}
+
+// adds a lane's pending cell sums (corner q: acc[3q .. 3q+2]) to the window
+__device__ __forceinline__ void flush_cell(unsigned long long* win, int key, int g, int ox, int oy,
+                                           const unsigned long long* acc) {
+  const int x0 = key % g, y0 = key / g;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int nx = x0 + (q & 1) - ox, ny = y0 + (q >> 1) - oy;
+    if (nx >= 0 && nx < kTile && ny >= 0 && ny < kTile) {
+      const int t = ny * kTile + nx;
+      atomicAdd(win + t, acc[3 * q]);
+      atomicAdd(win + kTileNodes + t, acc[3 * q + 1]);
+      atomicAdd(win + 2 * kTileNodes + t, acc[3 * q + 2]);
+    }
+  }
+}
+
+__device__ __forceinline__ float4 node_value(long long m, long long mxv, long long myv,
+                                             double inv_m, double inv_xs, double inv_xy) {
+  return make_float4(static_cast<float>(static_cast<double>(m) * inv_m),
+                     static_cast<float>(static_cast<double>(mxv) * inv_xs),
+                     static_cast<float>(static_cast<double>(myv) * inv_xy), 0.f);
+}
+
+__device__ __forceinline__ void splat_tiles_body(const TsneStats* __restrict__ st, SplatBuffers sb,
+                                                 int g, int item) {
+  __shared__ unsigned long long win[3 * kTileNodes];
+  __shared__ bool last;
+  const int ts = tiles_side(g), tiles = ts * ts;
+  if (item >= sb.tile_items[tiles]) return;
+  int lo = 0, hi = tiles - 1;  // tile: last t with tile_items[t] <= item
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (sb.tile_items[mid] <= item) lo = mid;
+    else hi = mid - 1;
+  }
+  const int tile = lo;
+  const int first = sb.tile_items[tile], nitems = sb.tile_items[tile + 1] - first;
+  int b0[4], b1[4];
+  tile_ranges(sb.bin_start, tile, ts, b0, b1);
+  const int len0 = b1[0] - b0[0], len1 = b1[1] - b0[1], len2 = b1[2] - b0[2];
+  const int pts = len0 + len1 + len2 + (b1[3] - b0[3]);
+  const int per = (pts + nitems - 1) / nitems;
+  const int s0 = min(pts, (item - first) * per), s1 = min(pts, s0 + per);
+  const int ox = (tile % ts) * kTile, oy = (tile / ts) * kTile;
+  const float lx = st->lower_x, ly = st->lower_y, inv = st->inv_spacing;
+  const double sm = st->fx_mass, sxy = st->fx_xy;
+  for (int t = threadIdx.x; t < 3 * kTileNodes; t += kThreads) win[t] = 0ull;
+  __syncthreads();
+  int pend = -1;
+  unsigned long long acc[12];
+#pragma unroll
+  for (int q = 0; q < 12; ++q) acc[q] = 0ull;
+  for (int v = s0 + threadIdx.x; v < s1; v += kThreads) {
+    // position v of the concatenated ranges own ++ left ++ upper ++ upper-left
+    int idx;
+    if (v < len0) idx = b0[0] + v;
+    else if (v < len0 + len1) idx = b0[1] + (v - len0);
+    else if (v < len0 + len1 + len2) idx = b0[2] + (v - len0 - len1);
+    else idx = b0[3] + (v - len0 - len1 - len2);
+    const float2 p = sb.binned[idx];
+    const Cell c = grid_cell(p.x, p.y, lx, ly, inv, g);
+    const int key = c.y0 * g + c.x0;
+    if (key != pend) {
+      if (pend >= 0) flush_cell(win, pend, g, ox, oy, acc);
+      pend = key;
+#pragma unroll
+      for (int q = 0; q < 12; ++q) acc[q] = 0ull;
+    }
+    const float w[4] = {(1.0f - c.tx) * (1.0f - c.ty), c.tx * (1.0f - c.ty),
+                        (1.0f - c.tx) * c.ty, c.tx * c.ty};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      acc[3 * q] += to_fixed(w[q], sm);
+      acc[3 * q + 1] += to_fixed(w[q] * p.x, sxy);
+      acc[3 * q + 2] += to_fixed(w[q] * p.y, sxy);
+    }
+  }
+  if (pend >= 0) flush_cell(win, pend, g, ox, oy, acc);
+  __syncthreads();
+  const double inv_m = 1.0 / st->fx_mass, inv_xy = 1.0 / st->fx_xy;
+  const double inv_xs = inv_xy / st->x_scale;
+  const size_t gg = static_cast<size_t>(g) * g;
+  if (nitems == 1) {
+    for (int t = threadIdx.x; t < kTileNodes; t += kThreads) {
+      const int nx = ox + (t % kTile), ny = oy + t / kTile;
+      if (nx < g && ny < g)
+        sb.sgrid[static_cast<size_t>(ny) * g + nx] =
+            node_value(static_cast<long long>(win[t]), static_cast<long long>(win[kTileNodes + t]),
+                       static_cast<long long>(win[2 * kTileNodes + t]), inv_m, inv_xs, inv_xy);
+    }
+    return;
+  }
+  for (int t = threadIdx.x; t < kTileNodes; t += kThreads) {
+    const int nx = ox + (t % kTile), ny = oy + t / kTile;
+    if (nx < g && ny < g) {
+      const size_t node = static_cast<size_t>(ny) * g + nx;
+#pragma unroll
+      for (int c3 = 0; c3 < 3; ++c3) {
+        const unsigned long long v = win[c3 * kTileNodes + t];
+        if (v) atomicAdd(sb.wide + c3 * gg + node, v);
+      }
+    }
+  }
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0)
+    last = atomicAdd(sb.tile_done + tile, 1u) == static_cast<unsigned>(nitems - 1);
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  volatile unsigned long long* vw = sb.wide;
+  for (int t = threadIdx.x; t < kTileNodes; t += kThreads) {
+    const int nx = ox + (t % kTile), ny = oy + t / kTile;
+    if (nx < g && ny < g) {
+      const size_t node = static_cast<size_t>(ny) * g + nx;
+      const long long m = static_cast<long long>(vw[node]);
+      const long long mxv = static_cast<long long>(vw[gg + node]);
+      const long long myv = static_cast<long long>(vw[2 * gg + node]);
+      vw[node] = 0ull;
+      vw[gg + node] = 0ull;
+      vw[2 * gg + node] = 0ull;
+      sb.sgrid[node] = node_value(m, mxv, myv, inv_m, inv_xs, inv_xy);
+    }
+  }
+  if (threadIdx.x == 0) sb.tile_done[tile] = 0u;
+}
+
+__global__ __launch_bounds__(kThreads) void splat_count(const float2* __restrict__ y,
+                                                        TsneStats* st, SplatBuffers sb, int n,
+                                                        int g) {
+  splat_count_body(y, st, sb, n, g, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(kThreads) void splat_scatter(const float2* __restrict__ y,
+                                                          const TsneStats* __restrict__ st,
+                                                          SplatBuffers sb, int n, int g) {
+  splat_scatter_body(y, st, sb, n, g, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(kThreads) void splat_tiles(const TsneStats* __restrict__ st,
+                                                        SplatBuffers sb, int g) {
+  splat_tiles_body(st, sb, g, blockIdx.x);
 }
 
 // ----------------------------------------------------------------------------
@@ -360,7 +647,7 @@ __global__ __launch_bounds__(kThreads) void splat_kernel(const float2* __restric
 //    kt[k2 * g + dy] for k2 in [0, g], so conv_cols reads them contiguously.
 // ----------------------------------------------------------------------------
 template <int F>
-__device__ __forceinline__ void conv_rows_fwd_body(unsigned long long* grids,
+__device__ __forceinline__ void conv_rows_fwd_body(const float4* __restrict__ sgrid,
                                                    const TsneStats* __restrict__ st,
                                                    const float2* __restrict__ tw,
                                                    float2* __restrict__ z1,
@@ -370,22 +657,12 @@ __device__ __forceinline__ void conv_rows_fwd_body(unsigned long long* grids,
   __shared__ float2 buf[2 * F];
   if (b < G) {
     const int row = b;
-    const double inv_m = 1.0 / st->fx_mass, inv_xy = 1.0 / st->fx_xy;
-    const double inv_xs = inv_xy / st->x_scale;
-    constexpr size_t gg = static_cast<size_t>(G) * G;
     for (int x = threadIdx.x; x < F; x += kThreads) {
       float2 a = make_float2(0.f, 0.f), c = make_float2(0.f, 0.f);
       if (x < G) {
-        const size_t node = static_cast<size_t>(row) * G + x;
-        const long long m = static_cast<long long>(grids[node]);
-        const long long mxv = static_cast<long long>(grids[gg + node]);
-        const long long myv = static_cast<long long>(grids[2 * gg + node]);
-        grids[node] = 0ull;
-        grids[gg + node] = 0ull;
-        grids[2 * gg + node] = 0ull;
-        a = make_float2(static_cast<float>(static_cast<double>(m) * inv_m),
-                        static_cast<float>(static_cast<double>(mxv) * inv_xs));
-        c = make_float2(static_cast<float>(static_cast<double>(myv) * inv_xy), 0.f);
+        const float4 v = sgrid[static_cast<size_t>(row) * G + x];
+        a = make_float2(v.x, v.y);
+        c = make_float2(v.z, 0.f);
       }
       buf[x] = a;
       buf[F + x] = c;
@@ -425,13 +702,13 @@ __device__ __forceinline__ void conv_rows_fwd_body(unsigned long long* grids,
 }
 
 template <int F>
-__global__ __launch_bounds__(kThreads) void conv_rows_fwd(unsigned long long* grids,
+__global__ __launch_bounds__(kThreads) void conv_rows_fwd(const float4* __restrict__ sgrid,
                                                           const TsneStats* __restrict__ st,
                                                           const float2* __restrict__ tw,
                                                           float2* __restrict__ z1,
                                                           float2* __restrict__ z2,
                                                           float2* __restrict__ kt) {
-  conv_rows_fwd_body<F>(grids, st, tw, z1, z2, kt, blockIdx.x);
+  conv_rows_fwd_body<F>(sgrid, st, tw, z1, z2, kt, blockIdx.x);
 }
 
 // ----------------------------------------------------------------------------
@@ -526,14 +803,22 @@ __device__ __forceinline__ void conv_cols_body(const float2* __restrict__ z1,
     last = (atomicAdd(&st->done_cols, 1u) == static_cast<unsigned>(nblk - 1));
   }
   __syncthreads();
-  if (last && threadIdx.x == 0) {
-    __threadfence();
-    const volatile double* vp = partials;
-    double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += vp[b];
+  if (!last) return;
+  __threadfence();
+  // whole-block fold of the column partials in a fixed order
+  const volatile double* vp = partials;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += kThreads) s += vp[b];
+  s = wave_sum(s);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kThreads / 64; ++w) t += red[w];
     // Parseval: sum_nodes mass*q = (1/F^2) sum_k |A(k)|^2 Kq_hat(k); kq above
     // already carries the 1/F^2.
-    st->sum_q_total = s;
+    st->sum_q_total = t;
     st->done_cols = 0u;
   }
 }
@@ -627,10 +912,11 @@ __device__ __forceinline__ void attract_body(const float2* __restrict__ y_in,
                                              float2* __restrict__ attr, int n, int blk, int nblk) {
   constexpr int U = kAttrUnroll;
   const int lane = threadIdx.x % TPR;
-  const int group = (blk * kThreads + static_cast<int>(threadIdx.x)) / TPR;
-  const int n_groups = nblk * (kThreads / TPR);
+  // a contiguous range of rows per block, the ranges of one XCD adjacent
+  int beg, end;
+  point_range(n, xcd_remap(blk, nblk), nblk, &beg, &end);
   const float mxf = st->mean_xf, myf = st->mean_yf;
-  for (int i = group; i < n; i += n_groups) {
+  for (int i = beg + static_cast<int>(threadIdx.x) / TPR; i < end; i += kThreads / TPR) {
     const float2 pi = y_in[i];
     const float xi = pi.x - mxf, yi = pi.y - myf;
     float ax = 0.0f, ay = 0.0f;
@@ -678,6 +964,180 @@ __global__ __launch_bounds__(kThreads) void attract_kernel(const float2* __restr
 }
 
 // ----------------------------------------------------------------------------
+// 5a'. entry-balanced attraction (the terms of attract_body summed in another
+//     fixed order).  Wave w owns the whole rows wave_row[w] .. wave_row[w+1]-1,
+//     cut so that every wave has about the same number of entries, and walks
+//     them in aligned steps of 512 entries: lane l takes the 8 consecutive
+//     entries 512*s + 8*l .. +7 (two 16-byte loads each of col and val).  Bit
+//     i of row_start[b] is set when entry 64*b + i begins a row, so the row of
+//     an entry is the wave's first row plus the row starts seen before it.  A
+//     lane sums its entries row by row in registers: rows that begin and end
+//     inside the lane are written at once; the piece before the lane's first
+//     row start (the head) closes the row that was open when the lane began,
+//     and the piece after its last row start (the tail) stays open.  One
+//     segmented scan over the 64 lanes per step (a lane adds the lane o below
+//     it while no lane in between has a row start) carries the open row from
+//     lane to lane, and the step's last lane carries it into the next step.
+//     The entry e1 that ends the wave's range is treated as a row start, so
+//     the last row closes like any other.  Every row must have at least one
+//     entry (checked when the plan is built); col, val and row_start carry
+//     padding so whole steps can be loaded.
+// ----------------------------------------------------------------------------
+constexpr int kFlatLane = 8;                 // consecutive entries per lane
+constexpr int kFlatStep = 64 * kFlatLane;    // entries per wave step
+constexpr int kFlatWords = kFlatStep / 64;   // row-start words per step
+
+__device__ __forceinline__ void attract_flat_body(const float2* __restrict__ y_in,
+                                                  const int* __restrict__ row_ptr,
+                                                  const int* __restrict__ col,
+                                                  const float* __restrict__ val,
+                                                  const unsigned long long* __restrict__ row_start,
+                                                  const int* __restrict__ wave_row,
+                                                  const TsneStats* __restrict__ st,
+                                                  float exaggeration, float2* __restrict__ attr,
+                                                  int wave) {
+  const int lane = threadIdx.x & 63;
+  const int r0 = wave_row[wave], r1 = wave_row[wave + 1];
+  if (r0 >= r1) return;
+  const int e0 = row_ptr[r0], e1 = row_ptr[r1];
+  const float mxf = st->mean_xf, myf = st->mean_yf;
+  const int wi = lane >> 3, sh = (lane & 7) * kFlatLane;
+  float cx = 0.0f, cy = 0.0f;  // the row open at the start of the step
+  int rowc = r0 - 1;           // row of the entry before the step
+  for (int base = (e0 / kFlatStep) * kFlatStep; base <= e1; base += kFlatStep) {
+    const int eb = base + lane * kFlatLane;
+    const int4 ja = *reinterpret_cast<const int4*>(col + eb);
+    const int4 jb = *reinterpret_cast<const int4*>(col + eb + 4);
+    const float4 va = *reinterpret_cast<const float4*>(val + eb);
+    const float4 vb = *reinterpret_cast<const float4*>(val + eb + 4);
+    // row starts of the step inside [e0, e1), plus the terminator at e1
+    unsigned long long mine = 0ull;
+    int before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kFlatWords; ++w) {
+      const int wb = base + 64 * w;
+      unsigned long long m = row_start[(base >> 6) + w];
+      if (e0 > wb) m &= e0 - wb >= 64 ? 0ull : ~((1ull << (e0 - wb)) - 1ull);
+      if (e1 < wb + 64) {
+        m &= e1 <= wb ? 0ull : ((1ull << (e1 - wb)) - 1ull);
+        if (e1 >= wb) m |= 1ull << (e1 - wb);
+      }
+      const int pc = __popcll(m);
+      before += w < wi ? pc : 0;
+      total += pc;
+      mine = w == wi ? m : mine;
+    }
+    const unsigned int bits = static_cast<unsigned int>(mine >> sh) & 0xffu;
+    const int open_row = rowc + before + __popcll(mine & ((1ull << sh) - 1ull));
+    const int j[kFlatLane] = {ja.x, ja.y, ja.z, ja.w, jb.x, jb.y, jb.z, jb.w};
+    const float v[kFlatLane] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
+    int row[kFlatLane];
+    bool ok[kFlatLane];
+    float2 pi[kFlatLane], pj[kFlatLane];
+#pragma unroll
+    for (int u = 0; u < kFlatLane; ++u) {
+      row[u] = open_row + __popc(bits & ((2u << u) - 1u));
+      ok[u] = eb + u >= e0 && eb + u < e1;
+      pi[u] = ok[u] ? y_in[row[u]] : make_float2(0.f, 0.f);
+      pj[u] = ok[u] ? y_in[j[u]] : make_float2(0.f, 0.f);
+    }
+    float hx = 0.f, hy = 0.f, tx = 0.f, ty = 0.f;
+    bool seen = false;
+#pragma unroll
+    for (int u = 0; u < kFlatLane; ++u) {
+      if ((bits >> u) & 1u) {
+        if (!seen) {
+          hx = tx;
+          hy = ty;
+          seen = true;
+        } else {
+          attr[row[u] - 1] = make_float2(tx, ty);
+        }
+        tx = 0.f;
+        ty = 0.f;
+      }
+      if (ok[u]) {
+        const float xi = pi[u].x - mxf, yi = pi[u].y - myf;
+        const float dx = xi - (pj[u].x - mxf), dy = yi - (pj[u].y - myf);
+        const float q = 1.0f / (1.0f + dx * dx + dy * dy);
+        const float cf = exaggeration * v[u] * q;
+        tx += cf * dx;
+        ty += cf * dy;
+      }
+    }
+    // segmented scan of the open-row pieces: a lane with a row start passes on
+    // its tail, any other lane adds its whole sum to the open row
+    const unsigned long long flags = __ballot(seen);
+    float sx = tx, sy = ty;  // tail, or the whole lane when it has no row start
+    if (lane == 0 && !seen) {
+      sx += cx;
+      sy += cy;
+    }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const float ux = __shfl_up(sx, o, 64), uy = __shfl_up(sy, o, 64);
+      const bool joined = lane >= o && ((flags >> (lane - o + 1)) & ((1ull << o) - 1ull)) == 0ull;
+      if (joined) {
+        sx += ux;
+        sy += uy;
+      }
+    }
+    // open row arriving at this lane: the carry for lane 0, else the scan of
+    // the lane below
+    float ix = __shfl_up(sx, 1, 64), iy = __shfl_up(sy, 1, 64);
+    if (lane == 0) {
+      ix = cx;
+      iy = cy;
+    }
+    if (seen && open_row >= r0) attr[open_row] = make_float2(ix + hx, iy + hy);
+    cx = __shfl(sx, 63, 64);
+    cy = __shfl(sy, 63, 64);
+    rowc += total;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void attract_flat(
+    const float2* __restrict__ y_in, const int* __restrict__ row_ptr, const int* __restrict__ col,
+    const float* __restrict__ val, const unsigned long long* __restrict__ row_start,
+    const int* __restrict__ wave_row, const TsneStats* __restrict__ st, float exaggeration,
+    float2* __restrict__ attr, int n_waves) {
+  const int wave = xcd_remap(blockIdx.x, gridDim.x) * (kThreads / 64) +
+                   static_cast<int>(threadIdx.x >> 6);
+  if (wave >= n_waves) return;
+  attract_flat_body(y_in, row_ptr, col, val, row_start, wave_row, st, exaggeration, attr, wave);
+}
+
+// set-up of the entry-balanced attraction: row-start bits and empty-row count
+__global__ __launch_bounds__(kThreads) void attract_row_starts(const int* __restrict__ row_ptr,
+                                                               int n,
+                                                               unsigned long long* row_start,
+                                                               unsigned int* empty_rows) {
+  const int r = blockIdx.x * kThreads + threadIdx.x;
+  if (r >= n) return;
+  const int s = row_ptr[r], e = row_ptr[r + 1];
+  if (e > s)
+    atomicOr(row_start + (s >> 6), 1ull << (s & 63));
+  else
+    atomicAdd(empty_rows, 1u);
+}
+
+// wave_row[w] = first row starting at or after entry w * nnz / n_waves
+__global__ __launch_bounds__(kThreads) void attract_wave_rows(const int* __restrict__ row_ptr,
+                                                              int n, long long nnz, int n_waves,
+                                                              int* __restrict__ wave_row) {
+  const int w = blockIdx.x * kThreads + threadIdx.x;
+  if (w > n_waves) return;
+  const long long target = static_cast<long long>(w) * nnz / n_waves;
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (row_ptr[mid] < target) lo = mid + 1;
+    else hi = mid;
+  }
+  wave_row[w] = lo;
+}
+
+// ----------------------------------------------------------------------------
 // 5b. gather + update, one thread per point (coalesced).  Back-interpolation
 //     of (q2, xq2, yq2) at the point's cell (tsne_neighbors.cpp:1247-1258),
 //     rep = -(y q2 - xq2, y q2 - yq2) / Z, grad = rep + attr.
@@ -700,7 +1160,10 @@ __device__ __forceinline__ bool update_body(const float2* __restrict__ y_in,
   const float inv_z = static_cast<float>(
       1.0 / fmax(st->sum_q_total - static_cast<double>(n), static_cast<double>(FLT_MIN)));
   LayoutPartial m{0.0, 0.0, FLT_MAX, -FLT_MAX, FLT_MAX, -FLT_MAX};
-  for (int i = blk * kThreads + threadIdx.x; i < n; i += nblk * kThreads) {
+  // a contiguous range of points per block, the ranges of one XCD adjacent
+  int beg, end;
+  point_range(n, xcd_remap(blk, nblk), nblk, &beg, &end);
+  for (int i = beg + threadIdx.x; i < end; i += kThreads) {
     const float2 pi = y_in[i];
     const float xi = pi.x - mxf, yi = pi.y - myf;
     const Cell c = grid_cell(xi, yi, lx, ly, inv, g);

@@ -22,6 +22,12 @@ def _alone(idx, dist, y0, grid, early, normal):
                        early_exaggeration_iter=early, n_iter=normal)
 
 
+def _add(b, job, grid, early, normal):
+    idx, dist, y0 = job
+    return b.add(idx, dist, perplexity=10, Y_init=y0, grid_size=grid,
+                 early_exaggeration_iter=early, n_iter=normal)
+
+
 def test_batch_equals_alone_bitwise():
     from fastembedr_amd.batch import TsneBatch
 
@@ -48,15 +54,15 @@ def test_jobs_join_and_leave_mid_run():
 
     a, c = _job(2000, 21), _job(1200, 22)
     b = TsneBatch(max_jobs=2)
-    sa = b.add(*a, perplexity=10, grid_size=128, early_exaggeration_iter=40, n_iter=40)
+    sa = _add(b, a, 128, 40, 40)
     b.step(30)
-    sc = b.add(*c, perplexity=10, grid_size=128, early_exaggeration_iter=30, n_iter=10)
+    sc = _add(b, c, 128, 30, 10)
     b.run_to_completion(chunk=10)
     ya, yc = b.layout(sa), b.layout(sc)
     b.remove(sa)
     # the freed slot takes a new job
     d = _job(1000, 23)
-    sd = b.add(*d, perplexity=10, grid_size=128, early_exaggeration_iter=10, n_iter=10)
+    sd = _add(b, d, 128, 10, 10)
     b.run_to_completion(chunk=10)
     yd = b.layout(sd)
     b.close()
@@ -71,7 +77,7 @@ def test_budget_admission():
     a = _job(2000, 31)
     need = job_bytes(2000, 30, 128)
     b = TsneBatch(max_jobs=4, hbm_budget_bytes=need + need // 4)
-    b.add(*a, perplexity=10, grid_size=128, early_exaggeration_iter=5, n_iter=5)
+    _add(b, a, 128, 5, 5)
     with pytest.raises(NotAdmitted):
-        b.add(*a, perplexity=10, grid_size=128, early_exaggeration_iter=5, n_iter=5)
+        _add(b, a, 128, 5, 5)
     b.close()

@@ -223,3 +223,41 @@ def test_end_to_end_quality_matches_cpu_reference_algorithm():
     assert abs(mh[0] - mc[0]) <= 0.01, (mh, mc)
     assert abs(mh[2] - mc[2]) <= 0.01, (mh, mc)
     assert abs(hip.mean(0)).max() < 1e-3
+
+
+def test_attraction_kernels_agree_on_hub_rows(monkeypatch):
+    # The centre of a 20-d Gaussian is every point's nearest neighbour, so its
+    # row of the symmetric P holds all n points: rows of every length, one
+    # spanning many 64-entry rounds.
+    rng = np.random.default_rng(711)
+    x = np.vstack([np.zeros((1, 20)), rng.normal(size=(4999, 20))])
+    idx, dist = exact_knn(x, 30)
+    y = rng.normal(scale=3.0, size=(5000, 2)).astype(np.float32)
+    csr = oracle.build_p(idx, dist.astype(np.float32).astype(np.float64), 10)
+    assert np.diff(csr.row_ptr).max() >= 500
+    ref = oracle.attractive(csr, y.astype(np.float64), 4.0)
+    flat = fe().gradient_diagnostic(idx, dist.astype(np.float32), y, 10, 4.0, 128)
+    monkeypatch.setenv("FASTEMBEDR_HIP_ATTRACTION", "rows")
+    rows = fe().gradient_diagnostic(idx, dist.astype(np.float32), y, 10, 4.0, 128)
+    assert relative_l2(flat.attractive_force, ref) < 5e-5
+    assert relative_l2(rows.attractive_force, ref) < 5e-5
+    assert relative_l2(flat.attractive_force, rows.attractive_force) < 1e-5
+    assert np.array_equal(flat.repulsive_force_fft, rows.repulsive_force_fft)
+
+
+def test_collapsed_layout_splits_heavy_tiles():
+    # 90% of the points inside one grid cell: their tile is cut into several
+    # work items whose sums meet in the 64-bit grid
+    n = 20000
+    x, _ = gaussian_mixture(n, 10, 5, seed=712)
+    idx, dist = exact_knn_clusters(x, 30)
+    rng = np.random.default_rng(713)
+    y = np.where(rng.random((n, 1)) < 0.9, rng.normal(scale=1e-3, size=(n, 2)),
+                 rng.normal(scale=10.0, size=(n, 2))).astype(np.float32)
+    dg = fe().gradient_diagnostic(idx, dist, y, 10, 1.0, 256)
+    yd = y.astype(np.float64)
+    rep, z = oracle.fft_repulsive(yd, 256)
+    assert relative_l2(dg.repulsive_force_fft, rep) < 1e-4
+    assert abs(dg.sum_q - z) / z < 1e-5
+    again = fe().gradient_diagnostic(idx, dist, y, 10, 1.0, 256)
+    assert np.array_equal(dg.repulsive_force_fft, again.repulsive_force_fft)
