@@ -191,6 +191,7 @@ class TsnePlan {
   // every wave's range (used when every row of P has an entry)
   Arena attr_store;
   unsigned long long* row_start = nullptr;
+  int* word_row = nullptr;
   int* wave_row = nullptr;
   int attr_waves = 0;
   bool flat_attr = false;
@@ -252,13 +253,15 @@ int TsnePlan::setup_attraction(hipStream_t s) {
   if (nnz <= 0 || !entry_balanced_requested()) return 0;
   int waves = static_cast<int>(std::min<long long>(std::max<long long>(nnz / 2048, 4), 32768));
   waves = (waves + 3) / 4 * 4;
-  const size_t words = static_cast<size_t>((nnz + 63) / 64) + kFlatWords;
+  const size_t words = static_cast<size_t>((nnz + 63) / 64) + 2 * kFlatWords;
   ArenaPlan ap;
   ap.add<unsigned long long>(words);
+  ap.add<int>(words);
   ap.add<int>(waves + 1);
   ap.add<unsigned int>(1);
   if (attr_store.init(ap.bytes, "t-SNE attraction tables")) return 1;
   row_start = attr_store.take<unsigned long long>(words);
+  word_row = attr_store.take<int>(words);
   wave_row = attr_store.take<int>(waves + 1);
   unsigned int* d_empty = attr_store.take<unsigned int>(1);
   FE_CHECK(hipMemsetAsync(row_start, 0, words * sizeof(unsigned long long), s));
@@ -269,12 +272,16 @@ int TsnePlan::setup_attraction(hipStream_t s) {
   attract_wave_rows<<<(waves + kThreads) / kThreads, kThreads, 0, s>>>(csr.row_ptr, n, nnz, waves,
                                                                       wave_row);
   FE_LAUNCH_CHECK("attract_wave_rows");
+  attract_word_rows<<<static_cast<int>((words + kThreads - 1) / kThreads), kThreads, 0, s>>>(
+      csr.row_ptr, n, static_cast<int>(words), word_row);
+  FE_LAUNCH_CHECK("attract_word_rows");
   unsigned int empty = 0;
   FE_CHECK(hipMemcpyAsync(&empty, d_empty, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
   FE_CHECK(hipStreamSynchronize(s));
   if (empty != 0) {
     attr_store.release();
     row_start = nullptr;
+    word_row = nullptr;
     wave_row = nullptr;
     return 0;
   }
@@ -316,7 +323,8 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
   }
   auto alloc_csr = [&](int nnz, int** row_ptr, int** col, float** val) -> int {
     // padded so the entry-balanced attraction can load whole 512-entry steps
-    const size_t m = static_cast<size_t>(nnz > 0 ? nnz : 0) + kFlatStep;
+    // and prefetch the next one
+    const size_t m = static_cast<size_t>(nnz > 0 ? nnz : 0) + 2 * kFlatStep;
     ArenaPlan cp;
     cp.add<int>(n + 1);
     cp.add<int>(m);
@@ -476,8 +484,8 @@ static int launch_attract(TsnePlan& p, float exaggeration, hipStream_t s) {
   const float2* yin = p.y[p.cur];
   if (p.flat_attr) {
     attract_flat<<<p.attr_waves / (kThreads / 64), kThreads, 0, s>>>(
-        yin, p.csr.row_ptr, p.csr.col, p.csr.val, p.row_start, p.wave_row, p.st, exaggeration,
-        p.diag, p.attr_waves);
+        yin, p.csr.row_ptr, p.csr.col, p.csr.val, p.row_start, p.word_row, p.wave_row, p.st,
+        exaggeration, p.diag, p.n, p.attr_waves);
     FE_LAUNCH_CHECK("attract_flat");
     return 0;
   }
@@ -792,6 +800,7 @@ class TsneBatch {
     d.col = p.csr.col;
     d.val = p.csr.val;
     d.row_start = p.row_start;
+    d.word_row = p.word_row;
     d.wave_row = p.wave_row;
     d.splat = p.splat;
     d.z1 = p.z1;

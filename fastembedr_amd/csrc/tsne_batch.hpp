@@ -26,6 +26,7 @@ struct JobDesc {
   const int* col;
   const float* val;
   const unsigned long long* row_start;  // entry-balanced attraction tables
+  const int* word_row;
   const int* wave_row;
   SplatBuffers splat;
   float2* z1;
@@ -128,10 +129,11 @@ __global__ __launch_bounds__(kThreads) void batch_attract_flat(JobDesc* jobs,
   JobDesc* J = jobs + slot[e];
   if (!J->active) return;
   const float exag = J->iter < J->early_iter ? J->early_exag : J->exag;
-  const int wave = (static_cast<int>(blockIdx.x) - first[e]) * (kThreads / 64) +
+  const int wave = xcd_remap(static_cast<int>(blockIdx.x) - first[e], first[e + 1] - first[e]) *
+                       (kThreads / 64) +
                    static_cast<int>(threadIdx.x >> 6);
-  attract_flat_body(J->y[J->cur], J->row_ptr, J->col, J->val, J->row_start, J->wave_row, J->st,
-                    exag, J->attr, wave);
+  attract_flat_body(J->y[J->cur], J->row_ptr, J->col, J->val, J->row_start, J->word_row,
+                    J->wave_row, J->st, exag, J->attr, J->n, wave);
 }
 
 template <int TPR>

@@ -969,106 +969,172 @@ __global__ __launch_bounds__(kThreads) void attract_kernel(const float2* __restr
 //     cut so that every wave has about the same number of entries, and walks
 //     them in aligned steps of 512 entries: lane l takes the 8 consecutive
 //     entries 512*s + 8*l .. +7 (two 16-byte loads each of col and val).  Bit
-//     i of row_start[b] is set when entry 64*b + i begins a row, so the row of
-//     an entry is the wave's first row plus the row starts seen before it.  A
-//     lane sums its entries row by row in registers: rows that begin and end
-//     inside the lane are written at once; the piece before the lane's first
-//     row start (the head) closes the row that was open when the lane began,
-//     and the piece after its last row start (the tail) stays open.  One
-//     segmented scan over the 64 lanes per step (a lane adds the lane o below
-//     it while no lane in between has a row start) carries the open row from
-//     lane to lane, and the step's last lane carries it into the next step.
-//     The entry e1 that ends the wave's range is treated as a row start, so
-//     the last row closes like any other.  Every row must have at least one
-//     entry (checked when the plan is built); col, val and row_start carry
-//     padding so whole steps can be loaded.
+//     i of row_start[b] is set when entry 64*b + i begins a row and
+//     word_row[b] is the row of entry 64*b, so a lane finds the rows of its
+//     entries from one word and one row number, without talking to the other
+//     lanes.  A lane sums its entries row by row in registers: rows that begin
+//     and end inside the lane are written at once; the piece before the lane's
+//     first row start (the head) closes the row that was open when the lane
+//     began, and the piece after its last row start (the tail) stays open.
+//     One segmented scan over the 64 lanes per step (lane l adds lane l - o
+//     while both lie in one segment, a segment starting at a lane with a row
+//     start) carries the open row from lane to lane, and lane 63 carries it
+//     into the next step.  The entry e1 that ends the wave's range counts as a
+//     row start, so the last row closes like any other.  Every row must have at
+//     least one entry (checked when the plan is built); col, val, row_start and
+//     word_row carry two steps of padding so that whole steps, and the next
+//     step's prefetch, can be loaded unconditionally.
 // ----------------------------------------------------------------------------
 constexpr int kFlatLane = 8;                 // consecutive entries per lane
 constexpr int kFlatStep = 64 * kFlatLane;    // entries per wave step
 constexpr int kFlatWords = kFlatStep / 64;   // row-start words per step
+
+// the loads that decide a step's gathers: a lane's 8 column indices and the
+// row-start word and row number of the 64-entry group holding them
+struct FlatCols {
+  int4 ja, jb;
+  unsigned long long word;
+  int wrow;
+};
+
+__device__ __forceinline__ FlatCols flat_cols(const int* __restrict__ col,
+                                              const unsigned long long* __restrict__ row_start,
+                                              const int* __restrict__ word_row, int base,
+                                              int lane) {
+  const int eb = base + lane * kFlatLane;
+  const int wix = (base >> 6) + (lane >> 3);
+  FlatCols c;
+  c.ja = *reinterpret_cast<const int4*>(col + eb);
+  c.jb = *reinterpret_cast<const int4*>(col + eb + 4);
+  c.word = row_start[wix];
+  c.wrow = word_row[wix];
+  return c;
+}
+
+// One step of one lane: issues the step's value loads and gathers and the
+// next step's column loads, then sums the lane's entries row by row.  FAST:
+// the lane's entries lie in at most two rows (its first and its last), so
+// two point loads serve all of them; otherwise (rows of fewer than eight
+// entries) each entry loads its own point.
+template <bool FAST>
+__device__ __forceinline__ void flat_entries(const float2* __restrict__ y_in,
+                                             const int* __restrict__ col,
+                                             const float* __restrict__ val,
+                                             const unsigned long long* __restrict__ row_start,
+                                             const int* __restrict__ word_row,
+                                             float2* __restrict__ attr, const FlatCols& cur,
+                                             FlatCols* next, int base, int lane, int open_row,
+                                             unsigned int rbits, unsigned int mbits,
+                                             unsigned int vmask, int n, float mxf, float myf,
+                                             float exaggeration, float* hx, float* hy,
+                                             float* tx, float* ty, int* hrow) {
+  const int eb = base + lane * kFlatLane;
+  const float4 va = *reinterpret_cast<const float4*>(val + eb);
+  const float4 vb = *reinterpret_cast<const float4*>(val + eb + 4);
+  const int jv[kFlatLane] = {cur.ja.x, cur.ja.y, cur.ja.z, cur.ja.w,
+                             cur.jb.x, cur.jb.y, cur.jb.z, cur.jb.w};
+  int row[kFlatLane];
+#pragma unroll
+  for (int u = 0; u < kFlatLane; ++u) row[u] = open_row + __popc(rbits & ((2u << u) - 1u));
+  const int ra = min(max(row[0], 0), n - 1), rb = min(max(row[kFlatLane - 1], 0), n - 1);
+  // gathers, all unconditional (an invalid entry reads a valid point)
+  float2 pj[kFlatLane];
+#pragma unroll
+  for (int u = 0; u < kFlatLane; ++u) pj[u] = y_in[((vmask >> u) & 1u) ? jv[u] : ra];
+  float2 pa = make_float2(0.f, 0.f), pb = pa;
+  if constexpr (FAST) {
+    pa = y_in[ra];
+    pb = y_in[rb];
+    // the next step's column loads fly while this step computes
+    *next = flat_cols(col, row_start, word_row, base + kFlatStep, lane);
+  }
+  const float v[kFlatLane] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
+  float sx = 0.f, sy = 0.f, ax = 0.f, ay = 0.f;
+  int h = -1;
+#pragma unroll
+  for (int u = 0; u < kFlatLane; ++u) {
+    if ((mbits >> u) & 1u) {
+      if (h == -1) {
+        ax = sx;
+        ay = sy;
+        // -2: the start is entry 0 of P, so no row is open before it
+        h = row[u] > 0 ? row[u] - 1 : -2;
+      } else {
+        attr[row[u] - 1] = make_float2(sx, sy);
+      }
+      sx = 0.f;
+      sy = 0.f;
+    }
+    float2 p;
+    if constexpr (FAST) p = row[u] == row[0] ? pa : pb;
+    else p = y_in[min(max(row[u], 0), n - 1)];  // rows of fewer than 8 entries: rare
+    const float xi = p.x - mxf, yi = p.y - myf;
+    const float dx = xi - (pj[u].x - mxf), dy = yi - (pj[u].y - myf);
+    const float q = __builtin_amdgcn_rcpf(1.0f + dx * dx + dy * dy);
+    const float cf = ((vmask >> u) & 1u) ? exaggeration * v[u] * q : 0.0f;
+    sx += cf * dx;
+    sy += cf * dy;
+  }
+  if constexpr (!FAST) *next = flat_cols(col, row_start, word_row, base + kFlatStep, lane);
+  *hx = ax;
+  *hy = ay;
+  *tx = sx;
+  *ty = sy;
+  *hrow = h;
+}
 
 __device__ __forceinline__ void attract_flat_body(const float2* __restrict__ y_in,
                                                   const int* __restrict__ row_ptr,
                                                   const int* __restrict__ col,
                                                   const float* __restrict__ val,
                                                   const unsigned long long* __restrict__ row_start,
+                                                  const int* __restrict__ word_row,
                                                   const int* __restrict__ wave_row,
                                                   const TsneStats* __restrict__ st,
                                                   float exaggeration, float2* __restrict__ attr,
-                                                  int wave) {
+                                                  int n, int wave) {
   const int lane = threadIdx.x & 63;
   const int r0 = wave_row[wave], r1 = wave_row[wave + 1];
   if (r0 >= r1) return;
   const int e0 = row_ptr[r0], e1 = row_ptr[r1];
   const float mxf = st->mean_xf, myf = st->mean_yf;
-  const int wi = lane >> 3, sh = (lane & 7) * kFlatLane;
+  const int sh = (lane & 7) * kFlatLane;
+  const unsigned long long upto_lane = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
   float cx = 0.0f, cy = 0.0f;  // the row open at the start of the step
-  int rowc = r0 - 1;           // row of the entry before the step
-  for (int base = (e0 / kFlatStep) * kFlatStep; base <= e1; base += kFlatStep) {
+  int base = (e0 / kFlatStep) * kFlatStep;
+  FlatCols cur = flat_cols(col, row_start, word_row, base, lane);
+  for (; base <= e1; base += kFlatStep) {
     const int eb = base + lane * kFlatLane;
-    const int4 ja = *reinterpret_cast<const int4*>(col + eb);
-    const int4 jb = *reinterpret_cast<const int4*>(col + eb + 4);
-    const float4 va = *reinterpret_cast<const float4*>(val + eb);
-    const float4 vb = *reinterpret_cast<const float4*>(val + eb + 4);
-    // row starts of the step inside [e0, e1), plus the terminator at e1
-    unsigned long long mine = 0ull;
-    int before = 0, total = 0;
-#pragma unroll
-    for (int w = 0; w < kFlatWords; ++w) {
-      const int wb = base + 64 * w;
-      unsigned long long m = row_start[(base >> 6) + w];
-      if (e0 > wb) m &= e0 - wb >= 64 ? 0ull : ~((1ull << (e0 - wb)) - 1ull);
-      if (e1 < wb + 64) {
-        m &= e1 <= wb ? 0ull : ((1ull << (e1 - wb)) - 1ull);
-        if (e1 >= wb) m |= 1ull << (e1 - wb);
-      }
-      const int pc = __popcll(m);
-      before += w < wi ? pc : 0;
-      total += pc;
-      mine = w == wi ? m : mine;
-    }
-    const unsigned int bits = static_cast<unsigned int>(mine >> sh) & 0xffu;
-    const int open_row = rowc + before + __popcll(mine & ((1ull << sh) - 1ull));
-    const int j[kFlatLane] = {ja.x, ja.y, ja.z, ja.w, jb.x, jb.y, jb.z, jb.w};
-    const float v[kFlatLane] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
-    int row[kFlatLane];
-    bool ok[kFlatLane];
-    float2 pi[kFlatLane], pj[kFlatLane];
-#pragma unroll
-    for (int u = 0; u < kFlatLane; ++u) {
-      row[u] = open_row + __popc(bits & ((2u << u) - 1u));
-      ok[u] = eb + u >= e0 && eb + u < e1;
-      pi[u] = ok[u] ? y_in[row[u]] : make_float2(0.f, 0.f);
-      pj[u] = ok[u] ? y_in[j[u]] : make_float2(0.f, 0.f);
-    }
-    float hx = 0.f, hy = 0.f, tx = 0.f, ty = 0.f;
-    bool seen = false;
-#pragma unroll
-    for (int u = 0; u < kFlatLane; ++u) {
-      if ((bits >> u) & 1u) {
-        if (!seen) {
-          hx = tx;
-          hy = ty;
-          seen = true;
-        } else {
-          attr[row[u] - 1] = make_float2(tx, ty);
-        }
-        tx = 0.f;
-        ty = 0.f;
-      }
-      if (ok[u]) {
-        const float xi = pi[u].x - mxf, yi = pi[u].y - myf;
-        const float dx = xi - (pj[u].x - mxf), dy = yi - (pj[u].y - myf);
-        const float q = 1.0f / (1.0f + dx * dx + dy * dy);
-        const float cf = exaggeration * v[u] * q;
-        tx += cf * dx;
-        ty += cf * dy;
-      }
-    }
+    // valid entries [vlo, vhi) of the lane; the terminator e1 at position t
+    const int vlo = min(max(e0 - eb, 0), kFlatLane), vhi = min(max(e1 - eb, 0), kFlatLane);
+    const unsigned int vmask = ((1u << vhi) - 1u) & ~((1u << vlo) - 1u);
+    const int t = e1 - eb;
+    const unsigned int term = (t >= 0 && t < kFlatLane) ? (1u << t) : 0u;
+    const unsigned int rbits = (static_cast<unsigned int>(cur.word >> sh) & 0xffu) | term;
+    const unsigned int mbits = rbits & (vmask | term);
+    // row open before the lane (the row of entry eb - 1; entry 0 of a word
+    // already counts in word_row); the terminator counts as a row start
+    const int open_row = cur.wrow + __popcll(cur.word & ((1ull << sh) - 1ull) & ~1ull) -
+                         (sh == 0 ? static_cast<int>(cur.word & 1ull) : 0);
+    float hx, hy, tx, ty;
+    int hrow;  // row closed by the lane's first row start (-1: none)
+    FlatCols next;
+    if (__ballot(__popc(rbits & 0xfeu) > 1) == 0ull)
+      flat_entries<true>(y_in, col, val, row_start, word_row, attr, cur, &next, base, lane,
+                         open_row, rbits, mbits, vmask, n, mxf, myf, exaggeration, &hx, &hy, &tx,
+                         &ty, &hrow);
+    else
+      flat_entries<false>(y_in, col, val, row_start, word_row, attr, cur, &next, base, lane,
+                          open_row, rbits, mbits, vmask, n, mxf, myf, exaggeration, &hx, &hy,
+                          &tx, &ty, &hrow);
+    cur = next;
+    const bool seen = hrow != -1;
     // segmented scan of the open-row pieces: a lane with a row start passes on
     // its tail, any other lane adds its whole sum to the open row
     const unsigned long long flags = __ballot(seen);
-    float sx = tx, sy = ty;  // tail, or the whole lane when it has no row start
+    const unsigned long long upto = flags & upto_lane;
+    const int lim = upto ? 63 - __clzll(upto) : 0;  // first lane of this lane's segment
+    float sx = tx, sy = ty;
     if (lane == 0 && !seen) {
       sx += cx;
       sy += cy;
@@ -1076,8 +1142,7 @@ __device__ __forceinline__ void attract_flat_body(const float2* __restrict__ y_i
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
       const float ux = __shfl_up(sx, o, 64), uy = __shfl_up(sy, o, 64);
-      const bool joined = lane >= o && ((flags >> (lane - o + 1)) & ((1ull << o) - 1ull)) == 0ull;
-      if (joined) {
+      if (lane - o >= lim) {
         sx += ux;
         sy += uy;
       }
@@ -1089,22 +1154,23 @@ __device__ __forceinline__ void attract_flat_body(const float2* __restrict__ y_i
       ix = cx;
       iy = cy;
     }
-    if (seen && open_row >= r0) attr[open_row] = make_float2(ix + hx, iy + hy);
+    if (seen && hrow >= r0) attr[hrow] = make_float2(ix + hx, iy + hy);
     cx = __shfl(sx, 63, 64);
     cy = __shfl(sy, 63, 64);
-    rowc += total;
   }
 }
 
 __global__ __launch_bounds__(kThreads) void attract_flat(
     const float2* __restrict__ y_in, const int* __restrict__ row_ptr, const int* __restrict__ col,
     const float* __restrict__ val, const unsigned long long* __restrict__ row_start,
-    const int* __restrict__ wave_row, const TsneStats* __restrict__ st, float exaggeration,
-    float2* __restrict__ attr, int n_waves) {
+    const int* __restrict__ word_row, const int* __restrict__ wave_row,
+    const TsneStats* __restrict__ st, float exaggeration, float2* __restrict__ attr, int n,
+    int n_waves) {
   const int wave = xcd_remap(blockIdx.x, gridDim.x) * (kThreads / 64) +
                    static_cast<int>(threadIdx.x >> 6);
   if (wave >= n_waves) return;
-  attract_flat_body(y_in, row_ptr, col, val, row_start, wave_row, st, exaggeration, attr, wave);
+  attract_flat_body(y_in, row_ptr, col, val, row_start, word_row, wave_row, st, exaggeration,
+                    attr, n, wave);
 }
 
 // set-up of the entry-balanced attraction: row-start bits and empty-row count
@@ -1119,6 +1185,22 @@ __global__ __launch_bounds__(kThreads) void attract_row_starts(const int* __rest
     atomicOr(row_start + (s >> 6), 1ull << (s & 63));
   else
     atomicAdd(empty_rows, 1u);
+}
+
+// word_row[b] = the row holding entry 64*b (the last row for padding words)
+__global__ __launch_bounds__(kThreads) void attract_word_rows(const int* __restrict__ row_ptr,
+                                                              int n, int words,
+                                                              int* __restrict__ word_row) {
+  const int b = blockIdx.x * kThreads + threadIdx.x;
+  if (b >= words) return;
+  const long long e = 64ll * b;
+  int lo = 0, hi = n - 1;  // last r with row_ptr[r] <= e
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (row_ptr[mid] <= e) lo = mid;
+    else hi = mid - 1;
+  }
+  word_row[b] = lo;
 }
 
 // wave_row[w] = first row starting at or after entry w * nnz / n_waves
@@ -1136,6 +1218,7 @@ __global__ __launch_bounds__(kThreads) void attract_wave_rows(const int* __restr
   }
   wave_row[w] = lo;
 }
+
 
 // ----------------------------------------------------------------------------
 // 5b. gather + update, one thread per point (coalesced).  Back-interpolation
