@@ -194,7 +194,7 @@ class TsnePlan {
   int* word_row = nullptr;
   int* wave_row = nullptr;
   int attr_waves = 0;
-  bool flat_attr = false;
+  int flat_attr = 0;  // FlatVariant, 0: row-per-lane-group kernel
   float2* y[2] = {nullptr, nullptr};
   float2* gains = nullptr;
   float2* upd = nullptr;
@@ -240,20 +240,28 @@ static int choose_tpr(double avg_row) {
   return 64;
 }
 
-// FASTEMBEDR_HIP_ATTRACTION=rows keeps the row-per-lane-group kernel (both
-// kernels compute the same per-entry terms; only the summation order differs)
-static bool entry_balanced_requested() {
+// FASTEMBEDR_HIP_ATTRACTION selects the attraction kernel: rows (one lane
+// group per CSR row), flat8p (default), flat8, flat4p, flat4 (entry-balanced,
+// 8 or 4 entries per lane, with or without prefetch).  All compute the same
+// per-entry terms; only the summation order differs.
+static int attraction_variant() {
   const char* v = std::getenv("FASTEMBEDR_HIP_ATTRACTION");
-  return !(v && std::strcmp(v, "rows") == 0);
+  if (!v || !*v || std::strcmp(v, "flat8p") == 0) return kFlat8Prefetch;
+  if (std::strcmp(v, "rows") == 0) return 0;
+  if (std::strcmp(v, "flat8") == 0) return kFlat8;
+  if (std::strcmp(v, "flat4p") == 0) return kFlat4Prefetch;
+  if (std::strcmp(v, "flat4") == 0) return kFlat4;
+  return kFlat8Prefetch;
 }
 
 int TsnePlan::setup_attraction(hipStream_t s) {
   const long long nnz = csr.nnz;
-  flat_attr = false;
-  if (nnz <= 0 || !entry_balanced_requested()) return 0;
+  flat_attr = 0;
+  const int variant = attraction_variant();
+  if (nnz <= 0 || variant == 0) return 0;
   int waves = static_cast<int>(std::min<long long>(std::max<long long>(nnz / 2048, 4), 32768));
   waves = (waves + 3) / 4 * 4;
-  const size_t words = static_cast<size_t>((nnz + 63) / 64) + 2 * kFlatWords;
+  const size_t words = static_cast<size_t>((nnz + 63) / 64) + kFlatPadWords;
   ArenaPlan ap;
   ap.add<unsigned long long>(words);
   ap.add<int>(words);
@@ -286,7 +294,7 @@ int TsnePlan::setup_attraction(hipStream_t s) {
     return 0;
   }
   attr_waves = waves;
-  flat_attr = true;
+  flat_attr = variant;
   return 0;
 }
 
@@ -324,7 +332,7 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
   auto alloc_csr = [&](int nnz, int** row_ptr, int** col, float** val) -> int {
     // padded so the entry-balanced attraction can load whole 512-entry steps
     // and prefetch the next one
-    const size_t m = static_cast<size_t>(nnz > 0 ? nnz : 0) + 2 * kFlatStep;
+    const size_t m = static_cast<size_t>(nnz > 0 ? nnz : 0) + kFlatPad;
     ArenaPlan cp;
     cp.add<int>(n + 1);
     cp.add<int>(m);
@@ -483,9 +491,17 @@ static int launch_conv_any(TsnePlan& p, hipStream_t s) {
 static int launch_attract(TsnePlan& p, float exaggeration, hipStream_t s) {
   const float2* yin = p.y[p.cur];
   if (p.flat_attr) {
-    attract_flat<<<p.attr_waves / (kThreads / 64), kThreads, 0, s>>>(
-        yin, p.csr.row_ptr, p.csr.col, p.csr.val, p.row_start, p.word_row, p.wave_row, p.st,
-        exaggeration, p.diag, p.n, p.attr_waves);
+#define FE_FLAT(L, PF)                                                                       \
+  attract_flat<L, PF><<<p.attr_waves / (kThreads / 64), kThreads, 0, s>>>(                   \
+      yin, p.csr.row_ptr, p.csr.col, p.csr.val, p.row_start, p.word_row, p.wave_row, p.st,   \
+      exaggeration, p.diag, p.n, p.attr_waves)
+    switch (p.flat_attr) {
+      case kFlat8: FE_FLAT(8, false); break;
+      case kFlat4Prefetch: FE_FLAT(4, true); break;
+      case kFlat4: FE_FLAT(4, false); break;
+      default: FE_FLAT(8, true); break;
+    }
+#undef FE_FLAT
     FE_LAUNCH_CHECK("attract_flat");
     return 0;
   }
@@ -751,7 +767,7 @@ class TsneBatch {
     slot_bytes.assign(max_jobs, 0);
     desc.resize(max_jobs);
     std::memset(desc.data(), 0, desc.size() * sizeof(JobDesc));
-    tab_cap = 2 * (8 * max_jobs + 40) + 64;  // every job is in 8 launches
+    tab_cap = 2 * (8 * max_jobs + 48) + 64;  // every job is in 8 launches
     ArenaPlan p;
     p.add<JobDesc>(max_jobs);
     p.add<int>(tab_cap);
@@ -891,12 +907,12 @@ class TsneBatch {
         add_launch(kind, F, ent);
       }
     }
-    {
+    for (int v = kFlat8Prefetch; v <= kFlat4; ++v) {
       std::vector<std::pair<int, int>> ent;
       for (int s = 0; s < max_jobs; ++s)
-        if (slots[s] && slots[s]->flat_attr)
+        if (slots[s] && slots[s]->flat_attr == v)
           ent.push_back({s, slots[s]->attr_waves / (kThreads / 64)});
-      add_launch(4, 0, ent);
+      add_launch(4, -v, ent);  // negative parameter: entry-balanced variant
     }
     for (int tpr = 4; tpr <= 64; tpr *= 2) {
       std::vector<std::pair<int, int>> ent;
@@ -958,7 +974,18 @@ class TsneBatch {
 #undef FE_B_CONV
         case 4:
           switch (L.param) {
-            case 0: batch_attract_flat<<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
+            case -kFlat8Prefetch:
+              batch_attract_flat<8, true><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
+              break;
+            case -kFlat8:
+              batch_attract_flat<8, false><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
+              break;
+            case -kFlat4Prefetch:
+              batch_attract_flat<4, true><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
+              break;
+            case -kFlat4:
+              batch_attract_flat<4, false><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
+              break;
             case 4: batch_attract<4><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
             case 8: batch_attract<8><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
             case 16: batch_attract<16><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;

@@ -121,6 +121,7 @@ __global__ __launch_bounds__(kThreads) void batch_rows_inv(JobDesc* jobs, const 
   conv_rows_inv_body<F>(J->o1, J->o2, J->tw, J->st, J->field, blockIdx.x - first[e]);
 }
 
+template <int LANE, bool PREFETCH>
 __global__ __launch_bounds__(kThreads) void batch_attract_flat(JobDesc* jobs,
                                                                const int* __restrict__ slot,
                                                                const int* __restrict__ first,
@@ -132,8 +133,8 @@ __global__ __launch_bounds__(kThreads) void batch_attract_flat(JobDesc* jobs,
   const int wave = xcd_remap(static_cast<int>(blockIdx.x) - first[e], first[e + 1] - first[e]) *
                        (kThreads / 64) +
                    static_cast<int>(threadIdx.x >> 6);
-  attract_flat_body(J->y[J->cur], J->row_ptr, J->col, J->val, J->row_start, J->word_row,
-                    J->wave_row, J->st, exag, J->attr, J->n, wave);
+  attract_flat_body<LANE, PREFETCH>(J->y[J->cur], J->row_ptr, J->col, J->val, J->row_start,
+                                    J->word_row, J->wave_row, J->st, exag, J->attr, J->n, wave);
 }
 
 template <int TPR>

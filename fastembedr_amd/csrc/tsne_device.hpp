@@ -967,92 +967,113 @@ __global__ __launch_bounds__(kThreads) void attract_kernel(const float2* __restr
 // 5a'. entry-balanced attraction (the terms of attract_body summed in another
 //     fixed order).  Wave w owns the whole rows wave_row[w] .. wave_row[w+1]-1,
 //     cut so that every wave has about the same number of entries, and walks
-//     them in aligned steps of 512 entries: lane l takes the 8 consecutive
-//     entries 512*s + 8*l .. +7 (two 16-byte loads each of col and val).  Bit
-//     i of row_start[b] is set when entry 64*b + i begins a row and
-//     word_row[b] is the row of entry 64*b, so a lane finds the rows of its
-//     entries from one word and one row number, without talking to the other
-//     lanes.  A lane sums its entries row by row in registers: rows that begin
-//     and end inside the lane are written at once; the piece before the lane's
-//     first row start (the head) closes the row that was open when the lane
-//     began, and the piece after its last row start (the tail) stays open.
-//     One segmented scan over the 64 lanes per step (lane l adds lane l - o
-//     while both lie in one segment, a segment starting at a lane with a row
-//     start) carries the open row from lane to lane, and lane 63 carries it
+//     them in aligned steps of 64*LANE entries: lane l takes the LANE
+//     consecutive entries 64*LANE*s + LANE*l .. (16-byte loads of col and
+//     val).  Bit i of row_start[b] is set when entry 64*b + i begins a row
+//     and word_row[b] is the row of entry 64*b, so a lane finds the rows of
+//     its entries from one word and one row number, without talking to the
+//     other lanes.  A lane sums its entries row by row in registers: rows that
+//     begin and end inside the lane are written at once; the piece before the
+//     lane's first row start (the head) closes the row that was open when the
+//     lane began, and the piece after its last row start (the tail) stays
+//     open.  One segmented scan over the 64 lanes per step (lane l adds lane
+//     l - o while both lie in one segment, a segment starting at a lane with a
+//     row start) carries the open row from lane to lane, and lane 63 carries it
 //     into the next step.  The entry e1 that ends the wave's range counts as a
-//     row start, so the last row closes like any other.  Every row must have at
-//     least one entry (checked when the plan is built); col, val, row_start and
-//     word_row carry two steps of padding so that whole steps, and the next
-//     step's prefetch, can be loaded unconditionally.
+//     row start, so the last row closes like any other.  PREFETCH issues the
+//     next step's column loads before this step computes.  Every row must have
+//     at least one entry (checked when the plan is built); col, val, row_start
+//     and word_row carry kFlatPad entries of padding so that whole steps and
+//     the prefetch load unconditionally.
 // ----------------------------------------------------------------------------
-constexpr int kFlatLane = 8;                 // consecutive entries per lane
-constexpr int kFlatStep = 64 * kFlatLane;    // entries per wave step
-constexpr int kFlatWords = kFlatStep / 64;   // row-start words per step
+constexpr int kFlatMaxLane = 8;
+constexpr int kFlatPad = 2 * 64 * kFlatMaxLane;  // padding entries after P
+constexpr int kFlatPadWords = kFlatPad / 64;
 
-// the loads that decide a step's gathers: a lane's 8 column indices and the
-// row-start word and row number of the 64-entry group holding them
+template <int LANE>
 struct FlatCols {
-  int4 ja, jb;
-  unsigned long long word;
-  int wrow;
+  int j[LANE];
+  unsigned long long word;  // row starts of the 64-entry group of the lane
+  int wrow;                 // row of that group's first entry
 };
 
-__device__ __forceinline__ FlatCols flat_cols(const int* __restrict__ col,
-                                              const unsigned long long* __restrict__ row_start,
-                                              const int* __restrict__ word_row, int base,
-                                              int lane) {
-  const int eb = base + lane * kFlatLane;
-  const int wix = (base >> 6) + (lane >> 3);
-  FlatCols c;
-  c.ja = *reinterpret_cast<const int4*>(col + eb);
-  c.jb = *reinterpret_cast<const int4*>(col + eb + 4);
+template <int LANE>
+__device__ __forceinline__ void load_lane(const int* __restrict__ p, int (&out)[LANE]) {
+#pragma unroll
+  for (int q = 0; q < LANE / 4; ++q) {
+    const int4 v = *reinterpret_cast<const int4*>(p + 4 * q);
+    out[4 * q] = v.x;
+    out[4 * q + 1] = v.y;
+    out[4 * q + 2] = v.z;
+    out[4 * q + 3] = v.w;
+  }
+}
+
+template <int LANE>
+__device__ __forceinline__ void load_lane(const float* __restrict__ p, float (&out)[LANE]) {
+#pragma unroll
+  for (int q = 0; q < LANE / 4; ++q) {
+    const float4 v = *reinterpret_cast<const float4*>(p + 4 * q);
+    out[4 * q] = v.x;
+    out[4 * q + 1] = v.y;
+    out[4 * q + 2] = v.z;
+    out[4 * q + 3] = v.w;
+  }
+}
+
+template <int LANE>
+__device__ __forceinline__ FlatCols<LANE> flat_cols(const int* __restrict__ col,
+                                                    const unsigned long long* __restrict__ row_start,
+                                                    const int* __restrict__ word_row, int base,
+                                                    int lane) {
+  const int eb = base + lane * LANE;
+  const int wix = eb >> 6;
+  FlatCols<LANE> c;
+  load_lane<LANE>(col + eb, c.j);
   c.word = row_start[wix];
   c.wrow = word_row[wix];
   return c;
 }
 
-// One step of one lane: issues the step's value loads and gathers and the
-// next step's column loads, then sums the lane's entries row by row.  FAST:
-// the lane's entries lie in at most two rows (its first and its last), so
-// two point loads serve all of them; otherwise (rows of fewer than eight
-// entries) each entry loads its own point.
-template <bool FAST>
+// One step of one lane: issues the step's value loads and gathers (and with
+// PREFETCH the next step's column loads), then sums the lane's entries row by
+// row.  FAST: the lane's entries lie in at most two rows (its first and its
+// last), so two point loads serve all of them; otherwise (rows shorter than
+// LANE entries) each entry loads its own point.
+template <int LANE, bool PREFETCH, bool FAST>
 __device__ __forceinline__ void flat_entries(const float2* __restrict__ y_in,
                                              const int* __restrict__ col,
                                              const float* __restrict__ val,
                                              const unsigned long long* __restrict__ row_start,
                                              const int* __restrict__ word_row,
-                                             float2* __restrict__ attr, const FlatCols& cur,
-                                             FlatCols* next, int base, int lane, int open_row,
-                                             unsigned int rbits, unsigned int mbits,
-                                             unsigned int vmask, int n, float mxf, float myf,
-                                             float exaggeration, float* hx, float* hy,
-                                             float* tx, float* ty, int* hrow) {
-  const int eb = base + lane * kFlatLane;
-  const float4 va = *reinterpret_cast<const float4*>(val + eb);
-  const float4 vb = *reinterpret_cast<const float4*>(val + eb + 4);
-  const int jv[kFlatLane] = {cur.ja.x, cur.ja.y, cur.ja.z, cur.ja.w,
-                             cur.jb.x, cur.jb.y, cur.jb.z, cur.jb.w};
-  int row[kFlatLane];
+                                             float2* __restrict__ attr, const FlatCols<LANE>& cur,
+                                             FlatCols<LANE>* next, int base, int lane,
+                                             int open_row, unsigned int rbits,
+                                             unsigned int mbits, unsigned int vmask, int n,
+                                             float mxf, float myf, float exaggeration, float* hx,
+                                             float* hy, float* tx, float* ty, int* hrow) {
+  constexpr int kStep = 64 * LANE;
+  const int eb = base + lane * LANE;
+  float v[LANE];
+  load_lane<LANE>(val + eb, v);
+  int row[LANE];
 #pragma unroll
-  for (int u = 0; u < kFlatLane; ++u) row[u] = open_row + __popc(rbits & ((2u << u) - 1u));
-  const int ra = min(max(row[0], 0), n - 1), rb = min(max(row[kFlatLane - 1], 0), n - 1);
+  for (int u = 0; u < LANE; ++u) row[u] = open_row + __popc(rbits & ((2u << u) - 1u));
+  const int ra = min(max(row[0], 0), n - 1), rb = min(max(row[LANE - 1], 0), n - 1);
   // gathers, all unconditional (an invalid entry reads a valid point)
-  float2 pj[kFlatLane];
+  float2 pj[LANE];
 #pragma unroll
-  for (int u = 0; u < kFlatLane; ++u) pj[u] = y_in[((vmask >> u) & 1u) ? jv[u] : ra];
+  for (int u = 0; u < LANE; ++u) pj[u] = y_in[((vmask >> u) & 1u) ? cur.j[u] : ra];
   float2 pa = make_float2(0.f, 0.f), pb = pa;
   if constexpr (FAST) {
     pa = y_in[ra];
     pb = y_in[rb];
-    // the next step's column loads fly while this step computes
-    *next = flat_cols(col, row_start, word_row, base + kFlatStep, lane);
   }
-  const float v[kFlatLane] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
+  if constexpr (PREFETCH && FAST) *next = flat_cols<LANE>(col, row_start, word_row, base + kStep, lane);
   float sx = 0.f, sy = 0.f, ax = 0.f, ay = 0.f;
   int h = -1;
 #pragma unroll
-  for (int u = 0; u < kFlatLane; ++u) {
+  for (int u = 0; u < LANE; ++u) {
     if ((mbits >> u) & 1u) {
       if (h == -1) {
         ax = sx;
@@ -1067,7 +1088,7 @@ __device__ __forceinline__ void flat_entries(const float2* __restrict__ y_in,
     }
     float2 p;
     if constexpr (FAST) p = row[u] == row[0] ? pa : pb;
-    else p = y_in[min(max(row[u], 0), n - 1)];  // rows of fewer than 8 entries: rare
+    else p = y_in[min(max(row[u], 0), n - 1)];  // rows shorter than LANE entries: rare
     const float xi = p.x - mxf, yi = p.y - myf;
     const float dx = xi - (pj[u].x - mxf), dy = yi - (pj[u].y - myf);
     const float q = __builtin_amdgcn_rcpf(1.0f + dx * dx + dy * dy);
@@ -1075,7 +1096,7 @@ __device__ __forceinline__ void flat_entries(const float2* __restrict__ y_in,
     sx += cf * dx;
     sy += cf * dy;
   }
-  if constexpr (!FAST) *next = flat_cols(col, row_start, word_row, base + kFlatStep, lane);
+  if constexpr (PREFETCH && !FAST) *next = flat_cols<LANE>(col, row_start, word_row, base + kStep, lane);
   *hx = ax;
   *hy = ay;
   *tx = sx;
@@ -1083,6 +1104,7 @@ __device__ __forceinline__ void flat_entries(const float2* __restrict__ y_in,
   *hrow = h;
 }
 
+template <int LANE, bool PREFETCH>
 __device__ __forceinline__ void attract_flat_body(const float2* __restrict__ y_in,
                                                   const int* __restrict__ row_ptr,
                                                   const int* __restrict__ col,
@@ -1093,24 +1115,26 @@ __device__ __forceinline__ void attract_flat_body(const float2* __restrict__ y_i
                                                   const TsneStats* __restrict__ st,
                                                   float exaggeration, float2* __restrict__ attr,
                                                   int n, int wave) {
+  constexpr int kStep = 64 * LANE;
+  constexpr unsigned int kLaneMask = (1u << LANE) - 1u;
   const int lane = threadIdx.x & 63;
   const int r0 = wave_row[wave], r1 = wave_row[wave + 1];
   if (r0 >= r1) return;
   const int e0 = row_ptr[r0], e1 = row_ptr[r1];
   const float mxf = st->mean_xf, myf = st->mean_yf;
-  const int sh = (lane & 7) * kFlatLane;
+  const int sh = (lane * LANE) & 63;
   const unsigned long long upto_lane = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
   float cx = 0.0f, cy = 0.0f;  // the row open at the start of the step
-  int base = (e0 / kFlatStep) * kFlatStep;
-  FlatCols cur = flat_cols(col, row_start, word_row, base, lane);
-  for (; base <= e1; base += kFlatStep) {
-    const int eb = base + lane * kFlatLane;
+  int base = (e0 / kStep) * kStep;
+  FlatCols<LANE> cur = flat_cols<LANE>(col, row_start, word_row, base, lane);
+  for (; base <= e1; base += kStep) {
+    const int eb = base + lane * LANE;
     // valid entries [vlo, vhi) of the lane; the terminator e1 at position t
-    const int vlo = min(max(e0 - eb, 0), kFlatLane), vhi = min(max(e1 - eb, 0), kFlatLane);
+    const int vlo = min(max(e0 - eb, 0), LANE), vhi = min(max(e1 - eb, 0), LANE);
     const unsigned int vmask = ((1u << vhi) - 1u) & ~((1u << vlo) - 1u);
     const int t = e1 - eb;
-    const unsigned int term = (t >= 0 && t < kFlatLane) ? (1u << t) : 0u;
-    const unsigned int rbits = (static_cast<unsigned int>(cur.word >> sh) & 0xffu) | term;
+    const unsigned int term = (t >= 0 && t < LANE) ? (1u << t) : 0u;
+    const unsigned int rbits = (static_cast<unsigned int>(cur.word >> sh) & kLaneMask) | term;
     const unsigned int mbits = rbits & (vmask | term);
     // row open before the lane (the row of entry eb - 1; entry 0 of a word
     // already counts in word_row); the terminator counts as a row start
@@ -1118,16 +1142,17 @@ __device__ __forceinline__ void attract_flat_body(const float2* __restrict__ y_i
                          (sh == 0 ? static_cast<int>(cur.word & 1ull) : 0);
     float hx, hy, tx, ty;
     int hrow;  // row closed by the lane's first row start (-1: none)
-    FlatCols next;
-    if (__ballot(__popc(rbits & 0xfeu) > 1) == 0ull)
-      flat_entries<true>(y_in, col, val, row_start, word_row, attr, cur, &next, base, lane,
-                         open_row, rbits, mbits, vmask, n, mxf, myf, exaggeration, &hx, &hy, &tx,
-                         &ty, &hrow);
+    FlatCols<LANE> next;
+    if (__ballot(__popc(rbits & (kLaneMask - 1u)) > 1) == 0ull)
+      flat_entries<LANE, PREFETCH, true>(y_in, col, val, row_start, word_row, attr, cur, &next,
+                                         base, lane, open_row, rbits, mbits, vmask, n, mxf, myf,
+                                         exaggeration, &hx, &hy, &tx, &ty, &hrow);
     else
-      flat_entries<false>(y_in, col, val, row_start, word_row, attr, cur, &next, base, lane,
-                          open_row, rbits, mbits, vmask, n, mxf, myf, exaggeration, &hx, &hy,
-                          &tx, &ty, &hrow);
-    cur = next;
+      flat_entries<LANE, PREFETCH, false>(y_in, col, val, row_start, word_row, attr, cur, &next,
+                                          base, lane, open_row, rbits, mbits, vmask, n, mxf,
+                                          myf, exaggeration, &hx, &hy, &tx, &ty, &hrow);
+    if constexpr (PREFETCH) cur = next;
+    else cur = flat_cols<LANE>(col, row_start, word_row, base + kStep, lane);
     const bool seen = hrow != -1;
     // segmented scan of the open-row pieces: a lane with a row start passes on
     // its tail, any other lane adds its whole sum to the open row
@@ -1160,6 +1185,10 @@ __device__ __forceinline__ void attract_flat_body(const float2* __restrict__ y_i
   }
 }
 
+// variants of the entry-balanced attraction (entries per lane, prefetch)
+enum FlatVariant { kFlat8Prefetch = 1, kFlat8 = 2, kFlat4Prefetch = 3, kFlat4 = 4 };
+
+template <int LANE, bool PREFETCH>
 __global__ __launch_bounds__(kThreads) void attract_flat(
     const float2* __restrict__ y_in, const int* __restrict__ row_ptr, const int* __restrict__ col,
     const float* __restrict__ val, const unsigned long long* __restrict__ row_start,
@@ -1169,8 +1198,8 @@ __global__ __launch_bounds__(kThreads) void attract_flat(
   const int wave = xcd_remap(blockIdx.x, gridDim.x) * (kThreads / 64) +
                    static_cast<int>(threadIdx.x >> 6);
   if (wave >= n_waves) return;
-  attract_flat_body(y_in, row_ptr, col, val, row_start, word_row, wave_row, st, exaggeration,
-                    attr, n, wave);
+  attract_flat_body<LANE, PREFETCH>(y_in, row_ptr, col, val, row_start, word_row, wave_row, st,
+                                    exaggeration, attr, n, wave);
 }
 
 // set-up of the entry-balanced attraction: row-start bits and empty-row count
