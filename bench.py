@@ -45,6 +45,10 @@ def parse():
                     help="rocprofv3-derived HBM bytes per launch of the dominant kernel")
     ap.add_argument("--quality", action="store_true",
                     help="also report trustworthiness / kNN preservation on a 2000-point sample")
+    ap.add_argument("--batched-jobs", type=int, default=256,
+                    help="jobs of the batched workload (0: skip it)")
+    ap.add_argument("--served-requests", type=int, default=48,
+                    help="requests of the served workload (0: skip it)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run a plain device copy of this many MiB (3 times) so hardware byte "
                          "counters can be scaled against a known transfer")
@@ -53,6 +57,132 @@ def parse():
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+SCHEDULE = dict(early=250, normal=500, early_exaggeration=12.0, momentum=0.8)
+
+
+def run_plan_alone(fe, ki, kd, n, k, y0):
+    """One job through the plain single-job path: P build, 250 + 500
+    iterations, layout read back."""
+    plan = fe.TsnePlan(ki, kd, perplexity=30.0, grid_size=0, Y_init=y0, device=True, n=n, k=k,
+                       index_offset=0)
+    lr = n / SCHEDULE["early_exaggeration"]
+    plan.run(SCHEDULE["early"], SCHEDULE["early_exaggeration"], SCHEDULE["momentum"], lr)
+    plan.run(SCHEDULE["normal"], 1.0, SCHEDULE["momentum"], lr)
+    out = plan.layout()
+    plan.close()
+    return out
+
+
+def bench_batched(fe, workloads, count, k=90):
+    """Batched t-SNE: `count` jobs of 2k-100k points (log-uniform), each a
+    5-50-cluster mixture in 30-50 dimensions, k = 90, perplexity 30, 750
+    iterations.  Times all jobs run one after another through the plain
+    single-job path, then all jobs admitted into one job-table batch (graph-
+    captured iterations), from KNN resident on the device to every layout on
+    the host."""
+    import torch
+
+    specs = workloads.batch_job_specs(count, seed=301)
+    jobs = []
+    t0 = time.perf_counter()
+    for sp in specs:
+        ki, kd = workloads.job_knn_device(sp, k)
+        y0 = workloads.centred_normal_init(sp["n"], seed=sp["seed"] + 1)
+        jobs.append((ki, kd, sp["n"], y0))
+    torch.cuda.synchronize()
+    total_points = sum(j[2] for j in jobs)
+    log(f"[batched] {count} jobs, {total_points} points, KNN in {time.perf_counter() - t0:.1f}s")
+
+    t0 = time.perf_counter()
+    alone = [run_plan_alone(fe, ki, kd, n, k, y0) for ki, kd, n, y0 in jobs]
+    t_seq = time.perf_counter() - t0
+    log(f"[batched] sequential plain path: {t_seq:.2f}s")
+
+    from fastembedr_amd.batch import TsneBatch
+
+    t0 = time.perf_counter()
+    b = TsneBatch(max_jobs=count)
+    slots = [b.add(ki, kd, perplexity=30.0, Y_init=y0, device=True, n=n, k=k, index_offset=0,
+                   early_exaggeration_iter=SCHEDULE["early"], n_iter=SCHEDULE["normal"],
+                   early_exaggeration=SCHEDULE["early_exaggeration"],
+                   initial_momentum=SCHEDULE["momentum"], final_momentum=SCHEDULE["momentum"])
+             for ki, kd, n, y0 in jobs]
+    t_admit = time.perf_counter() - t0
+    b.run_to_completion(chunk=50)
+    outs = [b.layout(s) for s in slots]
+    t_batch = time.perf_counter() - t0
+    b.close()
+    same = sum(int(np.array_equal(a, o)) for a, o in zip(alone, outs))
+    log(f"[batched] batched: {t_batch:.2f}s (admission {t_admit:.2f}s); "
+        f"{same}/{count} layouts bit-identical to the plain path")
+    return {"workload": f"{count} t-SNE jobs, n log-uniform in [2k, 100k] ({total_points} points), "
+                        f"5-50-cluster mixtures in 30-50 dims, k={k}, perplexity 30, 750 iterations",
+            "jobs": count, "points": total_points,
+            "batched_jobs_per_s": round(count / t_batch, 3),
+            "sequential_jobs_per_s": round(count / t_seq, 3),
+            "speedup_vs_sequential": round(t_seq / t_batch, 3),
+            "batched_s": round(t_batch, 3), "sequential_s": round(t_seq, 3),
+            "identical_to_plain": same}
+
+
+def bench_served(fe, workloads, n_requests, k=90):
+    """Served stream: a 500k-point reference embedding resident on the device;
+    Poisson arrivals of transform queries (1k-50k points, 50-d, k = 90 to the
+    reference) mixed with 10k-200k-point t-SNE jobs; admission by HBM budget;
+    latency is arrival to result on the host."""
+    import torch
+
+    from fastembedr_amd import serve
+
+    rng = np.random.default_rng(401)
+    n_ref = 500_000
+    xr, lr_, centres = workloads.gaussian_mixture(n_ref, 50, 50, seed=402, dirichlet=True)
+    weights = np.bincount(lr_, minlength=len(centres)) / n_ref
+    ki, kd = workloads.mixture_knn_device(xr, lr_, centres, k)
+    ref_layout = run_plan_alone(fe, ki, kd, n_ref, k, workloads.centred_normal_init(n_ref, 403))
+    del ki, kd
+    torch.cuda.empty_cache()
+    reqs = []
+    t = 0.0
+    rate = 8.0  # requests per second
+    for rid in range(n_requests):
+        t += rng.exponential(1.0 / rate)
+        if rng.random() < 0.25:
+            sp = dict(n=int(workloads.log_uniform_sizes(1, 10_000, 200_000, rng)[0]),
+                      d=int(rng.integers(30, 51)), clusters=int(rng.integers(5, 51)),
+                      seed=int(rng.integers(1 << 30)))
+            ji, jd = workloads.job_knn_device(sp, k)
+            payload = dict(indices=ji, distances=jd, device=True, n=sp["n"], k=k, index_offset=0,
+                           perplexity=30.0, Y_init=workloads.centred_normal_init(sp["n"], sp["seed"]),
+                           early_exaggeration_iter=SCHEDULE["early"], n_iter=SCHEDULE["normal"])
+            reqs.append(serve.Request(rid, "embed", t, payload))
+        else:
+            m = int(workloads.log_uniform_sizes(1, 1_000, 50_000, rng)[0])
+            xq, lq = workloads.sample_mixture(m, centres, weights, rng)
+            qi, qd = workloads.knn_to_reference(xq, lq, xr, lr_, centres, k)
+            reqs.append(serve.Request(rid, "transform", t, dict(indices=qi, distances=qd)))
+    torch.cuda.synchronize()
+    n_embed = sum(r.kind == "embed" for r in reqs)
+    log(f"[served] {n_requests} requests ({n_embed} embed), trace span {t:.2f}s")
+    free, _ = torch.cuda.mem_get_info()
+    budget = int(0.5 * free)
+    srv = serve.Server(reference_layout=ref_layout, max_jobs=64, hbm_budget_bytes=budget, tick=25)
+    stats = srv.run(reqs, realtime=True)
+    srv.close()
+    summ = stats.summary()
+    lat_e = [stats.latencies[r.rid] for r in reqs if r.kind == "embed"]
+    lat_t = [stats.latencies[r.rid] for r in reqs if r.kind == "transform"]
+    log(f"[served] {summ}")
+    return {"workload": f"{n_requests} Poisson requests at {rate}/s: transform_tsne of 1k-50k "
+                        f"queries against a resident {n_ref}-point reference (k={k}) and "
+                        f"10k-200k-point t-SNE jobs (750 iterations), HBM budget {budget >> 20} MiB",
+            **summ,
+            "embed_p50_ms": round(1e3 * float(np.percentile(lat_e, 50)), 3) if lat_e else None,
+            "embed_p99_ms": round(1e3 * float(np.percentile(lat_e, 99)), 3) if lat_e else None,
+            "transform_p50_ms": round(1e3 * float(np.percentile(lat_t, 50)), 3) if lat_t else None,
+            "transform_p99_ms": round(1e3 * float(np.percentile(lat_t, 99)), 3) if lat_t else None}
 
 
 def main():
@@ -168,6 +298,11 @@ def main():
         m = oracle.structure_metrics(x[s].astype(np.float64), layout[s].astype(np.float64), 15)
         quality = {"trustworthiness_k15": float(m[0]), "knn_preservation_k15": float(m[2])}
     plan.close()
+    torch.cuda.empty_cache()
+
+    batched = bench_batched(fe, workloads, args.batched_jobs) if args.batched_jobs > 0 else None
+    torch.cuda.empty_cache()
+    served = bench_served(fe, workloads, args.served_requests) if args.served_requests > 0 else None
 
     out = {
         "metric": "t-SNE iters/s @1M pts 2D (frac of HBM roofline); batched jobs/s + p99 latency",
@@ -202,6 +337,8 @@ def main():
     }
     if quality:
         out["quality"] = quality
+    out["batched"] = batched
+    out["served"] = served
     print(json.dumps(out), flush=True)
 
 

@@ -16,7 +16,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import _fp, _ip, check, ptr
-from .tsne import _column_major, _index_offset, _knn_arrays
+from .tsne import _column_major, _device_inputs_ready, _index_offset, _knn_arrays
 
 
 class JobStruct(ctypes.Structure):
@@ -53,23 +53,35 @@ class TsneBatch:
     def add(self, indices, distances, perplexity=30.0, Y_init=None, seed=4, grid_size=0,
             early_exaggeration_iter=250, n_iter=500, early_exaggeration=12.0, exaggeration=1.0,
             learning_rate=None, initial_momentum=0.8, final_momentum=0.8, min_gain=0.01,
-            max_step_norm=5.0):
-        """Add a job from host KNN arrays (n x k); returns its slot.  The
-        learning rate defaults to n / early_exaggeration for both phases (the
-        reference's auto_config policy)."""
-        idx, dist = _knn_arrays(indices, distances)
-        n, k = idx.shape
-        ci, cd = _column_major(idx, dist, np.float32)
+            max_step_norm=5.0, device=False, n=None, k=None, index_offset=None):
+        """Add a job; returns its slot.  ``indices``/``distances`` are host
+        n x k arrays or, with ``device=True``, objects exposing ``data_ptr()``
+        to column-major device memory (a k x n torch tensor), as for
+        ``TsnePlan``.  The learning rate defaults to n / early_exaggeration for
+        both phases (the reference's auto_config policy)."""
+        if device:
+            n, k = int(n), int(k)
+            off = 0 if index_offset is None else int(index_offset)
+            _device_inputs_ready(indices, distances)
+            ip, dp = int(indices.data_ptr()), int(distances.data_ptr())
+        else:
+            idx, dist = _knn_arrays(indices, distances)
+            n, k = idx.shape
+            ci, cd = _column_major(idx, dist, np.float32)
+            off = _index_offset(idx) if index_offset is None else int(index_offset)
+            ip, dp = ci.ctypes.data, cd.ctypes.data
         init = None if Y_init is None else np.ascontiguousarray(Y_init, dtype=np.float32)
+        if init is not None and init.shape != (n, 2):
+            raise ValueError("`Y_init` has the wrong shape.")
         lr = n / early_exaggeration if learning_rate is None else float(learning_rate)
-        j = JobStruct(ci.ctypes.data, cd.ctypes.data, n, k, _index_offset(idx), int(grid_size),
+        j = JobStruct(ip, dp, n, k, off, int(grid_size),
                       float(perplexity), init.ctypes.data if init is not None else None,
                       seed & 0xFFFFFFFF, int(early_exaggeration_iter), int(n_iter),
                       float(early_exaggeration), float(exaggeration), lr, lr,
                       float(initial_momentum), float(final_momentum), float(min_gain),
                       float(max_step_norm))
         slot = ctypes.c_int(-1)
-        rc = _lib.lib().fastembedr_hip_tsne_batch_add(self._h, ctypes.byref(j), 0,
+        rc = _lib.lib().fastembedr_hip_tsne_batch_add(self._h, ctypes.byref(j), int(bool(device)),
                                                       ctypes.byref(slot))
         if rc == 2:
             raise NotAdmitted(_lib.last_error())

@@ -367,7 +367,6 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
   sp.add<int>(nbins + 1);                 // bin_start
   sp.add<int>(nbins);                     // bin_cursor
   sp.add<int>(ntiles + 1);                // tile_items
-  sp.add<unsigned int>(ntiles);           // tile_done
   sp.add<float2>(n);                      // binned
   sp.add<unsigned long long>(3 * gg);     // wide
   sp.add<float4>(gg);                     // sgrid
@@ -391,7 +390,6 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
   splat.bin_start = state.take<int>(nbins + 1);
   splat.bin_cursor = state.take<int>(nbins);
   splat.tile_items = state.take<int>(ntiles + 1);
-  splat.tile_done = state.take<unsigned int>(ntiles);
   splat.binned = state.take<float2>(n);
   splat.wide = state.take<unsigned long long>(3 * gg);
   splat.sgrid = state.take<float4>(gg);
@@ -428,14 +426,15 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
   FE_CHECK(hipMemcpyAsync(gains, ones.data(), n * sizeof(float2), hipMemcpyHostToDevice, s));
   FE_CHECK(hipMemsetAsync(upd, 0, n * sizeof(float2), s));
   FE_CHECK(hipMemsetAsync(splat.bin_count, 0, nbins * sizeof(int), s));
-  FE_CHECK(hipMemsetAsync(splat.tile_done, 0, ntiles * sizeof(unsigned int), s));
   FE_CHECK(hipMemsetAsync(splat.wide, 0, 3 * gg * sizeof(unsigned long long), s));
   FE_CHECK(hipMemsetAsync(st, 0, sizeof(TsneStats), s));
-  layout_stats_kernel<<<stat_blocks, kThreads, 0, s>>>(y[0], lpart, st, n, g, centre ? 1 : 0);
+  layout_stats_kernel<<<stat_blocks, kThreads, 0, s>>>(y[0], lpart, n);
   FE_LAUNCH_CHECK("layout_stats_kernel");
+  layout_finalize<<<1, kThreads, 0, s>>>(lpart, stat_blocks, st, n, g, centre ? 1 : 0);
+  FE_LAUNCH_CHECK("layout_finalize");
   if (f > 1024) {
-    // conv_cols uses 5*F*8 bytes of dynamic LDS
-    const int bytes = 5 * f * static_cast<int>(sizeof(float2));
+    // conv_cols uses 4*F*8 bytes of dynamic LDS
+    const int bytes = 4 * f * static_cast<int>(sizeof(float2));
     hipError_t rc = hipSuccess;
     if (f == 2048)
       rc = hipFuncSetAttribute(reinterpret_cast<const void*>(&conv_cols<2048>),
@@ -451,13 +450,13 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
 template <int F>
 static int launch_conv(TsnePlan& p, hipStream_t s) {
   constexpr int G = F / 2;
-  conv_rows_fwd<F><<<G + G / 2, kThreads, 0, s>>>(p.splat.sgrid, p.st, p.tw, p.z1, p.z2, p.kt);
+  conv_rows_fwd<F><<<G + G / 2, kThreads, 0, s>>>(p.splat, p.st, p.tw, p.z1, p.z2, p.kt);
   FE_LAUNCH_CHECK("conv_rows_fwd");
-  conv_cols<F><<<conv_cols_blocks(G), kThreads, 5 * F * sizeof(float2), s>>>(p.z1, p.z2, p.kt, p.tw,
+  conv_cols<F><<<conv_cols_blocks(G), kThreads, 4 * F * sizeof(float2), s>>>(p.z1, p.z2, p.kt, p.tw,
                                                                              p.o1, p.o2,
-                                                               p.zpart, p.st);
+                                                                             p.zpart);
   FE_LAUNCH_CHECK("conv_cols");
-  conv_rows_inv<F><<<G, kThreads, 0, s>>>(p.o1, p.o2, p.tw, p.st, p.field);
+  conv_rows_inv<F><<<G, kThreads, 0, s>>>(p.o1, p.o2, p.tw, p.zpart, p.st, p.field);
   FE_LAUNCH_CHECK("conv_rows_inv");
   return 0;
 }
@@ -466,6 +465,8 @@ static int launch_splat(TsnePlan& p, hipStream_t s) {
   const float2* yin = p.y[p.cur];
   splat_count<<<p.splat_blocks, kThreads, 0, s>>>(yin, p.st, p.splat, p.n, p.g);
   FE_LAUNCH_CHECK("splat_count");
+  splat_prep<<<1, kThreads, 0, s>>>(p.splat, p.g);
+  FE_LAUNCH_CHECK("splat_prep");
   splat_scatter<<<p.splat_blocks, kThreads, 0, s>>>(yin, p.st, p.splat, p.n, p.g);
   FE_LAUNCH_CHECK("splat_scatter");
   splat_tiles<<<p.item_blocks, kThreads, 0, s>>>(p.st, p.splat, p.g);
@@ -527,6 +528,10 @@ static int launch_update(TsnePlan& p, const StepParams& sp, hipStream_t s) {
                                                          p.upd, p.diag, p.field, p.st, p.lpart,
                                                          p.n, p.g, sp, p.diag + p.n);
   FE_LAUNCH_CHECK("update_kernel");
+  if constexpr (MODE == 0) {
+    layout_finalize<<<1, kThreads, 0, s>>>(p.lpart, p.stat_blocks, p.st, p.n, p.g, 1);
+    FE_LAUNCH_CHECK("layout_finalize");
+  }
   return 0;
 }
 
@@ -618,12 +623,12 @@ int TsnePlan::profile(int iters, const StepParams& sp, double* ms) {
 #define FE_PROF_CONV(FF)                                                                      \
   case FF: {                                                                                  \
     constexpr int G = FF / 2;                                                                 \
-    conv_rows_fwd<FF><<<G + G / 2, kThreads, 0, s>>>(splat.sgrid, st, tw, z1, z2, kt);        \
+    conv_rows_fwd<FF><<<G + G / 2, kThreads, 0, s>>>(splat, st, tw, z1, z2, kt);              \
     (void)hipEventRecord(e[2], s);                                                            \
-    conv_cols<FF><<<conv_cols_blocks(G), kThreads, 5 * FF * sizeof(float2), s>>>(            \
-        z1, z2, kt, tw, o1, o2, zpart, st);                                                   \
+    conv_cols<FF><<<conv_cols_blocks(G), kThreads, 4 * FF * sizeof(float2), s>>>(            \
+        z1, z2, kt, tw, o1, o2, zpart);                                                       \
     (void)hipEventRecord(e[3], s);                                                            \
-    conv_rows_inv<FF><<<G, kThreads, 0, s>>>(o1, o2, tw, st, field);                          \
+    conv_rows_inv<FF><<<G, kThreads, 0, s>>>(o1, o2, tw, zpart, st, field);                   \
     break;                                                                                    \
   }
     switch (f) {
@@ -750,14 +755,15 @@ class TsneBatch {
   int tab_cap = 0;
   bool dirty = true;
   struct Launch {
-    int kind;   // 0 splat_count, 6 splat_scatter, 7 splat_tiles, 1 rows_fwd, 2 cols,
-                // 3 rows_inv, 4 attract, 5 update
+    int kind;   // 0 splat_count, 8 splat_prep, 6 splat_scatter, 7 splat_tiles,
+                // 1 rows_fwd, 2 cols, 3 rows_inv, 4 attract, 5 update, 9 finalize
     int param;  // F or TPR
     int off;    // offset of slot[] in d_tab; first[] follows it
     int m;      // entries
     int blocks;
   };
   std::vector<Launch> launches;
+  std::unique_ptr<GraphExec> graph;  // captured iterations of the current table
 
   int init(int max_jobs_, long long budget_) {
     max_jobs = max_jobs_;
@@ -767,7 +773,7 @@ class TsneBatch {
     slot_bytes.assign(max_jobs, 0);
     desc.resize(max_jobs);
     std::memset(desc.data(), 0, desc.size() * sizeof(JobDesc));
-    tab_cap = 2 * (8 * max_jobs + 48) + 64;  // every job is in 8 launches
+    tab_cap = 2 * (10 * max_jobs + 48) + 64;  // every job is in 10 launches
     ArenaPlan p;
     p.add<JobDesc>(max_jobs);
     p.add<int>(tab_cap);
@@ -832,6 +838,7 @@ class TsneBatch {
     d.attr = p.diag;
     d.n = p.n;
     d.g = p.g;
+    d.upd_blocks = p.stat_blocks;
     d.cur = p.cur;
     d.iter = 0;
     d.early_iter = j.early_exaggeration_iter;
@@ -887,13 +894,15 @@ class TsneBatch {
       L.blocks = acc;
       launches.push_back(L);
     };
-    std::vector<std::pair<int, int>> splat, tiles;
+    std::vector<std::pair<int, int>> splat, tiles, single;
     for (int s = 0; s < max_jobs; ++s)
       if (slots[s]) {
         splat.push_back({s, slots[s]->splat_blocks});
         tiles.push_back({s, slots[s]->item_blocks});
+        single.push_back({s, 1});
       }
     add_launch(0, 0, splat);
+    add_launch(8, 0, single);
     add_launch(6, 0, splat);
     add_launch(7, 0, tiles);
     for (int kind = 1; kind <= 3; ++kind) {
@@ -925,6 +934,7 @@ class TsneBatch {
     for (int s = 0; s < max_jobs; ++s)
       if (slots[s]) upd.push_back({s, slots[s]->stat_blocks});
     add_launch(5, 0, upd);
+    add_launch(9, 0, single);
     if (static_cast<int>(tab.size()) > tab_cap) return fail("batch table overflow");
     if (!tab.empty())
       FE_CHECK(hipMemcpy(d_tab, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -947,12 +957,18 @@ class TsneBatch {
         case 7:
           batch_splat_tiles<<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
           break;
+        case 8:
+          batch_splat_prep<<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
+          break;
+        case 9:
+          batch_finalize<<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
+          break;
 #define FE_B_CONV(FF)                                                                          \
   case FF:                                                                                     \
     if (L.kind == 1)                                                                           \
       batch_rows_fwd<FF><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);                   \
     else if (L.kind == 2)                                                                      \
-      batch_cols<FF><<<L.blocks, kThreads, 5 * FF * sizeof(float2), s>>>(d_desc, sl, fi, L.m); \
+      batch_cols<FF><<<L.blocks, kThreads, 4 * FF * sizeof(float2), s>>>(d_desc, sl, fi, L.m); \
     else                                                                                       \
       batch_rows_inv<FF><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);                   \
     break;
@@ -1005,16 +1021,47 @@ class TsneBatch {
   }
 
   int step(int iters) {
-    if (dirty && rebuild()) return 1;
+    if (dirty) {
+      graph.reset();
+      if (rebuild()) return 1;
+    }
     if (launches.empty()) return 0;
     static bool attr_done = false;
     if (!attr_done) {
       FE_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&batch_cols<2048>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   5 * 2048 * static_cast<int>(sizeof(float2))));
+                                   4 * 2048 * static_cast<int>(sizeof(float2))));
       attr_done = true;
     }
-    for (int it = 0; it < iters; ++it)
+    // Captured chunks of kChunk iterations replay the current launch table;
+    // a job that finishes inside a chunk goes inactive on the device and its
+    // blocks exit, so the graph stays valid until a job joins or leaves.
+    constexpr int kChunk = 25;
+    hipStream_t s = stream.get();
+    int left = iters;
+    if (graph_capture_enabled() && left >= kChunk) {
+      if (!graph) {
+        hipGraph_t g = nullptr;
+        bool ok = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        for (int it = 0; ok && it < kChunk; ++it) ok = enqueue_iteration() == 0;
+        const hipError_t end = hipStreamEndCapture(s, &g);
+        if (ok && end == hipSuccess && g) {
+          auto exec = std::make_unique<GraphExec>();
+          if (exec->instantiate(g) == 0) graph = std::move(exec);
+        } else if (g) {
+          (void)hipGraphDestroy(g);
+        }
+        (void)hipGetLastError();
+        clear_error();
+      }
+      if (graph) {
+        while (left >= kChunk) {
+          FE_CHECK(hipGraphLaunch(graph->get(), s));
+          left -= kChunk;
+        }
+      }
+    }
+    for (; left > 0; --left)
       if (enqueue_iteration()) return 1;
     return 0;
   }

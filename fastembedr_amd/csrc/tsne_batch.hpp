@@ -5,12 +5,12 @@
 // would use), finds its descriptor by a binary search of blockIdx.x over the
 // prefix table, and runs the single-job stage body with its job-local block
 // index.  Jobs therefore give bit-identical results alone and in a batch,
-// and a batch of S jobs costs 4 + 3 * (#grid sizes) + (#lane widths)
-// launches per iteration instead of 8 * S.  The descriptor's iteration counter, the
-// phase constants and the layout-buffer parity live on the device: the block
-// that finalises a job's layout statistics advances them, and a job whose
-// schedule is complete goes inactive, so captured sequences of iterations
-// can run jobs with different schedules.
+// and a batch of S jobs costs 6 + 3 * (#grid sizes) + (#attraction variants)
+// launches per iteration instead of 10 * S.  The descriptor's iteration
+// counter, the phase constants and the layout-buffer parity live on the
+// device: batch_finalize advances them after the job's update, and a job
+// whose schedule is complete goes inactive, so captured sequences of
+// iterations can run jobs with different schedules.
 #pragma once
 
 #include "tsne_device.hpp"
@@ -41,7 +41,8 @@ struct JobDesc {
   double* zpart;
   float2* attr;
   int n, g;
-  int cur, iter, total_iter, early_iter, active, pad;
+  int upd_blocks;  // blocks of the job's update launch (layout partials)
+  int cur, iter, total_iter, early_iter, active;
   float early_exag, exag, lr_early, lr_normal, mom0, mom1, min_gain, max_step;
 };
 
@@ -65,6 +66,16 @@ __global__ __launch_bounds__(kThreads) void batch_splat_count(JobDesc* jobs,
   if (!J->active) return;
   splat_count_body(J->y[J->cur], J->st, J->splat, J->n, J->g, blockIdx.x - first[e],
                    first[e + 1] - first[e]);
+}
+
+// one block per job
+__global__ __launch_bounds__(kThreads) void batch_splat_prep(JobDesc* jobs,
+                                                             const int* __restrict__ slot,
+                                                             const int* __restrict__ first, int m) {
+  const int e = find_entry(first, m);
+  JobDesc* J = jobs + slot[e];
+  if (!J->active) return;
+  splat_prep_body(J->splat, J->g);
 }
 
 __global__ __launch_bounds__(kThreads) void batch_splat_scatter(JobDesc* jobs,
@@ -93,7 +104,7 @@ __global__ __launch_bounds__(kThreads) void batch_rows_fwd(JobDesc* jobs, const 
   const int e = find_entry(first, m);
   JobDesc* J = jobs + slot[e];
   if (!J->active) return;
-  conv_rows_fwd_body<F>(J->splat.sgrid, J->st, J->tw, J->z1, J->z2, J->kt, blockIdx.x - first[e]);
+  conv_rows_fwd_body<F>(J->splat, J->st, J->tw, J->z1, J->z2, J->kt, blockIdx.x - first[e]);
 }
 
 // blocks of a job's range follow the XCD-aware column order of conv_cols;
@@ -109,7 +120,7 @@ __global__ __launch_bounds__(kThreads) void batch_cols(JobDesc* jobs, const int*
   if (!J->active) return;
   const int c = xcd_column(blockIdx.x - first[e], G);
   if (c > G) return;
-  conv_cols_body<F>(J->z1, J->z2, J->kt, J->tw, J->o1, J->o2, J->zpart, J->st, lds, c, G + 1);
+  conv_cols_body<F>(J->z1, J->z2, J->kt, J->tw, J->o1, J->o2, J->zpart, lds, c);
 }
 
 template <int F>
@@ -118,7 +129,7 @@ __global__ __launch_bounds__(kThreads) void batch_rows_inv(JobDesc* jobs, const 
   const int e = find_entry(first, m);
   JobDesc* J = jobs + slot[e];
   if (!J->active) return;
-  conv_rows_inv_body<F>(J->o1, J->o2, J->tw, J->st, J->field, blockIdx.x - first[e]);
+  conv_rows_inv_body<F>(J->o1, J->o2, J->tw, J->zpart, J->st, J->field, blockIdx.x - first[e]);
 }
 
 template <int LANE, bool PREFETCH>
@@ -157,13 +168,24 @@ __global__ __launch_bounds__(kThreads) void batch_update(JobDesc* jobs, const in
   const bool early = it < J->early_iter;
   const StepParams sp{early ? J->early_exag : J->exag, early ? J->lr_early : J->lr_normal,
                       early ? J->mom0 : J->mom1, J->min_gain, J->max_step};
-  const bool last = update_body<0>(J->y[cur], J->y[cur ^ 1], J->gains, J->upd, J->attr, J->field,
-                                   J->st, J->lpart, J->n, J->g, sp, nullptr,
-                                   blockIdx.x - first[e], first[e + 1] - first[e]);
-  if (last) {
-    J->cur = cur ^ 1;
-    J->iter = it + 1;
-    if (it + 1 >= J->total_iter) J->active = 0;
+  update_body<0>(J->y[cur], J->y[cur ^ 1], J->gains, J->upd, J->attr, J->field, J->st, J->lpart,
+                 J->n, J->g, sp, nullptr, blockIdx.x - first[e], first[e + 1] - first[e]);
+}
+
+// one block per job: the next iteration's geometry from the update's layout
+// partials, then the job's iteration counter, buffer parity and activity
+__global__ __launch_bounds__(kThreads) void batch_finalize(JobDesc* jobs,
+                                                           const int* __restrict__ slot,
+                                                           const int* __restrict__ first, int m) {
+  const int e = find_entry(first, m);
+  JobDesc* J = jobs + slot[e];
+  if (!J->active) return;
+  fold_layout(J->lpart, J->upd_blocks, J->st, J->n, J->g, true);
+  if (threadIdx.x == 0) {
+    const int it = J->iter + 1;
+    J->cur ^= 1;
+    J->iter = it;
+    if (it >= J->total_iter) J->active = 0;
   }
 }
 

@@ -51,9 +51,6 @@ struct TsneStats {
   // the mass channel, and is carried as mass*x / x_scale so both parts have
   // the magnitude of the mass (the Hermitian split then loses no digits)
   double x_scale;
-  unsigned int done_cols;  // last-block counters (reset by the last block)
-  unsigned int done_stats;
-  unsigned int done_bins;
 };
 
 struct LayoutPartial {
@@ -156,14 +153,25 @@ __device__ inline void finalize_geometry(TsneStats* st, double sx, double sy, fl
   st->x_scale = rx > 0.0 && isfinite(rx) ? ldexp(1.0, ilogb(rx) + 1) : 1.0;
 }
 
-// Block reduction of LayoutPartial then last-block finalize over the nblk
-// blocks that work on this layout (`partials` has nblk entries, this block is
-// blk).  Returns true in thread 0 of the block that finalised.
-__device__ inline bool reduce_layout_stats(LayoutPartial mine, LayoutPartial* partials,
-                                           TsneStats* st, int n, int g, bool centre, int blk,
-                                           int nblk) {
+// Cross-block results are combined at kernel boundaries: a block writes its
+// partial and a later launch folds the partials.  (A last-block reduction
+// inside one launch would need a device-scope release fence per block, which
+// on this GPU writes back the whole L2 of the block's XCD.)
+
+__device__ __forceinline__ LayoutPartial combine(LayoutPartial a, const LayoutPartial& b) {
+  a.sx += b.sx;
+  a.sy += b.sy;
+  a.mnx = fminf(a.mnx, b.mnx);
+  a.mxx = fmaxf(a.mxx, b.mxx);
+  a.mny = fminf(a.mny, b.mny);
+  a.mxy = fmaxf(a.mxy, b.mxy);
+  return a;
+}
+
+// Block reduction of LayoutPartial (fixed wave/LDS tree); thread 0 returns
+// the block's value.
+__device__ inline LayoutPartial block_layout(LayoutPartial mine) {
   __shared__ LayoutPartial red[kThreads / 64];
-  __shared__ bool last;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   mine.sx = wave_sum(mine.sx);
   mine.sy = wave_sum(mine.sy);
@@ -171,64 +179,29 @@ __device__ inline bool reduce_layout_stats(LayoutPartial mine, LayoutPartial* pa
   mine.mxx = static_cast<float>(wave_max(mine.mxx));
   mine.mny = static_cast<float>(wave_min(mine.mny));
   mine.mxy = static_cast<float>(wave_max(mine.mxy));
+  __syncthreads();
   if (lane == 0) red[wid] = mine;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    LayoutPartial b = red[0];
-    for (int w = 1; w < kThreads / 64; ++w) {
-      b.sx += red[w].sx;
-      b.sy += red[w].sy;
-      b.mnx = fminf(b.mnx, red[w].mnx);
-      b.mxx = fmaxf(b.mxx, red[w].mxx);
-      b.mny = fminf(b.mny, red[w].mny);
-      b.mxy = fmaxf(b.mxy, red[w].mxy);
-    }
-    partials[blk] = b;
-    __threadfence();
-    const unsigned prev = atomicAdd(&st->done_stats, 1u);
-    last = (prev == static_cast<unsigned>(nblk - 1));
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  // the whole last block folds the partials: thread t takes entries t,
-  // t + kThreads, ... in order, then the fixed wave/LDS tree above
-  {
-    const volatile LayoutPartial* vp = partials;
-    LayoutPartial a{0.0, 0.0, FLT_MAX, -FLT_MAX, FLT_MAX, -FLT_MAX};
-    for (int b = threadIdx.x; b < nblk; b += kThreads) {
-      a.sx += vp[b].sx;
-      a.sy += vp[b].sy;
-      a.mnx = fminf(a.mnx, vp[b].mnx);
-      a.mxx = fmaxf(a.mxx, vp[b].mxx);
-      a.mny = fminf(a.mny, vp[b].mny);
-      a.mxy = fmaxf(a.mxy, vp[b].mxy);
-    }
-    a.sx = wave_sum(a.sx);
-    a.sy = wave_sum(a.sy);
-    a.mnx = static_cast<float>(wave_min(a.mnx));
-    a.mxx = static_cast<float>(wave_max(a.mxx));
-    a.mny = static_cast<float>(wave_min(a.mny));
-    a.mxy = static_cast<float>(wave_max(a.mxy));
-    __syncthreads();  // red[] of the block partial above is consumed
-    if (lane == 0) red[wid] = a;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    LayoutPartial b = red[0];
-    for (int w = 1; w < kThreads / 64; ++w) {
-      b.sx += red[w].sx;
-      b.sy += red[w].sy;
-      b.mnx = fminf(b.mnx, red[w].mnx);
-      b.mxx = fmaxf(b.mxx, red[w].mxx);
-      b.mny = fminf(b.mny, red[w].mny);
-      b.mxy = fmaxf(b.mxy, red[w].mxy);
-    }
-    finalize_geometry(st, b.sx, b.sy, b.mnx, b.mxx, b.mny, b.mxy, n, g, centre);
-    st->done_stats = 0u;
-    return true;
-  }
-  return false;
+  LayoutPartial b = red[0];
+  for (int w = 1; w < kThreads / 64; ++w) b = combine(b, red[w]);
+  return b;
+}
+
+// One block folds the nblk partials (thread t takes t, t + kThreads, ...,
+// then the block tree) and sets the geometry of the next iteration.
+__device__ inline void fold_layout(const LayoutPartial* __restrict__ partials, int nblk,
+                                   TsneStats* st, int n, int g, bool centre) {
+  LayoutPartial a{0.0, 0.0, FLT_MAX, -FLT_MAX, FLT_MAX, -FLT_MAX};
+  for (int b = threadIdx.x; b < nblk; b += kThreads) a = combine(a, partials[b]);
+  const LayoutPartial t = block_layout(a);
+  if (threadIdx.x == 0)
+    finalize_geometry(st, t.sx, t.sy, t.mnx, t.mxx, t.mny, t.mxy, n, g, centre);
+}
+
+__global__ __launch_bounds__(kThreads) void layout_finalize(const LayoutPartial* __restrict__ partials,
+                                                            int nblk, TsneStats* st, int n, int g,
+                                                            int centre) {
+  fold_layout(partials, nblk, st, n, g, centre != 0);
 }
 
 // ----------------------------------------------------------------------------
@@ -236,9 +209,7 @@ __device__ inline bool reduce_layout_stats(LayoutPartial mine, LayoutPartial* pa
 // bounds pass of the reference).
 // ----------------------------------------------------------------------------
 __global__ __launch_bounds__(kThreads) void layout_stats_kernel(const float2* __restrict__ y,
-                                                                LayoutPartial* partials,
-                                                                TsneStats* st, int n, int g,
-                                                                int centre) {
+                                                                LayoutPartial* partials, int n) {
   LayoutPartial m{0.0, 0.0, FLT_MAX, -FLT_MAX, FLT_MAX, -FLT_MAX};
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
     const float2 p = y[i];
@@ -249,7 +220,8 @@ __global__ __launch_bounds__(kThreads) void layout_stats_kernel(const float2* __
     m.mny = fminf(m.mny, p.y);
     m.mxy = fmaxf(m.mxy, p.y);
   }
-  reduce_layout_stats(m, partials, st, n, g, centre != 0, blockIdx.x, gridDim.x);
+  const LayoutPartial b = block_layout(m);
+  if (threadIdx.x == 0) partials[blockIdx.x] = b;
 }
 
 // ----------------------------------------------------------------------------
@@ -272,21 +244,22 @@ __global__ __launch_bounds__(kThreads) void layout_stats_kernel(const float2* __
 // of bins: own tile [0, 4), left neighbour [1, 3), upper neighbour [2, 4),
 // upper-left neighbour [2, 3).
 //   a. splat_count    per-block LDS histogram of the bins, added to the
-//                     global counts; the last block turns the counts into bin
-//                     starts and cursors (clearing the counts) and cuts every
-//                     tile's points into work items of at most kSplatItem;
-//   b. splat_scatter  each point takes a slot in its bin (one global atomic
+//                     global counts;
+//   b. splat_prep     one block turns the counts into bin starts and cursors
+//                     (clearing the counts) and cuts every tile's points into
+//                     work items of at most kSplatItem;
+//   c. splat_scatter  each point takes a slot in its bin (one global atomic
 //                     per block and non-empty bin) and stores its centred
 //                     coordinates there;
-//   c. splat_tiles    one block per work item accumulates its points in an
+//   d. splat_tiles    one block per work item accumulates its points in an
 //                     LDS window of the tile (a lane sums consecutive points of
 //                     one cell in registers before touching LDS, which keeps
 //                     collapsed clusters cheap).  A tile of one item writes its
-//                     nodes once with plain stores; the items of a larger tile
-//                     add their windows to the 64-bit `wide` grid and the last
-//                     of them converts and clears it.  The output is the input
-//                     of the row FFTs:
-//                       sgrid(node) = (mass, mass*x / x_scale, mass*y, 0).
+//                     nodes once with plain stores as the input of the row FFTs,
+//                       sgrid(node) = (mass, mass*x / x_scale, mass*y, 0);
+//                     the items of a larger tile add their windows to the
+//                     64-bit `wide` grid, which the row FFT pass converts and
+//                     clears.
 // Slots inside a bin are taken in atomic order, but all sums are integer
 // sums, so the grid is the same on every run and matches a direct splat.
 // ----------------------------------------------------------------------------
@@ -302,7 +275,6 @@ struct SplatBuffers {
   int* bin_start;            // bins + 1
   int* bin_cursor;           // bins
   int* tile_items;           // tiles + 1: first work item of every tile
-  unsigned int* tile_done;   // tiles; zero between passes
   float2* binned;            // n centred points in bin order
   unsigned long long* wide;  // 3 x g*g sums of multi-item tiles; zero between passes
   float4* sgrid;             // g*g input of the row FFTs
@@ -373,12 +345,12 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* total) {
   return off + incl - v;
 }
 
-__device__ __forceinline__ void splat_count_body(const float2* __restrict__ y, TsneStats* st,
+__device__ __forceinline__ void splat_count_body(const float2* __restrict__ y,
+                                                 const TsneStats* __restrict__ st,
                                                  SplatBuffers sb, int n, int g, int blk,
                                                  int nblk) {
-  __shared__ int hist[kMaxBins + 1];
-  __shared__ bool last;
-  const int ts = tiles_side(g), tiles = ts * ts, nbins = 4 * tiles;
+  __shared__ int hist[kMaxBins];
+  const int ts = tiles_side(g), nbins = 4 * ts * ts;
   const float mx = st->mean_xf, my = st->mean_yf;
   const float lx = st->lower_x, ly = st->lower_y, inv = st->inv_spacing;
   for (int t = threadIdx.x; t < nbins; t += kThreads) hist[t] = 0;
@@ -393,71 +365,68 @@ __device__ __forceinline__ void splat_count_body(const float2* __restrict__ y, T
   __syncthreads();
   for (int t = threadIdx.x; t < nbins; t += kThreads)
     if (hist[t]) atomicAdd(sb.bin_count + t, hist[t]);
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) last = atomicAdd(&st->done_bins, 1u) == static_cast<unsigned>(nblk - 1);
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  // bin starts: thread t scans bins [t*per, t*per + per) after the block scan
-  // of the per-thread sums; the starts are also kept in hist (LDS)
+}
+
+// One block, after splat_count: bin starts and cursors (clearing the counts
+// for the next pass) and the work items of every tile.
+__device__ __forceinline__ void splat_prep_body(SplatBuffers sb, int g) {
+  __shared__ int bs[kMaxBins + 1];
+  const int ts = tiles_side(g), tiles = ts * ts, nbins = 4 * tiles;
   {
-    const volatile int* vc = sb.bin_count;
+    // thread t scans bins [t*per, t*per + per) after the block scan of the
+    // per-thread sums
     const int per = (nbins + kThreads - 1) / kThreads;
     const int b0 = threadIdx.x * per;
     int mine = 0;
     for (int q = 0; q < per; ++q)
-      if (b0 + q < nbins) mine += vc[b0 + q];
+      if (b0 + q < nbins) mine += sb.bin_count[b0 + q];
     int total;
     int run = block_exclusive_scan(mine, &total);
     for (int q = 0; q < per; ++q) {
       const int b = b0 + q;
       if (b < nbins) {
-        const int cnt = vc[b];
+        const int cnt = sb.bin_count[b];
         sb.bin_start[b] = run;
         sb.bin_cursor[b] = run;
         sb.bin_count[b] = 0;
-        hist[b] = run;
+        bs[b] = run;
         run += cnt;
       }
     }
     if (threadIdx.x == 0) {
       sb.bin_start[nbins] = total;
-      hist[nbins] = total;
-      st->done_bins = 0u;
+      bs[nbins] = total;
     }
     __syncthreads();
   }
-  // work items per tile
-  {
-    const int per = (tiles + kThreads - 1) / kThreads;
-    const int t0 = threadIdx.x * per;
-    int items[(kMaxTiles + kThreads - 1) / kThreads];
-    int mine = 0;
+  constexpr int kTilesPer = (kMaxTiles + kThreads - 1) / kThreads;
+  const int per = (tiles + kThreads - 1) / kThreads;
+  const int t0 = threadIdx.x * per;
+  int items[kTilesPer];
+  int mine = 0;
 #pragma unroll
-    for (int q = 0; q < (kMaxTiles + kThreads - 1) / kThreads; ++q) {
-      items[q] = 0;
-      const int t = t0 + q;
-      if (q < per && t < tiles) {
-        int b0[4], b1[4];
-        tile_ranges(hist, t, ts, b0, b1);
-        const int pts = (b1[0] - b0[0]) + (b1[1] - b0[1]) + (b1[2] - b0[2]) + (b1[3] - b0[3]);
-        items[q] = max(1, (pts + kSplatItem - 1) / kSplatItem);
-        mine += items[q];
-      }
+  for (int q = 0; q < kTilesPer; ++q) {
+    items[q] = 0;
+    const int t = t0 + q;
+    if (q < per && t < tiles) {
+      int b0[4], b1[4];
+      tile_ranges(bs, t, ts, b0, b1);
+      const int pts = (b1[0] - b0[0]) + (b1[1] - b0[1]) + (b1[2] - b0[2]) + (b1[3] - b0[3]);
+      items[q] = max(1, (pts + kSplatItem - 1) / kSplatItem);
+      mine += items[q];
     }
-    int total;
-    int run = block_exclusive_scan(mine, &total);
-#pragma unroll
-    for (int q = 0; q < (kMaxTiles + kThreads - 1) / kThreads; ++q) {
-      const int t = t0 + q;
-      if (q < per && t < tiles) {
-        sb.tile_items[t] = run;
-        run += items[q];
-      }
-    }
-    if (threadIdx.x == 0) sb.tile_items[tiles] = total;
   }
+  int total;
+  int run = block_exclusive_scan(mine, &total);
+#pragma unroll
+  for (int q = 0; q < kTilesPer; ++q) {
+    const int t = t0 + q;
+    if (q < per && t < tiles) {
+      sb.tile_items[t] = run;
+      run += items[q];
+    }
+  }
+  if (threadIdx.x == 0) sb.tile_items[tiles] = total;
 }
 
 __device__ __forceinline__ void splat_scatter_body(const float2* __restrict__ y,
@@ -523,7 +492,6 @@ __device__ __forceinline__ float4 node_value(long long m, long long mxv, long lo
 __device__ __forceinline__ void splat_tiles_body(const TsneStats* __restrict__ st, SplatBuffers sb,
                                                  int g, int item) {
   __shared__ unsigned long long win[3 * kTileNodes];
-  __shared__ bool last;
   const int ts = tiles_side(g), tiles = ts * ts;
   if (item >= sb.tile_items[tiles]) return;
   int lo = 0, hi = tiles - 1;  // tile: last t with tile_items[t] <= item
@@ -589,6 +557,8 @@ __device__ __forceinline__ void splat_tiles_body(const TsneStats* __restrict__ s
     }
     return;
   }
+  // items of a larger tile add into the 64-bit grid; the row FFT pass
+  // converts and clears it
   for (int t = threadIdx.x; t < kTileNodes; t += kThreads) {
     const int nx = ox + (t % kTile), ny = oy + t / kTile;
     if (nx < g && ny < g) {
@@ -600,34 +570,16 @@ __device__ __forceinline__ void splat_tiles_body(const TsneStats* __restrict__ s
       }
     }
   }
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0)
-    last = atomicAdd(sb.tile_done + tile, 1u) == static_cast<unsigned>(nitems - 1);
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  volatile unsigned long long* vw = sb.wide;
-  for (int t = threadIdx.x; t < kTileNodes; t += kThreads) {
-    const int nx = ox + (t % kTile), ny = oy + t / kTile;
-    if (nx < g && ny < g) {
-      const size_t node = static_cast<size_t>(ny) * g + nx;
-      const long long m = static_cast<long long>(vw[node]);
-      const long long mxv = static_cast<long long>(vw[gg + node]);
-      const long long myv = static_cast<long long>(vw[2 * gg + node]);
-      vw[node] = 0ull;
-      vw[gg + node] = 0ull;
-      vw[2 * gg + node] = 0ull;
-      sb.sgrid[node] = node_value(m, mxv, myv, inv_m, inv_xs, inv_xy);
-    }
-  }
-  if (threadIdx.x == 0) sb.tile_done[tile] = 0u;
 }
 
 __global__ __launch_bounds__(kThreads) void splat_count(const float2* __restrict__ y,
-                                                        TsneStats* st, SplatBuffers sb, int n,
-                                                        int g) {
+                                                        const TsneStats* __restrict__ st,
+                                                        SplatBuffers sb, int n, int g) {
   splat_count_body(y, st, sb, n, g, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(kThreads) void splat_prep(SplatBuffers sb, int g) {
+  splat_prep_body(sb, g);
 }
 
 __global__ __launch_bounds__(kThreads) void splat_scatter(const float2* __restrict__ y,
@@ -647,7 +599,7 @@ __global__ __launch_bounds__(kThreads) void splat_tiles(const TsneStats* __restr
 //    kt[k2 * g + dy] for k2 in [0, g], so conv_cols reads them contiguously.
 // ----------------------------------------------------------------------------
 template <int F>
-__device__ __forceinline__ void conv_rows_fwd_body(const float4* __restrict__ sgrid,
+__device__ __forceinline__ void conv_rows_fwd_body(SplatBuffers sb,
                                                    const TsneStats* __restrict__ st,
                                                    const float2* __restrict__ tw,
                                                    float2* __restrict__ z1,
@@ -657,10 +609,28 @@ __device__ __forceinline__ void conv_rows_fwd_body(const float4* __restrict__ sg
   __shared__ float2 buf[2 * F];
   if (b < G) {
     const int row = b;
+    const int ts = tiles_side(G);
+    const int* trow = sb.tile_items + (row / kTile) * ts;
+    constexpr size_t gg = static_cast<size_t>(G) * G;
     for (int x = threadIdx.x; x < F; x += kThreads) {
       float2 a = make_float2(0.f, 0.f), c = make_float2(0.f, 0.f);
       if (x < G) {
-        const float4 v = sgrid[static_cast<size_t>(row) * G + x];
+        const size_t node = static_cast<size_t>(row) * G + x;
+        const int t = x / kTile;
+        float4 v;
+        if (trow[t + 1] - trow[t] > 1) {
+          // node of a tile split over several work items: its 64-bit sums
+          const double inv_m = 1.0 / st->fx_mass, inv_xy = 1.0 / st->fx_xy;
+          const long long m = static_cast<long long>(sb.wide[node]);
+          const long long mxv = static_cast<long long>(sb.wide[gg + node]);
+          const long long myv = static_cast<long long>(sb.wide[2 * gg + node]);
+          sb.wide[node] = 0ull;
+          sb.wide[gg + node] = 0ull;
+          sb.wide[2 * gg + node] = 0ull;
+          v = node_value(m, mxv, myv, inv_m, inv_xy / st->x_scale, inv_xy);
+        } else {
+          v = sb.sgrid[node];
+        }
         a = make_float2(v.x, v.y);
         c = make_float2(v.z, 0.f);
       }
@@ -671,9 +641,11 @@ __device__ __forceinline__ void conv_rows_fwd_body(const float4* __restrict__ sg
     fft_lds<F, 2, kThreads, false>(buf, tw);
     float2* o1 = z1 + static_cast<size_t>(row) * F;
     float2* o2 = z2 + static_cast<size_t>(row) * F;
+    // z2 is the transform of a real row: columns G+1 .. F-1 are the
+    // conjugates of columns G-1 .. 1 and are never read
     for (int x = threadIdx.x; x < F; x += kThreads) {
       o1[x] = buf[x];
-      o2[x] = buf[F + x];
+      if (x <= G) o2[x] = buf[F + x];
     }
   } else {
     const int dy0 = b - G;
@@ -702,22 +674,24 @@ __device__ __forceinline__ void conv_rows_fwd_body(const float4* __restrict__ sg
 }
 
 template <int F>
-__global__ __launch_bounds__(kThreads) void conv_rows_fwd(const float4* __restrict__ sgrid,
+__global__ __launch_bounds__(kThreads) void conv_rows_fwd(SplatBuffers sb,
                                                           const TsneStats* __restrict__ st,
                                                           const float2* __restrict__ tw,
                                                           float2* __restrict__ z1,
                                                           float2* __restrict__ z2,
                                                           float2* __restrict__ kt) {
-  conv_rows_fwd_body<F>(sgrid, st, tw, z1, z2, kt, blockIdx.x);
+  conv_rows_fwd_body<F>(sb, st, tw, z1, z2, kt, blockIdx.x);
 }
 
 // ----------------------------------------------------------------------------
 // 3. column pass for the pair (c, F - c), c in [0, G].  LDS arrays:
-//    0: z1 col c, 1: z2 col c, 2: z1 col F-c, 3: z2 col F-c, 4: kernel col.
+//    0: z1 col c, 1: z2 col c, 2: z1 col F-c, 3: kernel col.
 // Output O1 = A*Kq + i*X*Kq2 (-> q, xq2) and O2 = A*Kq2 + i*Y*Kq2 (-> q2,
 // yq2) for rows y < G, where A = FFT(mass), X = FFT(mass_x), Y = FFT(mass_y).
-// The 1/F^2 of the inverse transform is folded into the kernel spectra.
-// `lds` holds 5*F float2 (dynamic shared memory of the launching kernel).
+// mass and mass_y are real, so A(F-c, -k) = conj A(c, k) and
+// Y(F-c, -k) = conj Y(c, k): column F-c of z2 is never transformed.  The
+// 1/F^2 of the inverse transform is folded into the kernel spectra.
+// `lds` holds 4*F float2 (dynamic shared memory of the launching kernel).
 // ----------------------------------------------------------------------------
 template <int F>
 __device__ __forceinline__ void conv_cols_body(const float2* __restrict__ z1,
@@ -725,41 +699,51 @@ __device__ __forceinline__ void conv_cols_body(const float2* __restrict__ z1,
                                                const float2* __restrict__ kt,
                                                const float2* __restrict__ tw,
                                                float2* __restrict__ o1, float2* __restrict__ o2,
-                                               double* partials, TsneStats* st, float2* lds,
-                                               int c, int nblk) {
+                                               double* __restrict__ partials, float2* lds,
+                                               int c) {
   constexpr int G = F / 2;
+  constexpr int kPer = (F + kThreads - 1) / kThreads;  // spectrum entries per thread
   float2* b0 = lds;
   float2* b1 = lds + F;
   float2* b2 = lds + 2 * F;
-  float2* b3 = lds + 3 * F;
-  float2* bk = lds + 4 * F;
+  float2* bk = lds + 3 * F;
   const int cp = (F - c) & (F - 1);
   const bool pair = (cp != c);
   for (int y = threadIdx.x; y < F; y += kThreads) {
-    float2 a = make_float2(0.f, 0.f), bb = a, cc = a, dd = a;
+    float2 a = make_float2(0.f, 0.f), bb = a, cc = a;
     if (y < G) {
       const size_t r = static_cast<size_t>(y) * F;
       a = z1[r + c];
       bb = z2[r + c];
       cc = z1[r + cp];
-      dd = z2[r + cp];
     }
     b0[y] = a;
     b1[y] = bb;
     b2[y] = cc;
-    b3[y] = dd;
     const int d = y <= G ? y : F - y;
     bk[y] = d < G ? kt[static_cast<size_t>(c) * G + d] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  fft_lds<F, 5, kThreads, false>(lds, tw);
+  fft_lds<F, 4, kThreads, false>(lds, tw);
   constexpr float inv_ff = 1.0f / (static_cast<float>(F) * static_cast<float>(F));
+  // the kernel spectrum is read before its array takes column F-c of O2
+  float2 kv[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int t = threadIdx.x + j * kThreads;
+    kv[j] = t < F ? bk[t] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  float2* b3 = bk;
   double zpart = 0.0;
-  for (int t = threadIdx.x; t < F; t += kThreads) {
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int t = threadIdx.x + j * kThreads;
+    if (t >= F) continue;
     const int k1 = t, k1m = (F - t) & (F - 1);
     const float2 z1a = b0[k1], z1b = b2[k1m];
-    const float2 ya = b1[k1], yb = b3[k1m];
-    const float kq = bk[k1].x * inv_ff, k2 = bk[k1].y * inv_ff;
+    const float2 ya = b1[k1], yb = cconj(ya);
+    const float kq = kv[j].x * inv_ff, k2 = kv[j].y * inv_ff;
     // A = (z + conj(zm))/2, X = (z - conj(zm))/(2i)
     const float2 A = make_float2(0.5f * (z1a.x + z1b.x), 0.5f * (z1a.y - z1b.y));
     const float2 X = make_float2(0.5f * (z1a.y + z1b.y), -0.5f * (z1a.x - z1b.x));
@@ -789,37 +773,15 @@ __device__ __forceinline__ void conv_cols_body(const float2* __restrict__ z1,
       o2[r + cp] = b3[y];
     }
   }
-  // Z partial: block reduce then last-block sum in column order
+  // Z partial of the column pair; conv_rows_inv folds the partials
   __shared__ double red[kThreads / 64];
-  __shared__ bool last;
   zpart = wave_sum(zpart);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = zpart;
   __syncthreads();
   if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int w = 0; w < kThreads / 64; ++w) s += red[w];
-    partials[c] = s;
-    __threadfence();
-    last = (atomicAdd(&st->done_cols, 1u) == static_cast<unsigned>(nblk - 1));
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  // whole-block fold of the column partials in a fixed order
-  const volatile double* vp = partials;
-  double s = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += kThreads) s += vp[b];
-  s = wave_sum(s);
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
     double t = 0.0;
     for (int w = 0; w < kThreads / 64; ++w) t += red[w];
-    // Parseval: sum_nodes mass*q = (1/F^2) sum_k |A(k)|^2 Kq_hat(k); kq above
-    // already carries the 1/F^2.
-    st->sum_q_total = t;
-    st->done_cols = 0u;
+    partials[c] = t;
   }
 }
 
@@ -842,12 +804,12 @@ __global__ __launch_bounds__(kThreads) void conv_cols(const float2* __restrict__
                                                       const float2* __restrict__ tw,
                                                       float2* __restrict__ o1,
                                                       float2* __restrict__ o2,
-                                                      double* partials, TsneStats* st) {
+                                                      double* __restrict__ partials) {
   extern __shared__ float2 lds[];
   constexpr int G = F / 2;
   const int c = xcd_column(blockIdx.x, G);
   if (c > G) return;
-  conv_cols_body<F>(z1, z2, kt, tw, o1, o2, partials, st, lds, c, G + 1);
+  conv_cols_body<F>(z1, z2, kt, tw, o1, o2, partials, lds, c);
 }
 
 // ----------------------------------------------------------------------------
@@ -857,10 +819,27 @@ template <int F>
 __device__ __forceinline__ void conv_rows_inv_body(const float2* __restrict__ o1,
                                                    const float2* __restrict__ o2,
                                                    const float2* __restrict__ tw,
-                                                   const TsneStats* __restrict__ st,
-                                                   float4* __restrict__ field, int row) {
+                                                   const double* __restrict__ zpart,
+                                                   TsneStats* st, float4* __restrict__ field,
+                                                   int row) {
   constexpr int G = F / 2;
   __shared__ float2 buf[2 * F];
+  if (row == 0) {
+    // Z: fold the G + 1 column-pair partials in a fixed order.  Parseval:
+    // sum_nodes mass*q = (1/F^2) sum_k |A(k)|^2 Kq_hat(k) (the kernel
+    // spectra already carry the 1/F^2)
+    __shared__ double red[kThreads / 64];
+    double a = 0.0;
+    for (int c = threadIdx.x; c <= G; c += kThreads) a += zpart[c];
+    a = wave_sum(a);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int w = 0; w < kThreads / 64; ++w) t += red[w];
+      st->sum_q_total = t;
+    }
+  }
   const float2* s1 = o1 + static_cast<size_t>(row) * F;
   const float2* s2 = o2 + static_cast<size_t>(row) * F;
   for (int x = threadIdx.x; x < F; x += kThreads) {
@@ -881,9 +860,10 @@ template <int F>
 __global__ __launch_bounds__(kThreads) void conv_rows_inv(const float2* __restrict__ o1,
                                                           const float2* __restrict__ o2,
                                                           const float2* __restrict__ tw,
-                                                          const TsneStats* __restrict__ st,
+                                                          const double* __restrict__ zpart,
+                                                          TsneStats* st,
                                                           float4* __restrict__ field) {
-  conv_rows_inv_body<F>(o1, o2, tw, st, field, blockIdx.x);
+  conv_rows_inv_body<F>(o1, o2, tw, zpart, st, field, blockIdx.x);
 }
 
 // ----------------------------------------------------------------------------
@@ -1255,12 +1235,12 @@ __global__ __launch_bounds__(kThreads) void attract_wave_rows(const int* __restr
 //     rep = -(y q2 - xq2, y q2 - yq2) / Z, grad = rep + attr.
 // MODE 0: optimiser step (apply_open_tsne_update_f, tsne_neighbors.cpp:
 //         1297-1336; CUDA twin embedding_cuda_kernels.cpp:2819-2886) into
-//         y_out plus the layout statistics for the next iteration; returns
-//         true in thread 0 of the block that finalised them.
+//         y_out plus this block's partial of the layout statistics
+//         (layout_finalize folds them into the next iteration's geometry).
 // MODE 1: diagnostic; writes the repulsive force to diag_rep.
 // ----------------------------------------------------------------------------
 template <int MODE>
-__device__ __forceinline__ bool update_body(const float2* __restrict__ y_in,
+__device__ __forceinline__ void update_body(const float2* __restrict__ y_in,
                                             float2* __restrict__ y_out,
                                             float2* __restrict__ gains, float2* __restrict__ upd,
                                             const float2* __restrict__ attr,
@@ -1322,8 +1302,10 @@ __device__ __forceinline__ bool update_body(const float2* __restrict__ y_in,
       m.mxy = fmaxf(m.mxy, ny);
     }
   }
-  if constexpr (MODE == 0) return reduce_layout_stats(m, partials, st, n, g, true, blk, nblk);
-  return false;
+  if constexpr (MODE == 0) {
+    const LayoutPartial b = block_layout(m);
+    if (threadIdx.x == 0) partials[blk] = b;
+  }
 }
 
 template <int MODE>
@@ -1332,8 +1314,8 @@ __global__ __launch_bounds__(kThreads) void update_kernel(
     float2* __restrict__ upd, const float2* __restrict__ attr, const float4* __restrict__ field,
     TsneStats* st, LayoutPartial* partials, int n, int g, StepParams sp,
     float2* __restrict__ diag_rep) {
-  (void)update_body<MODE>(y_in, y_out, gains, upd, attr, field, st, partials, n, g, sp, diag_rep,
-                          blockIdx.x, gridDim.x);
+  update_body<MODE>(y_in, y_out, gains, upd, attr, field, st, partials, n, g, sp, diag_rep,
+                    blockIdx.x, gridDim.x);
 }
 
 // Write the centred layout (final output): out_i = fl(y_i - mean).

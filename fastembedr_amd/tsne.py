@@ -212,6 +212,16 @@ def gradient_diagnostic(indices, distances, layout, perplexity, exaggeration=1.0
                       val[:m].copy())
 
 
+def _device_inputs_ready(*arrays):
+    """The library reads device inputs on its own stream: wait for the
+    producer's work on torch tensors before handing their pointers over."""
+    for a in arrays:
+        if getattr(a, "is_cuda", False):
+            import torch
+
+            torch.cuda.current_stream(a.device).synchronize()
+
+
 class TsnePlan:
     """A resident t-SNE job: P, layout, optimizer state and FFT work arrays on
     the device.  ``indices``/``distances`` are host n x k arrays or, with
@@ -226,6 +236,7 @@ class TsnePlan:
         if device:
             self.n, self.k = int(n), int(k)
             off = 0 if index_offset is None else int(index_offset)
+            _device_inputs_ready(indices, distances)
             ip, dp = _lib.ctypes.c_void_p(indices.data_ptr()), _lib.ctypes.c_void_p(distances.data_ptr())
             self._keep = (indices, distances)
         else:

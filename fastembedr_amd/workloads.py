@@ -92,3 +92,76 @@ def mixture_knn_device(x, labels, centres, k, device="cuda", chunk=4096, margin=
             out_i[:, q] = torch.gather(nb, 1, sel).to(torch.int32).T
             out_d[:, q] = srt.values[:, :k].T
     return out_i, out_d
+
+
+def log_uniform_sizes(count, lo, hi, rng):
+    """`count` integers log-uniform in [lo, hi]."""
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), size=count)).astype(np.int64)
+
+
+def batch_job_specs(count, seed, n_lo=2000, n_hi=100000):
+    """Independent t-SNE jobs of uneven size: n log-uniform in [n_lo, n_hi],
+    each a 5-50-cluster Gaussian mixture in 30-50 dimensions."""
+    rng = np.random.default_rng(seed)
+    sizes = log_uniform_sizes(count, n_lo, n_hi, rng)
+    return [dict(n=int(n), d=int(rng.integers(30, 51)), clusters=int(rng.integers(5, 51)),
+                 seed=int(rng.integers(1 << 30))) for n in sizes]
+
+
+def job_knn_device(spec, k, device="cuda"):
+    """Data and exact device KNN of one job spec (see batch_job_specs)."""
+    x, labels, centres = gaussian_mixture(spec["n"], spec["d"], spec["clusters"], seed=spec["seed"],
+                                          dirichlet=True)
+    return mixture_knn_device(x, labels, centres, k, device=device)
+
+
+def knn_to_reference(xq, lq, xr, lr, centres, k, device="cuda", chunk=4096, margin=20.0):
+    """Exact Euclidean k-NN of query points among reference points of the same
+    mixture (queries are not part of the reference), on the GPU.  Returns host
+    (indices, distances) of shape (nq, k), int32 0-based and float32."""
+    import torch
+
+    torch.backends.cuda.matmul.allow_tf32 = False
+    xr_t = torch.from_numpy(np.ascontiguousarray(xr)).to(device)
+    sqr = (xr_t * xr_t).sum(1)
+    order = np.argsort(lr, kind="stable")
+    bounds = np.searchsorted(lr[order], np.arange(len(centres) + 1))
+    cd = np.sqrt(((centres[:, None, :] - centres[None, :, :]) ** 2).sum(-1))
+    out_i = np.zeros((len(xq), k), np.int32)
+    out_d = np.zeros((len(xq), k), np.float32)
+    for c in np.unique(lq):
+        qsel = np.nonzero(lq == c)[0]
+        others = np.argsort(cd[c])
+        cand_clusters = []
+        nearest = cd[c, others[1]] if len(others) > 1 else np.inf
+        for o in others:
+            if o == c or cd[c, o] <= nearest + margin or \
+                    sum(bounds[q + 1] - bounds[q] for q in cand_clusters) < k:
+                cand_clusters.append(o)
+            else:
+                break
+        cand = np.concatenate([order[bounds[q]:bounds[q + 1]] for q in cand_clusters])
+        if len(cand) < k:
+            raise ValueError("reference too small for the requested k")
+        cand_t = torch.from_numpy(cand.astype(np.int64)).to(device)
+        xc, sqc = xr_t[cand_t], sqr[cand_t]
+        for s in range(0, len(qsel), chunk):
+            qs = qsel[s:s + chunk]
+            xq_t = torch.from_numpy(np.ascontiguousarray(xq[qs])).to(device)
+            d2 = (xq_t * xq_t).sum(1)[:, None] + sqc[None, :] - 2.0 * (xq_t @ xc.T)
+            top = torch.topk(d2, min(k + 8, len(cand)), dim=1, largest=False).indices
+            nb = cand_t[top]
+            diff = xq_t[:, None, :] - xr_t[nb]
+            de = torch.sqrt((diff * diff).sum(-1))
+            srt = torch.sort(de, dim=1, stable=True)
+            out_i[qs] = torch.gather(nb, 1, srt.indices[:, :k]).to(torch.int32).cpu().numpy()
+            out_d[qs] = srt.values[:, :k].cpu().numpy()
+    return out_i, out_d
+
+
+def sample_mixture(n, centres, weights, rng):
+    """n new points of a Gaussian mixture with the given centres, unit
+    within-cluster variance and cluster probabilities; returns (x, labels)."""
+    labels = rng.choice(len(centres), size=n, p=weights)
+    x = (centres[labels] + rng.normal(size=(n, centres.shape[1]))).astype(np.float32)
+    return x, labels
