@@ -195,6 +195,12 @@ class TsnePlan {
   int* wave_row = nullptr;
   int attr_waves = 0;
   int flat_attr = 0;  // FlatVariant, 0: row-per-lane-group kernel
+  // LDS windows of the staged attraction (flat8p with staging), one per
+  // group of kStagedWaves waves
+  int* stage_lo = nullptr;
+  int* stage_m = nullptr;
+  int stage_groups = 0;
+  bool staged = false;
   float2* y[2] = {nullptr, nullptr};
   float2* gains = nullptr;
   float2* upd = nullptr;
@@ -241,12 +247,21 @@ static int choose_tpr(double avg_row) {
 }
 
 // FASTEMBEDR_HIP_ATTRACTION selects the attraction kernel: rows (one lane
-// group per CSR row), flat8p (default), flat8, flat4p, flat4 (entry-balanced,
-// 8 or 4 entries per lane, with or without prefetch).  All compute the same
-// per-entry terms; only the summation order differs.
-static int attraction_variant() {
+// group per CSR row), flat8p, flat8, flat4p, flat4 (entry-balanced, 8 or 4
+// entries per lane, with or without prefetch), staged (flat8p with an LDS
+// window of the layout per workgroup).  Unset: staged when the windows cover
+// at least half of the entries, else flat8p.  All compute the same per-entry
+// terms; the entry-balanced kernels sum them in the same order, so flat8p
+// and staged agree bit for bit.
+static int attraction_variant(int* staging) {
   const char* v = std::getenv("FASTEMBEDR_HIP_ATTRACTION");
-  if (!v || !*v || std::strcmp(v, "flat8p") == 0) return kFlat8Prefetch;
+  *staging = 1;  // 0 never, 1 when the windows cover enough, 2 always
+  if (!v || !*v) return kFlat8Prefetch;
+  if (std::strcmp(v, "staged") == 0) {
+    *staging = 2;
+    return kFlat8Prefetch;
+  }
+  *staging = 0;
   if (std::strcmp(v, "rows") == 0) return 0;
   if (std::strcmp(v, "flat8") == 0) return kFlat8;
   if (std::strcmp(v, "flat4p") == 0) return kFlat4Prefetch;
@@ -257,21 +272,34 @@ static int attraction_variant() {
 int TsnePlan::setup_attraction(hipStream_t s) {
   const long long nnz = csr.nnz;
   flat_attr = 0;
-  const int variant = attraction_variant();
+  staged = false;
+  int staging = 0;
+  const int variant = attraction_variant(&staging);
   if (nnz <= 0 || variant == 0) return 0;
-  int waves = static_cast<int>(std::min<long long>(std::max<long long>(nnz / 2048, 4), 32768));
-  waves = (waves + 3) / 4 * 4;
+  // wave ranges of about 4096 entries, a whole number of staged groups
+  int waves = static_cast<int>(
+      std::min<long long>(std::max<long long>((nnz + 4095) / 4096, kStagedWaves), 16384));
+  waves = (waves + kStagedWaves - 1) / kStagedWaves * kStagedWaves;
+  const int groups = waves / kStagedWaves;
   const size_t words = static_cast<size_t>((nnz + 63) / 64) + kFlatPadWords;
   ArenaPlan ap;
   ap.add<unsigned long long>(words);
   ap.add<int>(words);
   ap.add<int>(waves + 1);
   ap.add<unsigned int>(1);
+  ap.add<int2>(waves);
+  ap.add<int>(groups);
+  ap.add<int>(groups);
+  ap.add<unsigned long long>(1);
   if (attr_store.init(ap.bytes, "t-SNE attraction tables")) return 1;
   row_start = attr_store.take<unsigned long long>(words);
   word_row = attr_store.take<int>(words);
   wave_row = attr_store.take<int>(waves + 1);
   unsigned int* d_empty = attr_store.take<unsigned int>(1);
+  int2* bounds = attr_store.take<int2>(waves);
+  stage_lo = attr_store.take<int>(groups);
+  stage_m = attr_store.take<int>(groups);
+  unsigned long long* d_hits = attr_store.take<unsigned long long>(1);
   FE_CHECK(hipMemsetAsync(row_start, 0, words * sizeof(unsigned long long), s));
   FE_CHECK(hipMemsetAsync(d_empty, 0, sizeof(unsigned int), s));
   attract_row_starts<<<(n + kThreads - 1) / kThreads, kThreads, 0, s>>>(csr.row_ptr, n, row_start,
@@ -283,6 +311,20 @@ int TsnePlan::setup_attraction(hipStream_t s) {
   attract_word_rows<<<static_cast<int>((words + kThreads - 1) / kThreads), kThreads, 0, s>>>(
       csr.row_ptr, n, static_cast<int>(words), word_row);
   FE_LAUNCH_CHECK("attract_word_rows");
+  unsigned long long hits = 0;
+  if (staging > 0 && variant == kFlat8Prefetch) {
+    FE_CHECK(hipMemsetAsync(d_hits, 0, sizeof(unsigned long long), s));
+    const int wblocks = (waves + kThreads / 64 - 1) / (kThreads / 64);
+    attract_col_bounds<<<wblocks, kThreads, 0, s>>>(csr.row_ptr, csr.col, wave_row, waves, bounds);
+    FE_LAUNCH_CHECK("attract_col_bounds");
+    attract_stage_windows<<<(groups + kThreads - 1) / kThreads, kThreads, 0, s>>>(
+        bounds, wave_row, waves, n, stage_lo, stage_m);
+    FE_LAUNCH_CHECK("attract_stage_windows");
+    attract_stage_cover<<<wblocks, kThreads, 0, s>>>(csr.row_ptr, csr.col, wave_row, stage_lo,
+                                                     stage_m, waves, d_hits);
+    FE_LAUNCH_CHECK("attract_stage_cover");
+    FE_CHECK(hipMemcpyAsync(&hits, d_hits, sizeof(hits), hipMemcpyDeviceToHost, s));
+  }
   unsigned int empty = 0;
   FE_CHECK(hipMemcpyAsync(&empty, d_empty, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
   FE_CHECK(hipStreamSynchronize(s));
@@ -291,10 +333,22 @@ int TsnePlan::setup_attraction(hipStream_t s) {
     row_start = nullptr;
     word_row = nullptr;
     wave_row = nullptr;
+    stage_lo = stage_m = nullptr;
     return 0;
   }
   attr_waves = waves;
   flat_attr = variant;
+  stage_groups = groups;
+  staged = staging == 2 || (staging == 1 && 2 * hits >= static_cast<unsigned long long>(nnz));
+  if (staged) {
+    static bool attr_set = false;
+    if (!attr_set) {
+      FE_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&attract_staged<8, true>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   kStagedCap * static_cast<int>(sizeof(float2))));
+      attr_set = true;
+    }
+  }
   return 0;
 }
 
@@ -491,6 +545,13 @@ static int launch_conv_any(TsnePlan& p, hipStream_t s) {
 // the repulsive force into diag[n, 2n) (MODE 1)
 static int launch_attract(TsnePlan& p, float exaggeration, hipStream_t s) {
   const float2* yin = p.y[p.cur];
+  if (p.staged) {
+    attract_staged<8, true><<<p.stage_groups, kStagedThreads, kStagedCap * sizeof(float2), s>>>(
+        yin, p.csr.row_ptr, p.csr.col, p.csr.val, p.row_start, p.word_row, p.wave_row,
+        p.stage_lo, p.stage_m, p.st, exaggeration, p.diag, p.n, p.attr_waves);
+    FE_LAUNCH_CHECK("attract_staged");
+    return 0;
+  }
   if (p.flat_attr) {
 #define FE_FLAT(L, PF)                                                                       \
   attract_flat<L, PF><<<p.attr_waves / (kThreads / 64), kThreads, 0, s>>>(                   \

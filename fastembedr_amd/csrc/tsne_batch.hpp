@@ -144,8 +144,9 @@ __global__ __launch_bounds__(kThreads) void batch_attract_flat(JobDesc* jobs,
   const int wave = xcd_remap(static_cast<int>(blockIdx.x) - first[e], first[e + 1] - first[e]) *
                        (kThreads / 64) +
                    static_cast<int>(threadIdx.x >> 6);
-  attract_flat_body<LANE, PREFETCH>(J->y[J->cur], J->row_ptr, J->col, J->val, J->row_start,
-                                    J->word_row, J->wave_row, J->st, exag, J->attr, J->n, wave);
+  const GlobalPoints src{J->y[J->cur], J->st->mean_xf, J->st->mean_yf};
+  attract_flat_body<LANE, PREFETCH>(src, J->row_ptr, J->col, J->val, J->row_start, J->word_row,
+                                    J->wave_row, exag, J->attr, J->n, wave);
 }
 
 template <int TPR>

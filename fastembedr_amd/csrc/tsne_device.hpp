@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <climits>
 #include <cstdint>
 
 #include "fe_fft.hpp"
@@ -74,6 +75,16 @@ __device__ __forceinline__ double wave_min(double v) {
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
   return v;
 }
 
@@ -1020,8 +1031,34 @@ __device__ __forceinline__ FlatCols<LANE> flat_cols(const int* __restrict__ col,
 // row.  FAST: the lane's entries lie in at most two rows (its first and its
 // last), so two point loads serve all of them; otherwise (rows shorter than
 // LANE entries) each entry loads its own point.
-template <int LANE, bool PREFETCH, bool FAST>
-__device__ __forceinline__ void flat_entries(const float2* __restrict__ y_in,
+// Sources of centred points for the attraction: fl(y - mean) straight from
+// the layout, or from a window of the layout staged in LDS (points outside
+// the window still come from the layout).  Both give the same values.
+struct GlobalPoints {
+  const float2* __restrict__ y;
+  float mx, my;
+  __device__ __forceinline__ float2 operator()(int j) const {
+    const float2 p = y[j];
+    return make_float2(p.x - mx, p.y - my);
+  }
+};
+
+struct StagedPoints {
+  const float2* __restrict__ y;
+  const float2* lds;  // centred points lo .. lo + m - 1
+  int lo;
+  unsigned int m;
+  float mx, my;
+  __device__ __forceinline__ float2 operator()(int j) const {
+    const unsigned int o = static_cast<unsigned int>(j - lo);
+    if (o < m) return lds[o];
+    const float2 p = y[j];
+    return make_float2(p.x - mx, p.y - my);
+  }
+};
+
+template <int LANE, bool PREFETCH, bool FAST, class SRC>
+__device__ __forceinline__ void flat_entries(const SRC& src,
                                              const int* __restrict__ col,
                                              const float* __restrict__ val,
                                              const unsigned long long* __restrict__ row_start,
@@ -1030,8 +1067,8 @@ __device__ __forceinline__ void flat_entries(const float2* __restrict__ y_in,
                                              FlatCols<LANE>* next, int base, int lane,
                                              int open_row, unsigned int rbits,
                                              unsigned int mbits, unsigned int vmask, int n,
-                                             float mxf, float myf, float exaggeration, float* hx,
-                                             float* hy, float* tx, float* ty, int* hrow) {
+                                             float exaggeration, float* hx, float* hy, float* tx,
+                                             float* ty, int* hrow) {
   constexpr int kStep = 64 * LANE;
   const int eb = base + lane * LANE;
   float v[LANE];
@@ -1043,11 +1080,11 @@ __device__ __forceinline__ void flat_entries(const float2* __restrict__ y_in,
   // gathers, all unconditional (an invalid entry reads a valid point)
   float2 pj[LANE];
 #pragma unroll
-  for (int u = 0; u < LANE; ++u) pj[u] = y_in[((vmask >> u) & 1u) ? cur.j[u] : ra];
+  for (int u = 0; u < LANE; ++u) pj[u] = src(((vmask >> u) & 1u) ? cur.j[u] : ra);
   float2 pa = make_float2(0.f, 0.f), pb = pa;
   if constexpr (FAST) {
-    pa = y_in[ra];
-    pb = y_in[rb];
+    pa = src(ra);
+    pb = src(rb);
   }
   if constexpr (PREFETCH && FAST) *next = flat_cols<LANE>(col, row_start, word_row, base + kStep, lane);
   float sx = 0.f, sy = 0.f, ax = 0.f, ay = 0.f;
@@ -1068,9 +1105,8 @@ __device__ __forceinline__ void flat_entries(const float2* __restrict__ y_in,
     }
     float2 p;
     if constexpr (FAST) p = row[u] == row[0] ? pa : pb;
-    else p = y_in[min(max(row[u], 0), n - 1)];  // rows shorter than LANE entries: rare
-    const float xi = p.x - mxf, yi = p.y - myf;
-    const float dx = xi - (pj[u].x - mxf), dy = yi - (pj[u].y - myf);
+    else p = src(min(max(row[u], 0), n - 1));  // rows shorter than LANE entries: rare
+    const float dx = p.x - pj[u].x, dy = p.y - pj[u].y;
     const float q = __builtin_amdgcn_rcpf(1.0f + dx * dx + dy * dy);
     const float cf = ((vmask >> u) & 1u) ? exaggeration * v[u] * q : 0.0f;
     sx += cf * dx;
@@ -1084,15 +1120,14 @@ __device__ __forceinline__ void flat_entries(const float2* __restrict__ y_in,
   *hrow = h;
 }
 
-template <int LANE, bool PREFETCH>
-__device__ __forceinline__ void attract_flat_body(const float2* __restrict__ y_in,
+template <int LANE, bool PREFETCH, class SRC>
+__device__ __forceinline__ void attract_flat_body(const SRC& src,
                                                   const int* __restrict__ row_ptr,
                                                   const int* __restrict__ col,
                                                   const float* __restrict__ val,
                                                   const unsigned long long* __restrict__ row_start,
                                                   const int* __restrict__ word_row,
                                                   const int* __restrict__ wave_row,
-                                                  const TsneStats* __restrict__ st,
                                                   float exaggeration, float2* __restrict__ attr,
                                                   int n, int wave) {
   constexpr int kStep = 64 * LANE;
@@ -1101,7 +1136,6 @@ __device__ __forceinline__ void attract_flat_body(const float2* __restrict__ y_i
   const int r0 = wave_row[wave], r1 = wave_row[wave + 1];
   if (r0 >= r1) return;
   const int e0 = row_ptr[r0], e1 = row_ptr[r1];
-  const float mxf = st->mean_xf, myf = st->mean_yf;
   const int sh = (lane * LANE) & 63;
   const unsigned long long upto_lane = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
   float cx = 0.0f, cy = 0.0f;  // the row open at the start of the step
@@ -1124,13 +1158,13 @@ __device__ __forceinline__ void attract_flat_body(const float2* __restrict__ y_i
     int hrow;  // row closed by the lane's first row start (-1: none)
     FlatCols<LANE> next;
     if (__ballot(__popc(rbits & (kLaneMask - 1u)) > 1) == 0ull)
-      flat_entries<LANE, PREFETCH, true>(y_in, col, val, row_start, word_row, attr, cur, &next,
-                                         base, lane, open_row, rbits, mbits, vmask, n, mxf, myf,
+      flat_entries<LANE, PREFETCH, true>(src, col, val, row_start, word_row, attr, cur, &next,
+                                         base, lane, open_row, rbits, mbits, vmask, n,
                                          exaggeration, &hx, &hy, &tx, &ty, &hrow);
     else
-      flat_entries<LANE, PREFETCH, false>(y_in, col, val, row_start, word_row, attr, cur, &next,
-                                          base, lane, open_row, rbits, mbits, vmask, n, mxf,
-                                          myf, exaggeration, &hx, &hy, &tx, &ty, &hrow);
+      flat_entries<LANE, PREFETCH, false>(src, col, val, row_start, word_row, attr, cur, &next,
+                                          base, lane, open_row, rbits, mbits, vmask, n,
+                                          exaggeration, &hx, &hy, &tx, &ty, &hrow);
     if constexpr (PREFETCH) cur = next;
     else cur = flat_cols<LANE>(col, row_start, word_row, base + kStep, lane);
     const bool seen = hrow != -1;
@@ -1178,8 +1212,116 @@ __global__ __launch_bounds__(kThreads) void attract_flat(
   const int wave = xcd_remap(blockIdx.x, gridDim.x) * (kThreads / 64) +
                    static_cast<int>(threadIdx.x >> 6);
   if (wave >= n_waves) return;
-  attract_flat_body<LANE, PREFETCH>(y_in, row_ptr, col, val, row_start, word_row, wave_row, st,
+  const GlobalPoints src{y_in, st->mean_xf, st->mean_yf};
+  attract_flat_body<LANE, PREFETCH>(src, row_ptr, col, val, row_start, word_row, wave_row,
                                     exaggeration, attr, n, wave);
+}
+
+// 5a''. the entry-balanced attraction with a window of the layout staged in
+//      LDS.  A workgroup of kStagedWaves waves takes kStagedWaves consecutive
+//      wave ranges (consecutive rows); the window [stage_lo[w], stage_lo[w] +
+//      stage_m[w]) of centred points, chosen when the plan is built to cover
+//      the columns of those rows (their neighbours lie close in index order
+//      when the input groups similar points), is copied into LDS once and
+//      serves every gather inside it; columns outside still read the layout.
+//      Bit-identical to attract_flat.
+constexpr int kStagedWaves = 16;       // 1024 threads, one workgroup per CU
+constexpr int kStagedCap = 16384;      // points of the window (128 KB of LDS)
+constexpr int kStagedThreads = 64 * kStagedWaves;
+
+template <int LANE, bool PREFETCH>
+__global__ __launch_bounds__(kStagedThreads) void attract_staged(
+    const float2* __restrict__ y_in, const int* __restrict__ row_ptr, const int* __restrict__ col,
+    const float* __restrict__ val, const unsigned long long* __restrict__ row_start,
+    const int* __restrict__ word_row, const int* __restrict__ wave_row,
+    const int* __restrict__ stage_lo, const int* __restrict__ stage_m,
+    const TsneStats* __restrict__ st, float exaggeration, float2* __restrict__ attr, int n,
+    int n_waves) {
+  extern __shared__ float2 sy[];
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const int lo = stage_lo[wg], m = stage_m[wg];
+  const float mx = st->mean_xf, my = st->mean_yf;
+  for (int i = threadIdx.x; i < m; i += kStagedThreads) {
+    const float2 p = y_in[lo + i];
+    sy[i] = make_float2(p.x - mx, p.y - my);
+  }
+  __syncthreads();
+  const int wave = wg * kStagedWaves + static_cast<int>(threadIdx.x >> 6);
+  if (wave >= n_waves) return;
+  const StagedPoints src{y_in, sy, lo, static_cast<unsigned int>(m), mx, my};
+  attract_flat_body<LANE, PREFETCH>(src, row_ptr, col, val, row_start, word_row, wave_row,
+                                    exaggeration, attr, n, wave);
+}
+
+// set-up of the staged windows: column bounds of every wave range (one wave
+// each), then per workgroup the window, then the entries it covers
+__global__ __launch_bounds__(kThreads) void attract_col_bounds(const int* __restrict__ row_ptr,
+                                                               const int* __restrict__ col,
+                                                               const int* __restrict__ wave_row,
+                                                               int n_waves, int2* bounds) {
+  const int wave = blockIdx.x * (kThreads / 64) + static_cast<int>(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (wave >= n_waves) return;
+  const int e0 = row_ptr[wave_row[wave]], e1 = row_ptr[wave_row[wave + 1]];
+  int mn = INT_MAX, mx = -1;
+  for (int e = e0 + lane; e < e1; e += 64) {
+    const int j = col[e];
+    mn = min(mn, j);
+    mx = max(mx, j);
+  }
+  mn = wave_min_i(mn);
+  mx = wave_max_i(mx);
+  if (lane == 0) bounds[wave] = make_int2(mn, mx);
+}
+
+__global__ __launch_bounds__(kThreads) void attract_stage_windows(const int2* __restrict__ bounds,
+                                                                  const int* __restrict__ wave_row,
+                                                                  int n_waves, int n,
+                                                                  int* __restrict__ stage_lo,
+                                                                  int* __restrict__ stage_m) {
+  const int w = blockIdx.x * kThreads + threadIdx.x;
+  const int groups = (n_waves + kStagedWaves - 1) / kStagedWaves;
+  if (w >= groups) return;
+  const int w0 = w * kStagedWaves, w1 = min(n_waves, w0 + kStagedWaves);
+  int mn = INT_MAX, mx = -1;
+  for (int v = w0; v < w1; ++v) {
+    mn = min(mn, bounds[v].x);
+    mx = max(mx, bounds[v].y);
+  }
+  int lo = 0, m = 0;
+  if (mx >= mn) {
+    if (mx - mn + 1 <= kStagedCap) {
+      lo = mn;
+      m = mx - mn + 1;
+    } else {
+      // centred on the group's own rows, inside the column bounds
+      const int mid = (wave_row[w0] + wave_row[w1]) / 2;
+      lo = min(max(mid - kStagedCap / 2, mn), mx - kStagedCap + 1);
+      m = kStagedCap;
+    }
+  }
+  stage_lo[w] = lo;
+  stage_m[w] = min(m, n - lo);
+}
+
+__global__ __launch_bounds__(kThreads) void attract_stage_cover(const int* __restrict__ row_ptr,
+                                                                const int* __restrict__ col,
+                                                                const int* __restrict__ wave_row,
+                                                                const int* __restrict__ stage_lo,
+                                                                const int* __restrict__ stage_m,
+                                                                int n_waves,
+                                                                unsigned long long* hits) {
+  const int wave = blockIdx.x * (kThreads / 64) + static_cast<int>(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (wave >= n_waves) return;
+  const int g = wave / kStagedWaves;
+  const int lo = stage_lo[g];
+  const unsigned int m = static_cast<unsigned int>(stage_m[g]);
+  const int e0 = row_ptr[wave_row[wave]], e1 = row_ptr[wave_row[wave + 1]];
+  unsigned int c = 0;
+  for (int e = e0 + lane; e < e1; e += 64) c += static_cast<unsigned int>(col[e] - lo) < m ? 1u : 0u;
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if (lane == 0) atomicAdd(hits, static_cast<unsigned long long>(c));
 }
 
 // set-up of the entry-balanced attraction: row-start bits and empty-row count

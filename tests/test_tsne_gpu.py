@@ -236,13 +236,18 @@ def test_attraction_kernels_agree_on_hub_rows(monkeypatch):
     csr = oracle.build_p(idx, dist.astype(np.float32).astype(np.float64), 10)
     assert np.diff(csr.row_ptr).max() >= 500
     ref = oracle.attractive(csr, y.astype(np.float64), 4.0)
-    flat = fe().gradient_diagnostic(idx, dist.astype(np.float32), y, 10, 4.0, 128)
-    monkeypatch.setenv("FASTEMBEDR_HIP_ATTRACTION", "rows")
-    rows = fe().gradient_diagnostic(idx, dist.astype(np.float32), y, 10, 4.0, 128)
-    assert relative_l2(flat.attractive_force, ref) < 5e-5
-    assert relative_l2(rows.attractive_force, ref) < 5e-5
-    assert relative_l2(flat.attractive_force, rows.attractive_force) < 1e-5
-    assert np.array_equal(flat.repulsive_force_fft, rows.repulsive_force_fft)
+    got = {}
+    for kind in ("flat8p", "staged", "flat8", "flat4p", "rows"):
+        monkeypatch.setenv("FASTEMBEDR_HIP_ATTRACTION", kind)
+        got[kind] = fe().gradient_diagnostic(idx, dist.astype(np.float32), y, 10, 4.0, 128)
+        assert relative_l2(got[kind].attractive_force, ref) < 5e-5, kind
+    flat = got["flat8p"]
+    # the entry-balanced kernels sum in one order; the staged window changes
+    # only where the points are read from
+    assert np.array_equal(got["staged"].attractive_force, flat.attractive_force)
+    assert np.array_equal(got["flat8"].attractive_force, flat.attractive_force)
+    assert relative_l2(flat.attractive_force, got["rows"].attractive_force) < 1e-5
+    assert np.array_equal(flat.repulsive_force_fft, got["rows"].repulsive_force_fft)
 
 
 def test_collapsed_layout_splits_heavy_tiles():
