@@ -265,6 +265,11 @@ def main():
     dominant = max(algo, key=lambda kk: prof[kk])
     achieved = algo[dominant] / (prof[dominant] * 1e-3) / 1e9
     traffic = None
+    if args.traffic_json is None and n == 1_000_000:
+        # rocprofv3 FETCH_SIZE + WRITE_SIZE of this workload, calibrated
+        # against a plain copy (profiles/attract_staged_1M_counters.md)
+        args.traffic_json = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                                         "traffic_1M.json")
     if args.traffic_json and os.path.exists(args.traffic_json):
         with open(args.traffic_json) as fh:
             tj = json.load(fh)
