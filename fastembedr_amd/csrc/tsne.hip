@@ -276,9 +276,12 @@ int TsnePlan::setup_attraction(hipStream_t s) {
   int staging = 0;
   const int variant = attraction_variant(&staging);
   if (nnz <= 0 || variant == 0) return 0;
-  // wave ranges of about 4096 entries, a whole number of staged groups
-  int waves = static_cast<int>(
-      std::min<long long>(std::max<long long>((nnz + 4095) / 4096, kStagedWaves), 16384));
+  // wave ranges of about 4096 entries (small P: at least 512 waves of one
+  // 512-entry step or more, so a job alone still fills the GPU), a whole
+  // number of staged groups; the partition does not change any sum
+  long long w = (nnz + 4095) / 4096;
+  if (w < 512) w = std::min<long long>(512, std::max<long long>(nnz / kFlatStep8, 1));
+  int waves = static_cast<int>(std::min<long long>(std::max<long long>(w, kStagedWaves), 16384));
   waves = (waves + kStagedWaves - 1) / kStagedWaves * kStagedWaves;
   const int groups = waves / kStagedWaves;
   const size_t words = static_cast<size_t>((nnz + 63) / 64) + kFlatPadWords;
@@ -802,6 +805,10 @@ static long long job_setup_bytes(int n, int k) {
   return 8ll * nk /* upload */ + 8ll * nk /* nbr, pdir */ + 16ll * nk /* raw */ + 16ll * n;
 }
 
+// launch-table parameter of the staged attraction (other attraction launches
+// carry minus their FlatVariant or their lanes per row)
+constexpr int kStagedLaunch = -1000;
+
 class TsneBatch {
  public:
   int max_jobs = 0;
@@ -834,7 +841,7 @@ class TsneBatch {
     slot_bytes.assign(max_jobs, 0);
     desc.resize(max_jobs);
     std::memset(desc.data(), 0, desc.size() * sizeof(JobDesc));
-    tab_cap = 2 * (10 * max_jobs + 48) + 64;  // every job is in 10 launches
+    tab_cap = 2 * (10 * max_jobs + 56) + 64;  // every job is in 10 launches
     ArenaPlan p;
     p.add<JobDesc>(max_jobs);
     p.add<int>(tab_cap);
@@ -884,6 +891,9 @@ class TsneBatch {
     d.val = p.csr.val;
     d.row_start = p.row_start;
     d.word_row = p.word_row;
+    d.stage_lo = p.stage_lo;
+    d.stage_m = p.stage_m;
+    d.attr_waves = p.attr_waves;
     d.wave_row = p.wave_row;
     d.splat = p.splat;
     d.z1 = p.z1;
@@ -977,10 +987,16 @@ class TsneBatch {
         add_launch(kind, F, ent);
       }
     }
+    {
+      std::vector<std::pair<int, int>> ent;
+      for (int s = 0; s < max_jobs; ++s)
+        if (slots[s] && slots[s]->staged) ent.push_back({s, slots[s]->stage_groups});
+      add_launch(4, kStagedLaunch, ent);
+    }
     for (int v = kFlat8Prefetch; v <= kFlat4; ++v) {
       std::vector<std::pair<int, int>> ent;
       for (int s = 0; s < max_jobs; ++s)
-        if (slots[s] && slots[s]->flat_attr == v)
+        if (slots[s] && !slots[s]->staged && slots[s]->flat_attr == v)
           ent.push_back({s, slots[s]->attr_waves / (kThreads / 64)});
       add_launch(4, -v, ent);  // negative parameter: entry-balanced variant
     }
@@ -1051,6 +1067,11 @@ class TsneBatch {
 #undef FE_B_CONV
         case 4:
           switch (L.param) {
+            case kStagedLaunch:
+              batch_attract_staged<8, true><<<L.blocks, kStagedThreads,
+                                              kStagedCap * sizeof(float2), s>>>(d_desc, sl, fi,
+                                                                               L.m);
+              break;
             case -kFlat8Prefetch:
               batch_attract_flat<8, true><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
               break;
@@ -1092,6 +1113,9 @@ class TsneBatch {
       FE_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&batch_cols<2048>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize,
                                    4 * 2048 * static_cast<int>(sizeof(float2))));
+      FE_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&batch_attract_staged<8, true>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   kStagedCap * static_cast<int>(sizeof(float2))));
       attr_done = true;
     }
     // Captured chunks of kChunk iterations replay the current launch table;

@@ -978,6 +978,7 @@ __global__ __launch_bounds__(kThreads) void attract_kernel(const float2* __restr
 //     the prefetch load unconditionally.
 // ----------------------------------------------------------------------------
 constexpr int kFlatMaxLane = 8;
+constexpr int kFlatStep8 = 64 * 8;               // entries per step with 8 per lane
 constexpr int kFlatPad = 2 * 64 * kFlatMaxLane;  // padding entries after P
 constexpr int kFlatPadWords = kFlatPad / 64;
 
