@@ -1032,6 +1032,38 @@ __device__ __forceinline__ FlatCols<LANE> flat_cols(const int* __restrict__ col,
 // row.  FAST: the lane's entries lie in at most two rows (its first and its
 // last), so two point loads serve all of them; otherwise (rows shorter than
 // LANE entries) each entry loads its own point.
+// Wave-wide data movement by DPP (no LDS round trip): row_shr:o within rows
+// of 16 lanes, row_bcast:15 (lane 15 of each row to the next row, written to
+// rows 1 and 3) and row_bcast:31 (lane 31 to rows 2 and 3), wave_shr:1.
+// Lanes without a source read 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, true));
+}
+__device__ __forceinline__ float dpp_wave_shr1(float v) { return dpp_move<0x138, 0xF>(v); }
+
+// Segmented inclusive scan of (sx, sy) over the wave: lane l sums lanes
+// lim .. l, where lim (<= l) is the first lane of its segment.  The steps add
+// the value of a source lane below only while the source lies in the segment.
+__device__ __forceinline__ void segmented_wave_scan(float& sx, float& sy, int lane, int lim) {
+#define FE_SEG_STEP(CTRL, MASK, SRC)                   \
+  {                                                    \
+    const float ux = dpp_move<CTRL, MASK>(sx);          \
+    const float uy = dpp_move<CTRL, MASK>(sy);          \
+    if ((SRC) >= lim) {                                \
+      sx += ux;                                        \
+      sy += uy;                                        \
+    }                                                  \
+  }
+  FE_SEG_STEP(0x111, 0xF, lane - 1)
+  FE_SEG_STEP(0x112, 0xF, lane - 2)
+  FE_SEG_STEP(0x114, 0xF, lane - 4)
+  FE_SEG_STEP(0x118, 0xF, lane - 8)
+  FE_SEG_STEP(0x142, 0xA, (lane & ~15) - 1)
+  FE_SEG_STEP(0x143, 0xC, 31)
+#undef FE_SEG_STEP
+}
+
 // Sources of centred points for the attraction: fl(y - mean) straight from
 // the layout, or from a window of the layout staged in LDS (points outside
 // the window still come from the layout).  Both give the same values.
@@ -1179,24 +1211,17 @@ __device__ __forceinline__ void attract_flat_body(const SRC& src,
       sx += cx;
       sy += cy;
     }
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const float ux = __shfl_up(sx, o, 64), uy = __shfl_up(sy, o, 64);
-      if (lane - o >= lim) {
-        sx += ux;
-        sy += uy;
-      }
-    }
+    segmented_wave_scan(sx, sy, lane, lim);
     // open row arriving at this lane: the carry for lane 0, else the scan of
     // the lane below
-    float ix = __shfl_up(sx, 1, 64), iy = __shfl_up(sy, 1, 64);
+    float ix = dpp_wave_shr1(sx), iy = dpp_wave_shr1(sy);
     if (lane == 0) {
       ix = cx;
       iy = cy;
     }
     if (seen && hrow >= r0) attr[hrow] = make_float2(ix + hx, iy + hy);
-    cx = __shfl(sx, 63, 64);
-    cy = __shfl(sy, 63, 64);
+    cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sx), 63));
+    cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sy), 63));
   }
 }
 
