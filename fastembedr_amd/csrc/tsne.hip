@@ -187,20 +187,14 @@ class TsnePlan {
   DeviceCsr csr;
   Arena state;
   Arena csr_store;
-  // entry-balanced attraction: row-start bits per 64 entries, first row of
-  // every wave's range (used when every row of P has an entry)
+  // segment-blocked attraction: groups of rows, their windows, the padded
+  // segments and their step and wave tables
+  Arena group_store;
   Arena attr_store;
-  unsigned long long* row_start = nullptr;
-  int* word_row = nullptr;
-  int* wave_row = nullptr;
-  int attr_waves = 0;
-  int flat_attr = 0;  // FlatVariant, 0: row-per-lane-group kernel
-  // LDS windows of the staged attraction (flat8p with staging), one per
-  // group of kStagedWaves waves
-  int* stage_lo = nullptr;
-  int* stage_m = nullptr;
-  int stage_groups = 0;
-  bool staged = false;
+  BlockTables blk{};
+  long long seg_entries = 0;
+  int groups = 0;
+  int attr_kind = 0;  // AttractionKind
   float2* y[2] = {nullptr, nullptr};
   float2* gains = nullptr;
   float2* upd = nullptr;
@@ -247,110 +241,121 @@ static int choose_tpr(double avg_row) {
 }
 
 // FASTEMBEDR_HIP_ATTRACTION selects the attraction kernel: rows (one lane
-// group per CSR row), flat8p, flat8, flat4p, flat4 (entry-balanced, 8 or 4
-// entries per lane, with or without prefetch), staged (flat8p with an LDS
-// window of the layout per workgroup).  Unset: staged when the windows cover
-// at least half of the entries, else flat8p.  All compute the same per-entry
-// terms; the entry-balanced kernels sum them in the same order, so flat8p
-// and staged agree bit for bit.
-static int attraction_variant(int* staging) {
+// group per CSR row), lanes (segment-blocked, points read from the layout),
+// staged (segment-blocked, window passes read the LDS windows; the default).
+// lanes and staged sum the same terms in the same order and agree bit for
+// bit.
+static int attraction_kind() {
   const char* v = std::getenv("FASTEMBEDR_HIP_ATTRACTION");
-  *staging = 1;  // 0 never, 1 when the windows cover enough, 2 always
-  if (!v || !*v) return kFlat8Prefetch;
-  if (std::strcmp(v, "staged") == 0) {
-    *staging = 2;
-    return kFlat8Prefetch;
-  }
-  *staging = 0;
-  if (std::strcmp(v, "rows") == 0) return 0;
-  if (std::strcmp(v, "flat8") == 0) return kFlat8;
-  if (std::strcmp(v, "flat4p") == 0) return kFlat4Prefetch;
-  if (std::strcmp(v, "flat4") == 0) return kFlat4;
-  return kFlat8Prefetch;
+  if (!v || !*v) return kAttrStaged;
+  if (std::strcmp(v, "rows") == 0) return kAttrRows;
+  if (std::strcmp(v, "lanes") == 0) return kAttrLanes;
+  return kAttrStaged;
 }
 
 int TsnePlan::setup_attraction(hipStream_t s) {
   const long long nnz = csr.nnz;
-  flat_attr = 0;
-  staged = false;
-  int staging = 0;
-  const int variant = attraction_variant(&staging);
-  if (nnz <= 0 || variant == 0) return 0;
-  // wave ranges of about 4096 entries (small P: at least 512 waves of one
-  // 512-entry step or more, so a job alone still fills the GPU), a whole
-  // number of staged groups; the partition does not change any sum
-  long long w = (nnz + 4095) / 4096;
-  if (w < 512) w = std::min<long long>(512, std::max<long long>(nnz / kFlatStep8, 1));
-  int waves = static_cast<int>(std::min<long long>(std::max<long long>(w, kStagedWaves), 16384));
-  waves = (waves + kStagedWaves - 1) / kStagedWaves * kStagedWaves;
-  const int groups = waves / kStagedWaves;
-  const size_t words = static_cast<size_t>((nnz + 63) / 64) + kFlatPadWords;
-  ArenaPlan ap;
-  ap.add<unsigned long long>(words);
-  ap.add<int>(words);
-  ap.add<int>(waves + 1);
-  ap.add<unsigned int>(1);
-  ap.add<int2>(waves);
-  ap.add<int>(groups);
-  ap.add<int>(groups);
-  ap.add<unsigned long long>(1);
-  if (attr_store.init(ap.bytes, "t-SNE attraction tables")) return 1;
-  row_start = attr_store.take<unsigned long long>(words);
-  word_row = attr_store.take<int>(words);
-  wave_row = attr_store.take<int>(waves + 1);
-  unsigned int* d_empty = attr_store.take<unsigned int>(1);
-  int2* bounds = attr_store.take<int2>(waves);
-  stage_lo = attr_store.take<int>(groups);
-  stage_m = attr_store.take<int>(groups);
-  unsigned long long* d_hits = attr_store.take<unsigned long long>(1);
-  FE_CHECK(hipMemsetAsync(row_start, 0, words * sizeof(unsigned long long), s));
-  FE_CHECK(hipMemsetAsync(d_empty, 0, sizeof(unsigned int), s));
-  attract_row_starts<<<(n + kThreads - 1) / kThreads, kThreads, 0, s>>>(csr.row_ptr, n, row_start,
-                                                                      d_empty);
-  FE_LAUNCH_CHECK("attract_row_starts");
-  attract_wave_rows<<<(waves + kThreads) / kThreads, kThreads, 0, s>>>(csr.row_ptr, n, nnz, waves,
-                                                                      wave_row);
-  FE_LAUNCH_CHECK("attract_wave_rows");
-  attract_word_rows<<<static_cast<int>((words + kThreads - 1) / kThreads), kThreads, 0, s>>>(
-      csr.row_ptr, n, static_cast<int>(words), word_row);
-  FE_LAUNCH_CHECK("attract_word_rows");
-  unsigned long long hits = 0;
-  if (staging > 0 && variant == kFlat8Prefetch) {
-    FE_CHECK(hipMemsetAsync(d_hits, 0, sizeof(unsigned long long), s));
-    const int wblocks = (waves + kThreads / 64 - 1) / (kThreads / 64);
-    attract_col_bounds<<<wblocks, kThreads, 0, s>>>(csr.row_ptr, csr.col, wave_row, waves, bounds);
-    FE_LAUNCH_CHECK("attract_col_bounds");
-    attract_stage_windows<<<(groups + kThreads - 1) / kThreads, kThreads, 0, s>>>(
-        bounds, wave_row, waves, n, stage_lo, stage_m);
-    FE_LAUNCH_CHECK("attract_stage_windows");
-    attract_stage_cover<<<wblocks, kThreads, 0, s>>>(csr.row_ptr, csr.col, wave_row, stage_lo,
-                                                     stage_m, waves, d_hits);
-    FE_LAUNCH_CHECK("attract_stage_cover");
-    FE_CHECK(hipMemcpyAsync(&hits, d_hits, sizeof(hits), hipMemcpyDeviceToHost, s));
-  }
-  unsigned int empty = 0;
-  FE_CHECK(hipMemcpyAsync(&empty, d_empty, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
-  FE_CHECK(hipStreamSynchronize(s));
-  if (empty != 0) {
-    attr_store.release();
-    row_start = nullptr;
-    word_row = nullptr;
-    wave_row = nullptr;
-    stage_lo = stage_m = nullptr;
+  attr_kind = kAttrRows;
+  const int kind = attraction_kind();
+  // the padded segments must keep 32-bit offsets (at most kPasses segments
+  // per row, each padded by fewer than 8 entries)
+  const long long bound = nnz + static_cast<long long>(kPasses) * kLaneEntries * n + kLanePad;
+  if (nnz <= 0 || kind == kAttrRows || bound >= (1ll << 31) ||
+      static_cast<long long>(kPasses) * n + 1 >= (1ll << 31))
     return 0;
-  }
-  attr_waves = waves;
-  flat_attr = variant;
-  stage_groups = groups;
-  staged = staging == 2 || (staging == 1 && 2 * hits >= static_cast<unsigned long long>(nnz));
-  if (staged) {
-    static bool attr_set = false;
-    if (!attr_set) {
-      FE_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&attract_staged<8, true>),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   kStagedCap * static_cast<int>(sizeof(float2))));
-      attr_set = true;
-    }
+  const long long slots = static_cast<long long>(kPasses) * n;  // + 1 for the scans
+  // groups of about 16 * 4096 padded entries (small P: at least 32 groups,
+  // so a job alone still spreads over the GPU); the cut changes no sum
+  Arena scratch;
+  ArenaPlan sp;
+  const int nb_rows = (n + 1 + kScanBlock - 1) / kScanBlock;
+  const int nb_slots = static_cast<int>((slots + 1 + kScanBlock - 1) / kScanBlock);
+  sp.add<int>(n + 1);
+  sp.add<int>(n + 1);
+  sp.add<int>(std::max(nb_rows, nb_slots));
+  sp.add<int>(1);
+  sp.add<int>(slots + 1);
+  sp.add<int>(slots + 1);
+  sp.add<int>(slots + 1);
+  sp.add<int>(slots + 1);
+  if (scratch.init(sp.bytes, "t-SNE attraction set-up")) return 1;
+  int* len = scratch.take<int>(n + 1);
+  int* lrow = scratch.take<int>(n + 1);
+  int* sums = scratch.take<int>(std::max(nb_rows, nb_slots));
+  int* total = scratch.take<int>(1);
+  int* seg_len = scratch.take<int>(slots + 1);
+  int* seg_used = scratch.take<int>(slots + 1);
+  int* off = scratch.take<int>(slots + 1);
+  int* sid = scratch.take<int>(slots + 1);
+  lane_lengths<<<(n + kThreads) / kThreads, kThreads, 0, s>>>(csr.row_ptr, n, len);
+  FE_LAUNCH_CHECK("lane_lengths");
+  if (exclusive_scan(len, n + 1, lrow, sums, total, s)) return 1;
+  int h_lnnz = 0;
+  FE_CHECK(hipMemcpyAsync(&h_lnnz, lrow + n, sizeof(int), hipMemcpyDeviceToHost, s));
+  FE_CHECK(hipStreamSynchronize(s));
+  long long gcount = (h_lnnz + 16ll * 4096 - 1) / (16ll * 4096);
+  gcount = std::max<long long>(gcount, std::min<long long>(32, std::max(1, n / 64)));
+  groups = static_cast<int>(std::min<long long>(gcount, 1 << 16));
+  // group rows, windows and wave ranges; then the segments, whose size the
+  // windows decide
+  ArenaPlan wp;
+  wp.add<int>(groups + 1);
+  wp.add<int2>(static_cast<size_t>(groups) * kMaxWindows);
+  wp.add<int>(static_cast<size_t>(groups) * kPasses * (kStagedWaves + 1));
+  if (group_store.init(wp.bytes, "t-SNE attraction groups")) return 1;
+  int* group_row = group_store.take<int>(groups + 1);
+  int2* window = group_store.take<int2>(static_cast<size_t>(groups) * kMaxWindows);
+  int* pass_wave = group_store.take<int>(static_cast<size_t>(groups) * kPasses * (kStagedWaves + 1));
+  cut_rows<<<(groups + kThreads) / kThreads, kThreads, 0, s>>>(lrow, n, h_lnnz, groups, group_row);
+  FE_LAUNCH_CHECK("cut_rows");
+  blk_windows<<<groups, 256, 0, s>>>(csr.row_ptr, csr.col, group_row, n, window);
+  FE_LAUNCH_CHECK("blk_windows");
+  blk_seg_lengths<<<(n + kThreads) / kThreads, kThreads, 0, s>>>(
+      csr.row_ptr, csr.col, group_row, groups, window, n, seg_len, seg_used);
+  FE_LAUNCH_CHECK("blk_seg_lengths");
+  if (exclusive_scan(seg_len, static_cast<int>(slots + 1), off, sums, total, s)) return 1;
+  if (exclusive_scan(seg_used, static_cast<int>(slots + 1), sid, sums, total, s)) return 1;
+  int h_ent = 0, h_segs = 0;
+  FE_CHECK(hipMemcpyAsync(&h_ent, off + slots, sizeof(int), hipMemcpyDeviceToHost, s));
+  FE_CHECK(hipMemcpyAsync(&h_segs, sid + slots, sizeof(int), hipMemcpyDeviceToHost, s));
+  FE_CHECK(hipStreamSynchronize(s));
+  seg_entries = h_ent;
+  const size_t ent = static_cast<size_t>(h_ent) + kLanePad;
+  const int steps = h_ent / kLaneStep + 1 + kLanePadSteps;
+  ArenaPlan ap;
+  ap.add<int>(static_cast<size_t>(h_segs) + 1);
+  ap.add<int>(ent);
+  ap.add<float>(ent);
+  ap.add<int>(ent / kLaneEntries);
+  ap.add<unsigned long long>(steps);
+  if (attr_store.init(ap.bytes, "t-SNE attraction tables")) return 1;
+  int* seg_ptr = attr_store.take<int>(static_cast<size_t>(h_segs) + 1);
+  int* pcol = attr_store.take<int>(ent);
+  float* pval = attr_store.take<float>(ent);
+  int* lane_row = attr_store.take<int>(ent / kLaneEntries);
+  unsigned long long* starts = attr_store.take<unsigned long long>(steps);
+  FE_CHECK(hipMemcpyAsync(seg_ptr + h_segs, off + slots, sizeof(int), hipMemcpyDeviceToDevice, s));
+  FE_CHECK(hipMemsetAsync(pcol + h_ent, 0, kLanePad * sizeof(int), s));
+  FE_CHECK(hipMemsetAsync(pval + h_ent, 0, kLanePad * sizeof(float), s));
+  FE_CHECK(hipMemsetAsync(lane_row + h_ent / kLaneEntries, 0,
+                          kLanePad / kLaneEntries * sizeof(int), s));
+  FE_CHECK(hipMemsetAsync(starts, 0, steps * sizeof(unsigned long long), s));
+  blk_fill<<<(n + kThreads / 64 - 1) / (kThreads / 64), kThreads, 0, s>>>(
+      csr.row_ptr, csr.col, csr.val, group_row, groups, window, n, off, sid, seg_ptr, lane_row,
+      pcol, pval, starts);
+  FE_LAUNCH_CHECK("blk_fill");
+  blk_pass_waves<<<(groups * kPasses + kThreads - 1) / kThreads, kThreads, 0, s>>>(
+      group_row, groups, sid, seg_ptr, pass_wave);
+  FE_LAUNCH_CHECK("blk_pass_waves");
+  FE_CHECK(hipStreamSynchronize(s));
+  blk = BlockTables{seg_ptr, pcol, pval, lane_row, starts, group_row, window, pass_wave};
+  attr_kind = kind;
+  static bool attr_set = false;
+  if (!attr_set) {
+    FE_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&attract_blocked<true>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 static_cast<int>(kStagedLds)));
+    attr_set = true;
   }
   return 0;
 }
@@ -387,9 +392,7 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
     d_dist = ud;
   }
   auto alloc_csr = [&](int nnz, int** row_ptr, int** col, float** val) -> int {
-    // padded so the entry-balanced attraction can load whole 512-entry steps
-    // and prefetch the next one
-    const size_t m = static_cast<size_t>(nnz > 0 ? nnz : 0) + kFlatPad;
+    const size_t m = static_cast<size_t>(nnz > 0 ? nnz : 1);
     ArenaPlan cp;
     cp.add<int>(n + 1);
     cp.add<int>(m);
@@ -548,26 +551,16 @@ static int launch_conv_any(TsnePlan& p, hipStream_t s) {
 // the repulsive force into diag[n, 2n) (MODE 1)
 static int launch_attract(TsnePlan& p, float exaggeration, hipStream_t s) {
   const float2* yin = p.y[p.cur];
-  if (p.staged) {
-    attract_staged<8, true><<<p.stage_groups, kStagedThreads, kStagedCap * sizeof(float2), s>>>(
-        yin, p.csr.row_ptr, p.csr.col, p.csr.val, p.row_start, p.word_row, p.wave_row,
-        p.stage_lo, p.stage_m, p.st, exaggeration, p.diag, p.n, p.attr_waves);
-    FE_LAUNCH_CHECK("attract_staged");
+  if (p.attr_kind == kAttrStaged) {
+    attract_blocked<true><<<p.groups, kStagedThreads, kStagedLds, s>>>(yin, p.blk, p.st,
+                                                                       exaggeration, p.diag, p.n);
+    FE_LAUNCH_CHECK("attract_blocked");
     return 0;
   }
-  if (p.flat_attr) {
-#define FE_FLAT(L, PF)                                                                       \
-  attract_flat<L, PF><<<p.attr_waves / (kThreads / 64), kThreads, 0, s>>>(                   \
-      yin, p.csr.row_ptr, p.csr.col, p.csr.val, p.row_start, p.word_row, p.wave_row, p.st,   \
-      exaggeration, p.diag, p.n, p.attr_waves)
-    switch (p.flat_attr) {
-      case kFlat8: FE_FLAT(8, false); break;
-      case kFlat4Prefetch: FE_FLAT(4, true); break;
-      case kFlat4: FE_FLAT(4, false); break;
-      default: FE_FLAT(8, true); break;
-    }
-#undef FE_FLAT
-    FE_LAUNCH_CHECK("attract_flat");
+  if (p.attr_kind == kAttrLanes) {
+    attract_blocked<false><<<p.groups, kStagedThreads, 0, s>>>(yin, p.blk, p.st, exaggeration,
+                                                               p.diag, p.n);
+    FE_LAUNCH_CHECK("attract_blocked");
     return 0;
   }
 #define FE_ATTR(T)                                                                     \
@@ -795,19 +788,28 @@ static int tsne_one_shot(const int* indices, const DistT* distances, bool device
 // entries) and the transient peak while its P is built.
 static long long job_resident_bytes(int n, int k, int g) {
   const long long f = 2ll * g, gg = 1ll * g * g, nk = 1ll * n * k;
-  return 8ll * 2 * nk + 4ll * (n + 1) + 8ll * 7 * n + 16ll * (splat_bins(g) + 1) + 40ll * gg +
+  // P, its padded segments (at most kPasses per row) with their tables, the
+  // layout state, the splat bins and grids, the FFT buffers
+  const long long segs = std::min<long long>(2 * nk, 1ll * kPasses * n);
+  const long long seg_nnz = 2 * nk + 1ll * kLaneEntries * segs + kLanePad;
+  const long long groups = std::max<long long>(32, seg_nnz / (16 * 4096) + 1);
+  return 8ll * 2 * nk + 4ll * (n + 1) + 8ll * seg_nnz + seg_nnz / 2 + 8ll * (seg_nnz / kLaneStep + 4) +
+         4ll * (segs + 1) + 4ll * groups * (1 + 2 * kMaxWindows + kPasses * (kStagedWaves + 1)) +
+         8ll * 7 * n + 16ll * (splat_bins(g) + 1) + 40ll * gg +
          8ll * 4 * g * f + 8ll * (g + 1) * g + 16ll * gg + 8ll * f + 32ll * 2048 + 8ll * (g + 1) +
          4096;
 }
 
 static long long job_setup_bytes(int n, int k) {
   const long long nk = 1ll * n * k;
-  return 8ll * nk /* upload */ + 8ll * nk /* nbr, pdir */ + 16ll * nk /* raw */ + 16ll * n;
+  return 8ll * nk /* upload */ + 8ll * nk /* nbr, pdir */ + 16ll * nk /* raw */ + 16ll * n +
+         16ll * kPasses * n /* segment scans */;
 }
 
-// launch-table parameter of the staged attraction (other attraction launches
-// carry minus their FlatVariant or their lanes per row)
+// launch-table parameters of the segment-blocked attraction kernels (the
+// row-group kernel's launches carry their lanes per row)
 constexpr int kStagedLaunch = -1000;
+constexpr int kLanesLaunch = -1001;
 
 class TsneBatch {
  public:
@@ -889,12 +891,8 @@ class TsneBatch {
     d.row_ptr = p.csr.row_ptr;
     d.col = p.csr.col;
     d.val = p.csr.val;
-    d.row_start = p.row_start;
-    d.word_row = p.word_row;
-    d.stage_lo = p.stage_lo;
-    d.stage_m = p.stage_m;
-    d.attr_waves = p.attr_waves;
-    d.wave_row = p.wave_row;
+    d.blk = p.blk;
+    d.groups = p.groups;
     d.splat = p.splat;
     d.z1 = p.z1;
     d.z2 = p.z2;
@@ -990,20 +988,19 @@ class TsneBatch {
     {
       std::vector<std::pair<int, int>> ent;
       for (int s = 0; s < max_jobs; ++s)
-        if (slots[s] && slots[s]->staged) ent.push_back({s, slots[s]->stage_groups});
+        if (slots[s] && slots[s]->attr_kind == kAttrStaged) ent.push_back({s, slots[s]->groups});
       add_launch(4, kStagedLaunch, ent);
     }
-    for (int v = kFlat8Prefetch; v <= kFlat4; ++v) {
+    {
       std::vector<std::pair<int, int>> ent;
       for (int s = 0; s < max_jobs; ++s)
-        if (slots[s] && !slots[s]->staged && slots[s]->flat_attr == v)
-          ent.push_back({s, slots[s]->attr_waves / (kThreads / 64)});
-      add_launch(4, -v, ent);  // negative parameter: entry-balanced variant
+        if (slots[s] && slots[s]->attr_kind == kAttrLanes) ent.push_back({s, slots[s]->groups});
+      add_launch(4, kLanesLaunch, ent);
     }
     for (int tpr = 4; tpr <= 64; tpr *= 2) {
       std::vector<std::pair<int, int>> ent;
       for (int s = 0; s < max_jobs; ++s)
-        if (slots[s] && !slots[s]->flat_attr && slots[s]->tpr == tpr)
+        if (slots[s] && slots[s]->attr_kind == kAttrRows && slots[s]->tpr == tpr)
           ent.push_back({s, slots[s]->upd_blocks});
       add_launch(4, tpr, ent);
     }
@@ -1068,21 +1065,11 @@ class TsneBatch {
         case 4:
           switch (L.param) {
             case kStagedLaunch:
-              batch_attract_staged<8, true><<<L.blocks, kStagedThreads,
-                                              kStagedCap * sizeof(float2), s>>>(d_desc, sl, fi,
-                                                                               L.m);
+              batch_attract_blocked<true><<<L.blocks, kStagedThreads, kStagedLds, s>>>(
+                  d_desc, sl, fi, L.m);
               break;
-            case -kFlat8Prefetch:
-              batch_attract_flat<8, true><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
-              break;
-            case -kFlat8:
-              batch_attract_flat<8, false><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
-              break;
-            case -kFlat4Prefetch:
-              batch_attract_flat<4, true><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
-              break;
-            case -kFlat4:
-              batch_attract_flat<4, false><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
+            case kLanesLaunch:
+              batch_attract_blocked<false><<<L.blocks, kStagedThreads, 0, s>>>(d_desc, sl, fi, L.m);
               break;
             case 4: batch_attract<4><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
             case 8: batch_attract<8><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
@@ -1113,9 +1100,9 @@ class TsneBatch {
       FE_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&batch_cols<2048>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize,
                                    4 * 2048 * static_cast<int>(sizeof(float2))));
-      FE_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&batch_attract_staged<8, true>),
+      FE_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&batch_attract_blocked<true>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   kStagedCap * static_cast<int>(sizeof(float2))));
+                                   static_cast<int>(kStagedLds)));
       attr_done = true;
     }
     // Captured chunks of kChunk iterations replay the current launch table;
@@ -1331,7 +1318,7 @@ int fastembedr_hip_tsne_plan_info(fastembedr_hip_tsne_plan* plan, long long* inf
   info[1] = p.k;
   info[2] = p.g;
   info[3] = p.csr.nnz;
-  info[4] = p.flat_attr ? 0 : p.tpr;
+  info[4] = p.attr_kind == fe::kAttrRows ? p.tpr : 0;
   info[5] = static_cast<long long>(p.state.capacity() + p.csr_store.capacity());
   info[6] = p.iters_run;
   info[7] = static_cast<long long>(p.setup_us);

@@ -25,12 +25,8 @@ struct JobDesc {
   const int* row_ptr;
   const int* col;
   const float* val;
-  const unsigned long long* row_start;  // entry-balanced attraction tables
-  const int* word_row;
-  const int* stage_lo;  // LDS windows of the staged attraction
-  const int* stage_m;
-  int attr_waves;
-  const int* wave_row;
+  BlockTables blk;  // segment-blocked attraction tables
+  int groups;
   SplatBuffers splat;
   float2* z1;
   float2* z2;
@@ -135,48 +131,20 @@ __global__ __launch_bounds__(kThreads) void batch_rows_inv(JobDesc* jobs, const 
   conv_rows_inv_body<F>(J->o1, J->o2, J->tw, J->zpart, J->st, J->field, blockIdx.x - first[e]);
 }
 
-template <int LANE, bool PREFETCH>
-__global__ __launch_bounds__(kThreads) void batch_attract_flat(JobDesc* jobs,
-                                                               const int* __restrict__ slot,
-                                                               const int* __restrict__ first,
-                                                               int m) {
-  const int e = find_entry(first, m);
-  JobDesc* J = jobs + slot[e];
-  if (!J->active) return;
-  const float exag = J->iter < J->early_iter ? J->early_exag : J->exag;
-  const int wave = xcd_remap(static_cast<int>(blockIdx.x) - first[e], first[e + 1] - first[e]) *
-                       (kThreads / 64) +
-                   static_cast<int>(threadIdx.x >> 6);
-  const GlobalPoints src{J->y[J->cur], J->st->mean_xf, J->st->mean_yf};
-  attract_flat_body<LANE, PREFETCH>(src, J->row_ptr, J->col, J->val, J->row_start, J->word_row,
-                                    J->wave_row, exag, J->attr, J->n, wave);
-}
-
-// staged attraction: one block of kStagedThreads per 16-wave group of a job
-template <int LANE, bool PREFETCH>
-__global__ __launch_bounds__(kStagedThreads) void batch_attract_staged(JobDesc* jobs,
-                                                                       const int* __restrict__ slot,
-                                                                       const int* __restrict__ first,
-                                                                       int m) {
+// segment-blocked attraction: one workgroup per group of a job
+template <bool STAGE>
+__global__ __launch_bounds__(kStagedThreads) void batch_attract_blocked(JobDesc* jobs,
+                                                                        const int* __restrict__ slot,
+                                                                        const int* __restrict__ first,
+                                                                        int m) {
   extern __shared__ float2 sy[];
   const int e = find_entry(first, m);
   JobDesc* J = jobs + slot[e];
   if (!J->active) return;
-  const int wg = xcd_remap(static_cast<int>(blockIdx.x) - first[e], first[e + 1] - first[e]);
-  const int lo = J->stage_lo[wg], mm = J->stage_m[wg];
-  const float2* y = J->y[J->cur];
-  const float mx = J->st->mean_xf, my = J->st->mean_yf;
-  for (int i = threadIdx.x; i < mm; i += kStagedThreads) {
-    const float2 p = y[lo + i];
-    sy[i] = make_float2(p.x - mx, p.y - my);
-  }
-  __syncthreads();
-  const int wave = wg * kStagedWaves + static_cast<int>(threadIdx.x >> 6);
-  if (wave >= J->attr_waves) return;
   const float exag = J->iter < J->early_iter ? J->early_exag : J->exag;
-  const StagedPoints src{y, sy, lo, static_cast<unsigned int>(mm), mx, my};
-  attract_flat_body<LANE, PREFETCH>(src, J->row_ptr, J->col, J->val, J->row_start, J->word_row,
-                                    J->wave_row, exag, J->attr, J->n, wave);
+  const int g = xcd_remap(static_cast<int>(blockIdx.x) - first[e], first[e + 1] - first[e]);
+  attract_group<STAGE>(J->y[J->cur], J->blk, J->st->mean_xf, J->st->mean_yf, exag, J->attr, J->n,
+                       g, sy);
 }
 
 template <int TPR>

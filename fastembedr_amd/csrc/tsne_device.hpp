@@ -955,87 +955,96 @@ __global__ __launch_bounds__(kThreads) void attract_kernel(const float2* __restr
 }
 
 // ----------------------------------------------------------------------------
-// 5a'. entry-balanced attraction (the terms of attract_body summed in another
-//     fixed order).  Wave w owns the whole rows wave_row[w] .. wave_row[w+1]-1,
-//     cut so that every wave has about the same number of entries, and walks
-//     them in aligned steps of 64*LANE entries: lane l takes the LANE
-//     consecutive entries 64*LANE*s + LANE*l .. (16-byte loads of col and
-//     val).  Bit i of row_start[b] is set when entry 64*b + i begins a row
-//     and word_row[b] is the row of entry 64*b, so a lane finds the rows of
-//     its entries from one word and one row number, without talking to the
-//     other lanes.  A lane sums its entries row by row in registers: rows that
-//     begin and end inside the lane are written at once; the piece before the
-//     lane's first row start (the head) closes the row that was open when the
-//     lane began, and the piece after its last row start (the tail) stays
-//     open.  One segmented scan over the 64 lanes per step (lane l adds lane
-//     l - o while both lie in one segment, a segment starting at a lane with a
-//     row start) carries the open row from lane to lane, and lane 63 carries it
-//     into the next step.  The entry e1 that ends the wave's range counts as a
-//     row start, so the last row closes like any other.  PREFETCH issues the
-//     next step's column loads before this step computes.  Every row must have
-//     at least one entry (checked when the plan is built); col, val, row_start
-//     and word_row carry kFlatPad entries of padding so that whole steps and
-//     the prefetch load unconditionally.
+// 5a'. segment-blocked attraction (the terms of attract_body summed in another
+//     fixed order).
+//
+//     Groups.  The rows of P are cut into groups of consecutive rows with
+//     about the same number of entries; a workgroup of 16 waves takes one
+//     group.  Each group has up to kMaxWindows windows: ranges of
+//     kWindowPoints consecutive layout points chosen when the plan is built
+//     to hold many of the group's columns (neighbours lie close in index
+//     order when the input groups similar points).  The workgroup makes one
+//     pass per window, copying the window into LDS, and a last pass for the
+//     entries outside every window.
+//
+//     Segments.  Row r's entries whose columns fall in window w form the
+//     segment (r, w); the rest form its last segment.  A segment keeps the
+//     row's column order.  Segments are laid out group by group, pass by
+//     pass, row by row, each padded to a multiple of 8 entries with entries
+//     (column = the row itself, value 0) whose terms are exactly zero, so
+//     that every aligned group of 8 entries (one lane's share of a step)
+//     lies in one segment.
+//
+//     Waves.  Within a pass, wave w owns the whole segments pass_wave[w] ..
+//     pass_wave[w+1]-1 (about the same number of entries per wave) and walks
+//     them in aligned steps of 512 entries: lane l takes entries 512 s + 8 l
+//     .. + 7 and their row lane_row[64 s + l], sums their terms in order and
+//     scales the sum by the exaggeration.  Bit l of starts[s] is set when
+//     lane l of step s begins a segment.  One segmented scan over the 64
+//     lanes per step (a segment of lanes begins at a lane that begins a
+//     segment) carries each segment's sum to the lane where the next one
+//     begins, which adds it to the force of the row of the lane below it;
+//     lane 63 carries the open segment and its row into the next step.  The entry e1 that ends the wave's
+//     range counts as a segment start, so the last segment closes like any
+//     other.  A row's force is therefore zero plus its segment sums in pass
+//     order, whichever waves and workgroups run them.
+//
+//     The padded arrays carry kLanePad entries past the end so that whole
+//     steps and the prefetch of the next step load unconditionally.
 // ----------------------------------------------------------------------------
-constexpr int kFlatMaxLane = 8;
-constexpr int kFlatStep8 = 64 * 8;               // entries per step with 8 per lane
-constexpr int kFlatPad = 2 * 64 * kFlatMaxLane;  // padding entries after P
-constexpr int kFlatPadWords = kFlatPad / 64;
+constexpr int kLaneEntries = 8;
+constexpr int kLaneStep = 64 * kLaneEntries;  // entries per wave step
+constexpr int kLanePad = 2 * kLaneStep;       // padding entries after the segments
+constexpr int kLanePadSteps = kLanePad / kLaneStep;
+constexpr int kStagedWaves = 16;              // waves of a group's workgroup
+constexpr int kStagedThreads = 64 * kStagedWaves;
+constexpr int kWindowPoints = 20479;          // + one zero point: 160 KB of LDS
+constexpr size_t kStagedLds = (kWindowPoints + 1) * sizeof(float2);
+constexpr int kMaxWindows = 4;
+constexpr int kPasses = kMaxWindows + 1;      // the last pass: entries outside the windows
+constexpr int kWindowMinEntries = 2048;       // fewer entries do not pay for a window
+constexpr int kWindowBins = 1024;             // candidate windows counted per group
 
-template <int LANE>
-struct FlatCols {
-  int j[LANE];
-  unsigned long long word;  // row starts of the 64-entry group of the lane
-  int wrow;                 // row of that group's first entry
+// attraction kernels: one lane group per row of P (any P), and the
+// segment-blocked kernel reading the layout directly or through the windows
+enum AttractionKind { kAttrRows = 0, kAttrLanes = 1, kAttrStaged = 2 };
+
+// the segments and their tables
+struct BlockTables {
+  const int* seg_ptr;                // segment offsets (segments + 1)
+  const int* col;                    // padded entries (+ kLanePad)
+  const float* val;
+  const int* lane_row;               // row of every 8 padded entries
+  const unsigned long long* starts;  // per step: lanes that begin a segment
+  const int* group_row;              // first row of every group (groups + 1)
+  const int2* window;                // per group and window: (lo, points); 0 points: none
+  const int* pass_wave;              // per group, pass, wave: first segment (17 per pass)
 };
 
-template <int LANE>
-__device__ __forceinline__ void load_lane(const int* __restrict__ p, int (&out)[LANE]) {
-#pragma unroll
-  for (int q = 0; q < LANE / 4; ++q) {
-    const int4 v = *reinterpret_cast<const int4*>(p + 4 * q);
-    out[4 * q] = v.x;
-    out[4 * q + 1] = v.y;
-    out[4 * q + 2] = v.z;
-    out[4 * q + 3] = v.w;
-  }
-}
+struct LaneCols {
+  int j[kLaneEntries];
+  int row;
+  unsigned long long starts;
+};
 
-template <int LANE>
-__device__ __forceinline__ void load_lane(const float* __restrict__ p, float (&out)[LANE]) {
-#pragma unroll
-  for (int q = 0; q < LANE / 4; ++q) {
-    const float4 v = *reinterpret_cast<const float4*>(p + 4 * q);
-    out[4 * q] = v.x;
-    out[4 * q + 1] = v.y;
-    out[4 * q + 2] = v.z;
-    out[4 * q + 3] = v.w;
-  }
-}
-
-template <int LANE>
-__device__ __forceinline__ FlatCols<LANE> flat_cols(const int* __restrict__ col,
-                                                    const unsigned long long* __restrict__ row_start,
-                                                    const int* __restrict__ word_row, int base,
-                                                    int lane) {
-  const int eb = base + lane * LANE;
-  const int wix = eb >> 6;
-  FlatCols<LANE> c;
-  load_lane<LANE>(col + eb, c.j);
-  c.word = row_start[wix];
-  c.wrow = word_row[wix];
+__device__ __forceinline__ LaneCols lane_cols(const BlockTables& t, int base, int lane) {
+  LaneCols c;
+  const int* p = t.col + base + kLaneEntries * lane;
+  const int4 a = *reinterpret_cast<const int4*>(p);
+  const int4 b = *reinterpret_cast<const int4*>(p + 4);
+  c.j[0] = a.x;
+  c.j[1] = a.y;
+  c.j[2] = a.z;
+  c.j[3] = a.w;
+  c.j[4] = b.x;
+  c.j[5] = b.y;
+  c.j[6] = b.z;
+  c.j[7] = b.w;
+  c.row = t.lane_row[base / kLaneEntries + lane];
+  c.starts = t.starts[base / kLaneStep];
   return c;
 }
 
-// One step of one lane: issues the step's value loads and gathers (and with
-// PREFETCH the next step's column loads), then sums the lane's entries row by
-// row.  FAST: the lane's entries lie in at most two rows (its first and its
-// last), so two point loads serve all of them; otherwise (rows shorter than
-// LANE entries) each entry loads its own point.
-// Wave-wide data movement by DPP (no LDS round trip): row_shr:o within rows
-// of 16 lanes, row_bcast:15 (lane 15 of each row to the next row, written to
-// rows 1 and 3) and row_bcast:31 (lane 31 to rows 2 and 3), wave_shr:1.
-// Lanes without a source read 0.
 template <int CTRL, int ROW_MASK>
 __device__ __forceinline__ float dpp_move(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, true));
@@ -1064,336 +1073,440 @@ __device__ __forceinline__ void segmented_wave_scan(float& sx, float& sy, int la
 #undef FE_SEG_STEP
 }
 
-// Sources of centred points for the attraction: fl(y - mean) straight from
-// the layout, or from a window of the layout staged in LDS (points outside
-// the window still come from the layout).  Both give the same values.
+// Sources of layout points for the attraction: straight from the layout, or
+// from a window of the layout staged in LDS.  fetch() issues the loads of a
+// point and value() forms it from what they returned; a step issues every
+// fetch before it uses any value, so all its loads are in flight together.
+// Both give the stored point (the caller centres it, fl(y - mean)).
 struct GlobalPoints {
   const float2* __restrict__ y;
   float mx, my;
-  __device__ __forceinline__ float2 operator()(int j) const {
-    const float2 p = y[j];
-    return make_float2(p.x - mx, p.y - my);
-  }
+  using Fetch = float2;
+  __device__ __forceinline__ Fetch fetch(int j) const { return y[j]; }
+  __device__ __forceinline__ float2 value(const Fetch& f) const { return f; }
 };
 
+// A gather reads the window (slot m, which holds zeros, for points outside
+// it), and only the lanes whose point lies outside also load it from the
+// layout; the point is the sum of the two.
 struct StagedPoints {
   const float2* __restrict__ y;
-  const float2* lds;  // centred points lo .. lo + m - 1
+  const float2* lds;  // points lo .. lo + m - 1, then a zero point
   int lo;
   unsigned int m;
   float mx, my;
-  __device__ __forceinline__ float2 operator()(int j) const {
+  struct Fetch {
+    float2 a, b;
+  };
+  __device__ __forceinline__ Fetch fetch(int j) const {
     const unsigned int o = static_cast<unsigned int>(j - lo);
-    if (o < m) return lds[o];
-    const float2 p = y[j];
-    return make_float2(p.x - mx, p.y - my);
+    const bool in = o < m;
+    Fetch f;
+    f.a = lds[in ? o : m];
+    f.b = make_float2(0.f, 0.f);
+    if (!in) f.b = y[j];
+    return f;
+  }
+  __device__ __forceinline__ float2 value(const Fetch& f) const {
+    return make_float2(f.a.x + f.b.x, f.a.y + f.b.y);
   }
 };
 
-template <int LANE, bool PREFETCH, bool FAST, class SRC>
-__device__ __forceinline__ void flat_entries(const SRC& src,
-                                             const int* __restrict__ col,
-                                             const float* __restrict__ val,
-                                             const unsigned long long* __restrict__ row_start,
-                                             const int* __restrict__ word_row,
-                                             float2* __restrict__ attr, const FlatCols<LANE>& cur,
-                                             FlatCols<LANE>* next, int base, int lane,
-                                             int open_row, unsigned int rbits,
-                                             unsigned int mbits, unsigned int vmask, int n,
-                                             float exaggeration, float* hx, float* hy, float* tx,
-                                             float* ty, int* hrow) {
-  constexpr int kStep = 64 * LANE;
-  const int eb = base + lane * LANE;
-  float v[LANE];
-  load_lane<LANE>(val + eb, v);
-  int row[LANE];
-#pragma unroll
-  for (int u = 0; u < LANE; ++u) row[u] = open_row + __popc(rbits & ((2u << u) - 1u));
-  const int ra = min(max(row[0], 0), n - 1), rb = min(max(row[LANE - 1], 0), n - 1);
-  // gathers, all unconditional (an invalid entry reads a valid point)
-  float2 pj[LANE];
-#pragma unroll
-  for (int u = 0; u < LANE; ++u) pj[u] = src(((vmask >> u) & 1u) ? cur.j[u] : ra);
-  float2 pa = make_float2(0.f, 0.f), pb = pa;
-  if constexpr (FAST) {
-    pa = src(ra);
-    pb = src(rb);
-  }
-  if constexpr (PREFETCH && FAST) *next = flat_cols<LANE>(col, row_start, word_row, base + kStep, lane);
-  float sx = 0.f, sy = 0.f, ax = 0.f, ay = 0.f;
-  int h = -1;
-#pragma unroll
-  for (int u = 0; u < LANE; ++u) {
-    if ((mbits >> u) & 1u) {
-      if (h == -1) {
-        ax = sx;
-        ay = sy;
-        // -2: the start is entry 0 of P, so no row is open before it
-        h = row[u] > 0 ? row[u] - 1 : -2;
-      } else {
-        attr[row[u] - 1] = make_float2(sx, sy);
-      }
-      sx = 0.f;
-      sy = 0.f;
-    }
-    float2 p;
-    if constexpr (FAST) p = row[u] == row[0] ? pa : pb;
-    else p = src(min(max(row[u], 0), n - 1));  // rows shorter than LANE entries: rare
-    const float dx = p.x - pj[u].x, dy = p.y - pj[u].y;
-    const float q = __builtin_amdgcn_rcpf(1.0f + dx * dx + dy * dy);
-    const float cf = ((vmask >> u) & 1u) ? exaggeration * v[u] * q : 0.0f;
-    sx += cf * dx;
-    sy += cf * dy;
-  }
-  if constexpr (PREFETCH && !FAST) *next = flat_cols<LANE>(col, row_start, word_row, base + kStep, lane);
-  *hx = ax;
-  *hy = ay;
-  *tx = sx;
-  *ty = sy;
-  *hrow = h;
-}
-
-template <int LANE, bool PREFETCH, class SRC>
-__device__ __forceinline__ void attract_flat_body(const SRC& src,
-                                                  const int* __restrict__ row_ptr,
-                                                  const int* __restrict__ col,
-                                                  const float* __restrict__ val,
-                                                  const unsigned long long* __restrict__ row_start,
-                                                  const int* __restrict__ word_row,
-                                                  const int* __restrict__ wave_row,
-                                                  float exaggeration, float2* __restrict__ attr,
-                                                  int n, int wave) {
-  constexpr int kStep = 64 * LANE;
-  constexpr unsigned int kLaneMask = (1u << LANE) - 1u;
+// The segments r0 .. r1-1 of one wave; a closed segment's sum is added to
+// its row's force in attr.
+template <class SRC>
+__device__ __forceinline__ void attract_segments(const SRC& src, const BlockTables& t, int r0,
+                                                 int r1, float exaggeration,
+                                                 float2* __restrict__ attr) {
   const int lane = threadIdx.x & 63;
-  const int r0 = wave_row[wave], r1 = wave_row[wave + 1];
-  if (r0 >= r1) return;
-  const int e0 = row_ptr[r0], e1 = row_ptr[r1];
-  const int sh = (lane * LANE) & 63;
+  const int e0 = t.seg_ptr[r0], e1 = t.seg_ptr[r1];
   const unsigned long long upto_lane = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-  float cx = 0.0f, cy = 0.0f;  // the row open at the start of the step
-  int base = (e0 / kStep) * kStep;
-  FlatCols<LANE> cur = flat_cols<LANE>(col, row_start, word_row, base, lane);
-  for (; base <= e1; base += kStep) {
-    const int eb = base + lane * LANE;
-    // valid entries [vlo, vhi) of the lane; the terminator e1 at position t
-    const int vlo = min(max(e0 - eb, 0), LANE), vhi = min(max(e1 - eb, 0), LANE);
-    const unsigned int vmask = ((1u << vhi) - 1u) & ~((1u << vlo) - 1u);
-    const int t = e1 - eb;
-    const unsigned int term = (t >= 0 && t < LANE) ? (1u << t) : 0u;
-    const unsigned int rbits = (static_cast<unsigned int>(cur.word >> sh) & kLaneMask) | term;
-    const unsigned int mbits = rbits & (vmask | term);
-    // row open before the lane (the row of entry eb - 1; entry 0 of a word
-    // already counts in word_row); the terminator counts as a row start
-    const int open_row = cur.wrow + __popcll(cur.word & ((1ull << sh) - 1ull) & ~1ull) -
-                         (sh == 0 ? static_cast<int>(cur.word & 1ull) : 0);
-    float hx, hy, tx, ty;
-    int hrow;  // row closed by the lane's first row start (-1: none)
-    FlatCols<LANE> next;
-    if (__ballot(__popc(rbits & (kLaneMask - 1u)) > 1) == 0ull)
-      flat_entries<LANE, PREFETCH, true>(src, col, val, row_start, word_row, attr, cur, &next,
-                                         base, lane, open_row, rbits, mbits, vmask, n,
-                                         exaggeration, &hx, &hy, &tx, &ty, &hrow);
-    else
-      flat_entries<LANE, PREFETCH, false>(src, col, val, row_start, word_row, attr, cur, &next,
-                                          base, lane, open_row, rbits, mbits, vmask, n,
-                                          exaggeration, &hx, &hy, &tx, &ty, &hrow);
-    if constexpr (PREFETCH) cur = next;
-    else cur = flat_cols<LANE>(col, row_start, word_row, base + kStep, lane);
-    const bool seen = hrow != -1;
-    // segmented scan of the open-row pieces: a lane with a row start passes on
-    // its tail, any other lane adds its whole sum to the open row
-    const unsigned long long flags = __ballot(seen);
-    const unsigned long long upto = flags & upto_lane;
-    const int lim = upto ? 63 - __clzll(upto) : 0;  // first lane of this lane's segment
-    float sx = tx, sy = ty;
-    if (lane == 0 && !seen) {
-      sx += cx;
-      sy += cy;
+  const float mx = src.mx, my = src.my;
+  float cx = 0.0f, cy = 0.0f;  // the segment open at the start of the step
+  int crow = 0;                // and its row
+  int base = e0 & ~(kLaneStep - 1);
+  LaneCols cur = lane_cols(t, base, lane);
+  for (; base <= e1; base += kLaneStep) {
+    const int eb = base + kLaneEntries * lane;
+    const bool valid = eb >= e0 && eb < e1;
+    // a lane that begins a segment, or the end of the range, closes the one
+    // below; the one closed at e0 is not this wave's
+    const bool seen = eb == e1 || (valid && ((cur.starts >> lane) & 1ull));
+    const int row = cur.row;
+    float v[kLaneEntries];
+    {
+      const float4 a = *reinterpret_cast<const float4*>(t.val + eb);
+      const float4 b = *reinterpret_cast<const float4*>(t.val + eb + 4);
+      v[0] = a.x;
+      v[1] = a.y;
+      v[2] = a.z;
+      v[3] = a.w;
+      v[4] = b.x;
+      v[5] = b.y;
+      v[6] = b.z;
+      v[7] = b.w;
     }
-    segmented_wave_scan(sx, sy, lane, lim);
-    // open row arriving at this lane: the carry for lane 0, else the scan of
-    // the lane below
-    float ix = dpp_wave_shr1(sx), iy = dpp_wave_shr1(sy);
+    // the row's own point (adjacent rows: through the caches), the gathers,
+    // then the next step's columns and rows
+    const float2 pr = src.y[row];
+    typename SRC::Fetch fj[kLaneEntries];
+#pragma unroll
+    for (int u = 0; u < kLaneEntries; ++u) fj[u] = src.fetch(cur.j[u]);
+    cur = lane_cols(t, base + kLaneStep, lane);
+    // keep every load of the step above the first wait
+    __builtin_amdgcn_sched_barrier(0);
+    const float xi = pr.x - mx, yi = pr.y - my;
+    float sx = 0.0f, sy = 0.0f;
+#pragma unroll
+    for (int u = 0; u < kLaneEntries; ++u) {
+      const float2 pj = src.value(fj[u]);
+      const float dx = xi - (pj.x - mx), dy = yi - (pj.y - my);
+      const float q = __builtin_amdgcn_rcpf(1.0f + dx * dx + dy * dy);
+      const float c = v[u] * q;
+      sx += c * dx;
+      sy += c * dy;
+    }
+    float tx = valid ? exaggeration * sx : 0.0f, ty = valid ? exaggeration * sy : 0.0f;
+    const unsigned long long upto = __ballot(seen) & upto_lane;
+    const int lim = upto ? 63 - __clzll(upto) : 0;  // first lane of this lane's segment
+    if (lane == 0 && !seen) {
+      tx += cx;
+      ty += cy;
+    }
+    segmented_wave_scan(tx, ty, lane, lim);
+    // the open segment arriving at this lane and its row: the carry for lane
+    // 0, else the lane below
+    float ix = dpp_wave_shr1(tx), iy = dpp_wave_shr1(ty);
+    int irow = __builtin_amdgcn_update_dpp(0, row, 0x138, 0xF, 0xF, true);
     if (lane == 0) {
       ix = cx;
       iy = cy;
+      irow = crow;
     }
-    if (seen && hrow >= r0) attr[hrow] = make_float2(ix + hx, iy + hy);
-    cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sx), 63));
-    cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sy), 63));
+    if (seen && eb > e0) {
+      const float2 a = attr[irow];
+      attr[irow] = make_float2(a.x + ix, a.y + iy);
+    }
+    cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tx), 63));
+    cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ty), 63));
+    crow = __builtin_amdgcn_readlane(row, 63);
   }
 }
 
-// variants of the entry-balanced attraction (entries per lane, prefetch)
-enum FlatVariant { kFlat8Prefetch = 1, kFlat8 = 2, kFlat4Prefetch = 3, kFlat4 = 4 };
-
-template <int LANE, bool PREFETCH>
-__global__ __launch_bounds__(kThreads) void attract_flat(
-    const float2* __restrict__ y_in, const int* __restrict__ row_ptr, const int* __restrict__ col,
-    const float* __restrict__ val, const unsigned long long* __restrict__ row_start,
-    const int* __restrict__ word_row, const int* __restrict__ wave_row,
-    const TsneStats* __restrict__ st, float exaggeration, float2* __restrict__ attr, int n,
-    int n_waves) {
-  const int wave = xcd_remap(blockIdx.x, gridDim.x) * (kThreads / 64) +
-                   static_cast<int>(threadIdx.x >> 6);
-  if (wave >= n_waves) return;
-  const GlobalPoints src{y_in, st->mean_xf, st->mean_yf};
-  attract_flat_body<LANE, PREFETCH>(src, row_ptr, col, val, row_start, word_row, wave_row,
-                                    exaggeration, attr, n, wave);
-}
-
-// 5a''. the entry-balanced attraction with a window of the layout staged in
-//      LDS.  A workgroup of kStagedWaves waves takes kStagedWaves consecutive
-//      wave ranges (consecutive rows); the window [stage_lo[w], stage_lo[w] +
-//      stage_m[w]) of centred points, chosen when the plan is built to cover
-//      the columns of those rows (their neighbours lie close in index order
-//      when the input groups similar points), is copied into LDS once and
-//      serves every gather inside it; columns outside still read the layout.
-//      Bit-identical to attract_flat.
-constexpr int kStagedWaves = 16;       // 1024 threads, one workgroup per CU
-constexpr int kStagedCap = 16384;      // points of the window (128 KB of LDS)
-constexpr int kStagedThreads = 64 * kStagedWaves;
-
-template <int LANE, bool PREFETCH>
-__global__ __launch_bounds__(kStagedThreads) void attract_staged(
-    const float2* __restrict__ y_in, const int* __restrict__ row_ptr, const int* __restrict__ col,
-    const float* __restrict__ val, const unsigned long long* __restrict__ row_start,
-    const int* __restrict__ word_row, const int* __restrict__ wave_row,
-    const int* __restrict__ stage_lo, const int* __restrict__ stage_m,
-    const TsneStats* __restrict__ st, float exaggeration, float2* __restrict__ attr, int n,
-    int n_waves) {
-  extern __shared__ float2 sy[];
-  const int wg = xcd_remap(blockIdx.x, gridDim.x);
-  const int lo = stage_lo[wg], m = stage_m[wg];
-  const float mx = st->mean_xf, my = st->mean_yf;
-  for (int i = threadIdx.x; i < m; i += kStagedThreads) {
-    const float2 p = y_in[lo + i];
-    sy[i] = make_float2(p.x - mx, p.y - my);
-  }
+__device__ __forceinline__ void stage_window(const float2* __restrict__ y, int lo, int m,
+                                             float2* sy) {
+  for (int i = threadIdx.x; i < m; i += kStagedThreads) sy[i] = y[lo + i];
+  if (threadIdx.x == 0) sy[m] = make_float2(0.f, 0.f);
   __syncthreads();
-  const int wave = wg * kStagedWaves + static_cast<int>(threadIdx.x >> 6);
-  if (wave >= n_waves) return;
-  const StagedPoints src{y_in, sy, lo, static_cast<unsigned int>(m), mx, my};
-  attract_flat_body<LANE, PREFETCH>(src, row_ptr, col, val, row_start, word_row, wave_row,
-                                    exaggeration, attr, n, wave);
 }
 
-// set-up of the staged windows: column bounds of every wave range (one wave
-// each), then per workgroup the window, then the entries it covers
-__global__ __launch_bounds__(kThreads) void attract_col_bounds(const int* __restrict__ row_ptr,
-                                                               const int* __restrict__ col,
-                                                               const int* __restrict__ wave_row,
-                                                               int n_waves, int2* bounds) {
-  const int wave = blockIdx.x * (kThreads / 64) + static_cast<int>(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (wave >= n_waves) return;
-  const int e0 = row_ptr[wave_row[wave]], e1 = row_ptr[wave_row[wave + 1]];
-  int mn = INT_MAX, mx = -1;
-  for (int e = e0 + lane; e < e1; e += 64) {
-    const int j = col[e];
-    mn = min(mn, j);
-    mx = max(mx, j);
-  }
-  mn = wave_min_i(mn);
-  mx = wave_max_i(mx);
-  if (lane == 0) bounds[wave] = make_int2(mn, mx);
-}
-
-__global__ __launch_bounds__(kThreads) void attract_stage_windows(const int2* __restrict__ bounds,
-                                                                  const int* __restrict__ wave_row,
-                                                                  int n_waves, int n,
-                                                                  int* __restrict__ stage_lo,
-                                                                  int* __restrict__ stage_m) {
-  const int w = blockIdx.x * kThreads + threadIdx.x;
-  const int groups = (n_waves + kStagedWaves - 1) / kStagedWaves;
-  if (w >= groups) return;
-  const int w0 = w * kStagedWaves, w1 = min(n_waves, w0 + kStagedWaves);
-  int mn = INT_MAX, mx = -1;
-  for (int v = w0; v < w1; ++v) {
-    mn = min(mn, bounds[v].x);
-    mx = max(mx, bounds[v].y);
-  }
-  int lo = 0, m = 0;
-  if (mx >= mn) {
-    if (mx - mn + 1 <= kStagedCap) {
-      lo = mn;
-      m = mx - mn + 1;
+// One group: zero its rows' forces, then the passes in order.  STAGE: the
+// window passes read the staged windows (the same values as the layout).
+template <bool STAGE>
+__device__ __forceinline__ void attract_group(const float2* __restrict__ y, const BlockTables& t,
+                                              float mx, float my, float exaggeration,
+                                              float2* __restrict__ attr, int n, int g,
+                                              float2* sy) {
+  const int rb = t.group_row[g], re = t.group_row[g + 1];
+  for (int r = rb + static_cast<int>(threadIdx.x); r < re; r += kStagedThreads)
+    attr[r] = make_float2(0.f, 0.f);
+  __syncthreads();
+  const int w = static_cast<int>(threadIdx.x >> 6);
+  for (int k = 0; k < kPasses; ++k) {
+    const int* pw = t.pass_wave + (g * kPasses + k) * (kStagedWaves + 1);
+    if (pw[0] == pw[kStagedWaves]) continue;  // the same for the whole workgroup
+    const int r0 = __builtin_amdgcn_readfirstlane(pw[w]);
+    const int r1 = __builtin_amdgcn_readfirstlane(pw[w + 1]);
+    if (STAGE && k < kMaxWindows) {
+      const int2 win = t.window[g * kMaxWindows + k];
+      stage_window(y, win.x, win.y, sy);
+      const StagedPoints src{y, sy, win.x, static_cast<unsigned int>(win.y), mx, my};
+      if (r0 < r1) attract_segments(src, t, r0, r1, exaggeration, attr);
     } else {
-      // centred on the group's own rows, inside the column bounds
-      const int mid = (wave_row[w0] + wave_row[w1]) / 2;
-      lo = min(max(mid - kStagedCap / 2, mn), mx - kStagedCap + 1);
-      m = kStagedCap;
+      const GlobalPoints src{y, mx, my};
+      if (r0 < r1) attract_segments(src, t, r0, r1, exaggeration, attr);
     }
+    __syncthreads();
   }
-  stage_lo[w] = lo;
-  stage_m[w] = min(m, n - lo);
 }
 
-__global__ __launch_bounds__(kThreads) void attract_stage_cover(const int* __restrict__ row_ptr,
-                                                                const int* __restrict__ col,
-                                                                const int* __restrict__ wave_row,
-                                                                const int* __restrict__ stage_lo,
-                                                                const int* __restrict__ stage_m,
-                                                                int n_waves,
-                                                                unsigned long long* hits) {
-  const int wave = blockIdx.x * (kThreads / 64) + static_cast<int>(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (wave >= n_waves) return;
-  const int g = wave / kStagedWaves;
-  const int lo = stage_lo[g];
-  const unsigned int m = static_cast<unsigned int>(stage_m[g]);
-  const int e0 = row_ptr[wave_row[wave]], e1 = row_ptr[wave_row[wave + 1]];
-  unsigned int c = 0;
-  for (int e = e0 + lane; e < e1; e += 64) c += static_cast<unsigned int>(col[e] - lo) < m ? 1u : 0u;
-  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-  if (lane == 0) atomicAdd(hits, static_cast<unsigned long long>(c));
+template <bool STAGE>
+__global__ __launch_bounds__(kStagedThreads) void attract_blocked(
+    const float2* __restrict__ y_in, BlockTables t, const TsneStats* __restrict__ st,
+    float exaggeration, float2* __restrict__ attr, int n) {
+  extern __shared__ float2 sy[];
+  attract_group<STAGE>(y_in, t, st->mean_xf, st->mean_yf, exaggeration, attr, n,
+                       xcd_remap(blockIdx.x, gridDim.x), sy);
 }
 
-// set-up of the entry-balanced attraction: row-start bits and empty-row count
-__global__ __launch_bounds__(kThreads) void attract_row_starts(const int* __restrict__ row_ptr,
-                                                               int n,
-                                                               unsigned long long* row_start,
-                                                               unsigned int* empty_rows) {
+// ---- set-up of the segments ----
+
+// lengths of the rows rounded up to whole lanes (at least one), to cut the
+// groups by padded entries
+__global__ __launch_bounds__(kThreads) void lane_lengths(const int* __restrict__ row_ptr, int n,
+                                                         int* __restrict__ len) {
   const int r = blockIdx.x * kThreads + threadIdx.x;
-  if (r >= n) return;
-  const int s = row_ptr[r], e = row_ptr[r + 1];
-  if (e > s)
-    atomicOr(row_start + (s >> 6), 1ull << (s & 63));
-  else
-    atomicAdd(empty_rows, 1u);
-}
-
-// word_row[b] = the row holding entry 64*b (the last row for padding words)
-__global__ __launch_bounds__(kThreads) void attract_word_rows(const int* __restrict__ row_ptr,
-                                                              int n, int words,
-                                                              int* __restrict__ word_row) {
-  const int b = blockIdx.x * kThreads + threadIdx.x;
-  if (b >= words) return;
-  const long long e = 64ll * b;
-  int lo = 0, hi = n - 1;  // last r with row_ptr[r] <= e
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (row_ptr[mid] <= e) lo = mid;
-    else hi = mid - 1;
+  if (r > n) return;
+  if (r == n) {
+    len[n] = 0;
+    return;
   }
-  word_row[b] = lo;
+  const int l = row_ptr[r + 1] - row_ptr[r];
+  len[r] = max(kLaneEntries, (l + kLaneEntries - 1) / kLaneEntries * kLaneEntries);
 }
 
-// wave_row[w] = first row starting at or after entry w * nnz / n_waves
-__global__ __launch_bounds__(kThreads) void attract_wave_rows(const int* __restrict__ row_ptr,
-                                                              int n, long long nnz, int n_waves,
-                                                              int* __restrict__ wave_row) {
+// first row starting at or after entry w * nnz / parts (parts + 1 values)
+__global__ __launch_bounds__(kThreads) void cut_rows(const int* __restrict__ row_ptr, int n,
+                                                     long long nnz, int parts,
+                                                     int* __restrict__ first_row) {
   const int w = blockIdx.x * kThreads + threadIdx.x;
-  if (w > n_waves) return;
-  const long long target = static_cast<long long>(w) * nnz / n_waves;
+  if (w > parts) return;
+  const long long target = static_cast<long long>(w) * nnz / parts;
   int lo = 0, hi = n;
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
     if (row_ptr[mid] < target) lo = mid + 1;
     else hi = mid;
   }
-  wave_row[w] = lo;
+  first_row[w] = lo;
+}
+
+// Windows of one group (one block): the column bounds; when they span at most
+// kWindowPoints one window covers them, else the columns are counted in
+// candidate windows lo = min + b * kWindowPoints and the (at most
+// kMaxWindows) fullest with kWindowMinEntries or more are kept, in order.
+__global__ __launch_bounds__(256) void blk_windows(const int* __restrict__ row_ptr,
+                                                   const int* __restrict__ col,
+                                                   const int* __restrict__ group_row, int n,
+                                                   int2* __restrict__ window) {
+  __shared__ int hist[kWindowBins];
+  __shared__ int red[2][256 / 64];
+  const int g = blockIdx.x;
+  const int e0 = row_ptr[group_row[g]], e1 = row_ptr[group_row[g + 1]];
+  int mn = INT_MAX, mx = -1;
+  for (int e = e0 + static_cast<int>(threadIdx.x); e < e1; e += 256) {
+    const int j = col[e];
+    mn = min(mn, j);
+    mx = max(mx, j);
+  }
+  mn = wave_min_i(mn);
+  mx = wave_max_i(mx);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = mn;
+    red[1][threadIdx.x >> 6] = mx;
+  }
+  for (int b = threadIdx.x; b < kWindowBins; b += 256) hist[b] = 0;
+  __syncthreads();
+  mn = min(min(red[0][0], red[0][1]), min(red[0][2], red[0][3]));
+  mx = max(max(red[1][0], red[1][1]), max(red[1][2], red[1][3]));
+  int2* out = window + g * kMaxWindows;
+  if (mx < mn) {  // no entries
+    if (threadIdx.x < kMaxWindows) out[threadIdx.x] = make_int2(0, 0);
+    return;
+  }
+  if (mx - mn + 1 <= kWindowPoints) {
+    if (threadIdx.x < kMaxWindows)
+      out[threadIdx.x] = threadIdx.x == 0 ? make_int2(mn, mx - mn + 1) : make_int2(0, 0);
+    return;
+  }
+  for (int e = e0 + static_cast<int>(threadIdx.x); e < e1; e += 256) {
+    const int b = (col[e] - mn) / kWindowPoints;
+    if (b < kWindowBins) atomicAdd(&hist[b], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int pick[kMaxWindows];
+    int np = 0;
+    for (int q = 0; q < kMaxWindows; ++q) {
+      int best = -1, bc = kWindowMinEntries - 1;
+      for (int b = 0; b < kWindowBins; ++b) {
+        if (hist[b] > bc) {
+          bc = hist[b];
+          best = b;
+        }
+      }
+      if (best < 0) break;
+      pick[np++] = best;
+      hist[best] = -1;
+    }
+    // ascending window order
+    for (int a = 1; a < np; ++a)
+      for (int c = a; c > 0 && pick[c - 1] > pick[c]; --c) {
+        const int tmp = pick[c];
+        pick[c] = pick[c - 1];
+        pick[c - 1] = tmp;
+      }
+    for (int q = 0; q < kMaxWindows; ++q) {
+      if (q < np) {
+        const int lo = mn + pick[q] * kWindowPoints;
+        out[q] = make_int2(lo, min(kWindowPoints, n - lo));
+      } else {
+        out[q] = make_int2(0, 0);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ int lower_col(const int* __restrict__ col, int lo, int hi, int v) {
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (col[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int group_of_row(const int* __restrict__ group_row, int groups, int r) {
+  int lo = 0, hi = groups - 1;  // last g with group_row[g] <= r
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (group_row[mid] <= r) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// slot of segment (row r, pass k): groups in order, passes in order, rows in order
+__device__ __forceinline__ int seg_slot(const int* __restrict__ group_row, int g, int k, int r) {
+  const int rb = group_row[g], rows = group_row[g + 1] - rb;
+  return kPasses * rb + k * rows + (r - rb);
+}
+
+// entries [a[k], b[k]) of row r in window k (empty when the window is absent)
+__device__ __forceinline__ void row_windows(const int* __restrict__ col, int s, int e,
+                                            const int2* __restrict__ win, int* a, int* b) {
+#pragma unroll
+  for (int k = 0; k < kMaxWindows; ++k) {
+    const int2 w = win[k];
+    if (w.y > 0) {
+      a[k] = lower_col(col, s, e, w.x);
+      b[k] = lower_col(col, a[k], e, w.x + w.y);
+    } else {
+      a[k] = b[k] = s;
+    }
+  }
+}
+
+// padded length of every segment and whether it is used (one thread per row)
+__global__ __launch_bounds__(kThreads) void blk_seg_lengths(const int* __restrict__ row_ptr,
+                                                            const int* __restrict__ col,
+                                                            const int* __restrict__ group_row,
+                                                            int groups,
+                                                            const int2* __restrict__ window, int n,
+                                                            int* __restrict__ len,
+                                                            int* __restrict__ used) {
+  const int r = blockIdx.x * kThreads + threadIdx.x;
+  if (r > n) return;
+  if (r == n) {  // the scans' last element
+    len[kPasses * n] = 0;
+    used[kPasses * n] = 0;
+    return;
+  }
+  const int g = group_of_row(group_row, groups, r);
+  const int s = row_ptr[r], e = row_ptr[r + 1];
+  int a[kMaxWindows], b[kMaxWindows];
+  row_windows(col, s, e, window + g * kMaxWindows, a, b);
+  int rest = e - s;
+  for (int k = 0; k < kPasses; ++k) {
+    int l;
+    if (k < kMaxWindows) {
+      l = b[k] - a[k];
+      rest -= l;
+    } else {
+      l = rest;
+    }
+    const int slot = seg_slot(group_row, g, k, r);
+    len[slot] = (l + kLaneEntries - 1) / kLaneEntries * kLaneEntries;
+    used[slot] = l > 0 ? 1 : 0;
+  }
+}
+
+// the padded segments (one wave per row): entries, padding, segment starts
+__global__ __launch_bounds__(kThreads) void blk_fill(const int* __restrict__ row_ptr,
+                                                     const int* __restrict__ col,
+                                                     const float* __restrict__ val,
+                                                     const int* __restrict__ group_row, int groups,
+                                                     const int2* __restrict__ window, int n,
+                                                     const int* __restrict__ off,
+                                                     const int* __restrict__ sid,
+                                                     int* __restrict__ seg_ptr,
+                                                     int* __restrict__ lane_row,
+                                                     int* __restrict__ pcol,
+                                                     float* __restrict__ pval,
+                                                     unsigned long long* starts) {
+  const int r = blockIdx.x * (kThreads / 64) + static_cast<int>(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (r >= n) return;
+  const int g = group_of_row(group_row, groups, r);
+  const int s = row_ptr[r], e = row_ptr[r + 1];
+  int a[kMaxWindows], b[kMaxWindows];
+  row_windows(col, s, e, window + g * kMaxWindows, a, b);
+  int rest_out = 0;  // entries of the last segment written so far
+  for (int k = 0; k < kPasses; ++k) {
+    const int slot = seg_slot(group_row, g, k, r);
+    const int o = off[slot], plen = off[slot + 1] - o;
+    if (plen == 0) continue;
+    if (lane == 0) {
+      seg_ptr[sid[slot]] = o;
+      atomicOr(starts + o / kLaneStep, 1ull << ((o / kLaneEntries) & 63));
+    }
+    for (int q = lane; q < plen / kLaneEntries; q += 64) lane_row[o / kLaneEntries + q] = r;
+    if (k < kMaxWindows) {
+      const int l = b[k] - a[k];
+      for (int q = lane; q < plen; q += 64) {
+        const bool real = q < l;
+        pcol[o + q] = real ? col[a[k] + q] : r;
+        pval[o + q] = real ? val[a[k] + q] : 0.0f;
+      }
+    } else {
+      // the entries outside every window, in column order
+      for (int c0 = s; c0 < e; c0 += 64) {
+        const int q = c0 + lane;
+        bool out = q < e;
+        if (out) {
+#pragma unroll
+          for (int w = 0; w < kMaxWindows; ++w) out = out && (q < a[w] || q >= b[w]);
+        }
+        const unsigned long long m = __ballot(out);
+        if (out) {
+          const int at = rest_out + __popcll(m & ((1ull << lane) - 1ull));
+          pcol[o + at] = col[q];
+          pval[o + at] = val[q];
+        }
+        rest_out += __popcll(m);
+      }
+      for (int q = rest_out + lane; q < plen; q += 64) {
+        pcol[o + q] = r;
+        pval[o + q] = 0.0f;
+      }
+    }
+  }
+}
+
+// per group and pass, the wave ranges: about the same number of entries each
+__global__ __launch_bounds__(kThreads) void blk_pass_waves(const int* __restrict__ group_row,
+                                                           int groups,
+                                                           const int* __restrict__ sid,
+                                                           const int* __restrict__ seg_ptr,
+                                                           int* __restrict__ pass_wave) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= groups * kPasses) return;
+  const int g = i / kPasses, k = i % kPasses;
+  const int rb = group_row[g], rows = group_row[g + 1] - rb;
+  const int slot = kPasses * rb + k * rows;
+  const int sb = sid[slot], se = sid[slot + rows];  // segments of this pass
+  const long long ob = seg_ptr[sb], oe = seg_ptr[se];
+  int* out = pass_wave + i * (kStagedWaves + 1);
+  for (int w = 0; w <= kStagedWaves; ++w) {
+    const long long target = ob + (oe - ob) * w / kStagedWaves;
+    int lo = sb, hi = se;  // first segment starting at or after target
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (seg_ptr[mid] < target) lo = mid + 1;
+      else hi = mid;
+    }
+    out[w] = lo;
+  }
 }
 
 

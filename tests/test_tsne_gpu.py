@@ -237,17 +237,16 @@ def test_attraction_kernels_agree_on_hub_rows(monkeypatch):
     assert np.diff(csr.row_ptr).max() >= 500
     ref = oracle.attractive(csr, y.astype(np.float64), 4.0)
     got = {}
-    for kind in ("flat8p", "staged", "flat8", "flat4p", "rows"):
+    for kind in ("lanes", "staged", "rows"):
         monkeypatch.setenv("FASTEMBEDR_HIP_ATTRACTION", kind)
         got[kind] = fe().gradient_diagnostic(idx, dist.astype(np.float32), y, 10, 4.0, 128)
         assert relative_l2(got[kind].attractive_force, ref) < 5e-5, kind
-    flat = got["flat8p"]
-    # the entry-balanced kernels sum in one order; the staged window changes
+    lanes = got["lanes"]
+    # the lane-padded kernels sum in one order; the staged window changes
     # only where the points are read from
-    assert np.array_equal(got["staged"].attractive_force, flat.attractive_force)
-    assert np.array_equal(got["flat8"].attractive_force, flat.attractive_force)
-    assert relative_l2(flat.attractive_force, got["rows"].attractive_force) < 1e-5
-    assert np.array_equal(flat.repulsive_force_fft, got["rows"].repulsive_force_fft)
+    assert np.array_equal(got["staged"].attractive_force, lanes.attractive_force)
+    assert relative_l2(lanes.attractive_force, got["rows"].attractive_force) < 1e-5
+    assert np.array_equal(lanes.repulsive_force_fft, got["rows"].repulsive_force_fft)
 
 
 def test_collapsed_layout_splits_heavy_tiles():
