@@ -276,17 +276,13 @@ int TsnePlan::setup_attraction(hipStream_t s) {
   sp.add<int>(1);
   sp.add<int>(slots + 1);
   sp.add<int>(slots + 1);
-  sp.add<int>(slots + 1);
-  sp.add<int>(slots + 1);
   if (scratch.init(sp.bytes, "t-SNE attraction set-up")) return 1;
   int* len = scratch.take<int>(n + 1);
   int* lrow = scratch.take<int>(n + 1);
   int* sums = scratch.take<int>(std::max(nb_rows, nb_slots));
   int* total = scratch.take<int>(1);
   int* seg_len = scratch.take<int>(slots + 1);
-  int* seg_used = scratch.take<int>(slots + 1);
   int* off = scratch.take<int>(slots + 1);
-  int* sid = scratch.take<int>(slots + 1);
   lane_lengths<<<(n + kThreads) / kThreads, kThreads, 0, s>>>(csr.row_ptr, n, len);
   FE_LAUNCH_CHECK("lane_lengths");
   if (exclusive_scan(len, n + 1, lrow, sums, total, s)) return 1;
@@ -301,54 +297,49 @@ int TsnePlan::setup_attraction(hipStream_t s) {
   ArenaPlan wp;
   wp.add<int>(groups + 1);
   wp.add<int2>(static_cast<size_t>(groups) * kMaxWindows);
-  wp.add<int>(static_cast<size_t>(groups) * kPasses * (kStagedWaves + 1));
+  wp.add<int2>(static_cast<size_t>(groups) * kPasses);
   if (group_store.init(wp.bytes, "t-SNE attraction groups")) return 1;
   int* group_row = group_store.take<int>(groups + 1);
   int2* window = group_store.take<int2>(static_cast<size_t>(groups) * kMaxWindows);
-  int* pass_wave = group_store.take<int>(static_cast<size_t>(groups) * kPasses * (kStagedWaves + 1));
+  int2* pass_range = group_store.take<int2>(static_cast<size_t>(groups) * kPasses);
   cut_rows<<<(groups + kThreads) / kThreads, kThreads, 0, s>>>(lrow, n, h_lnnz, groups, group_row);
   FE_LAUNCH_CHECK("cut_rows");
   blk_windows<<<groups, 256, 0, s>>>(csr.row_ptr, csr.col, group_row, n, window);
   FE_LAUNCH_CHECK("blk_windows");
   blk_seg_lengths<<<(n + kThreads) / kThreads, kThreads, 0, s>>>(
-      csr.row_ptr, csr.col, group_row, groups, window, n, seg_len, seg_used);
+      csr.row_ptr, csr.col, group_row, groups, window, n, seg_len);
   FE_LAUNCH_CHECK("blk_seg_lengths");
   if (exclusive_scan(seg_len, static_cast<int>(slots + 1), off, sums, total, s)) return 1;
-  if (exclusive_scan(seg_used, static_cast<int>(slots + 1), sid, sums, total, s)) return 1;
-  int h_ent = 0, h_segs = 0;
+  int h_ent = 0;
   FE_CHECK(hipMemcpyAsync(&h_ent, off + slots, sizeof(int), hipMemcpyDeviceToHost, s));
-  FE_CHECK(hipMemcpyAsync(&h_segs, sid + slots, sizeof(int), hipMemcpyDeviceToHost, s));
   FE_CHECK(hipStreamSynchronize(s));
   seg_entries = h_ent;
   const size_t ent = static_cast<size_t>(h_ent) + kLanePad;
   const int steps = h_ent / kLaneStep + 1 + kLanePadSteps;
   ArenaPlan ap;
-  ap.add<int>(static_cast<size_t>(h_segs) + 1);
   ap.add<int>(ent);
   ap.add<float>(ent);
   ap.add<int>(ent / kLaneEntries);
   ap.add<unsigned long long>(steps);
   if (attr_store.init(ap.bytes, "t-SNE attraction tables")) return 1;
-  int* seg_ptr = attr_store.take<int>(static_cast<size_t>(h_segs) + 1);
   int* pcol = attr_store.take<int>(ent);
   float* pval = attr_store.take<float>(ent);
   int* lane_row = attr_store.take<int>(ent / kLaneEntries);
   unsigned long long* starts = attr_store.take<unsigned long long>(steps);
-  FE_CHECK(hipMemcpyAsync(seg_ptr + h_segs, off + slots, sizeof(int), hipMemcpyDeviceToDevice, s));
   FE_CHECK(hipMemsetAsync(pcol + h_ent, 0, kLanePad * sizeof(int), s));
   FE_CHECK(hipMemsetAsync(pval + h_ent, 0, kLanePad * sizeof(float), s));
   FE_CHECK(hipMemsetAsync(lane_row + h_ent / kLaneEntries, 0,
                           kLanePad / kLaneEntries * sizeof(int), s));
   FE_CHECK(hipMemsetAsync(starts, 0, steps * sizeof(unsigned long long), s));
   blk_fill<<<(n + kThreads / 64 - 1) / (kThreads / 64), kThreads, 0, s>>>(
-      csr.row_ptr, csr.col, csr.val, group_row, groups, window, n, off, sid, seg_ptr, lane_row,
+      csr.row_ptr, csr.col, csr.val, group_row, groups, window, n, off, lane_row,
       pcol, pval, starts);
   FE_LAUNCH_CHECK("blk_fill");
-  blk_pass_waves<<<(groups * kPasses + kThreads - 1) / kThreads, kThreads, 0, s>>>(
-      group_row, groups, sid, seg_ptr, pass_wave);
-  FE_LAUNCH_CHECK("blk_pass_waves");
+  blk_pass_ranges<<<(groups * kPasses + kThreads - 1) / kThreads, kThreads, 0, s>>>(
+      group_row, groups, off, pass_range);
+  FE_LAUNCH_CHECK("blk_pass_ranges");
   FE_CHECK(hipStreamSynchronize(s));
-  blk = BlockTables{seg_ptr, pcol, pval, lane_row, starts, group_row, window, pass_wave};
+  blk = BlockTables{pcol, pval, lane_row, starts, group_row, window, pass_range};
   attr_kind = kind;
   static bool attr_set = false;
   if (!attr_set) {
@@ -794,7 +785,7 @@ static long long job_resident_bytes(int n, int k, int g) {
   const long long seg_nnz = 2 * nk + 1ll * kLaneEntries * segs + kLanePad;
   const long long groups = std::max<long long>(32, seg_nnz / (16 * 4096) + 1);
   return 8ll * 2 * nk + 4ll * (n + 1) + 8ll * seg_nnz + seg_nnz / 2 + 8ll * (seg_nnz / kLaneStep + 4) +
-         4ll * (segs + 1) + 4ll * groups * (1 + 2 * kMaxWindows + kPasses * (kStagedWaves + 1)) +
+         4ll * groups * (1 + 2 * kMaxWindows + 2 * kPasses) +
          8ll * 7 * n + 16ll * (splat_bins(g) + 1) + 40ll * gg +
          8ll * 4 * g * f + 8ll * (g + 1) * g + 16ll * gg + 8ll * f + 32ll * 2048 + 8ll * (g + 1) +
          4096;
@@ -803,7 +794,7 @@ static long long job_resident_bytes(int n, int k, int g) {
 static long long job_setup_bytes(int n, int k) {
   const long long nk = 1ll * n * k;
   return 8ll * nk /* upload */ + 8ll * nk /* nbr, pdir */ + 16ll * nk /* raw */ + 16ll * n +
-         16ll * kPasses * n /* segment scans */;
+         8ll * kPasses * n /* segment scan */;
 }
 
 // launch-table parameters of the segment-blocked attraction kernels (the

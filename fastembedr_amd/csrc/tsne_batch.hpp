@@ -143,8 +143,8 @@ __global__ __launch_bounds__(kStagedThreads) void batch_attract_blocked(JobDesc*
   if (!J->active) return;
   const float exag = J->iter < J->early_iter ? J->early_exag : J->exag;
   const int g = xcd_remap(static_cast<int>(blockIdx.x) - first[e], first[e + 1] - first[e]);
-  attract_group<STAGE>(J->y[J->cur], J->blk, J->st->mean_xf, J->st->mean_yf, exag, J->attr, J->n,
-                       g, sy);
+  attract_group<STAGE>(J->y[J->cur], J->blk, J->st->mean_xf, J->st->mean_yf, exag, J->attr, g,
+                       sy);
 }
 
 template <int TPR>

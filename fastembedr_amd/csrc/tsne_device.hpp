@@ -975,19 +975,22 @@ __global__ __launch_bounds__(kThreads) void attract_kernel(const float2* __restr
 //     that every aligned group of 8 entries (one lane's share of a step)
 //     lies in one segment.
 //
-//     Waves.  Within a pass, wave w owns the whole segments pass_wave[w] ..
-//     pass_wave[w+1]-1 (about the same number of entries per wave) and walks
-//     them in aligned steps of 512 entries: lane l takes entries 512 s + 8 l
-//     .. + 7 and their row lane_row[64 s + l], sums their terms in order and
-//     scales the sum by the exaggeration.  Bit l of starts[s] is set when
-//     lane l of step s begins a segment.  One segmented scan over the 64
-//     lanes per step (a segment of lanes begins at a lane that begins a
-//     segment) carries each segment's sum to the lane where the next one
-//     begins, which adds it to the force of the row of the lane below it;
-//     lane 63 carries the open segment and its row into the next step.  The entry e1 that ends the wave's
-//     range counts as a segment start, so the last segment closes like any
-//     other.  A row's force is therefore zero plus its segment sums in pass
-//     order, whichever waves and workgroups run them.
+//     Waves.  A pass's entries are cut at 512-entry step boundaries into 16
+//     ranges of about the same number of steps, one per wave; lane l of a
+//     step takes entries 512 s + 8 l .. + 7 and their row lane_row[64 s + l],
+//     sums their terms in order and scales the sum by the exaggeration.  Bit
+//     l of starts[s] is set when lane l of step s begins a segment.  One
+//     segmented scan over the 64 lanes per step (a segment of lanes begins at
+//     a lane that begins a segment) carries each segment's sum to the lane
+//     where the next one begins, which adds it to the force of the row of the
+//     lane below it; lane 63 carries the open segment and its row into the
+//     next step.  The segment open where a wave's range begins (its head) and
+//     the one open where it ends (its tail) are left in LDS; after the pass
+//     one wave joins them across the wave boundaries (a tail, then the heads
+//     of the following waves up to the next segment start, in that order)
+//     and adds each joined segment to its row.  A row's force is therefore
+//     zero plus its segment sums in pass order, each formed in an order fixed
+//     by the layout, whichever workgroups run them.
 //
 //     The padded arrays carry kLanePad entries past the end so that whole
 //     steps and the prefetch of the next step load unconditionally.
@@ -998,7 +1001,7 @@ constexpr int kLanePad = 2 * kLaneStep;       // padding entries after the segme
 constexpr int kLanePadSteps = kLanePad / kLaneStep;
 constexpr int kStagedWaves = 16;              // waves of a group's workgroup
 constexpr int kStagedThreads = 64 * kStagedWaves;
-constexpr int kWindowPoints = 20479;          // + one zero point: 160 KB of LDS
+constexpr int kWindowPoints = 20400;          // + one zero point and the wave edges: 160 KB of LDS
 constexpr size_t kStagedLds = (kWindowPoints + 1) * sizeof(float2);
 constexpr int kMaxWindows = 4;
 constexpr int kPasses = kMaxWindows + 1;      // the last pass: entries outside the windows
@@ -1011,14 +1014,13 @@ enum AttractionKind { kAttrRows = 0, kAttrLanes = 1, kAttrStaged = 2 };
 
 // the segments and their tables
 struct BlockTables {
-  const int* seg_ptr;                // segment offsets (segments + 1)
   const int* col;                    // padded entries (+ kLanePad)
   const float* val;
   const int* lane_row;               // row of every 8 padded entries
   const unsigned long long* starts;  // per step: lanes that begin a segment
   const int* group_row;              // first row of every group (groups + 1)
   const int2* window;                // per group and window: (lo, points); 0 points: none
-  const int* pass_wave;              // per group, pass, wave: first segment (17 per pass)
+  const int2* pass_range;            // per group and pass: its entries [begin, end)
 };
 
 struct LaneCols {
@@ -1112,26 +1114,36 @@ struct StagedPoints {
   }
 };
 
-// The segments r0 .. r1-1 of one wave; a closed segment's sum is added to
-// its row's force in attr.
+// What a wave leaves for the pass's join: the sum of its entries before its
+// first segment start (head), whether it has a start (-1: empty range), the
+// segment open at its end (tail) and that segment's row.
+struct WaveEdge {
+  float2 head;
+  float2 tail;
+  int tail_row;
+  int has_start;
+};
+
+// Entries e0 .. e1-1 of one wave (e0 < e1): segments that begin and end in
+// the range are added to their rows' forces in attr; the head and tail go to
+// *edge.
 template <class SRC>
-__device__ __forceinline__ void attract_segments(const SRC& src, const BlockTables& t, int r0,
-                                                 int r1, float exaggeration,
-                                                 float2* __restrict__ attr) {
+__device__ __forceinline__ void attract_range(const SRC& src, const BlockTables& t, int e0, int e1,
+                                              float exaggeration, float2* __restrict__ attr,
+                                              WaveEdge* edge) {
   const int lane = threadIdx.x & 63;
-  const int e0 = t.seg_ptr[r0], e1 = t.seg_ptr[r1];
   const unsigned long long upto_lane = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
   const float mx = src.mx, my = src.my;
   float cx = 0.0f, cy = 0.0f;  // the segment open at the start of the step
   int crow = 0;                // and its row
+  bool head_open = true;       // no segment start seen yet
   int base = e0 & ~(kLaneStep - 1);
   LaneCols cur = lane_cols(t, base, lane);
-  for (; base <= e1; base += kLaneStep) {
+  for (; base < e1; base += kLaneStep) {
     const int eb = base + kLaneEntries * lane;
     const bool valid = eb >= e0 && eb < e1;
-    // a lane that begins a segment, or the end of the range, closes the one
-    // below; the one closed at e0 is not this wave's
-    const bool seen = eb == e1 || (valid && ((cur.starts >> lane) & 1ull));
+    // a lane that begins a segment closes the one below it
+    const bool seen = valid && ((cur.starts >> lane) & 1ull);
     const int row = cur.row;
     float v[kLaneEntries];
     {
@@ -1167,7 +1179,8 @@ __device__ __forceinline__ void attract_segments(const SRC& src, const BlockTabl
       sy += c * dy;
     }
     float tx = valid ? exaggeration * sx : 0.0f, ty = valid ? exaggeration * sy : 0.0f;
-    const unsigned long long upto = __ballot(seen) & upto_lane;
+    const unsigned long long seen_mask = __ballot(seen);
+    const unsigned long long upto = seen_mask & upto_lane;
     const int lim = upto ? 63 - __clzll(upto) : 0;  // first lane of this lane's segment
     if (lane == 0 && !seen) {
       tx += cx;
@@ -1183,21 +1196,66 @@ __device__ __forceinline__ void attract_segments(const SRC& src, const BlockTabl
       iy = cy;
       irow = crow;
     }
-    if (seen && eb > e0) {
-      const float2 a = attr[irow];
-      attr[irow] = make_float2(a.x + ix, a.y + iy);
+    if (seen) {
+      // the wave's first start closes its head
+      if (head_open && lane == __builtin_ctzll(seen_mask)) {
+        edge->head = make_float2(ix, iy);
+      } else {
+        const float2 a = attr[irow];
+        attr[irow] = make_float2(a.x + ix, a.y + iy);
+      }
     }
+    if (seen_mask) head_open = false;
     cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tx), 63));
     cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ty), 63));
     crow = __builtin_amdgcn_readlane(row, 63);
   }
+  if (lane == 0) {
+    if (head_open) {  // one segment runs through the whole range
+      edge->head = make_float2(cx, cy);
+      edge->tail = make_float2(0.f, 0.f);
+      edge->tail_row = 0;
+      edge->has_start = 0;
+    } else {
+      edge->tail = make_float2(cx, cy);
+      edge->tail_row = t.lane_row[(e1 - 1) / kLaneEntries];
+      edge->has_start = 1;
+    }
+  }
+}
+
+// The join of a pass (lanes 0 .. kStagedWaves of one wave): lane w >= 1 with
+// a segment start closes the segment open before it, whose sum is the tail
+// of the last wave with a start before w plus the heads of the waves after
+// that one up to w; lane kStagedWaves closes the segment open at the pass's
+// end the same way.
+__device__ __forceinline__ void join_waves(const WaveEdge* edges, float2* __restrict__ attr) {
+  const int lane = threadIdx.x & 63;
+  if (lane > kStagedWaves) return;
+  if (lane < kStagedWaves && edges[lane].has_start != 1) return;
+  int j = lane - 1;
+  while (j >= 0 && edges[j].has_start != 1) --j;
+  if (j < 0) return;  // nothing open before: the pass begins with a segment start
+  float2 sum = edges[j].tail;
+  for (int i = j + 1; i < lane; ++i) {
+    if (edges[i].has_start == 0) {
+      sum.x += edges[i].head.x;
+      sum.y += edges[i].head.y;
+    }
+  }
+  if (lane < kStagedWaves) {
+    sum.x += edges[lane].head.x;
+    sum.y += edges[lane].head.y;
+  }
+  const int r = edges[j].tail_row;
+  const float2 a = attr[r];
+  attr[r] = make_float2(a.x + sum.x, a.y + sum.y);
 }
 
 __device__ __forceinline__ void stage_window(const float2* __restrict__ y, int lo, int m,
                                              float2* sy) {
   for (int i = threadIdx.x; i < m; i += kStagedThreads) sy[i] = y[lo + i];
   if (threadIdx.x == 0) sy[m] = make_float2(0.f, 0.f);
-  __syncthreads();
 }
 
 // One group: zero its rows' forces, then the passes in order.  STAGE: the
@@ -1205,28 +1263,39 @@ __device__ __forceinline__ void stage_window(const float2* __restrict__ y, int l
 template <bool STAGE>
 __device__ __forceinline__ void attract_group(const float2* __restrict__ y, const BlockTables& t,
                                               float mx, float my, float exaggeration,
-                                              float2* __restrict__ attr, int n, int g,
-                                              float2* sy) {
+                                              float2* __restrict__ attr, int g, float2* sy) {
+  __shared__ WaveEdge edges[kStagedWaves];
   const int rb = t.group_row[g], re = t.group_row[g + 1];
   for (int r = rb + static_cast<int>(threadIdx.x); r < re; r += kStagedThreads)
     attr[r] = make_float2(0.f, 0.f);
-  __syncthreads();
   const int w = static_cast<int>(threadIdx.x >> 6);
   for (int k = 0; k < kPasses; ++k) {
-    const int* pw = t.pass_wave + (g * kPasses + k) * (kStagedWaves + 1);
-    if (pw[0] == pw[kStagedWaves]) continue;  // the same for the whole workgroup
-    const int r0 = __builtin_amdgcn_readfirstlane(pw[w]);
-    const int r1 = __builtin_amdgcn_readfirstlane(pw[w + 1]);
-    if (STAGE && k < kMaxWindows) {
-      const int2 win = t.window[g * kMaxWindows + k];
+    const int2 pr = t.pass_range[g * kPasses + k];
+    if (pr.x >= pr.y) continue;  // the same for the whole workgroup
+    const bool staged = STAGE && k < kMaxWindows;
+    int2 win = make_int2(0, 0);
+    if (staged) {
+      win = t.window[g * kMaxWindows + k];
       stage_window(y, win.x, win.y, sy);
-      const StagedPoints src{y, sy, win.x, static_cast<unsigned int>(win.y), mx, my};
-      if (r0 < r1) attract_segments(src, t, r0, r1, exaggeration, attr);
-    } else {
-      const GlobalPoints src{y, mx, my};
-      if (r0 < r1) attract_segments(src, t, r0, r1, exaggeration, attr);
+    }
+    // the window, the zeroed forces and the previous pass's join are done
+    __syncthreads();
+    const int s0 = pr.x / kLaneStep, steps = (pr.y - 1) / kLaneStep - s0 + 1;
+    const int e0 = max(pr.x, (s0 + w * steps / kStagedWaves) * kLaneStep);
+    const int e1 = min(pr.y, (s0 + (w + 1) * steps / kStagedWaves) * kLaneStep);
+    if (e0 < e1) {
+      if (staged) {
+        const StagedPoints src{y, sy, win.x, static_cast<unsigned int>(win.y), mx, my};
+        attract_range(src, t, e0, e1, exaggeration, attr, &edges[w]);
+      } else {
+        const GlobalPoints src{y, mx, my};
+        attract_range(src, t, e0, e1, exaggeration, attr, &edges[w]);
+      }
+    } else if ((threadIdx.x & 63) == 0) {
+      edges[w].has_start = -1;
     }
     __syncthreads();
+    if (w == 0) join_waves(edges, attr);
   }
 }
 
@@ -1235,7 +1304,7 @@ __global__ __launch_bounds__(kStagedThreads) void attract_blocked(
     const float2* __restrict__ y_in, BlockTables t, const TsneStats* __restrict__ st,
     float exaggeration, float2* __restrict__ attr, int n) {
   extern __shared__ float2 sy[];
-  attract_group<STAGE>(y_in, t, st->mean_xf, st->mean_yf, exaggeration, attr, n,
+  attract_group<STAGE>(y_in, t, st->mean_xf, st->mean_yf, exaggeration, attr,
                        xcd_remap(blockIdx.x, gridDim.x), sy);
 }
 
@@ -1387,19 +1456,17 @@ __device__ __forceinline__ void row_windows(const int* __restrict__ col, int s, 
   }
 }
 
-// padded length of every segment and whether it is used (one thread per row)
+// padded length of every segment (one thread per row)
 __global__ __launch_bounds__(kThreads) void blk_seg_lengths(const int* __restrict__ row_ptr,
                                                             const int* __restrict__ col,
                                                             const int* __restrict__ group_row,
                                                             int groups,
                                                             const int2* __restrict__ window, int n,
-                                                            int* __restrict__ len,
-                                                            int* __restrict__ used) {
+                                                            int* __restrict__ len) {
   const int r = blockIdx.x * kThreads + threadIdx.x;
   if (r > n) return;
   if (r == n) {  // the scans' last element
     len[kPasses * n] = 0;
-    used[kPasses * n] = 0;
     return;
   }
   const int g = group_of_row(group_row, groups, r);
@@ -1417,7 +1484,6 @@ __global__ __launch_bounds__(kThreads) void blk_seg_lengths(const int* __restric
     }
     const int slot = seg_slot(group_row, g, k, r);
     len[slot] = (l + kLaneEntries - 1) / kLaneEntries * kLaneEntries;
-    used[slot] = l > 0 ? 1 : 0;
   }
 }
 
@@ -1428,8 +1494,6 @@ __global__ __launch_bounds__(kThreads) void blk_fill(const int* __restrict__ row
                                                      const int* __restrict__ group_row, int groups,
                                                      const int2* __restrict__ window, int n,
                                                      const int* __restrict__ off,
-                                                     const int* __restrict__ sid,
-                                                     int* __restrict__ seg_ptr,
                                                      int* __restrict__ lane_row,
                                                      int* __restrict__ pcol,
                                                      float* __restrict__ pval,
@@ -1446,10 +1510,7 @@ __global__ __launch_bounds__(kThreads) void blk_fill(const int* __restrict__ row
     const int slot = seg_slot(group_row, g, k, r);
     const int o = off[slot], plen = off[slot + 1] - o;
     if (plen == 0) continue;
-    if (lane == 0) {
-      seg_ptr[sid[slot]] = o;
-      atomicOr(starts + o / kLaneStep, 1ull << ((o / kLaneEntries) & 63));
-    }
+    if (lane == 0) atomicOr(starts + o / kLaneStep, 1ull << ((o / kLaneEntries) & 63));
     for (int q = lane; q < plen / kLaneEntries; q += 64) lane_row[o / kLaneEntries + q] = r;
     if (k < kMaxWindows) {
       const int l = b[k] - a[k];
@@ -1483,30 +1544,17 @@ __global__ __launch_bounds__(kThreads) void blk_fill(const int* __restrict__ row
   }
 }
 
-// per group and pass, the wave ranges: about the same number of entries each
-__global__ __launch_bounds__(kThreads) void blk_pass_waves(const int* __restrict__ group_row,
-                                                           int groups,
-                                                           const int* __restrict__ sid,
-                                                           const int* __restrict__ seg_ptr,
-                                                           int* __restrict__ pass_wave) {
+// per group and pass, its entries [begin, end)
+__global__ __launch_bounds__(kThreads) void blk_pass_ranges(const int* __restrict__ group_row,
+                                                            int groups,
+                                                            const int* __restrict__ off,
+                                                            int2* __restrict__ pass_range) {
   const int i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= groups * kPasses) return;
   const int g = i / kPasses, k = i % kPasses;
   const int rb = group_row[g], rows = group_row[g + 1] - rb;
   const int slot = kPasses * rb + k * rows;
-  const int sb = sid[slot], se = sid[slot + rows];  // segments of this pass
-  const long long ob = seg_ptr[sb], oe = seg_ptr[se];
-  int* out = pass_wave + i * (kStagedWaves + 1);
-  for (int w = 0; w <= kStagedWaves; ++w) {
-    const long long target = ob + (oe - ob) * w / kStagedWaves;
-    int lo = sb, hi = se;  // first segment starting at or after target
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (seg_ptr[mid] < target) lo = mid + 1;
-      else hi = mid;
-    }
-    out[w] = lo;
-  }
+  pass_range[i] = make_int2(off[slot], off[slot + rows]);
 }
 
 
