@@ -318,28 +318,31 @@ int TsnePlan::setup_attraction(hipStream_t s) {
   const int steps = h_ent / kLaneStep + 1 + kLanePadSteps;
   ArenaPlan ap;
   ap.add<int>(ent);
+  ap.add<unsigned short>(ent);
   ap.add<float>(ent);
   ap.add<int>(ent / kLaneEntries);
   ap.add<unsigned long long>(steps);
   if (attr_store.init(ap.bytes, "t-SNE attraction tables")) return 1;
   int* pcol = attr_store.take<int>(ent);
+  unsigned short* pcol16 = attr_store.take<unsigned short>(ent);
   float* pval = attr_store.take<float>(ent);
   int* lane_row = attr_store.take<int>(ent / kLaneEntries);
   unsigned long long* starts = attr_store.take<unsigned long long>(steps);
   FE_CHECK(hipMemsetAsync(pcol + h_ent, 0, kLanePad * sizeof(int), s));
+  FE_CHECK(hipMemsetAsync(pcol16 + h_ent, 0, kLanePad * sizeof(unsigned short), s));
   FE_CHECK(hipMemsetAsync(pval + h_ent, 0, kLanePad * sizeof(float), s));
   FE_CHECK(hipMemsetAsync(lane_row + h_ent / kLaneEntries, 0,
                           kLanePad / kLaneEntries * sizeof(int), s));
   FE_CHECK(hipMemsetAsync(starts, 0, steps * sizeof(unsigned long long), s));
   blk_fill<<<(n + kThreads / 64 - 1) / (kThreads / 64), kThreads, 0, s>>>(
       csr.row_ptr, csr.col, csr.val, group_row, groups, window, n, off, lane_row,
-      pcol, pval, starts);
+      pcol, pcol16, pval, starts);
   FE_LAUNCH_CHECK("blk_fill");
   blk_pass_ranges<<<(groups * kPasses + kThreads - 1) / kThreads, kThreads, 0, s>>>(
       group_row, groups, off, pass_range);
   FE_LAUNCH_CHECK("blk_pass_ranges");
   FE_CHECK(hipStreamSynchronize(s));
-  blk = BlockTables{pcol, pval, lane_row, starts, group_row, window, pass_range};
+  blk = BlockTables{pcol, pcol16, pval, lane_row, starts, group_row, window, pass_range};
   attr_kind = kind;
   static bool attr_set = false;
   if (!attr_set) {
@@ -784,7 +787,7 @@ static long long job_resident_bytes(int n, int k, int g) {
   const long long segs = std::min<long long>(2 * nk, 1ll * kPasses * n);
   const long long seg_nnz = 2 * nk + 1ll * kLaneEntries * segs + kLanePad;
   const long long groups = std::max<long long>(32, seg_nnz / (16 * 4096) + 1);
-  return 8ll * 2 * nk + 4ll * (n + 1) + 8ll * seg_nnz + seg_nnz / 2 + 8ll * (seg_nnz / kLaneStep + 4) +
+  return 8ll * 2 * nk + 4ll * (n + 1) + 10ll * seg_nnz + seg_nnz / 2 + 8ll * (seg_nnz / kLaneStep + 4) +
          4ll * groups * (1 + 2 * kMaxWindows + 2 * kPasses) +
          8ll * 7 * n + 16ll * (splat_bins(g) + 1) + 40ll * gg +
          8ll * 4 * g * f + 8ll * (g + 1) * g + 16ll * gg + 8ll * f + 32ll * 2048 + 8ll * (g + 1) +

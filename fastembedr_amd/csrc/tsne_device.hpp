@@ -1001,8 +1001,8 @@ constexpr int kLanePad = 2 * kLaneStep;       // padding entries after the segme
 constexpr int kLanePadSteps = kLanePad / kLaneStep;
 constexpr int kStagedWaves = 16;              // waves of a group's workgroup
 constexpr int kStagedThreads = 64 * kStagedWaves;
-constexpr int kWindowPoints = 20400;          // + one zero point and the wave edges: 160 KB of LDS
-constexpr size_t kStagedLds = (kWindowPoints + 1) * sizeof(float2);
+constexpr int kWindowPoints = 20432;          // + the wave edges: 160 KB of LDS
+constexpr size_t kStagedLds = kWindowPoints * sizeof(float2);
 constexpr int kMaxWindows = 4;
 constexpr int kPasses = kMaxWindows + 1;      // the last pass: entries outside the windows
 constexpr int kWindowMinEntries = 2048;       // fewer entries do not pay for a window
@@ -1015,6 +1015,7 @@ enum AttractionKind { kAttrRows = 0, kAttrLanes = 1, kAttrStaged = 2 };
 // the segments and their tables
 struct BlockTables {
   const int* col;                    // padded entries (+ kLanePad)
+  const unsigned short* col16;       // window passes: column - window start
   const float* val;
   const int* lane_row;               // row of every 8 padded entries
   const unsigned long long* starts;  // per step: lanes that begin a segment
@@ -1029,19 +1030,32 @@ struct LaneCols {
   unsigned long long starts;
 };
 
+// LOCAL: the window passes' 16-bit columns (one 16-byte load per lane)
+template <bool LOCAL>
 __device__ __forceinline__ LaneCols lane_cols(const BlockTables& t, int base, int lane) {
   LaneCols c;
-  const int* p = t.col + base + kLaneEntries * lane;
-  const int4 a = *reinterpret_cast<const int4*>(p);
-  const int4 b = *reinterpret_cast<const int4*>(p + 4);
-  c.j[0] = a.x;
-  c.j[1] = a.y;
-  c.j[2] = a.z;
-  c.j[3] = a.w;
-  c.j[4] = b.x;
-  c.j[5] = b.y;
-  c.j[6] = b.z;
-  c.j[7] = b.w;
+  if constexpr (LOCAL) {
+    const int4 a = *reinterpret_cast<const int4*>(t.col16 + base + kLaneEntries * lane);
+    const unsigned int w[4] = {static_cast<unsigned int>(a.x), static_cast<unsigned int>(a.y),
+                               static_cast<unsigned int>(a.z), static_cast<unsigned int>(a.w)};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      c.j[2 * q] = static_cast<int>(w[q] & 0xffffu);
+      c.j[2 * q + 1] = static_cast<int>(w[q] >> 16);
+    }
+  } else {
+    const int* p = t.col + base + kLaneEntries * lane;
+    const int4 a = *reinterpret_cast<const int4*>(p);
+    const int4 b = *reinterpret_cast<const int4*>(p + 4);
+    c.j[0] = a.x;
+    c.j[1] = a.y;
+    c.j[2] = a.z;
+    c.j[3] = a.w;
+    c.j[4] = b.x;
+    c.j[5] = b.y;
+    c.j[6] = b.z;
+    c.j[7] = b.w;
+  }
   c.row = t.lane_row[base / kLaneEntries + lane];
   c.starts = t.starts[base / kLaneStep];
   return c;
@@ -1081,6 +1095,7 @@ __device__ __forceinline__ void segmented_wave_scan(float& sx, float& sy, int la
 // fetch before it uses any value, so all its loads are in flight together.
 // Both give the stored point (the caller centres it, fl(y - mean)).
 struct GlobalPoints {
+  static constexpr bool kLocal = false;
   const float2* __restrict__ y;
   float mx, my;
   using Fetch = float2;
@@ -1088,30 +1103,16 @@ struct GlobalPoints {
   __device__ __forceinline__ float2 value(const Fetch& f) const { return f; }
 };
 
-// A gather reads the window (slot m, which holds zeros, for points outside
-// it), and only the lanes whose point lies outside also load it from the
-// layout; the point is the sum of the two.
-struct StagedPoints {
+// A window pass: every column of the pass lies in the window staged in LDS
+// and is stored as its offset from the window's start.
+struct WindowPoints {
+  static constexpr bool kLocal = true;
   const float2* __restrict__ y;
-  const float2* lds;  // points lo .. lo + m - 1, then a zero point
-  int lo;
-  unsigned int m;
+  const float2* lds;
   float mx, my;
-  struct Fetch {
-    float2 a, b;
-  };
-  __device__ __forceinline__ Fetch fetch(int j) const {
-    const unsigned int o = static_cast<unsigned int>(j - lo);
-    const bool in = o < m;
-    Fetch f;
-    f.a = lds[in ? o : m];
-    f.b = make_float2(0.f, 0.f);
-    if (!in) f.b = y[j];
-    return f;
-  }
-  __device__ __forceinline__ float2 value(const Fetch& f) const {
-    return make_float2(f.a.x + f.b.x, f.a.y + f.b.y);
-  }
+  using Fetch = float2;
+  __device__ __forceinline__ Fetch fetch(int o) const { return lds[o]; }
+  __device__ __forceinline__ float2 value(const Fetch& f) const { return f; }
 };
 
 // What a wave leaves for the pass's join: the sum of its entries before its
@@ -1138,7 +1139,7 @@ __device__ __forceinline__ void attract_range(const SRC& src, const BlockTables&
   int crow = 0;                // and its row
   bool head_open = true;       // no segment start seen yet
   int base = e0 & ~(kLaneStep - 1);
-  LaneCols cur = lane_cols(t, base, lane);
+  LaneCols cur = lane_cols<SRC::kLocal>(t, base, lane);
   for (; base < e1; base += kLaneStep) {
     const int eb = base + kLaneEntries * lane;
     const bool valid = eb >= e0 && eb < e1;
@@ -1164,7 +1165,7 @@ __device__ __forceinline__ void attract_range(const SRC& src, const BlockTables&
     typename SRC::Fetch fj[kLaneEntries];
 #pragma unroll
     for (int u = 0; u < kLaneEntries; ++u) fj[u] = src.fetch(cur.j[u]);
-    cur = lane_cols(t, base + kLaneStep, lane);
+    cur = lane_cols<SRC::kLocal>(t, base + kLaneStep, lane);
     // keep every load of the step above the first wait
     __builtin_amdgcn_sched_barrier(0);
     const float xi = pr.x - mx, yi = pr.y - my;
@@ -1255,7 +1256,6 @@ __device__ __forceinline__ void join_waves(const WaveEdge* edges, float2* __rest
 __device__ __forceinline__ void stage_window(const float2* __restrict__ y, int lo, int m,
                                              float2* sy) {
   for (int i = threadIdx.x; i < m; i += kStagedThreads) sy[i] = y[lo + i];
-  if (threadIdx.x == 0) sy[m] = make_float2(0.f, 0.f);
 }
 
 // One group: zero its rows' forces, then the passes in order.  STAGE: the
@@ -1285,7 +1285,7 @@ __device__ __forceinline__ void attract_group(const float2* __restrict__ y, cons
     const int e1 = min(pr.y, (s0 + (w + 1) * steps / kStagedWaves) * kLaneStep);
     if (e0 < e1) {
       if (staged) {
-        const StagedPoints src{y, sy, win.x, static_cast<unsigned int>(win.y), mx, my};
+        const WindowPoints src{y, sy, mx, my};
         attract_range(src, t, e0, e1, exaggeration, attr, &edges[w]);
       } else {
         const GlobalPoints src{y, mx, my};
@@ -1496,6 +1496,7 @@ __global__ __launch_bounds__(kThreads) void blk_fill(const int* __restrict__ row
                                                      const int* __restrict__ off,
                                                      int* __restrict__ lane_row,
                                                      int* __restrict__ pcol,
+                                                     unsigned short* __restrict__ pcol16,
                                                      float* __restrict__ pval,
                                                      unsigned long long* starts) {
   const int r = blockIdx.x * (kThreads / 64) + static_cast<int>(threadIdx.x >> 6);
@@ -1514,9 +1515,12 @@ __global__ __launch_bounds__(kThreads) void blk_fill(const int* __restrict__ row
     for (int q = lane; q < plen / kLaneEntries; q += 64) lane_row[o / kLaneEntries + q] = r;
     if (k < kMaxWindows) {
       const int l = b[k] - a[k];
+      const int lo = window[g * kMaxWindows + k].x;
       for (int q = lane; q < plen; q += 64) {
         const bool real = q < l;
-        pcol[o + q] = real ? col[a[k] + q] : r;
+        const int j = real ? col[a[k] + q] : r;
+        pcol[o + q] = j;
+        pcol16[o + q] = static_cast<unsigned short>(real ? j - lo : 0);
         pval[o + q] = real ? val[a[k] + q] : 0.0f;
       }
     } else {
