@@ -45,6 +45,16 @@ int resolve_grid_size(int n, int requested) {
   return g;
 }
 
+// compute units of the current device (256 on MI355X)
+static int compute_units() {
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+      prop.multiProcessorCount > 0)
+    return prop.multiProcessorCount;
+  return 256;
+}
+
 static bool graph_capture_enabled() {
   const char* env = std::getenv("FASTEMBEDR_HIP_GRAPH_CAPTURE");
   return !(env && env[0] == '0');
@@ -264,8 +274,8 @@ int TsnePlan::setup_attraction(hipStream_t s) {
       static_cast<long long>(kPasses) * n + 1 >= (1ll << 31))
     return 0;
   const long long slots = static_cast<long long>(kPasses) * n;  // + 1 for the scans
-  // groups of about 16 * 4096 padded entries (small P: at least 32 groups,
-  // so a job alone still spreads over the GPU); the cut changes no sum
+  // (small P: at least 32 groups, so a job alone still spreads over the
+  // GPU); the cut changes no sum
   Arena scratch;
   ArenaPlan sp;
   const int nb_rows = (n + 1 + kScanBlock - 1) / kScanBlock;
@@ -289,7 +299,11 @@ int TsnePlan::setup_attraction(hipStream_t s) {
   int h_lnnz = 0;
   FE_CHECK(hipMemcpyAsync(&h_lnnz, lrow + n, sizeof(int), hipMemcpyDeviceToHost, s));
   FE_CHECK(hipStreamSynchronize(s));
-  long long gcount = (h_lnnz + 16ll * 4096 - 1) / (16ll * 4096);
+  // groups of about 16 * 8192 padded entries; past one group per compute
+  // unit, a whole number of rounds of them (one workgroup fits a CU)
+  long long gcount = (h_lnnz + 16ll * 8192 - 1) / (16ll * 8192);
+  const int cus = compute_units();
+  if (gcount > cus) gcount = (gcount + cus / 2) / cus * cus;
   gcount = std::max<long long>(gcount, std::min<long long>(32, std::max(1, n / 64)));
   groups = static_cast<int>(std::min<long long>(gcount, 1 << 16));
   // group rows, windows and wave ranges; then the segments, whose size the
