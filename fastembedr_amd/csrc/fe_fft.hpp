@@ -1,16 +1,21 @@
 // In-LDS Stockham FFT for one 256-thread workgroup (gfx950, wave64).
 //
-// A workgroup transforms NB independent length-N complex arrays that sit
-// contiguously in LDS (array b at buf + b*N).  Radix-8 stages are used while
-// at least 8 points remain per group, then one radix-4 or radix-2 stage.
-// Each stage loads every butterfly input of the thread into registers,
-// synchronises, applies the twiddles and the radix-R DFT, and writes the
-// outputs back in place (Stockham autosort: the result is in natural order,
-// no bit reversal).  Twiddles come from a table tw[t] = exp(-2*pi*i*t/N)
-// computed on the host in double precision.
+// A workgroup transforms NB independent length-N complex arrays held in LDS
+// with one padding slot after every 32 elements: element i of array b sits at
+// buf[fft_at<N>(b, i)] (the padding spreads the strided accesses of the
+// stages over the LDS banks).  Radix-8 stages are used while at least 8
+// points remain per group, then one radix-4 or radix-2 stage.  Each stage
+// loads every butterfly input of the thread into registers, synchronises,
+// applies the twiddles and the radix-R DFT, and writes the outputs back in
+// place (Stockham autosort: the result is in natural order, no bit reversal).
+// Twiddles come from a table computed on the host in double precision, laid
+// out stage by stage so that the R-1 twiddles of one butterfly are adjacent
+// (16-byte loads); fft_tw_size(N) entries, filled by fft_twiddles().
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <cmath>
 
 namespace fe {
 
@@ -78,12 +83,64 @@ __device__ __forceinline__ void dft_r(float2 (&v)[R]) {
   }
 }
 
+constexpr int kFftPadShift = 5;  // one padding slot per 32 elements
+
+// LDS float2 slots of one padded array of N elements
+template <int N>
+__host__ __device__ constexpr int fft_span() {
+  return N + (N >> kFftPadShift);
+}
+
+// slot of element i of array b
+template <int N>
+__device__ __forceinline__ int fft_at(int b, int i) {
+  return b * fft_span<N>() + i + (i >> kFftPadShift);
+}
+
+// radix of the stage whose butterflies span NS (radix 8 while 8 points remain)
+__host__ __device__ constexpr int fft_radix(int n, int ns) { return n / ns >= 8 ? 8 : n / ns; }
+
+// table slots per butterfly: R - 1 twiddles rounded up to an even count
+__host__ __device__ constexpr int fft_tw_slots(int r) { return r == 8 ? 8 : (r == 4 ? 4 : 2); }
+
+// first slot of the table of the stage with span NS (stages with NS > 1 only)
+__host__ __device__ constexpr int fft_tw_offset(int n, int ns) {
+  int off = 0;
+  for (int s = 1; s < ns; s *= fft_radix(n, s))
+    if (s > 1) off += s * fft_tw_slots(fft_radix(n, s));
+  return off;
+}
+
+__host__ __device__ constexpr int fft_tw_size(int n) { return fft_tw_offset(n, n); }
+
+// The table: for the stage with span NS and radix R, butterfly k in [0, NS)
+// and r in [1, R): exp(-2*pi*i*r*k/(NS*R)) at fft_tw_offset(n, NS) + k *
+// fft_tw_slots(R) + r - 1 (padding slots zero).  tw holds fft_tw_size(n).
+inline void fft_twiddles(int n, float2* tw) {
+  for (int s = 1; s < n; s *= fft_radix(n, s)) {
+    if (s == 1) continue;
+    const int r = fft_radix(n, s), sl = fft_tw_slots(r);
+    float2* t = tw + fft_tw_offset(n, s);
+    for (int k = 0; k < s; ++k) {
+      for (int q = 0; q < sl; ++q) {
+        const int rr = q + 1;
+        if (rr < r) {
+          const double a = -2.0 * 3.14159265358979323846 * static_cast<double>(rr) *
+                           static_cast<double>(k) / (static_cast<double>(s) * r);
+          t[k * sl + q] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+        } else {
+          t[k * sl + q] = make_float2(0.f, 0.f);
+        }
+      }
+    }
+  }
+}
+
 template <int N, int NB, int NT, bool INV, int R, int NS>
 __device__ __forceinline__ void fft_stage(float2* buf, const float2* __restrict__ tw) {
   constexpr int NBF = N / R;          // butterflies per array
   constexpr int TOT = NB * NBF;       // butterflies per stage
   constexpr int BPT = (TOT + NT - 1) / NT;
-  constexpr int TSTEP = N / (NS * R);  // twiddle index stride
   float2 v[BPT][R];
   const int tid = static_cast<int>(threadIdx.x);
 #pragma unroll
@@ -91,9 +148,8 @@ __device__ __forceinline__ void fft_stage(float2* buf, const float2* __restrict_
     const int idx = tid + s * NT;
     if (TOT % NT == 0 || idx < TOT) {
       const int b = idx / NBF, j = idx - b * NBF;
-      const float2* src = buf + b * N + j;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[s][r] = src[r * NBF];
+      for (int r = 0; r < R; ++r) v[s][r] = buf[fft_at<N>(b, j + r * NBF)];
     }
   }
   __syncthreads();
@@ -104,17 +160,27 @@ __device__ __forceinline__ void fft_stage(float2* buf, const float2* __restrict_
       const int b = idx / NBF, j = idx - b * NBF;
       const int k = j % NS;
       if constexpr (NS > 1) {
+        constexpr int SL = fft_tw_slots(R);
+        const float4* tp =
+            reinterpret_cast<const float4*>(tw + fft_tw_offset(N, NS) + k * SL);
+        float2 w[SL];
+#pragma unroll
+        for (int q = 0; q < SL / 2; ++q) {
+          const float4 t4 = tp[q];
+          w[2 * q] = make_float2(t4.x, t4.y);
+          w[2 * q + 1] = make_float2(t4.z, t4.w);
+        }
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          float2 w = tw[r * k * TSTEP];
-          if (INV) w.y = -w.y;
-          v[s][r] = cmul(v[s][r], w);
+          float2 wr = w[r - 1];
+          if (INV) wr.y = -wr.y;
+          v[s][r] = cmul(v[s][r], wr);
         }
       }
       dft_r<R, INV>(v[s]);
-      float2* dst = buf + b * N + (j / NS) * NS * R + k;
+      const int base = (j / NS) * NS * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) dst[r * NS] = v[s][r];
+      for (int r = 0; r < R; ++r) buf[fft_at<N>(b, base + r * NS)] = v[s][r];
     }
   }
   __syncthreads();
@@ -130,8 +196,8 @@ __device__ __forceinline__ void fft_stages(float2* buf, const float2* __restrict
   }
 }
 
-// Unnormalised transform (forward: exp(-i), inverse: exp(+i)) of NB arrays of
-// N points in LDS.  Caller must __syncthreads() after filling buf.
+// Unnormalised transform (forward: exp(-i), inverse: exp(+i)) of NB padded
+// arrays of N points in LDS.  Caller must __syncthreads() after filling buf.
 template <int N, int NB, int NT, bool INV>
 __device__ __forceinline__ void fft_lds(float2* buf, const float2* __restrict__ tw) {
   fft_stages<N, NB, NT, INV, 1>(buf, tw);

@@ -444,7 +444,7 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
   sp.add<float2>(gf);  // o1
   sp.add<float2>(gf);  // o2
   sp.add<float4>(gg);  // field
-  sp.add<float2>(f);   // twiddles
+  sp.add<float2>(fft_tw_size(f));  // twiddles
   sp.add<TsneStats>(1);
   sp.add<LayoutPartial>(stat_blocks);
   sp.add<double>(g + 1);
@@ -467,17 +467,14 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
   o1 = state.take<float2>(gf);
   o2 = state.take<float2>(gf);
   field = state.take<float4>(gg);
-  tw = state.take<float2>(f);
+  tw = state.take<float2>(fft_tw_size(f));
   st = state.take<TsneStats>(1);
   lpart = state.take<LayoutPartial>(stat_blocks);
   zpart = state.take<double>(g + 1);
 
-  std::vector<float2> htw(f);
-  for (int t = 0; t < f; ++t) {
-    const double a = -2.0 * M_PI * static_cast<double>(t) / static_cast<double>(f);
-    htw[t] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
-  }
-  FE_CHECK(hipMemcpyAsync(tw, htw.data(), f * sizeof(float2), hipMemcpyHostToDevice, s));
+  std::vector<float2> htw(fft_tw_size(f));
+  fft_twiddles(f, htw.data());
+  FE_CHECK(hipMemcpyAsync(tw, htw.data(), htw.size() * sizeof(float2), hipMemcpyHostToDevice, s));
   std::vector<float> y0(2 * static_cast<size_t>(n));
   if (init) {
     std::memcpy(y0.data(), init, y0.size() * sizeof(float));
@@ -501,8 +498,8 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
   layout_finalize<<<1, kThreads, 0, s>>>(lpart, stat_blocks, st, n, g, centre ? 1 : 0);
   FE_LAUNCH_CHECK("layout_finalize");
   if (f > 1024) {
-    // conv_cols uses 4*F*8 bytes of dynamic LDS
-    const int bytes = 4 * f * static_cast<int>(sizeof(float2));
+    // conv_cols uses four padded arrays of dynamic LDS
+    const int bytes = static_cast<int>(conv_cols_lds<2048>());
     hipError_t rc = hipSuccess;
     if (f == 2048)
       rc = hipFuncSetAttribute(reinterpret_cast<const void*>(&conv_cols<2048>),
@@ -520,7 +517,7 @@ static int launch_conv(TsnePlan& p, hipStream_t s) {
   constexpr int G = F / 2;
   conv_rows_fwd<F><<<G + G / 2, kThreads, 0, s>>>(p.splat, p.st, p.tw, p.z1, p.z2, p.kt);
   FE_LAUNCH_CHECK("conv_rows_fwd");
-  conv_cols<F><<<conv_cols_blocks(G), kThreads, 4 * F * sizeof(float2), s>>>(p.z1, p.z2, p.kt, p.tw,
+  conv_cols<F><<<conv_cols_blocks(G), kThreads, conv_cols_lds<F>(), s>>>(p.z1, p.z2, p.kt, p.tw,
                                                                              p.o1, p.o2,
                                                                              p.zpart);
   FE_LAUNCH_CHECK("conv_cols");
@@ -690,7 +687,7 @@ int TsnePlan::profile(int iters, const StepParams& sp, double* ms) {
     constexpr int G = FF / 2;                                                                 \
     conv_rows_fwd<FF><<<G + G / 2, kThreads, 0, s>>>(splat, st, tw, z1, z2, kt);              \
     (void)hipEventRecord(e[2], s);                                                            \
-    conv_cols<FF><<<conv_cols_blocks(G), kThreads, 4 * FF * sizeof(float2), s>>>(            \
+    conv_cols<FF><<<conv_cols_blocks(G), kThreads, conv_cols_lds<FF>(), s>>>(            \
         z1, z2, kt, tw, o1, o2, zpart);                                                       \
     (void)hipEventRecord(e[3], s);                                                            \
     conv_rows_inv<FF><<<G, kThreads, 0, s>>>(o1, o2, tw, zpart, st, field);                   \
@@ -804,7 +801,8 @@ static long long job_resident_bytes(int n, int k, int g) {
   return 8ll * 2 * nk + 4ll * (n + 1) + 10ll * seg_nnz + seg_nnz / 2 + 8ll * (seg_nnz / kLaneStep + 4) +
          4ll * groups * (1 + 2 * kMaxWindows + 2 * kPasses) +
          8ll * 7 * n + 16ll * (splat_bins(g) + 1) + 40ll * gg +
-         8ll * 4 * g * f + 8ll * (g + 1) * g + 16ll * gg + 8ll * f + 32ll * 2048 + 8ll * (g + 1) +
+         8ll * 4 * g * f + 8ll * (g + 1) * g + 16ll * gg + 8ll * fft_tw_size(2 * g) + 32ll * 2048 +
+         8ll * (g + 1) +
          4096;
 }
 
@@ -1050,7 +1048,7 @@ class TsneBatch {
     if (L.kind == 1)                                                                           \
       batch_rows_fwd<FF><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);                   \
     else if (L.kind == 2)                                                                      \
-      batch_cols<FF><<<L.blocks, kThreads, 4 * FF * sizeof(float2), s>>>(d_desc, sl, fi, L.m); \
+      batch_cols<FF><<<L.blocks, kThreads, conv_cols_lds<FF>(), s>>>(d_desc, sl, fi, L.m); \
     else                                                                                       \
       batch_rows_inv<FF><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);                   \
     break;
@@ -1107,7 +1105,7 @@ class TsneBatch {
     if (!attr_done) {
       FE_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&batch_cols<2048>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   4 * 2048 * static_cast<int>(sizeof(float2))));
+                                   static_cast<int>(conv_cols_lds<2048>())));
       FE_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&batch_attract_blocked<true>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize,
                                    static_cast<int>(kStagedLds)));

@@ -617,7 +617,7 @@ __device__ __forceinline__ void conv_rows_fwd_body(SplatBuffers sb,
                                                    float2* __restrict__ z2,
                                                    float2* __restrict__ kt, int b) {
   constexpr int G = F / 2;
-  __shared__ float2 buf[2 * F];
+  __shared__ float2 buf[2 * fft_span<F>()];
   if (b < G) {
     const int row = b;
     const int ts = tiles_side(G);
@@ -645,8 +645,8 @@ __device__ __forceinline__ void conv_rows_fwd_body(SplatBuffers sb,
         a = make_float2(v.x, v.y);
         c = make_float2(v.z, 0.f);
       }
-      buf[x] = a;
-      buf[F + x] = c;
+      buf[fft_at<F>(0, x)] = a;
+      buf[fft_at<F>(1, x)] = c;
     }
     __syncthreads();
     fft_lds<F, 2, kThreads, false>(buf, tw);
@@ -655,8 +655,8 @@ __device__ __forceinline__ void conv_rows_fwd_body(SplatBuffers sb,
     // z2 is the transform of a real row: columns G+1 .. F-1 are the
     // conjugates of columns G-1 .. 1 and are never read
     for (int x = threadIdx.x; x < F; x += kThreads) {
-      o1[x] = buf[x];
-      if (x <= G) o2[x] = buf[F + x];
+      o1[x] = buf[fft_at<F>(0, x)];
+      if (x <= G) o2[x] = buf[fft_at<F>(1, x)];
     }
   } else {
     const int dy0 = b - G;
@@ -672,14 +672,14 @@ __device__ __forceinline__ void conv_rows_fwd_body(SplatBuffers sb,
         const float q = 1.0f / (1.0f + (xo * xo + yo * yo));
         v = make_float2(q, q * q);
       }
-      buf[t] = v;
+      buf[fft_at<F>(which, xx)] = v;
     }
     __syncthreads();
     fft_lds<F, 2, kThreads, false>(buf, tw);
     for (int t = threadIdx.x; t < 2 * (G + 1); t += kThreads) {
       const int which = t / (G + 1), k2 = t - which * (G + 1);
       const int dy = dy0 + which * (G / 2);
-      kt[static_cast<size_t>(k2) * G + dy] = buf[which * F + k2];
+      kt[static_cast<size_t>(k2) * G + dy] = buf[fft_at<F>(which, k2)];
     }
   }
 }
@@ -702,8 +702,14 @@ __global__ __launch_bounds__(kThreads) void conv_rows_fwd(SplatBuffers sb,
 // mass and mass_y are real, so A(F-c, -k) = conj A(c, k) and
 // Y(F-c, -k) = conj Y(c, k): column F-c of z2 is never transformed.  The
 // 1/F^2 of the inverse transform is folded into the kernel spectra.
-// `lds` holds 4*F float2 (dynamic shared memory of the launching kernel).
+// `lds` holds 4 padded arrays, conv_cols_lds<F>() bytes (dynamic shared
+// memory of the launching kernel).
 // ----------------------------------------------------------------------------
+template <int F>
+__host__ __device__ constexpr size_t conv_cols_lds() {
+  return 4 * static_cast<size_t>(fft_span<F>()) * sizeof(float2);
+}
+
 template <int F>
 __device__ __forceinline__ void conv_cols_body(const float2* __restrict__ z1,
                                                const float2* __restrict__ z2,
@@ -714,10 +720,6 @@ __device__ __forceinline__ void conv_cols_body(const float2* __restrict__ z1,
                                                int c) {
   constexpr int G = F / 2;
   constexpr int kPer = (F + kThreads - 1) / kThreads;  // spectrum entries per thread
-  float2* b0 = lds;
-  float2* b1 = lds + F;
-  float2* b2 = lds + 2 * F;
-  float2* bk = lds + 3 * F;
   const int cp = (F - c) & (F - 1);
   const bool pair = (cp != c);
   for (int y = threadIdx.x; y < F; y += kThreads) {
@@ -728,11 +730,11 @@ __device__ __forceinline__ void conv_cols_body(const float2* __restrict__ z1,
       bb = z2[r + c];
       cc = z1[r + cp];
     }
-    b0[y] = a;
-    b1[y] = bb;
-    b2[y] = cc;
+    lds[fft_at<F>(0, y)] = a;
+    lds[fft_at<F>(1, y)] = bb;
+    lds[fft_at<F>(2, y)] = cc;
     const int d = y <= G ? y : F - y;
-    bk[y] = d < G ? kt[static_cast<size_t>(c) * G + d] : make_float2(0.f, 0.f);
+    lds[fft_at<F>(3, y)] = d < G ? kt[static_cast<size_t>(c) * G + d] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   fft_lds<F, 4, kThreads, false>(lds, tw);
@@ -742,18 +744,17 @@ __device__ __forceinline__ void conv_cols_body(const float2* __restrict__ z1,
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     const int t = threadIdx.x + j * kThreads;
-    kv[j] = t < F ? bk[t] : make_float2(0.f, 0.f);
+    kv[j] = t < F ? lds[fft_at<F>(3, t)] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  float2* b3 = bk;
   double zpart = 0.0;
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     const int t = threadIdx.x + j * kThreads;
     if (t >= F) continue;
     const int k1 = t, k1m = (F - t) & (F - 1);
-    const float2 z1a = b0[k1], z1b = b2[k1m];
-    const float2 ya = b1[k1], yb = cconj(ya);
+    const float2 z1a = lds[fft_at<F>(0, k1)], z1b = lds[fft_at<F>(2, k1m)];
+    const float2 ya = lds[fft_at<F>(1, k1)], yb = cconj(ya);
     const float kq = kv[j].x * inv_ff, k2 = kv[j].y * inv_ff;
     // A = (z + conj(zm))/2, X = (z - conj(zm))/(2i)
     const float2 A = make_float2(0.5f * (z1a.x + z1b.x), 0.5f * (z1a.y - z1b.y));
@@ -766,10 +767,10 @@ __device__ __forceinline__ void conv_cols_body(const float2* __restrict__ z1,
     const float2 o2b = make_float2(Ab.x * k2 - yb.y * k2, Ab.y * k2 + yb.x * k2);
     zpart += 2.0 * (static_cast<double>(A.x) * A.x + static_cast<double>(A.y) * A.y) *
              static_cast<double>(kq);
-    b0[k1] = o1a;
-    b1[k1] = o2a;
-    b2[k1m] = o1b;
-    b3[k1m] = o2b;
+    lds[fft_at<F>(0, k1)] = o1a;
+    lds[fft_at<F>(1, k1)] = o2a;
+    lds[fft_at<F>(2, k1m)] = o1b;
+    lds[fft_at<F>(3, k1m)] = o2b;
   }
   // self-paired columns (c = 0, c = G) visit every element twice
   if (!pair) zpart *= 0.5;
@@ -777,11 +778,11 @@ __device__ __forceinline__ void conv_cols_body(const float2* __restrict__ z1,
   fft_lds<F, 4, kThreads, true>(lds, tw);
   for (int y = threadIdx.x; y < G; y += kThreads) {
     const size_t r = static_cast<size_t>(y) * F;
-    o1[r + c] = b0[y];
-    o2[r + c] = b1[y];
+    o1[r + c] = lds[fft_at<F>(0, y)];
+    o2[r + c] = lds[fft_at<F>(1, y)];
     if (pair) {
-      o1[r + cp] = b2[y];
-      o2[r + cp] = b3[y];
+      o1[r + cp] = lds[fft_at<F>(2, y)];
+      o2[r + cp] = lds[fft_at<F>(3, y)];
     }
   }
   // Z partial of the column pair; conv_rows_inv folds the partials
@@ -834,7 +835,7 @@ __device__ __forceinline__ void conv_rows_inv_body(const float2* __restrict__ o1
                                                    TsneStats* st, float4* __restrict__ field,
                                                    int row) {
   constexpr int G = F / 2;
-  __shared__ float2 buf[2 * F];
+  __shared__ float2 buf[2 * fft_span<F>()];
   if (row == 0) {
     // Z: fold the G + 1 column-pair partials in a fixed order.  Parseval:
     // sum_nodes mass*q = (1/F^2) sum_k |A(k)|^2 Kq_hat(k) (the kernel
@@ -854,15 +855,15 @@ __device__ __forceinline__ void conv_rows_inv_body(const float2* __restrict__ o1
   const float2* s1 = o1 + static_cast<size_t>(row) * F;
   const float2* s2 = o2 + static_cast<size_t>(row) * F;
   for (int x = threadIdx.x; x < F; x += kThreads) {
-    buf[x] = s1[x];
-    buf[F + x] = s2[x];
+    buf[fft_at<F>(0, x)] = s1[x];
+    buf[fft_at<F>(1, x)] = s2[x];
   }
   __syncthreads();
   fft_lds<F, 2, kThreads, true>(buf, tw);
   float4* out = field + static_cast<size_t>(row) * G;
   const double xs = st->x_scale;
   for (int x = threadIdx.x; x < G; x += kThreads) {
-    const float2 a = buf[x], b = buf[F + x];
+    const float2 a = buf[fft_at<F>(0, x)], b = buf[fft_at<F>(1, x)];
     out[x] = make_float4(b.x, static_cast<float>(static_cast<double>(a.y) * xs), b.y, a.x);
   }
 }
