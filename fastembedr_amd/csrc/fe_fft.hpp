@@ -141,8 +141,12 @@ __device__ __forceinline__ void fft_stage(float2* buf, const float2* __restrict_
   constexpr int NBF = N / R;          // butterflies per array
   constexpr int TOT = NB * NBF;       // butterflies per stage
   constexpr int BPT = (TOT + NT - 1) / NT;
+  constexpr int SL = fft_tw_slots(R);
+  constexpr int NW = NS > 1 ? SL : 1;  // twiddle registers per butterfly
   float2 v[BPT][R];
+  float2 w[BPT][NW];
   const int tid = static_cast<int>(threadIdx.x);
+  // the butterfly inputs and, before the barrier, the twiddle loads
 #pragma unroll
   for (int s = 0; s < BPT; ++s) {
     const int idx = tid + s * NT;
@@ -150,6 +154,16 @@ __device__ __forceinline__ void fft_stage(float2* buf, const float2* __restrict_
       const int b = idx / NBF, j = idx - b * NBF;
 #pragma unroll
       for (int r = 0; r < R; ++r) v[s][r] = buf[fft_at<N>(b, j + r * NBF)];
+      if constexpr (NS > 1) {
+        const float4* tp =
+            reinterpret_cast<const float4*>(tw + fft_tw_offset(N, NS) + (j % NS) * SL);
+#pragma unroll
+        for (int q = 0; q < SL / 2; ++q) {
+          const float4 t4 = tp[q];
+          w[s][2 * q] = make_float2(t4.x, t4.y);
+          w[s][2 * q + 1] = make_float2(t4.z, t4.w);
+        }
+      }
     }
   }
   __syncthreads();
@@ -160,19 +174,9 @@ __device__ __forceinline__ void fft_stage(float2* buf, const float2* __restrict_
       const int b = idx / NBF, j = idx - b * NBF;
       const int k = j % NS;
       if constexpr (NS > 1) {
-        constexpr int SL = fft_tw_slots(R);
-        const float4* tp =
-            reinterpret_cast<const float4*>(tw + fft_tw_offset(N, NS) + k * SL);
-        float2 w[SL];
-#pragma unroll
-        for (int q = 0; q < SL / 2; ++q) {
-          const float4 t4 = tp[q];
-          w[2 * q] = make_float2(t4.x, t4.y);
-          w[2 * q + 1] = make_float2(t4.z, t4.w);
-        }
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          float2 wr = w[r - 1];
+          float2 wr = w[s][r - 1];
           if (INV) wr.y = -wr.y;
           v[s][r] = cmul(v[s][r], wr);
         }

@@ -515,13 +515,13 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
 template <int F>
 static int launch_conv(TsnePlan& p, hipStream_t s) {
   constexpr int G = F / 2;
-  conv_rows_fwd<F><<<G + G / 2, kThreads, 0, s>>>(p.splat, p.st, p.tw, p.z1, p.z2, p.kt);
+  conv_rows_fwd<F><<<G + G / 2, kFftThreads, 0, s>>>(p.splat, p.st, p.tw, p.z1, p.z2, p.kt);
   FE_LAUNCH_CHECK("conv_rows_fwd");
-  conv_cols<F><<<conv_cols_blocks(G), kThreads, conv_cols_lds<F>(), s>>>(p.z1, p.z2, p.kt, p.tw,
+  conv_cols<F><<<conv_cols_blocks(G), kFftThreads, conv_cols_lds<F>(), s>>>(p.z1, p.z2, p.kt, p.tw,
                                                                              p.o1, p.o2,
                                                                              p.zpart);
   FE_LAUNCH_CHECK("conv_cols");
-  conv_rows_inv<F><<<G, kThreads, 0, s>>>(p.o1, p.o2, p.tw, p.zpart, p.st, p.field);
+  conv_rows_inv<F><<<G, kFftThreads, 0, s>>>(p.o1, p.o2, p.tw, p.zpart, p.st, p.field);
   FE_LAUNCH_CHECK("conv_rows_inv");
   return 0;
 }
@@ -530,7 +530,7 @@ static int launch_splat(TsnePlan& p, hipStream_t s) {
   const float2* yin = p.y[p.cur];
   splat_count<<<p.splat_blocks, kThreads, 0, s>>>(yin, p.st, p.splat, p.n, p.g);
   FE_LAUNCH_CHECK("splat_count");
-  splat_prep<<<1, kThreads, 0, s>>>(p.splat, p.g);
+  splat_prep<<<1, kPrepThreads, 0, s>>>(p.splat, p.g);
   FE_LAUNCH_CHECK("splat_prep");
   splat_scatter<<<p.splat_blocks, kThreads, 0, s>>>(yin, p.st, p.splat, p.n, p.g);
   FE_LAUNCH_CHECK("splat_scatter");
@@ -685,12 +685,12 @@ int TsnePlan::profile(int iters, const StepParams& sp, double* ms) {
 #define FE_PROF_CONV(FF)                                                                      \
   case FF: {                                                                                  \
     constexpr int G = FF / 2;                                                                 \
-    conv_rows_fwd<FF><<<G + G / 2, kThreads, 0, s>>>(splat, st, tw, z1, z2, kt);              \
+    conv_rows_fwd<FF><<<G + G / 2, kFftThreads, 0, s>>>(splat, st, tw, z1, z2, kt);              \
     (void)hipEventRecord(e[2], s);                                                            \
-    conv_cols<FF><<<conv_cols_blocks(G), kThreads, conv_cols_lds<FF>(), s>>>(            \
+    conv_cols<FF><<<conv_cols_blocks(G), kFftThreads, conv_cols_lds<FF>(), s>>>(            \
         z1, z2, kt, tw, o1, o2, zpart);                                                       \
     (void)hipEventRecord(e[3], s);                                                            \
-    conv_rows_inv<FF><<<G, kThreads, 0, s>>>(o1, o2, tw, zpart, st, field);                   \
+    conv_rows_inv<FF><<<G, kFftThreads, 0, s>>>(o1, o2, tw, zpart, st, field);                   \
     break;                                                                                    \
   }
     switch (f) {
@@ -1038,7 +1038,7 @@ class TsneBatch {
           batch_splat_tiles<<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
           break;
         case 8:
-          batch_splat_prep<<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
+          batch_splat_prep<<<L.blocks, kPrepThreads, 0, s>>>(d_desc, sl, fi, L.m);
           break;
         case 9:
           batch_finalize<<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
@@ -1046,11 +1046,11 @@ class TsneBatch {
 #define FE_B_CONV(FF)                                                                          \
   case FF:                                                                                     \
     if (L.kind == 1)                                                                           \
-      batch_rows_fwd<FF><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);                   \
+      batch_rows_fwd<FF><<<L.blocks, kFftThreads, 0, s>>>(d_desc, sl, fi, L.m);                   \
     else if (L.kind == 2)                                                                      \
-      batch_cols<FF><<<L.blocks, kThreads, conv_cols_lds<FF>(), s>>>(d_desc, sl, fi, L.m); \
+      batch_cols<FF><<<L.blocks, kFftThreads, conv_cols_lds<FF>(), s>>>(d_desc, sl, fi, L.m); \
     else                                                                                       \
-      batch_rows_inv<FF><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);                   \
+      batch_rows_inv<FF><<<L.blocks, kFftThreads, 0, s>>>(d_desc, sl, fi, L.m);                   \
     break;
         case 1:
         case 2:

@@ -68,7 +68,7 @@ __global__ __launch_bounds__(kThreads) void batch_splat_count(JobDesc* jobs,
 }
 
 // one block per job
-__global__ __launch_bounds__(kThreads) void batch_splat_prep(JobDesc* jobs,
+__global__ __launch_bounds__(kPrepThreads) void batch_splat_prep(JobDesc* jobs,
                                                              const int* __restrict__ slot,
                                                              const int* __restrict__ first, int m) {
   const int e = find_entry(first, m);
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(kThreads) void batch_splat_tiles(JobDesc* jobs,
 }
 
 template <int F>
-__global__ __launch_bounds__(kThreads) void batch_rows_fwd(JobDesc* jobs, const int* __restrict__ slot,
+__global__ __launch_bounds__(kFftThreads) void batch_rows_fwd(JobDesc* jobs, const int* __restrict__ slot,
                                                            const int* __restrict__ first, int m) {
   const int e = find_entry(first, m);
   JobDesc* J = jobs + slot[e];
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(kThreads) void batch_rows_fwd(JobDesc* jobs, const 
 // every entry of this launch has a multiple of 8 blocks, so job-local block
 // ids keep the hardware's XCD assignment
 template <int F>
-__global__ __launch_bounds__(kThreads) void batch_cols(JobDesc* jobs, const int* __restrict__ slot,
+__global__ __launch_bounds__(kFftThreads) void batch_cols(JobDesc* jobs, const int* __restrict__ slot,
                                                        const int* __restrict__ first, int m) {
   extern __shared__ float2 lds[];
   constexpr int G = F / 2;
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(kThreads) void batch_cols(JobDesc* jobs, const int*
 }
 
 template <int F>
-__global__ __launch_bounds__(kThreads) void batch_rows_inv(JobDesc* jobs, const int* __restrict__ slot,
+__global__ __launch_bounds__(kFftThreads) void batch_rows_inv(JobDesc* jobs, const int* __restrict__ slot,
                                                            const int* __restrict__ first, int m) {
   const int e = find_entry(first, m);
   JobDesc* J = jobs + slot[e];

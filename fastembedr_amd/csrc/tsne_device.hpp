@@ -378,66 +378,71 @@ __device__ __forceinline__ void splat_count_body(const float2* __restrict__ y,
     if (hist[t]) atomicAdd(sb.bin_count + t, hist[t]);
 }
 
-// One block, after splat_count: bin starts and cursors (clearing the counts
-// for the next pass) and the work items of every tile.
+// exclusive prefix sum of v over a block of NT threads in thread order
+template <int NT>
+__device__ __forceinline__ int block_exclusive_scan_nt(int v, int* total) {
+  __shared__ int wsum[NT / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int incl = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += u;
+  }
+  if (lane == 63) wsum[wid] = incl;
+  __syncthreads();
+  int off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) {
+    const int s = wsum[w];
+    off += w < wid ? s : 0;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + incl - v;
+}
+
+// One block of kPrepThreads, after splat_count: bin starts and cursors
+// (clearing the counts for the next pass) and the work items of every tile.
+// Thread t owns bins 4t .. 4t+3 (one 16-byte load and store each) and tile t.
+constexpr int kPrepThreads = kMaxTiles;
+static_assert(kMaxBins == 4 * kPrepThreads, "four bins per thread");
+
 __device__ __forceinline__ void splat_prep_body(SplatBuffers sb, int g) {
   __shared__ int bs[kMaxBins + 1];
   const int ts = tiles_side(g), tiles = ts * ts, nbins = 4 * tiles;
-  {
-    // thread t scans bins [t*per, t*per + per) after the block scan of the
-    // per-thread sums
-    const int per = (nbins + kThreads - 1) / kThreads;
-    const int b0 = threadIdx.x * per;
-    int mine = 0;
-    for (int q = 0; q < per; ++q)
-      if (b0 + q < nbins) mine += sb.bin_count[b0 + q];
-    int total;
-    int run = block_exclusive_scan(mine, &total);
-    for (int q = 0; q < per; ++q) {
-      const int b = b0 + q;
-      if (b < nbins) {
-        const int cnt = sb.bin_count[b];
-        sb.bin_start[b] = run;
-        sb.bin_cursor[b] = run;
-        sb.bin_count[b] = 0;
-        bs[b] = run;
-        run += cnt;
-      }
-    }
-    if (threadIdx.x == 0) {
-      sb.bin_start[nbins] = total;
-      bs[nbins] = total;
-    }
-    __syncthreads();
-  }
-  constexpr int kTilesPer = (kMaxTiles + kThreads - 1) / kThreads;
-  const int per = (tiles + kThreads - 1) / kThreads;
-  const int t0 = threadIdx.x * per;
-  int items[kTilesPer];
-  int mine = 0;
-#pragma unroll
-  for (int q = 0; q < kTilesPer; ++q) {
-    items[q] = 0;
-    const int t = t0 + q;
-    if (q < per && t < tiles) {
-      int b0[4], b1[4];
-      tile_ranges(bs, t, ts, b0, b1);
-      const int pts = (b1[0] - b0[0]) + (b1[1] - b0[1]) + (b1[2] - b0[2]) + (b1[3] - b0[3]);
-      items[q] = max(1, (pts + kSplatItem - 1) / kSplatItem);
-      mine += items[q];
-    }
-  }
+  const int t = threadIdx.x, b0 = 4 * t;
+  int4 cnt = make_int4(0, 0, 0, 0);
+  if (b0 < nbins) cnt = *reinterpret_cast<const int4*>(sb.bin_count + b0);
   int total;
-  int run = block_exclusive_scan(mine, &total);
-#pragma unroll
-  for (int q = 0; q < kTilesPer; ++q) {
-    const int t = t0 + q;
-    if (q < per && t < tiles) {
-      sb.tile_items[t] = run;
-      run += items[q];
-    }
+  const int run = block_exclusive_scan_nt<kPrepThreads>(cnt.x + cnt.y + cnt.z + cnt.w, &total);
+  if (b0 < nbins) {
+    const int4 st = make_int4(run, run + cnt.x, run + cnt.x + cnt.y, run + cnt.x + cnt.y + cnt.z);
+    *reinterpret_cast<int4*>(sb.bin_start + b0) = st;
+    *reinterpret_cast<int4*>(sb.bin_cursor + b0) = st;
+    *reinterpret_cast<int4*>(sb.bin_count + b0) = make_int4(0, 0, 0, 0);
+    bs[b0] = st.x;
+    bs[b0 + 1] = st.y;
+    bs[b0 + 2] = st.z;
+    bs[b0 + 3] = st.w;
   }
-  if (threadIdx.x == 0) sb.tile_items[tiles] = total;
+  if (t == 0) {
+    sb.bin_start[nbins] = total;
+    bs[nbins] = total;
+  }
+  __syncthreads();
+  int items = 0;
+  if (t < tiles) {
+    int r0[4], r1[4];
+    tile_ranges(bs, t, ts, r0, r1);
+    const int pts = (r1[0] - r0[0]) + (r1[1] - r0[1]) + (r1[2] - r0[2]) + (r1[3] - r0[3]);
+    items = max(1, (pts + kSplatItem - 1) / kSplatItem);
+  }
+  int all;
+  const int first = block_exclusive_scan_nt<kPrepThreads>(items, &all);
+  if (t < tiles) sb.tile_items[t] = first;
+  if (t == 0) sb.tile_items[tiles] = all;
 }
 
 __device__ __forceinline__ void splat_scatter_body(const float2* __restrict__ y,
@@ -589,7 +594,7 @@ __global__ __launch_bounds__(kThreads) void splat_count(const float2* __restrict
   splat_count_body(y, st, sb, n, g, blockIdx.x, gridDim.x);
 }
 
-__global__ __launch_bounds__(kThreads) void splat_prep(SplatBuffers sb, int g) {
+__global__ __launch_bounds__(kPrepThreads) void splat_prep(SplatBuffers sb, int g) {
   splat_prep_body(sb, g);
 }
 
@@ -609,6 +614,9 @@ __global__ __launch_bounds__(kThreads) void splat_tiles(const TsneStats* __restr
 //    dy and dy + g/2.  Kernel row spectra are stored transposed,
 //    kt[k2 * g + dy] for k2 in [0, g], so conv_cols reads them contiguously.
 // ----------------------------------------------------------------------------
+// threads of the FFT pass workgroups
+constexpr int kFftThreads = 512;
+
 template <int F>
 __device__ __forceinline__ void conv_rows_fwd_body(SplatBuffers sb,
                                                    const TsneStats* __restrict__ st,
@@ -623,7 +631,7 @@ __device__ __forceinline__ void conv_rows_fwd_body(SplatBuffers sb,
     const int ts = tiles_side(G);
     const int* trow = sb.tile_items + (row / kTile) * ts;
     constexpr size_t gg = static_cast<size_t>(G) * G;
-    for (int x = threadIdx.x; x < F; x += kThreads) {
+    for (int x = threadIdx.x; x < F; x += kFftThreads) {
       float2 a = make_float2(0.f, 0.f), c = make_float2(0.f, 0.f);
       if (x < G) {
         const size_t node = static_cast<size_t>(row) * G + x;
@@ -649,19 +657,19 @@ __device__ __forceinline__ void conv_rows_fwd_body(SplatBuffers sb,
       buf[fft_at<F>(1, x)] = c;
     }
     __syncthreads();
-    fft_lds<F, 2, kThreads, false>(buf, tw);
+    fft_lds<F, 2, kFftThreads, false>(buf, tw);
     float2* o1 = z1 + static_cast<size_t>(row) * F;
     float2* o2 = z2 + static_cast<size_t>(row) * F;
     // z2 is the transform of a real row: columns G+1 .. F-1 are the
     // conjugates of columns G-1 .. 1 and are never read
-    for (int x = threadIdx.x; x < F; x += kThreads) {
+    for (int x = threadIdx.x; x < F; x += kFftThreads) {
       o1[x] = buf[fft_at<F>(0, x)];
       if (x <= G) o2[x] = buf[fft_at<F>(1, x)];
     }
   } else {
     const int dy0 = b - G;
     const float h = st->spacing;
-    for (int t = threadIdx.x; t < 2 * F; t += kThreads) {
+    for (int t = threadIdx.x; t < 2 * F; t += kFftThreads) {
       const int which = t / F, xx = t - which * F;
       const int dy = dy0 + which * (G / 2);
       float2 v = make_float2(0.f, 0.f);
@@ -675,8 +683,8 @@ __device__ __forceinline__ void conv_rows_fwd_body(SplatBuffers sb,
       buf[fft_at<F>(which, xx)] = v;
     }
     __syncthreads();
-    fft_lds<F, 2, kThreads, false>(buf, tw);
-    for (int t = threadIdx.x; t < 2 * (G + 1); t += kThreads) {
+    fft_lds<F, 2, kFftThreads, false>(buf, tw);
+    for (int t = threadIdx.x; t < 2 * (G + 1); t += kFftThreads) {
       const int which = t / (G + 1), k2 = t - which * (G + 1);
       const int dy = dy0 + which * (G / 2);
       kt[static_cast<size_t>(k2) * G + dy] = buf[fft_at<F>(which, k2)];
@@ -685,7 +693,7 @@ __device__ __forceinline__ void conv_rows_fwd_body(SplatBuffers sb,
 }
 
 template <int F>
-__global__ __launch_bounds__(kThreads) void conv_rows_fwd(SplatBuffers sb,
+__global__ __launch_bounds__(kFftThreads) void conv_rows_fwd(SplatBuffers sb,
                                                           const TsneStats* __restrict__ st,
                                                           const float2* __restrict__ tw,
                                                           float2* __restrict__ z1,
@@ -719,38 +727,50 @@ __device__ __forceinline__ void conv_cols_body(const float2* __restrict__ z1,
                                                double* __restrict__ partials, float2* lds,
                                                int c) {
   constexpr int G = F / 2;
-  constexpr int kPer = (F + kThreads - 1) / kThreads;  // spectrum entries per thread
+  constexpr int kPer = (F + kFftThreads - 1) / kFftThreads;  // spectrum entries per thread
   const int cp = (F - c) & (F - 1);
   const bool pair = (cp != c);
-  for (int y = threadIdx.x; y < F; y += kThreads) {
-    float2 a = make_float2(0.f, 0.f), bb = a, cc = a;
+  // every load of the column is issued before the first LDS write
+  constexpr int kIt = (F + kFftThreads - 1) / kFftThreads;
+  float2 la[kIt], lb[kIt], lc[kIt], lk[kIt];
+#pragma unroll
+  for (int it = 0; it < kIt; ++it) {
+    const int y = threadIdx.x + it * kFftThreads;
+    la[it] = lb[it] = lc[it] = lk[it] = make_float2(0.f, 0.f);
     if (y < G) {
       const size_t r = static_cast<size_t>(y) * F;
-      a = z1[r + c];
-      bb = z2[r + c];
-      cc = z1[r + cp];
+      la[it] = z1[r + c];
+      lb[it] = z2[r + c];
+      lc[it] = z1[r + cp];
     }
-    lds[fft_at<F>(0, y)] = a;
-    lds[fft_at<F>(1, y)] = bb;
-    lds[fft_at<F>(2, y)] = cc;
     const int d = y <= G ? y : F - y;
-    lds[fft_at<F>(3, y)] = d < G ? kt[static_cast<size_t>(c) * G + d] : make_float2(0.f, 0.f);
+    if (y < F && d < G) lk[it] = kt[static_cast<size_t>(c) * G + d];
+  }
+#pragma unroll
+  for (int it = 0; it < kIt; ++it) {
+    const int y = threadIdx.x + it * kFftThreads;
+    if (y < F) {
+      lds[fft_at<F>(0, y)] = la[it];
+      lds[fft_at<F>(1, y)] = lb[it];
+      lds[fft_at<F>(2, y)] = lc[it];
+      lds[fft_at<F>(3, y)] = lk[it];
+    }
   }
   __syncthreads();
-  fft_lds<F, 4, kThreads, false>(lds, tw);
+  fft_lds<F, 4, kFftThreads, false>(lds, tw);
   constexpr float inv_ff = 1.0f / (static_cast<float>(F) * static_cast<float>(F));
   // the kernel spectrum is read before its array takes column F-c of O2
   float2 kv[kPer];
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
-    const int t = threadIdx.x + j * kThreads;
+    const int t = threadIdx.x + j * kFftThreads;
     kv[j] = t < F ? lds[fft_at<F>(3, t)] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   double zpart = 0.0;
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
-    const int t = threadIdx.x + j * kThreads;
+    const int t = threadIdx.x + j * kFftThreads;
     if (t >= F) continue;
     const int k1 = t, k1m = (F - t) & (F - 1);
     const float2 z1a = lds[fft_at<F>(0, k1)], z1b = lds[fft_at<F>(2, k1m)];
@@ -775,8 +795,8 @@ __device__ __forceinline__ void conv_cols_body(const float2* __restrict__ z1,
   // self-paired columns (c = 0, c = G) visit every element twice
   if (!pair) zpart *= 0.5;
   __syncthreads();
-  fft_lds<F, 4, kThreads, true>(lds, tw);
-  for (int y = threadIdx.x; y < G; y += kThreads) {
+  fft_lds<F, 4, kFftThreads, true>(lds, tw);
+  for (int y = threadIdx.x; y < G; y += kFftThreads) {
     const size_t r = static_cast<size_t>(y) * F;
     o1[r + c] = lds[fft_at<F>(0, y)];
     o2[r + c] = lds[fft_at<F>(1, y)];
@@ -786,13 +806,13 @@ __device__ __forceinline__ void conv_cols_body(const float2* __restrict__ z1,
     }
   }
   // Z partial of the column pair; conv_rows_inv folds the partials
-  __shared__ double red[kThreads / 64];
+  __shared__ double red[kFftThreads / 64];
   zpart = wave_sum(zpart);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = zpart;
   __syncthreads();
   if (threadIdx.x == 0) {
     double t = 0.0;
-    for (int w = 0; w < kThreads / 64; ++w) t += red[w];
+    for (int w = 0; w < kFftThreads / 64; ++w) t += red[w];
     partials[c] = t;
   }
 }
@@ -810,7 +830,7 @@ __device__ __forceinline__ int xcd_column(int b, int g) {
 }
 
 template <int F>
-__global__ __launch_bounds__(kThreads) void conv_cols(const float2* __restrict__ z1,
+__global__ __launch_bounds__(kFftThreads) void conv_cols(const float2* __restrict__ z1,
                                                       const float2* __restrict__ z2,
                                                       const float2* __restrict__ kt,
                                                       const float2* __restrict__ tw,
@@ -840,36 +860,36 @@ __device__ __forceinline__ void conv_rows_inv_body(const float2* __restrict__ o1
     // Z: fold the G + 1 column-pair partials in a fixed order.  Parseval:
     // sum_nodes mass*q = (1/F^2) sum_k |A(k)|^2 Kq_hat(k) (the kernel
     // spectra already carry the 1/F^2)
-    __shared__ double red[kThreads / 64];
+    __shared__ double red[kFftThreads / 64];
     double a = 0.0;
-    for (int c = threadIdx.x; c <= G; c += kThreads) a += zpart[c];
+    for (int c = threadIdx.x; c <= G; c += kFftThreads) a += zpart[c];
     a = wave_sum(a);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
     __syncthreads();
     if (threadIdx.x == 0) {
       double t = 0.0;
-      for (int w = 0; w < kThreads / 64; ++w) t += red[w];
+      for (int w = 0; w < kFftThreads / 64; ++w) t += red[w];
       st->sum_q_total = t;
     }
   }
   const float2* s1 = o1 + static_cast<size_t>(row) * F;
   const float2* s2 = o2 + static_cast<size_t>(row) * F;
-  for (int x = threadIdx.x; x < F; x += kThreads) {
+  for (int x = threadIdx.x; x < F; x += kFftThreads) {
     buf[fft_at<F>(0, x)] = s1[x];
     buf[fft_at<F>(1, x)] = s2[x];
   }
   __syncthreads();
-  fft_lds<F, 2, kThreads, true>(buf, tw);
+  fft_lds<F, 2, kFftThreads, true>(buf, tw);
   float4* out = field + static_cast<size_t>(row) * G;
   const double xs = st->x_scale;
-  for (int x = threadIdx.x; x < G; x += kThreads) {
+  for (int x = threadIdx.x; x < G; x += kFftThreads) {
     const float2 a = buf[fft_at<F>(0, x)], b = buf[fft_at<F>(1, x)];
     out[x] = make_float4(b.x, static_cast<float>(static_cast<double>(a.y) * xs), b.y, a.x);
   }
 }
 
 template <int F>
-__global__ __launch_bounds__(kThreads) void conv_rows_inv(const float2* __restrict__ o1,
+__global__ __launch_bounds__(kFftThreads) void conv_rows_inv(const float2* __restrict__ o1,
                                                           const float2* __restrict__ o2,
                                                           const float2* __restrict__ tw,
                                                           const double* __restrict__ zpart,
