@@ -631,30 +631,48 @@ __device__ __forceinline__ void conv_rows_fwd_body(SplatBuffers sb,
     const int ts = tiles_side(G);
     const int* trow = sb.tile_items + (row / kTile) * ts;
     constexpr size_t gg = static_cast<size_t>(G) * G;
-    for (int x = threadIdx.x; x < F; x += kFftThreads) {
-      float2 a = make_float2(0.f, 0.f), c = make_float2(0.f, 0.f);
+    // the row's nodes x < G (all loads issued before the LDS writes); the
+    // zero-padded half x >= G
+    constexpr int kIt = (G + kFftThreads - 1) / kFftThreads;
+    float4 v[kIt];
+    bool split[kIt];
+#pragma unroll
+    for (int it = 0; it < kIt; ++it) {
+      const int x = threadIdx.x + it * kFftThreads;
+      const int t = min(x, G - 1) / kTile;
+      split[it] = x < G && trow[t + 1] - trow[t] > 1;
+    }
+#pragma unroll
+    for (int it = 0; it < kIt; ++it) {
+      const int x = threadIdx.x + it * kFftThreads;
+      v[it] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (x < G && !split[it]) v[it] = sb.sgrid[static_cast<size_t>(row) * G + x];
+    }
+#pragma unroll
+    for (int it = 0; it < kIt; ++it) {
+      if (!split[it]) continue;
+      // node of a tile split over several work items: its 64-bit sums
+      const size_t node = static_cast<size_t>(row) * G + threadIdx.x + it * kFftThreads;
+      const double inv_m = 1.0 / st->fx_mass, inv_xy = 1.0 / st->fx_xy;
+      const long long m = static_cast<long long>(sb.wide[node]);
+      const long long mxv = static_cast<long long>(sb.wide[gg + node]);
+      const long long myv = static_cast<long long>(sb.wide[2 * gg + node]);
+      sb.wide[node] = 0ull;
+      sb.wide[gg + node] = 0ull;
+      sb.wide[2 * gg + node] = 0ull;
+      v[it] = node_value(m, mxv, myv, inv_m, inv_xy / st->x_scale, inv_xy);
+    }
+#pragma unroll
+    for (int it = 0; it < kIt; ++it) {
+      const int x = threadIdx.x + it * kFftThreads;
       if (x < G) {
-        const size_t node = static_cast<size_t>(row) * G + x;
-        const int t = x / kTile;
-        float4 v;
-        if (trow[t + 1] - trow[t] > 1) {
-          // node of a tile split over several work items: its 64-bit sums
-          const double inv_m = 1.0 / st->fx_mass, inv_xy = 1.0 / st->fx_xy;
-          const long long m = static_cast<long long>(sb.wide[node]);
-          const long long mxv = static_cast<long long>(sb.wide[gg + node]);
-          const long long myv = static_cast<long long>(sb.wide[2 * gg + node]);
-          sb.wide[node] = 0ull;
-          sb.wide[gg + node] = 0ull;
-          sb.wide[2 * gg + node] = 0ull;
-          v = node_value(m, mxv, myv, inv_m, inv_xy / st->x_scale, inv_xy);
-        } else {
-          v = sb.sgrid[node];
-        }
-        a = make_float2(v.x, v.y);
-        c = make_float2(v.z, 0.f);
+        buf[fft_at<F>(0, x)] = make_float2(v[it].x, v[it].y);
+        buf[fft_at<F>(1, x)] = make_float2(v[it].z, 0.f);
       }
-      buf[fft_at<F>(0, x)] = a;
-      buf[fft_at<F>(1, x)] = c;
+    }
+    for (int x = G + threadIdx.x; x < F; x += kFftThreads) {
+      buf[fft_at<F>(0, x)] = make_float2(0.f, 0.f);
+      buf[fft_at<F>(1, x)] = make_float2(0.f, 0.f);
     }
     __syncthreads();
     fft_lds<F, 2, kFftThreads, false>(buf, tw);
@@ -874,9 +892,25 @@ __device__ __forceinline__ void conv_rows_inv_body(const float2* __restrict__ o1
   }
   const float2* s1 = o1 + static_cast<size_t>(row) * F;
   const float2* s2 = o2 + static_cast<size_t>(row) * F;
-  for (int x = threadIdx.x; x < F; x += kFftThreads) {
-    buf[fft_at<F>(0, x)] = s1[x];
-    buf[fft_at<F>(1, x)] = s2[x];
+  {
+    constexpr int kIt = (F + kFftThreads - 1) / kFftThreads;
+    float2 a[kIt], b[kIt];
+#pragma unroll
+    for (int it = 0; it < kIt; ++it) {
+      const int x = threadIdx.x + it * kFftThreads;
+      if (x < F) {
+        a[it] = s1[x];
+        b[it] = s2[x];
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < kIt; ++it) {
+      const int x = threadIdx.x + it * kFftThreads;
+      if (x < F) {
+        buf[fft_at<F>(0, x)] = a[it];
+        buf[fft_at<F>(1, x)] = b[it];
+      }
+    }
   }
   __syncthreads();
   fft_lds<F, 2, kFftThreads, true>(buf, tw);
