@@ -43,8 +43,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-json", default=None,
                     help="rocprofv3-derived HBM bytes per launch of the dominant kernel")
-    ap.add_argument("--quality", action="store_true",
-                    help="also report trustworthiness / kNN preservation on a 2000-point sample")
+    ap.add_argument("--quality-sample", type=int, default=5000,
+                    help="points of the exact structure-metric sample of the final layout "
+                         "(device metric, timed beside the CPU restatement); 0 skips it")
     ap.add_argument("--batched-jobs", type=int, default=256,
                     help="jobs of the batched workload (0: skip it)")
     ap.add_argument("--served-requests", type=int, default=48,
@@ -295,13 +296,29 @@ def main():
                          f"1M-point P and layout, {threads} threads, {sec:.1f}s"}
         log(f"[bench] cpu baseline {cpu}")
     quality = None
-    if args.quality:
-        import oracle
+    if args.quality_sample > 0:
+        from fastembedr_amd import quality as fq
 
+        qs = min(args.quality_sample, n, 5000)
         rng = np.random.default_rng(7)
-        s = rng.choice(n, 2000, replace=False)
-        m = oracle.structure_metrics(x[s].astype(np.float64), layout[s].astype(np.float64), 15)
-        quality = {"trustworthiness_k15": float(m[0]), "knn_preservation_k15": float(m[2])}
+        sidx = np.sort(rng.choice(n, qs, replace=False))
+        hs, ls = x[sidx].astype(np.float64), layout[sidx].astype(np.float64)
+        fq.exact_structure_metrics(hs[:64], ls[:64], [15])  # warm-up (module load, LDS attribute)
+        t0 = time.perf_counter()
+        m = fq.exact_structure_metrics(hs, ls, [15])[0]
+        dev_s = time.perf_counter() - t0
+        quality = {"sample": int(qs), "k": 15, "trustworthiness": float(m[0]),
+                   "continuity": float(m[1]), "knn_preservation": float(m[2]),
+                   "mean_neighbor_rank_error": float(m[3]), "device_ms": round(1e3 * dev_s, 3)}
+        if not args.no_cpu_baseline:
+            import oracle
+
+            t0 = time.perf_counter()
+            mc = oracle.structure_metrics(hs, ls, 15)
+            quality["cpu_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
+            quality["cpu_kind"] = "oracle restatement of exact_structure_metrics_cpp, 1 thread"
+            quality["identical_to_cpu"] = bool(np.array_equal(np.asarray(mc), m))
+        log(f"[bench] quality {quality}")
     plan.close()
     torch.cuda.empty_cache()
 

@@ -77,6 +77,8 @@ SIGNATURES = {
                 c_double, c_double, c_double, c_double, c_double, c_double, c_int, c_int, c_int,
                 _fp]),
     "fastembedr_hip_transform_reference_destroy": (None, [_vp]),
+    "fastembedr_hip_exact_structure_metrics": (
+        c_int, [_dp, c_int, c_int, _dp, c_int, _ip, c_int, _dp]),
 }
 
 _lib = None

@@ -266,6 +266,17 @@ int fastembedr_hip_transform_reference_run(fastembedr_hip_transform_reference* r
 
 void fastembedr_hip_transform_reference_destroy(fastembedr_hip_transform_reference* ref);
 
+/* ---- embedding quality ---------------------------------------------------
+ * Replaces exact_structure_metrics_cpp (src/embedding_utils.cpp:1582-1752):
+ * high (n x high_cols) and low (n x low_cols) are column-major FP64 (R
+ * matrices), 3 <= n <= 5000, every requested k with 1 <= k < n and
+ * 2n - 3k - 1 > 0.  out is column-major nk x 4: trustworthiness,
+ * continuity, knn_preservation, mean_neighbor_rank_error (the reference's
+ * column order); equal to the reference's values bit for bit. */
+int fastembedr_hip_exact_structure_metrics(const double* high, int n, int high_cols,
+                                           const double* low, int low_cols,
+                                           const int* requested_k, int nk, double* out);
+
 #ifdef __cplusplus
 }
 #endif

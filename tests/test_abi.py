@@ -54,3 +54,25 @@ def test_auto_parameters_known_answers():
     b = auto_parameters(70000, 90)
     assert b["perplexity"] == 30 and b["n_neighbors"] == 90
     assert b["early_exaggeration_iter"] == 250 and b["n_iter"] == 500
+
+
+def test_structure_metrics_argument_errors_need_no_device():
+    # embedding_utils.cpp:1587-1605: the checks run before any device work
+    import numpy as np
+
+    from fastembedr_amd import HipError, quality
+
+    x = np.zeros((10, 3))
+    cases = [
+        (np.zeros((2, 3)), np.zeros((2, 2)), [1], "at least three observations are required"),
+        (np.zeros((5001, 1)), np.zeros((5001, 1)), [5], "at most 5000 sampled observations"),
+        (x, np.zeros((10, 2)), [], "requested_k must not be empty"),
+        (x, np.zeros((10, 2)), [0], "each k must satisfy"),
+        (x, np.zeros((10, 2)), [7], "each k must satisfy"),
+        (x, np.zeros((10, 2)), [3, 10], "each k must satisfy"),
+    ]
+    for high, low, ks, msg in cases:
+        with pytest.raises(HipError, match=msg):
+            quality.exact_structure_metrics(high, low, ks)
+    with pytest.raises(ValueError, match="same row count"):
+        quality.exact_structure_metrics(x, np.zeros((9, 2)), [2])
