@@ -77,6 +77,12 @@ SIGNATURES = {
                 c_double, c_double, c_double, c_double, c_double, c_double, c_int, c_int, c_int,
                 _fp]),
     "fastembedr_hip_transform_reference_destroy": (None, [_vp]),
+    "fastembedr_hip_umap_graph_create": (
+        c_int, [_vp, _vp, c_int, c_int, c_int, c_int, c_int, c_int, c_int, _vpp]),
+    "fastembedr_hip_umap_graph_info": (c_int, [_vp, _llp, _fp]),
+    "fastembedr_hip_umap_graph_export": (c_int, [_vp, _ip, _ip, _fp, _fp]),
+    "fastembedr_hip_umap_graph_device": (c_int, [_vp, _vpp, _vpp, _vpp, _vpp]),
+    "fastembedr_hip_umap_graph_destroy": (None, [_vp]),
     "fastembedr_hip_exact_structure_metrics": (
         c_int, [_dp, c_int, c_int, _dp, c_int, _ip, c_int, _dp]),
 }

@@ -6,11 +6,16 @@ reached from umap_cuda_optimize_csr_cpp / umap_cuda_optimize_coo_impl,
 src/embedding_cuda_bridge.cpp:345-378, src/embedding_cuda_impl.cpp:
 2048-2107).  ``csr_to_coo`` does the bridge's CSR -> COO expansion with
 epochs_per_sample = max_w / max(w, 1e-6) (:3019-3042; CPU
-attach_csr_epoch_schedule, src/fast_knn_umap.cpp:1264-1282).
+attach_csr_epoch_schedule, src/fast_knn_umap.cpp:1264-1282).  ``FuzzyGraph``
+builds the reference's default fuzzy graph (umap_graph_csr_float_cpp,
+src/fast_knn_umap.cpp:3995-4048) on the device and can feed the epochs
+without a host round trip.
 """
 from __future__ import annotations
 
 import numpy as np
+
+import ctypes
 
 from . import _lib
 from ._lib import _fp, _ip, check, ptr
@@ -69,3 +74,101 @@ def schedule(heads, tails, epochs_per_sample, n, epoch, negative_sample_rate=5, 
         int(seed) & 0xFFFFFFFF, mn, ptr(npos, _ip), ptr(nneg, _ip), ptr(ids, _ip))
     check(rc, "HIP UMAP schedule")
     return npos, nneg, ids[:, :mn]
+
+
+def _device_ptr(a):
+    if hasattr(a, "data_ptr"):  # torch tensor on the GPU
+        import torch
+
+        if a.is_cuda:
+            torch.cuda.current_stream(a.device).synchronize()
+        return a.data_ptr()
+    return int(a)
+
+
+class FuzzyGraph:
+    """The default UMAP fuzzy graph of a KNN (rows = points), kept on the GPU.
+
+    ``indices`` / ``distances`` are n x k arrays (host), or with
+    ``device=True`` column-major k x n device buffers (torch tensors or raw
+    pointers) with ``n`` and ``k`` given.  ``index_offset`` None detects it
+    as the reference does.  ``export()`` returns the reference's
+    list(offsets, neighbors, weights, epochs_per_sample); ``optimize`` runs
+    the epochs from the resident graph."""
+
+    def __init__(self, indices, distances, col_start=0, n_cols=None, edge_budget=None,
+                 index_offset=None, device=False, n=None, k=None):
+        L = _lib.lib()
+        if device:
+            n, k_total = int(n), int(k)
+            ip, dp = _device_ptr(indices), _device_ptr(distances)
+            keep = ()
+        else:
+            idx = np.asarray(indices)
+            dist = np.asarray(distances)
+            if idx.ndim != 2 or idx.shape != dist.shape:
+                raise ValueError("indices and distances must have the same dimensions")
+            n, k_total = idx.shape
+            ic = np.ascontiguousarray(idx.astype(np.int32).T)
+            # the reference's float conversion of double distances
+            dc = np.ascontiguousarray(np.asarray(dist, np.float64).astype(np.float32).T)
+            ip, dp = ic.ctypes.data, dc.ctypes.data
+            keep = (ic, dc)
+        n_cols = k_total - col_start if n_cols is None else int(n_cols)
+        edge_budget = n_cols if edge_budget is None else int(edge_budget)
+        off = -1 if index_offset is None else int(index_offset)
+        h = ctypes.c_void_p()
+        check(L.fastembedr_hip_umap_graph_create(ip, dp, n, k_total, int(col_start), n_cols,
+                                                 edge_budget, off, 1 if device else 0,
+                                                 ctypes.byref(h)), "HIP UMAP fuzzy graph")
+        del keep
+        self._h = h
+        self.n = n
+        nnz = ctypes.c_longlong()
+        mw = ctypes.c_float()
+        check(L.fastembedr_hip_umap_graph_info(h, ctypes.byref(nnz), ctypes.byref(mw)),
+              "HIP UMAP fuzzy graph info")
+        self.nnz = int(nnz.value)
+        self.max_weight = float(mw.value)
+
+    def export(self):
+        off = np.zeros(self.n + 1, np.int32)
+        m = max(self.nnz, 1)
+        nb = np.zeros(m, np.int32)
+        w = np.zeros(m, np.float32)
+        e = np.zeros(m, np.float32)
+        check(_lib.lib().fastembedr_hip_umap_graph_export(self._h, ptr(off, _ip), ptr(nb, _ip),
+                                                          ptr(w, _fp), ptr(e, _fp)),
+              "HIP UMAP fuzzy graph export")
+        k = self.nnz
+        return {"offsets": off, "neighbors": nb[:k], "weights": w[:k],
+                "epochs_per_sample": e[:k], "max_weight": self.max_weight, "n": self.n,
+                "nnz": k}
+
+    def optimize(self, init, n_epochs, negative_sample_rate=5, learning_rate=1.0,
+                 a=A_MIN_DIST_001, b=B_MIN_DIST_001, repulsion_strength=1.0, seed=0,
+                 optimizer_mode=1):
+        """UMAP epochs on the resident graph; returns the float32 n x 2 layout."""
+        ptrs = [ctypes.c_void_p() for _ in range(4)]
+        L = _lib.lib()
+        check(L.fastembedr_hip_umap_graph_device(self._h, *[ctypes.byref(p) for p in ptrs]),
+              "HIP UMAP fuzzy graph device")
+        y = np.ascontiguousarray(init, dtype=np.float32)
+        out = np.zeros_like(y)
+        rc = L.fastembedr_hip_umap_optimize_coo_device(
+            ptrs[0], ptrs[1], ptrs[2], ptrs[3], ptr(y, _fp), self.n, self.nnz, int(n_epochs),
+            int(negative_sample_rate), float(learning_rate), float(a), float(b),
+            float(repulsion_strength), int(seed) & 0xFFFFFFFF, int(optimizer_mode), ptr(out, _fp))
+        check(rc, "HIP COO UMAP optimizer (device graph)")
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().fastembedr_hip_umap_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -225,6 +225,42 @@ int fastembedr_hip_umap_schedule(const int* heads, const int* tails,
                                  int negative_sample_rate, unsigned int seed, int max_neg,
                                  int* n_pos, int* n_neg, int* neg_ids);
 
+/* The UMAP fuzzy graph, built and kept on the device.  Replaces
+ * umap_graph_csr_float_cpp (src/fast_knn_umap.cpp:3995-4048) at the default
+ * edge budget (edge_budget >= n_cols; fast_knn_umap.R:611-640 passes
+ * n_neighbors), i.e. build_graph_csr_native_direct_values (:1562-1780):
+ * smooth_knn_dist sigmas and rhos, memberships exp(-(d - rho)/sigma), the
+ * union f + r - f*r above 1e-6 and epochs_per_sample = max_w / max(w, 1e-6).
+ * indices / distances: column-major n x k_total (float32 distances, as the
+ * reference converts them), columns col_start .. col_start + n_cols - 1;
+ * index_offset -1 detects it as the reference does (1 when every used index
+ * lies in 1..n); device_inputs: the two arrays are device pointers. */
+typedef struct fastembedr_hip_umap_graph fastembedr_hip_umap_graph;
+
+int fastembedr_hip_umap_graph_create(const int* indices, const float* distances, int n,
+                                     int k_total, int col_start, int n_cols, int edge_budget,
+                                     int index_offset, int device_inputs,
+                                     fastembedr_hip_umap_graph** graph);
+
+/* nnz and max_weight of the graph. */
+int fastembedr_hip_umap_graph_info(fastembedr_hip_umap_graph* graph, long long* nnz,
+                                   float* max_weight);
+
+/* The CSR to host buffers (offsets n+1, the rest nnz; any may be NULL), as
+ * the reference's list(offsets, neighbors, weights, epochs_per_sample),
+ * 0-based neighbours in ascending order within a row. */
+int fastembedr_hip_umap_graph_export(fastembedr_hip_umap_graph* graph, int* offsets,
+                                     int* neighbors, float* weights, float* epochs_per_sample);
+
+/* Borrowed device pointers to the graph as COO (heads = row of each entry),
+ * valid until destroy: the inputs of fastembedr_hip_umap_optimize_coo_device.
+ * No reference counterpart (the reference returns the graph to the host). */
+int fastembedr_hip_umap_graph_device(fastembedr_hip_umap_graph* graph, const int** heads,
+                                     const int** tails, const float** weights,
+                                     const float** epochs_per_sample);
+
+void fastembedr_hip_umap_graph_destroy(fastembedr_hip_umap_graph* graph);
+
 /* ---- fixed-reference t-SNE transform ------------------------------------
  * The reference binds transform_tsne_cuda_cpp (R/transform_tsne.R:275) but
  * defines it nowhere; the argument list follows transform_tsne_metal_cpp

@@ -86,6 +86,8 @@ def _declare(L):
                                     c_double, c_double, c_double, c_int, c_int, _dp]
     if hasattr(L, "or_umap_epoch"):
         _declare_umap(L)
+    if hasattr(L, "or_umap_fuzzy_graph"):
+        _declare_umap_graph(L)
     if hasattr(L, "or_transform"):
         _declare_transform(L)
 
@@ -108,6 +110,13 @@ def _declare_umap(L):
                                    c_double, c_double, c_uint, c_int, _fp]
     L.or_umap_schedule.argtypes = [_ip, _ip, _fp, c_int, c_int, c_int, c_int, c_uint, c_int, _ip,
                                    _ip, _ip]
+
+
+def _declare_umap_graph(L):
+    c_int, c_ll = ctypes.c_int, ctypes.c_longlong
+    L.or_umap_fuzzy_graph.argtypes = [_ip, _fp, c_int, c_int, c_int, c_int, c_int, _ip, _ip, _fp,
+                                      _fp, _fp, _fp, _fp, c_ll]
+    L.or_umap_fuzzy_graph.restype = c_int
 
 
 def _declare_transform(L):
@@ -337,6 +346,39 @@ def port_tsne_run(indices, distances, y_init, perplexity=30.0, early_iter=250, n
 # min_dist 0.01, spread 1 (find_ab_params, src/fast_knn_umap.cpp:437-497)
 UMAP_A = 1.895605865596314
 UMAP_B = 0.8006377738365004
+
+
+def umap_fuzzy_graph(indices, distances, col_start=0, n_cols=None, index_offset=None):
+    """The reference's default fuzzy graph (umap_graph_csr_float_cpp) of a KNN
+    given as n x k arrays (rows = points).  index_offset None: detected as the
+    reference does (1 when every used index lies in 1..n).  Returns a dict
+    with offsets, neighbors, weights, epochs_per_sample, max_weight, sigmas,
+    rhos."""
+    idx = np.asarray(indices)
+    n, k_total = idx.shape
+    n_cols = k_total - col_start if n_cols is None else n_cols
+    used = idx[:, col_start:col_start + n_cols]
+    if index_offset is None:
+        index_offset = 1 if (used.min() >= 1 and used.max() <= n) else 0
+    ic = np.ascontiguousarray(idx.astype(np.int32).T)
+    dc = np.ascontiguousarray(np.asarray(distances, np.float64).astype(np.float32).T)
+    cap = n * n_cols * 2 + 1
+    off = np.zeros(n + 1, np.int32)
+    nb = np.zeros(cap, np.int32)
+    w = np.zeros(cap, np.float32)
+    eps = np.zeros(cap, np.float32)
+    mw = np.zeros(1, np.float32)
+    sig = np.zeros(n, np.float32)
+    rho = np.zeros(n, np.float32)
+    nnz = lib().or_umap_fuzzy_graph(_ptr(ic, _ip), _ptr(dc, _fp), n, k_total, col_start, n_cols,
+                                    int(index_offset), _ptr(off, _ip), _ptr(nb, _ip), _ptr(w, _fp),
+                                    _ptr(eps, _fp), _ptr(mw, _fp), _ptr(sig, _fp), _ptr(rho, _fp),
+                                    cap)
+    if nnz < 0:
+        raise RuntimeError("or_umap_fuzzy_graph capacity")
+    return {"offsets": off, "neighbors": nb[:nnz].copy(), "weights": w[:nnz].copy(),
+            "epochs_per_sample": eps[:nnz].copy(), "max_weight": float(mw[0]), "sigmas": sig,
+            "rhos": rho}
 
 
 def umap_schedule(heads, tails, period, n, epoch, nsr, seed, max_neg):

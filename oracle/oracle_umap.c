@@ -187,3 +187,157 @@ void or_umap_schedule(const int* heads, const int* tails, const float* period, i
     }
   }
 }
+
+/* ---- fuzzy graph (umap_graph_csr_float_cpp, src/fast_knn_umap.cpp:3995-4048,
+ * building with build_graph_csr_native_direct_values :1562-1780 at the default
+ * edge budget = n_cols) ------------------------------------------------------
+ * smooth_knn_dist :876-1003 (one thread: the global mean in long double, row
+ * order); directed memberships :1608-1634; per-row sort by neighbour and
+ * max-dedupe (compact_directed_rows_inplace :1291-1364); incoming lists in
+ * source order :1641-1700; union f + r - f*r with the 1e-6 floor :1702-1768;
+ * epochs_per_sample :1264-1282.  indices / distances: column-major n x k_total
+ * (R matrices), columns col_start .. col_start+n_cols-1 used.  Returns nnz, or
+ * -1 when cap is too small. */
+static void or_smooth_knn_rows(const float* dist, int n, int k, float* sigmas, float* rhos) {
+  const double target = log2((double)k);
+  long double gsum = 0.0L;
+  size_t gcount = 0;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < k; ++j) {
+      const float d = dist[(size_t)i * k + j];
+      if (isfinite(d) && d >= 0.0f) { gsum += (long double)d; ++gcount; }
+    }
+  const double gmean = gcount > 0 ? (double)(gsum / (long double)gcount) : 1.0;
+  for (int i = 0; i < n; ++i) {
+    const float* row = dist + (size_t)i * k;
+    double rho = INFINITY, rsum = 0.0;
+    int rcount = 0;
+    for (int j = 0; j < k; ++j) {
+      const float d = row[j];
+      if (!isfinite(d)) continue;
+      if (d >= 0.0f) { rsum += (double)d; ++rcount; }
+      if (d > 0.0f && (double)d < rho) rho = (double)d;
+    }
+    if (!isfinite(rho)) rho = 0.0;
+    rhos[i] = (float)rho;
+    const double big = 1.79769313486231570815e+308;
+    double sigma = 1.0, best = 1.0, best_diff = big, lo = 0.0, hi = big;
+    for (int it = 0; it < 64; ++it) {
+      const double safe = sigma > 1.0e-12 ? sigma : 1.0e-12;
+      double psum = 0.0;
+      for (int j = 0; j < k; ++j) {
+        if (!isfinite(row[j])) continue;
+        const double d = (double)row[j] - rho;
+        psum += d <= 0.0 ? 1.0 : exp(-d / safe);
+      }
+      const double diff = fabs(psum - target);
+      if (diff < best_diff) { best_diff = diff; best = sigma; }
+      if (psum > target) {
+        hi = sigma;
+        sigma = 0.5 * (lo + hi);
+      } else {
+        lo = sigma;
+        sigma = hi == big ? sigma * 2.0 : 0.5 * (lo + hi);
+      }
+      if (diff < 1.0e-5) break;
+    }
+    const double rmean = rcount > 0 ? rsum / (double)rcount : gmean;
+    const double floor_s = 1.0e-3 * (rho > 0.0 ? rmean : gmean);
+    if (best < floor_s) best = floor_s;
+    sigmas[i] = (float)(best > 1.0e-12 ? best : 1.0e-12);
+  }
+}
+
+int or_umap_fuzzy_graph(const int* indices, const float* distances, int n, int k_total,
+                        int col_start, int n_cols, int index_offset, int* offsets,
+                        int* neighbors, float* weights, float* eps, float* max_weight,
+                        float* sigmas, float* rhos, long long cap) {
+  const int k = n_cols;
+  float* dist = (float*)malloc(sizeof(float) * (size_t)n * k);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < k; ++j) dist[(size_t)i * k + j] = distances[(size_t)(j + col_start) * n + i];
+  or_smooth_knn_rows(dist, n, k, sigmas, rhos);
+  /* directed rows, sorted by neighbour (insertion, stable), equal neighbours
+   * merged by max */
+  int* dn = (int*)malloc(sizeof(int) * (size_t)n * k);
+  float* dw = (float*)malloc(sizeof(float) * (size_t)n * k);
+  int* dc = (int*)calloc((size_t)n, sizeof(int));
+  int* incnt = (int*)calloc((size_t)n + 1, sizeof(int));
+  for (int i = 0; i < n; ++i) {
+    int* rn = dn + (size_t)i * k;
+    float* rw = dw + (size_t)i * k;
+    int c = 0;
+    for (int j = 0; j < k; ++j) {
+      const int nb = indices[(size_t)(j + col_start) * n + i] - index_offset;
+      const float d = dist[(size_t)i * k + j];
+      if (nb < 0 || nb >= n || nb == i || !isfinite(d)) continue;
+      const float w = d <= rhos[i] ? 1.0f : expf(-(d - rhos[i]) / sigmas[i]);
+      if (!isfinite(w) || w <= 0.0f) continue;
+      int at = c++;
+      while (at > 0 && rn[at - 1] > nb) { rn[at] = rn[at - 1]; rw[at] = rw[at - 1]; --at; }
+      rn[at] = nb;
+      rw[at] = w;
+    }
+    int u = 0;
+    for (int r = 0; r < c;) {
+      const int nb = rn[r];
+      float w = rw[r++];
+      while (r < c && rn[r] == nb) { if (rw[r] > w) w = rw[r]; ++r; }
+      rn[u] = nb;
+      rw[u] = w;
+      ++u;
+    }
+    dc[i] = u;
+    for (int r = 0; r < u; ++r) incnt[rn[r] + 1]++;
+  }
+  for (int i = 0; i < n; ++i) incnt[i + 1] += incnt[i];
+  long long tot_in = incnt[n];
+  int* in_src = (int*)malloc(sizeof(int) * (size_t)(tot_in > 0 ? tot_in : 1));
+  float* in_w = (float*)malloc(sizeof(float) * (size_t)(tot_in > 0 ? tot_in : 1));
+  int* cur = (int*)malloc(sizeof(int) * (size_t)n);
+  memcpy(cur, incnt, sizeof(int) * (size_t)n);
+  for (int i = 0; i < n; ++i) /* sources in ascending order */
+    for (int r = 0; r < dc[i]; ++r) {
+      const int t = dn[(size_t)i * k + r];
+      in_src[cur[t]] = i;
+      in_w[cur[t]] = dw[(size_t)i * k + r];
+      cur[t]++;
+    }
+  long long nnz = 0;
+  offsets[0] = 0;
+  for (int i = 0; i < n; ++i) {
+    int o = 0, p = incnt[i];
+    const int oe = dc[i], pe = incnt[i + 1];
+    const int* rn = dn + (size_t)i * k;
+    const float* rw = dw + (size_t)i * k;
+    while (o < oe || p < pe) {
+      const int a = o < oe ? rn[o] : 2147483647, b = p < pe ? in_src[p] : 2147483647;
+      int nb;
+      float f = 0.0f, r = 0.0f;
+      if (a == b) { nb = a; f = rw[o++]; r = in_w[p++]; }
+      else if (a < b) { nb = a; f = rw[o++]; }
+      else { nb = b; r = in_w[p++]; }
+      if (nb == i) continue;
+      const float s = f + r, pr = f * r;
+      const float w = s - pr;
+      if (!isfinite(w) || w <= 1.0e-6f) continue;
+      if (nnz >= cap) { nnz = -1; goto done; }
+      neighbors[nnz] = nb;
+      weights[nnz] = w;
+      ++nnz;
+    }
+    offsets[i + 1] = (int)nnz;
+  }
+  {
+    float mw = 0.0f;
+    for (long long e = 0; e < nnz; ++e) if (isfinite(weights[e]) && weights[e] > mw) mw = weights[e];
+    *max_weight = mw;
+    for (long long e = 0; e < nnz; ++e) {
+      const float w = weights[e] > 1.0e-6f ? weights[e] : 1.0e-6f;
+      eps[e] = mw > 0.0f ? mw / w : INFINITY;
+    }
+  }
+done:
+  free(dist); free(dn); free(dw); free(dc); free(incnt); free(in_src); free(in_w); free(cur);
+  return (int)nnz;
+}

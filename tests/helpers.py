@@ -96,3 +96,27 @@ def gaussian_mixture(n, d, n_clusters, seed, center_sd=5.0, dirichlet=False):
     labels = np.repeat(np.arange(n_clusters), sizes)
     x = centres[labels] + rng.normal(size=(n, d))
     return x.astype(np.float32), labels
+
+
+class DeviceArray:
+    """A device copy of a numpy array through the HIP runtime the backend
+    library links (ctypes; no torch), for device-input tests."""
+
+    def __init__(self, a):
+        import ctypes
+
+        from fastembedr_amd import _lib
+
+        _lib.lib()  # the backend's runtime is loaded first
+        self._hip = ctypes.CDLL("libamdhip64.so.7")
+        a = np.ascontiguousarray(a)
+        self.ptr = ctypes.c_void_p()
+        assert self._hip.hipMalloc(ctypes.byref(self.ptr), ctypes.c_size_t(max(a.nbytes, 1))) == 0
+        assert self._hip.hipMemcpy(self.ptr, a.ctypes.data_as(ctypes.c_void_p),
+                                   ctypes.c_size_t(a.nbytes), 1) == 0  # host to device
+        self.nbytes = a.nbytes
+
+    def free(self):
+        if self.ptr:
+            self._hip.hipFree(self.ptr)
+            self.ptr = None

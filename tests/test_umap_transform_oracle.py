@@ -164,3 +164,85 @@ def test_transform_rejects_bad_knn():
     ref = np.zeros((10, 2))
     with pytest.raises(ValueError):
         oracle.transform(ref, np.array([[11]]), np.array([[1.0]]))
+
+
+def _dense_fuzzy(idx, dist, sig, rho, offset=0):
+    """Dense restatement of the reference's memberships and union
+    (fast_knn_umap.cpp:1608-1634, 1702-1768) from the oracle's sigmas/rhos,
+    float32 arithmetic, exp of the float argument correctly rounded via
+    float64."""
+    n = idx.shape[0]
+    D = np.zeros((n, n), np.float32)
+    d32 = np.asarray(dist, np.float64).astype(np.float32)
+    for i in range(n):
+        for j in range(idx.shape[1]):
+            nb = int(idx[i, j]) - offset
+            d = d32[i, j]
+            if nb < 0 or nb >= n or nb == i or not np.isfinite(d):
+                continue
+            if d <= rho[i]:
+                w = np.float32(1.0)
+            else:
+                x = np.float32(-(d - rho[i])) / sig[i]
+                w = np.float32(np.exp(np.float64(np.float32(x))))
+            if w > 0:
+                D[i, nb] = max(D[i, nb], w)
+    s = (D + D.T).astype(np.float32)
+    p = (D * D.T).astype(np.float32)
+    return (s - p).astype(np.float32)
+
+
+def test_fuzzy_graph_oracle_pinned():
+    rng = np.random.default_rng(41)
+    x = np.vstack([rng.normal(size=(120, 6)), rng.normal(loc=4.0, size=(80, 6))])
+    idx, dist = exact_knn(x, 15)
+    g = oracle.umap_fuzzy_graph(idx, dist, index_offset=0)
+    n, k = idx.shape
+    sig, rho = g["sigmas"], g["rhos"]
+    # smooth_knn_dist: rho is the smallest positive distance and the search
+    # meets sum_j exp(-max(d - rho, 0)/sigma) = log2(k) to its tolerance
+    d32 = dist.astype(np.float32)
+    assert np.array_equal(rho, d32.min(1))
+    psum = np.exp(-np.maximum(d32.astype(np.float64) - rho[:, None], 0) / sig[:, None]).sum(1)
+    assert np.abs(psum - np.log2(k)).max() < 1e-4
+    # the union against a dense restatement
+    W = _dense_fuzzy(idx, dist, sig, rho)
+    assert np.array_equal(W, W.T)
+    rows, cols = np.nonzero(W > np.float32(1e-6))
+    assert np.array_equal(np.diff(g["offsets"]), np.bincount(rows, minlength=n))
+    assert np.array_equal(g["neighbors"], cols.astype(np.int32))
+    ref_w = W[rows, cols]
+    assert np.abs(g["weights"] - ref_w).max() <= 2 * np.spacing(np.float32(1.0))
+    # epochs_per_sample = max_w / max(w, 1e-6) (fast_knn_umap.cpp:1264-1282)
+    mw = np.float32(g["weights"].max())
+    assert g["max_weight"] == mw
+    assert np.array_equal(g["epochs_per_sample"],
+                          (mw / np.maximum(g["weights"], np.float32(1e-6))).astype(np.float32))
+
+
+def test_fuzzy_graph_oracle_known_answers():
+    # k = 2: the target log2(2) = 1 is met by the nearest neighbour alone, so
+    # every directed edge but the nearest vanishes; the union of a mutual
+    # nearest pair is 1 + 1 - 1 = 1
+    x = np.array([[0.0], [1.0], [3.0], [10.0]])
+    idx, dist = exact_knn(x, 2)
+    g = oracle.umap_fuzzy_graph(idx, dist, index_offset=0)
+    W = np.zeros((4, 4), np.float32)
+    for i in range(4):
+        for p in range(g["offsets"][i], g["offsets"][i + 1]):
+            W[i, g["neighbors"][p]] = g["weights"][p]
+    # nearest: 0->1, 1->0, 2->1, 3->2
+    assert W[0, 1] == 1.0 and W[1, 0] == 1.0 and W[1, 2] == 1.0 and W[2, 3] == 1.0
+    assert (W == W.T).all()
+    assert W[0, 2] == 0.0 and W[0, 3] == 0.0
+    # 1-based indices are detected like the reference (all in 1..n)
+    g1 = oracle.umap_fuzzy_graph(idx + 1, dist)
+    assert np.array_equal(g1["neighbors"], g["neighbors"])
+    # duplicate points: rho falls back to 0 for an all-zero row and sigma to
+    # its floor 1e-3 * global mean
+    y = np.array([[0.0], [0.0], [0.0], [5.0]])
+    idy, dsy = exact_knn(y, 2)
+    gd = oracle.umap_fuzzy_graph(idy, dsy, index_offset=0)
+    assert gd["rhos"][0] == 0.0
+    gmean = dsy[np.isfinite(dsy)].astype(np.float32).astype(np.float64).mean()
+    assert abs(gd["sigmas"][0] - 1e-3 * gmean) / (1e-3 * gmean) < 1e-6 or gd["sigmas"][0] >= 1e-3 * gmean
