@@ -43,6 +43,10 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-json", default=None,
                     help="rocprofv3-derived HBM bytes per launch of the dominant kernel")
+    ap.add_argument("--umap-graph-k", type=int, default=15,
+                    help="build the UMAP fuzzy graph of the workload's first k neighbours on the "
+                         "device (timed beside the CPU restatement on a 100k-point slice); 0 "
+                         "skips it")
     ap.add_argument("--quality-sample", type=int, default=5000,
                     help="points of the exact structure-metric sample of the final layout "
                          "(device metric, timed beside the CPU restatement); 0 skips it")
@@ -186,6 +190,46 @@ def bench_served(fe, workloads, n_requests, k=90):
             "transform_p99_ms": round(1e3 * float(np.percentile(lat_t, 99)), 3) if lat_t else None}
 
 
+def bench_umap_graph(fe, knn_i, knn_d, n, kk, cpu):
+    """The UMAP fuzzy graph of the first kk neighbours of the workload's KNN:
+    device build from the device-resident KNN (whole graph), and the CPU
+    restatement (oracle) beside the device build on the first 100k points."""
+    import torch
+
+    from fastembedr_amd import umap
+
+    warm = umap.FuzzyGraph(knn_i, knn_d, device=True, n=n, k=knn_i.shape[0], n_cols=kk,
+                           index_offset=0)
+    warm.close()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g = umap.FuzzyGraph(knn_i, knn_d, device=True, n=n, k=knn_i.shape[0], n_cols=kk,
+                        index_offset=0)
+    dev_s = time.perf_counter() - t0
+    out = {"n": int(n), "k": int(kk), "nnz": g.nnz, "device_ms": round(1e3 * dev_s, 2),
+           "device_points_per_s": round(n / dev_s, 1)}
+    g.close()
+    if cpu:
+        import oracle
+
+        m = min(n, 100_000)
+        hi = knn_i[:kk, :m].T.cpu().numpy()
+        hd = knn_d[:kk, :m].T.cpu().numpy().astype(np.float64)
+        t0 = time.perf_counter()
+        ref = oracle.umap_fuzzy_graph(hi, hd, index_offset=0)
+        cpu_s = time.perf_counter() - t0
+        small = umap.FuzzyGraph(hi, hd, index_offset=0).export()
+        same = (np.array_equal(small["offsets"], ref["offsets"]) and
+                np.array_equal(small["neighbors"], ref["neighbors"]) and
+                float(np.abs(small["weights"] - ref["weights"]).max(initial=0.0)) <= 2.4e-7)
+        out.update({"cpu_sample_points": int(m), "cpu_ms": round(1e3 * cpu_s, 1),
+                    "cpu_points_per_s": round(m / cpu_s, 1), "cpu_cores": 1,
+                    "cpu_kind": "oracle restatement of umap_graph_csr_float_cpp, 1 thread",
+                    "sample_matches_cpu": bool(same)})
+    log(f"[bench] umap graph {out}")
+    return out
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -223,6 +267,10 @@ def main():
                        n=n, k=k, index_offset=0)
     setup_s = time.perf_counter() - t0
     info = plan.info()
+    umap_graph = None
+    if args.umap_graph_k > 0:
+        umap_graph = bench_umap_graph(fe, knn_i, knn_d, n, min(args.umap_graph_k, k),
+                                      not args.no_cpu_baseline)
     del knn_i, knn_d
     torch.cuda.empty_cache()
     log(f"[bench] plan: {info} (P build + init {setup_s:.2f}s)")
@@ -359,6 +407,8 @@ def main():
     }
     if quality:
         out["quality"] = quality
+    if umap_graph:
+        out["umap_graph"] = umap_graph
     out["batched"] = batched
     out["served"] = served
     print(json.dumps(out), flush=True)
