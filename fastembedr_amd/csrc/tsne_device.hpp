@@ -1308,9 +1308,23 @@ __device__ __forceinline__ void join_waves(const WaveEdge* edges, float2* __rest
   attr[r] = make_float2(a.x + sum.x, a.y + sum.y);
 }
 
+// Every load of a thread's share of the window is issued before the first LDS
+// write, so the window arrives in one round trip to memory rather than one
+// per point a thread copies.
+constexpr int kStageLoads = (kWindowPoints + kStagedThreads - 1) / kStagedThreads;
 __device__ __forceinline__ void stage_window(const float2* __restrict__ y, int lo, int m,
                                              float2* sy) {
-  for (int i = threadIdx.x; i < m; i += kStagedThreads) sy[i] = y[lo + i];
+  float2 r[kStageLoads];
+#pragma unroll
+  for (int u = 0; u < kStageLoads; ++u) {
+    const int i = static_cast<int>(threadIdx.x) + u * kStagedThreads;
+    if (i < m) r[u] = y[lo + i];
+  }
+#pragma unroll
+  for (int u = 0; u < kStageLoads; ++u) {
+    const int i = static_cast<int>(threadIdx.x) + u * kStagedThreads;
+    if (i < m) sy[i] = r[u];
+  }
 }
 
 // One group: zero its rows' forces, then the passes in order.  STAGE: the
