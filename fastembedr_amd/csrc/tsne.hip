@@ -299,11 +299,15 @@ int TsnePlan::setup_attraction(hipStream_t s) {
   int h_lnnz = 0;
   FE_CHECK(hipMemcpyAsync(&h_lnnz, lrow + n, sizeof(int), hipMemcpyDeviceToHost, s));
   FE_CHECK(hipStreamSynchronize(s));
-  // groups of about 16 * 8192 padded entries; past one group per compute
-  // unit, a whole number of rounds of them (one workgroup fits a CU)
+  // groups of about 16 * 8192 padded entries; when that gives more groups
+  // than compute units, groups of about twice that (fewer window copies) in
+  // a whole number of rounds of the compute units (one workgroup fits a CU)
   long long gcount = (h_lnnz + 16ll * 8192 - 1) / (16ll * 8192);
   const int cus = compute_units();
-  if (gcount > cus) gcount = (gcount + cus / 2) / cus * cus;
+  if (gcount > cus) {
+    gcount = (h_lnnz + 16ll * 16384 - 1) / (16ll * 16384);
+    gcount = std::max<long long>(cus, (gcount + cus / 2) / cus * cus);
+  }
   gcount = std::max<long long>(gcount, std::min<long long>(32, std::max(1, n / 64)));
   groups = static_cast<int>(std::min<long long>(gcount, 1 << 16));
   // group rows, windows and wave ranges; then the segments, whose size the

@@ -1079,6 +1079,19 @@ struct BlockTables {
   const int2* pass_range;            // per group and pass: its entries [begin, end)
 };
 
+// The segments are read once per launch: their loads carry the non-temporal
+// hint so they do not push the layout windows out of L2.
+__device__ __forceinline__ int4 stream_load(const int4* p) {
+  typedef int v4 __attribute__((ext_vector_type(4)));
+  const v4 v = __builtin_nontemporal_load(reinterpret_cast<const v4*>(p));
+  return make_int4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float4 stream_load(const float4* p) {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  const v4 v = __builtin_nontemporal_load(reinterpret_cast<const v4*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
 struct LaneCols {
   int j[kLaneEntries];
   int row;
@@ -1090,7 +1103,7 @@ template <bool LOCAL>
 __device__ __forceinline__ LaneCols lane_cols(const BlockTables& t, int base, int lane) {
   LaneCols c;
   if constexpr (LOCAL) {
-    const int4 a = *reinterpret_cast<const int4*>(t.col16 + base + kLaneEntries * lane);
+    const int4 a = stream_load(reinterpret_cast<const int4*>(t.col16 + base + kLaneEntries * lane));
     const unsigned int w[4] = {static_cast<unsigned int>(a.x), static_cast<unsigned int>(a.y),
                                static_cast<unsigned int>(a.z), static_cast<unsigned int>(a.w)};
 #pragma unroll
@@ -1100,8 +1113,8 @@ __device__ __forceinline__ LaneCols lane_cols(const BlockTables& t, int base, in
     }
   } else {
     const int* p = t.col + base + kLaneEntries * lane;
-    const int4 a = *reinterpret_cast<const int4*>(p);
-    const int4 b = *reinterpret_cast<const int4*>(p + 4);
+    const int4 a = stream_load(reinterpret_cast<const int4*>(p));
+    const int4 b = stream_load(reinterpret_cast<const int4*>(p + 4));
     c.j[0] = a.x;
     c.j[1] = a.y;
     c.j[2] = a.z;
@@ -1111,7 +1124,7 @@ __device__ __forceinline__ LaneCols lane_cols(const BlockTables& t, int base, in
     c.j[6] = b.z;
     c.j[7] = b.w;
   }
-  c.row = t.lane_row[base / kLaneEntries + lane];
+  c.row = __builtin_nontemporal_load(t.lane_row + base / kLaneEntries + lane);
   c.starts = t.starts[base / kLaneStep];
   return c;
 }
@@ -1146,20 +1159,23 @@ __device__ __forceinline__ void segmented_wave_scan(float& sx, float& sy, int la
 
 // Sources of layout points for the attraction: straight from the layout, or
 // from a window of the layout staged in LDS.  fetch() issues the loads of a
-// point and value() forms it from what they returned; a step issues every
-// fetch before it uses any value, so all its loads are in flight together.
-// Both give the stored point (the caller centres it, fl(y - mean)).
+// point and value() forms the centred point fl(y - mean) from what they
+// returned; a step issues every fetch before it uses any value, so all its
+// loads are in flight together.
 struct GlobalPoints {
   static constexpr bool kLocal = false;
   const float2* __restrict__ y;
   float mx, my;
   using Fetch = float2;
   __device__ __forceinline__ Fetch fetch(int j) const { return y[j]; }
-  __device__ __forceinline__ float2 value(const Fetch& f) const { return f; }
+  __device__ __forceinline__ float2 value(const Fetch& f) const {
+    return make_float2(f.x - mx, f.y - my);
+  }
 };
 
 // A window pass: every column of the pass lies in the window staged in LDS
-// and is stored as its offset from the window's start.
+// (already centred, the same fl(y - mean)) and is stored as its offset from
+// the window's start.
 struct WindowPoints {
   static constexpr bool kLocal = true;
   const float2* __restrict__ y;
@@ -1203,8 +1219,8 @@ __device__ __forceinline__ void attract_range(const SRC& src, const BlockTables&
     const int row = cur.row;
     float v[kLaneEntries];
     {
-      const float4 a = *reinterpret_cast<const float4*>(t.val + eb);
-      const float4 b = *reinterpret_cast<const float4*>(t.val + eb + 4);
+      const float4 a = stream_load(reinterpret_cast<const float4*>(t.val + eb));
+      const float4 b = stream_load(reinterpret_cast<const float4*>(t.val + eb + 4));
       v[0] = a.x;
       v[1] = a.y;
       v[2] = a.z;
@@ -1228,7 +1244,7 @@ __device__ __forceinline__ void attract_range(const SRC& src, const BlockTables&
 #pragma unroll
     for (int u = 0; u < kLaneEntries; ++u) {
       const float2 pj = src.value(fj[u]);
-      const float dx = xi - (pj.x - mx), dy = yi - (pj.y - my);
+      const float dx = xi - pj.x, dy = yi - pj.y;
       const float q = __builtin_amdgcn_rcpf(1.0f + dx * dx + dy * dy);
       const float c = v[u] * q;
       sx += c * dx;
@@ -1308,12 +1324,13 @@ __device__ __forceinline__ void join_waves(const WaveEdge* edges, float2* __rest
   attr[r] = make_float2(a.x + sum.x, a.y + sum.y);
 }
 
-// Every load of a thread's share of the window is issued before the first LDS
-// write, so the window arrives in one round trip to memory rather than one
-// per point a thread copies.
+// The window's points, centred (fl(y - mean), as every reader forms them).
+// Every load of a thread's share is issued before the first LDS write, so
+// the window arrives in one round trip to memory rather than one per point a
+// thread copies.
 constexpr int kStageLoads = (kWindowPoints + kStagedThreads - 1) / kStagedThreads;
 __device__ __forceinline__ void stage_window(const float2* __restrict__ y, int lo, int m,
-                                             float2* sy) {
+                                             float mx, float my, float2* sy) {
   float2 r[kStageLoads];
 #pragma unroll
   for (int u = 0; u < kStageLoads; ++u) {
@@ -1323,7 +1340,7 @@ __device__ __forceinline__ void stage_window(const float2* __restrict__ y, int l
 #pragma unroll
   for (int u = 0; u < kStageLoads; ++u) {
     const int i = static_cast<int>(threadIdx.x) + u * kStagedThreads;
-    if (i < m) sy[i] = r[u];
+    if (i < m) sy[i] = make_float2(r[u].x - mx, r[u].y - my);
   }
 }
 
@@ -1345,7 +1362,7 @@ __device__ __forceinline__ void attract_group(const float2* __restrict__ y, cons
     int2 win = make_int2(0, 0);
     if (staged) {
       win = t.window[g * kMaxWindows + k];
-      stage_window(y, win.x, win.y, sy);
+      stage_window(y, win.x, win.y, mx, my, sy);
     }
     // the window, the zeroed forces and the previous pass's join are done
     __syncthreads();
