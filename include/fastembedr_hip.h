@@ -313,6 +313,25 @@ int fastembedr_hip_exact_structure_metrics(const double* high, int n, int high_c
                                            const double* low, int low_cols,
                                            const int* requested_k, int nk, double* out);
 
+/* ---- exact k-nearest neighbours -----------------------------------------
+ * Replaces the exact route of native_cuda_knn_impl
+ * (src/native_knn_cuda_impl.cpp:520-700: FAISS bfKnn or cuVS brute force
+ * for the k + 1 nearest rows, then finalize_cuvs_knn_kernel,
+ * src/embedding_cuda_kernels.cpp:1428-1480, dropping the row itself) for
+ * the Euclidean metric.  data: column-major n x p float32 (an R matrix);
+ * 1 <= k <= min(n - 1, 256).  indices/distances: column-major n x k (the
+ * GPU-resident list contract of embedding_cuda_impl.cpp:2435-2523:
+ * "column_major_query_by_k"), indices 0- or 1-based by index_base (the
+ * reference's list uses 1), the row itself excluded, ties by the smaller
+ * index, distances Euclidean float32.  The neighbours are exact: squared
+ * distances in FP64 from the float32 inputs, not the float32 expanded form
+ * of the reference's external search.  device_inputs / device_outputs: the
+ * pointers are device memory.  scanned_rows (may be NULL): how many rows the
+ * error bound did not settle and were found by a full FP64 scan. */
+int fastembedr_hip_exact_knn(const float* data, int n, int p, int k, int device_inputs,
+                             int index_base, int* indices, float* distances, int device_outputs,
+                             int* scanned_rows);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,6 +14,9 @@ It wraps ``liboracle.so`` built from:
   (``src/embedding_cuda_kernels.cpp:253-274,364-407,3044-3154``);
 * ``oracle_transform.c`` -- the fixed-reference transform
   (``src/tsne_neighbors.cpp:2124-2373``);
+* ``oracle_knn.c`` -- exact Euclidean k-nearest neighbours by FP64 brute
+  force (what the reference's exact route, ``src/native_knn_cuda_impl.cpp``
+  with ``finalize_cuvs_knn_kernel``, approximates in float32);
 * ``cpu_port.c`` -- the float32, multi-threaded restatement of the reference
   CPU optimiser used as the timed CPU baseline.
 
@@ -90,6 +93,9 @@ def _declare(L):
         _declare_umap_graph(L)
     if hasattr(L, "or_transform"):
         _declare_transform(L)
+    if hasattr(L, "or_exact_knn"):
+        L.or_exact_knn.argtypes = [_fp, c_int, c_int, c_int, _ip, c_int, _ip, _fp]
+        L.or_exact_knn.restype = c_int
 
 
 def _declare_umap(L):
@@ -300,6 +306,27 @@ def structure_metrics(high, low, k):
     if rc != 0:
         raise ValueError("each k must satisfy 1 <= k < (2 * n - 1) / 3")
     return out
+
+
+def exact_knn(data, k, queries=None):
+    """Exact k-NN of the rows of ``data`` (n x p), self excluded.
+
+    Returns (indices, distances), row-major len(queries) x k, 0-based, ties
+    by the smaller index, distances (float32) sqrt of the FP64 squared
+    distance.
+    """
+    x = np.asfortranarray(np.asarray(data, dtype=np.float32))
+    n, p = x.shape
+    q = None if queries is None else np.ascontiguousarray(queries, dtype=np.int32)
+    nq = n if q is None else len(q)
+    idx = np.zeros((nq, k), dtype=np.int32)
+    dist = np.zeros((nq, k), dtype=np.float32)
+    flat = x.ravel(order="F")
+    rc = lib().or_exact_knn(_ptr(flat, _fp), n, p, int(k), None if q is None else _ptr(q, _ip), nq,
+                            _ptr(idx, _ip), _ptr(dist, _fp))
+    if rc != 0:
+        raise ValueError("exact KNN needs n >= 2, p >= 1 and 1 <= k < n")
+    return idx, dist
 
 
 def port_tsne_iterate(row_ptr, col, val, y, iters, exag=12.0, learning_rate=None, momentum=0.8,

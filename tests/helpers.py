@@ -116,6 +116,15 @@ class DeviceArray:
                                    ctypes.c_size_t(a.nbytes), 1) == 0  # host to device
         self.nbytes = a.nbytes
 
+    def to_host(self, dtype, shape):
+        import ctypes
+
+        out = np.zeros(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        assert self._hip.hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), self.ptr,
+                                   ctypes.c_size_t(out.nbytes), 2) == 0  # device to host
+        return out
+
     def free(self):
         if self.ptr:
             self._hip.hipFree(self.ptr)

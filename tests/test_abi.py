@@ -76,3 +76,24 @@ def test_structure_metrics_argument_errors_need_no_device():
             quality.exact_structure_metrics(high, low, ks)
     with pytest.raises(ValueError, match="same row count"):
         quality.exact_structure_metrics(x, np.zeros((9, 2)), [2])
+
+
+def test_exact_knn_argument_errors_need_no_device():
+    # native_knn_cuda_impl's checks (src/native_knn_cuda_impl.cpp:536-543)
+    # run before any device work
+    import numpy as np
+
+    from fastembedr_amd import HipError, knn
+
+    x = np.zeros((10, 3), np.float32)
+    with pytest.raises(HipError, match="at least two rows and one column"):
+        knn.exact_knn(np.zeros((1, 3), np.float32), 1)
+    with pytest.raises(HipError, match="at least two rows and one column"):
+        knn.exact_knn(np.zeros((10, 0), np.float32), 1)
+    for k in (0, 10, 11):
+        with pytest.raises(HipError, match=r"k in \[1, min\(n - 1, 256\)\]"):
+            knn.exact_knn(x, k)
+    with pytest.raises(HipError, match=r"k in \[1, min\(n - 1, 256\)\]"):
+        knn.exact_knn(np.zeros((300, 2), np.float32), 257)
+    with pytest.raises(HipError, match="index_base must be 0 or 1"):
+        knn.exact_knn(x, 3, index_base=2)
