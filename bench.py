@@ -4,7 +4,9 @@
 Workload (the single-job configuration): 1,000,000 points from a 100-cluster
 Gaussian mixture in 50 dimensions (centres N(0, 5^2 I), within N(0, I),
 cluster sizes Multinomial(n, Dirichlet(1)), seed 201); exact k = 90 KNN
-(computed on the device as an untimed fixture); perplexity 30; 2D; FFT grid
+(computed by the backend's exact KNN producer, timed and reported apart from
+the iterations, with an FP64 brute-force check of a sample of rows);
+perplexity 30; 2D; FFT grid
 1024 x 1024 (the backend's rule at this size); schedule 250 early-exaggeration
 iterations (12) then 500 normal ones, learning rate n/12, momentum 0.8.
 
@@ -47,6 +49,9 @@ def parse():
                     help="build the UMAP fuzzy graph of the workload's first k neighbours on the "
                          "device (timed beside the CPU restatement on a 100k-point slice); 0 "
                          "skips it")
+    ap.add_argument("--knn-check-rows", type=int, default=32,
+                    help="query rows of the workload's exact KNN recomputed by the FP64 brute "
+                         "force on the CPU (timed, compared); 0 skips it")
     ap.add_argument("--quality-sample", type=int, default=5000,
                     help="points of the exact structure-metric sample of the final layout "
                          "(device metric, timed beside the CPU restatement); 0 skips it")
@@ -190,6 +195,54 @@ def bench_served(fe, workloads, n_requests, k=90):
             "transform_p99_ms": round(1e3 * float(np.percentile(lat_t, 99)), 3) if lat_t else None}
 
 
+MFMA_F32_PEAK_TFLOPS = 157.3
+
+
+def bench_knn(x, k, cpu, check_rows):
+    """Exact KNN of the workload on the device (the producer feeding the
+    t-SNE plan), with the FP64 brute force (oracle) timed on a sample of
+    query rows beside it and compared with the device rows."""
+    import torch
+
+    from fastembedr_amd import knn
+
+    n, p = x.shape
+    xs = np.ascontiguousarray(x[:2048].T, dtype=np.float32)
+    wi = torch.empty((k, 2048), dtype=torch.int32, device="cuda")
+    wd = torch.empty((k, 2048), dtype=torch.float32, device="cuda")
+    knn.exact_knn_device(torch.from_numpy(xs).cuda(), 2048, p, k, wi, wd)  # module load, LDS attributes
+    xd = torch.from_numpy(np.ascontiguousarray(x.T, dtype=np.float32)).cuda()
+    ki = torch.empty((k, n), dtype=torch.int32, device="cuda")
+    kd = torch.empty((k, n), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    scanned = knn.exact_knn_device(xd, n, p, k, ki, kd)
+    sec = time.perf_counter() - t0
+    del xd
+    padded = (p + 7) // 8 * 8
+    tflops = 2.0 * n * n * padded / sec / 1e12
+    out = {"n": int(n), "p": int(p), "k": int(k), "seconds": round(sec, 4),
+           "points_per_s": round(n / sec, 1), "scanned_rows": int(scanned),
+           "mfma": {"bound": "mfma", "achieved": round(tflops, 2), "peak": MFMA_F32_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(tflops / MFMA_F32_PEAK_TFLOPS, 4),
+                    "flops": f"2 n^2 P, P = {padded} (columns padded to a multiple of 8)"}}
+    if cpu and check_rows > 0:
+        import oracle
+
+        rows = np.linspace(0, n - 1, check_rows).astype(np.int32)
+        t0 = time.perf_counter()
+        oi, od = oracle.exact_knn(x.astype(np.float32), k, queries=rows)
+        cpu_s = time.perf_counter() - t0
+        gi = ki[:, torch.from_numpy(rows.astype(np.int64)).cuda()].T.cpu().numpy()
+        gd = kd[:, torch.from_numpy(rows.astype(np.int64)).cuda()].T.cpu().numpy()
+        out.update({"cpu_sample_rows": int(check_rows), "cpu_s": round(cpu_s, 2),
+                    "cpu_points_per_s": round(check_rows / cpu_s, 2), "cpu_cores": 1,
+                    "cpu_kind": "oracle FP64 brute force, 1 thread",
+                    "sample_matches_cpu": bool(np.array_equal(gi, oi) and np.array_equal(gd, od))})
+    log(f"[bench] exact knn {out}")
+    return ki, kd, out
+
+
 def bench_umap_graph(fe, knn_i, knn_d, n, kk, cpu):
     """The UMAP fuzzy graph of the first kk neighbours of the workload's KNN:
     device build from the device-resident KNN (whole graph), and the CPU
@@ -257,10 +310,7 @@ def main():
     t0 = time.perf_counter()
     x, labels, centres = workloads.gaussian_mixture(n, 50, 100, seed=201, dirichlet=True)
     log(f"[bench] data {x.shape} in {time.perf_counter() - t0:.1f}s")
-    t0 = time.perf_counter()
-    knn_i, knn_d = workloads.mixture_knn_device(x, labels, centres, k)
-    torch.cuda.synchronize()
-    log(f"[bench] device KNN k={k} in {time.perf_counter() - t0:.1f}s")
+    knn_i, knn_d, knn_report = bench_knn(x, k, not args.no_cpu_baseline, args.knn_check_rows)
     y0 = workloads.centred_normal_init(n, seed=202)
     t0 = time.perf_counter()
     plan = fe.TsnePlan(knn_i, knn_d, perplexity=30.0, grid_size=0, Y_init=y0, device=True,
@@ -405,6 +455,7 @@ def main():
         "layout_finite": finite,
         "cpu_baseline": cpu,
     }
+    out["knn"] = knn_report
     if quality:
         out["quality"] = quality
     if umap_graph:

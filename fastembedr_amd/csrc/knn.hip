@@ -11,7 +11,7 @@
 //
 //   knn_col_partials / knn_col_fold   column means (FP64, fixed order)
 //   knn_pack          row-major copies: centred float32 rows padded to a
-//                     multiple of 4 columns (the MFMA operands) with their
+//                     multiple of 8 columns (the MFMA operands) with their
 //                     float32 norms, the original rows, and FP64 norms
 //   knn_select        MFMA 16x16x4 f32 tiles of q . c over all candidates;
 //                     key fl(|c|^2 - 2 q . c) with the index in the low bits;
@@ -203,17 +203,21 @@ __global__ __launch_bounds__(kKnnThreads) void knn_norm_fold(const double* __res
 template <int KL>
 struct SelectShape {
   static constexpr int kWaves = KL <= 128 ? 4 : (KL == 256 ? 2 : 1);
+  static constexpr int kThreads = 64 * kWaves;
   static constexpr int kQueries = kKnnQ * kWaves;
   static constexpr int kSort = pow2_at_least(KL + kKnnBuf) / 64;  // keys per lane in a merge
-  // LDS: query chunk, candidate chunk, candidate norms, lists, pending keys, counts
-  static constexpr size_t kQueryBytes = sizeof(float) * kKnnChunk * kQueries;
-  static constexpr size_t kCandBytes = sizeof(float) * kKnnChunk * kKnnTile;
-  static constexpr size_t kNormBytes = sizeof(float) * kKnnTile;
+  // float4 loads per thread for one candidate chunk (64 rows x 64 columns)
+  static constexpr int kFetch = (kKnnTile * kKnnChunk / 4 + kThreads - 1) / kThreads;
+  // LDS: lists, pending keys, two candidate chunks, a query chunk, two norm
+  // tiles, counts
   static constexpr size_t kListBytes = sizeof(unsigned long long) * KL * kQueries;
   static constexpr size_t kBufBytes = sizeof(unsigned long long) * kKnnBuf * kQueries;
+  static constexpr size_t kCandBytes = 2 * sizeof(float) * kKnnChunk * kKnnTile;
+  static constexpr size_t kQueryBytes = sizeof(float) * kKnnChunk * kQueries;
+  static constexpr size_t kNormBytes = 2 * sizeof(float) * kKnnTile;
   static constexpr size_t kCountBytes = sizeof(int) * kQueries;
   static constexpr size_t kLds =
-      kQueryBytes + kCandBytes + kNormBytes + kListBytes + kBufBytes + kCountBytes;
+      kListBytes + kBufBytes + kCandBytes + kQueryBytes + kNormBytes + kCountBytes;
 };
 
 // Merge the pending keys of query slot qs into its sorted list (whole wave).
@@ -237,112 +241,204 @@ __device__ __forceinline__ void merge_pending(unsigned long long* list, unsigned
   if (lane == 0) cnt[qs] = 0;
 }
 
+// The keys of one MFMA block (candidates t0 + 16 b + 4 (lane / 16) + r)
+// below the query's current last key go to its pending buffer.  Candidates
+// come in increasing index order, so a key is below the last key exactly when
+// its value is below the last value (tau_v; -inf for a padding query slot);
+// padding candidates have norm +inf and never pass.
+__device__ __forceinline__ void offer_keys(const v4f& acc, int b, int t0, int lane,
+                                           const float* snrm, float tau_v, int* count,
+                                           unsigned long long* pending) {
+  const int cl = 16 * b + 4 * (lane >> 4);
+  const v4f nr = *reinterpret_cast<const v4f*>(snrm + cl);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const float v = __fsub_rn(nr[r], __fmul_rn(2.0f, acc[r]));
+    if (v < tau_v) {
+      const unsigned long long key = (static_cast<unsigned long long>(order_bits(v)) << 32) |
+                                     static_cast<unsigned int>(t0 + cl + r);
+      pending[atomicAdd(count, 1)] = key;
+    }
+  }
+}
+
+// the value part of a list's last key (+inf while the list is not full;
+// -inf for a padding query slot, which takes no keys)
+__device__ __forceinline__ float last_value(unsigned long long key, bool qvalid) {
+  if (!qvalid) return -__int_as_float(0x7f800000);
+  return key == kNoKey ? __int_as_float(0x7f800000)
+                       : order_float(static_cast<unsigned int>(key >> 32));
+}
+
+// Merge every query of wave w with more than `limit` pending keys; returns
+// the lane's query's last list key (tau unchanged when nothing merged).
+template <int KL, int R>
+__device__ __forceinline__ unsigned long long merge_full(unsigned long long* list,
+                                                         unsigned long long* buf, int* cnt, int w,
+                                                         int lane, int qs, int limit,
+                                                         unsigned long long tau) {
+  const bool need = lane < kKnnQ && cnt[w * kKnnQ + lane] > limit;
+  unsigned long long m = __ballot(need);
+  if (!m) return tau;
+  while (m) {
+    const int j = __builtin_ctzll(m);
+    m &= m - 1;
+    merge_pending<KL, R>(list, buf, cnt, w * kKnnQ + j);
+  }
+  return list[qs * KL + KL - 1];
+}
+
+// One step of the walk = (candidate tile, column chunk).  While the MFMAs
+// of a step run, the next step's candidate chunk (and, past 64 columns, the
+// next query chunk) is already on its way into registers; it goes to the
+// other LDS buffer afterwards, so one barrier per step suffices.
+template <int KL>
+__device__ __forceinline__ void fetch_chunk(const float* __restrict__ xc,
+                                            const float* __restrict__ nrm, int P, int nchunks,
+                                            int step, int tid,
+                                            v4f (&f)[SelectShape<KL>::kFetch], float& fn) {
+  using S = SelectShape<KL>;
+  const int t0 = (step / nchunks) * kKnnTile, c0 = (step % nchunks) * kKnnChunk;
+  const int kq = min(kKnnChunk, P - c0) / 4;
+  // slot e: candidate e / 16, float4 e % 16 of the chunk
+#pragma unroll
+  for (int u = 0; u < S::kFetch; ++u) {
+    const int e = tid + u * S::kThreads;
+    const int c = e >> 4, q = e & 15;
+    if (q < kq)
+      f[u] = *reinterpret_cast<const v4f*>(xc + static_cast<size_t>(t0 + c) * P + c0 + 4 * q);
+  }
+  if (c0 == 0 && tid < kKnnTile) fn = nrm[t0 + tid];
+}
+
+template <int KL>
+__device__ __forceinline__ void store_chunk(int P, int nchunks, int step, int tid,
+                                            const v4f (&f)[SelectShape<KL>::kFetch], float fn,
+                                            float* sc, float* snrm) {
+  using S = SelectShape<KL>;
+  const int c0 = (step % nchunks) * kKnnChunk;
+  const int kq = min(kKnnChunk, P - c0) / 4;
+#pragma unroll
+  for (int u = 0; u < S::kFetch; ++u) {
+    const int e = tid + u * S::kThreads;
+    const int c = e >> 4, q = e & 15;
+    if (q < kq) *reinterpret_cast<v4f*>(sc + (q * kKnnTile + c) * 4) = f[u];
+  }
+  if (c0 == 0 && tid < kKnnTile) snrm[tid] = fn;
+}
+
+// The workgroup's query rows, columns c0 .. c0 + 63, into LDS as
+// [k-step][query][4] (the MFMA's B operand layout).
+template <int KL>
+__device__ __forceinline__ void stage_queries(const float* __restrict__ xc, int P, int c0,
+                                              const int* __restrict__ qlist, int nq, int tid,
+                                              float* sq) {
+  using S = SelectShape<KL>;
+  const int kq = min(kKnnChunk, P - c0) / 4;
+  for (int e = tid; e < S::kQueries * kq; e += S::kThreads) {
+    const int q = e / kq, s = e - q * kq;
+    const int g = blockIdx.x * S::kQueries + q;
+    const int row = g < nq ? (qlist ? qlist[g] : g) : 0;
+    *reinterpret_cast<v4f*>(sq + (s * S::kQueries + q) * 4) =
+        *reinterpret_cast<const v4f*>(xc + static_cast<size_t>(row) * P + c0 + 4 * s);
+  }
+}
+
 // Stage one: each wave owns kKnnQ queries; the workgroup walks every
 // candidate tile; out: nq lists of KL keys (ascending).
 template <int KL>
-__global__ __launch_bounds__(64 * SelectShape<KL>::kWaves) void knn_select(
+__global__ __launch_bounds__(SelectShape<KL>::kThreads) void knn_select(
     const float* __restrict__ xc, const float* __restrict__ nrm, int n, int n_pad, int P,
     const int* __restrict__ qlist, int nq, unsigned long long* __restrict__ out) {
   using S = SelectShape<KL>;
   constexpr int NQ = S::kQueries;
   constexpr int R = S::kSort;
   extern __shared__ unsigned long long knn_smem[];
-  unsigned char* base = reinterpret_cast<unsigned char*>(knn_smem);
-  unsigned long long* list = reinterpret_cast<unsigned long long*>(base);
+  // the MFMA operands first (small LDS offsets), then the lists
+  float* sc0 = reinterpret_cast<float*>(knn_smem);          // [step][candidate][4], x2
+  float* sq = sc0 + 2 * kKnnChunk * kKnnTile;               // [step][query][4]
+  float* snrm0 = sq + kKnnChunk * NQ;                        // x2
+  unsigned long long* list = reinterpret_cast<unsigned long long*>(snrm0 + 2 * kKnnTile);
   unsigned long long* buf = list + KL * NQ;
-  float* sq = reinterpret_cast<float*>(buf + kKnnBuf * NQ);  // [step][query][4]
-  float* sc = sq + kKnnChunk * NQ;                           // [step][candidate][4]
-  float* snrm = sc + kKnnChunk * kKnnTile;
-  int* cnt = reinterpret_cast<int*>(snrm + kKnnTile);
+  int* cnt = reinterpret_cast<int*>(buf + kKnnBuf * NQ);
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int nthreads = 64 * S::kWaves;
   const int qs = w * kKnnQ + (lane & 15);  // this lane's query slot
   const int gq = blockIdx.x * NQ + qs;
   const bool qvalid = gq < nq;
 
-  for (int s = tid; s < KL * NQ; s += nthreads) list[s] = kNoKey;
-  for (int s = tid; s < NQ; s += nthreads) cnt[s] = 0;
+  for (int s = tid; s < KL * NQ; s += S::kThreads) list[s] = kNoKey;
+  for (int s = tid; s < NQ; s += S::kThreads) cnt[s] = 0;
   unsigned long long tau = kNoKey;
-  const bool one_chunk = P <= kKnnChunk;
+  float tau_v = qvalid ? __int_as_float(0x7f800000) : -__int_as_float(0x7f800000);
 
-  for (int t0 = 0; t0 < n_pad; t0 += kKnnTile) {
-    v4f acc[4];
-#pragma unroll
-    for (int b = 0; b < 4; ++b) acc[b] = v4f{0.f, 0.f, 0.f, 0.f};
-    for (int c0 = 0; c0 < P; c0 += kKnnChunk) {
-      const int kc = min(kKnnChunk, P - c0);
-      const int kq = kc / 4;
-      __syncthreads();  // the previous chunk's readers are done
-      if (!one_chunk || t0 == 0) {
-        for (int f = tid; f < NQ * kq; f += nthreads) {
-          const int q = f / kq, s = f - q * kq;
-          const int g = blockIdx.x * NQ + q;
-          const int row = g < nq ? (qlist ? qlist[g] : g) : 0;
-          const float4 v =
-              *reinterpret_cast<const float4*>(xc + static_cast<size_t>(row) * P + c0 + 4 * s);
-          *reinterpret_cast<float4*>(sq + (s * NQ + q) * 4) = v;
-        }
-      }
-      for (int f = tid; f < kKnnTile * kq; f += nthreads) {
-        const int c = f / kq, s = f - c * kq;
-        const float4 v =
-            *reinterpret_cast<const float4*>(xc + static_cast<size_t>(t0 + c) * P + c0 + 4 * s);
-        *reinterpret_cast<float4*>(sc + (s * kKnnTile + c) * 4) = v;
-      }
-      if (c0 == 0)
-        for (int c = tid; c < kKnnTile; c += nthreads) snrm[c] = nrm[t0 + c];
-      __syncthreads();
-      for (int s = 0; s < kq; ++s) {
-        const float bq = sq[(s * NQ + w * kKnnQ + (lane & 15)) * 4 + (lane >> 4)];
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          const float ac = sc[(s * kKnnTile + 16 * b + (lane & 15)) * 4 + (lane >> 4)];
-          acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac, bq, acc[b], 0, 0, 0);
-        }
-      }
+  const int nchunks = (P + kKnnChunk - 1) / kKnnChunk;
+  const int nsteps = (n_pad / kKnnTile) * nchunks;
+  v4f pf[S::kFetch];
+  float pfn = 0.0f;
+  stage_queries<KL>(xc, P, 0, qlist, nq, tid, sq);
+  fetch_chunk<KL>(xc, nrm, P, nchunks, 0, tid, pf, pfn);
+  store_chunk<KL>(P, nchunks, 0, tid, pf, pfn, sc0, snrm0);
+  __syncthreads();
+
+  v4f acc[4];
+  for (int step = 0; step < nsteps; ++step) {
+    const int tile = step / nchunks, chunk = step - tile * nchunks;
+    const int t0 = tile * kKnnTile;
+    const float* sc = sc0 + (step & 1) * (kKnnChunk * kKnnTile);
+    const float* snrm = snrm0 + (tile & 1) * kKnnTile;
+    const bool more = step + 1 < nsteps;
+    if (more) {
+      fetch_chunk<KL>(xc, nrm, P, nchunks, step + 1, tid, pf, pfn);
     }
-    // lane: query slot qs, candidates t0 + 16 b + 4 (lane / 16) + r
+    if (chunk == 0) {
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
+      for (int b = 0; b < 4; ++b) acc[b] = v4f{0.f, 0.f, 0.f, 0.f};
+    }
+    // P is a multiple of 8: the chunk has an even number of k-steps, taken
+    // two at a time (no branch between the MFMAs of a chunk)
+    const int kq = min(kKnnChunk, P - chunk * kKnnChunk) / 4;
+    const float* qb = sq + (w * kKnnQ + (lane & 15)) * 4 + (lane >> 4);
+    const float* cb = sc + (lane & 15) * 4 + (lane >> 4);
+    for (int s = 0; s < kq; s += 2) {
+      const float b0 = qb[s * NQ * 4], b1 = qb[(s + 1) * NQ * 4];
+      float a0[4], a1[4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int cl = 16 * b + 4 * (lane >> 4) + r;
-        const int c = t0 + cl;
-        if (qvalid && c < n) {
-          const float v = __fsub_rn(snrm[cl], __fmul_rn(2.0f, acc[b][r]));
-          const unsigned long long key =
-              (static_cast<unsigned long long>(order_bits(v)) << 32) | static_cast<unsigned int>(c);
-          if (key < tau) {
-            const int pos = atomicAdd(&cnt[qs], 1);
-            buf[qs * kKnnBuf + pos] = key;
-          }
-        }
+      for (int b = 0; b < 4; ++b) {
+        a0[b] = cb[(s * kKnnTile + 16 * b) * 4];
+        a1[b] = cb[((s + 1) * kKnnTile + 16 * b) * 4];
       }
-      // a query with more than kKnnBuf - 16 pending keys is merged before
-      // the next block can add 16 more
-      const bool need = lane < kKnnQ && cnt[w * kKnnQ + lane] > kKnnBuf - 16;
-      unsigned long long m = __ballot(need);
-      if (m) {
-        while (m) {
-          const int j = __builtin_ctzll(m);
-          m &= m - 1;
-          merge_pending<KL, R>(list, buf, cnt, w * kKnnQ + j);
-        }
-        tau = list[qs * KL + KL - 1];
-      }
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[b], b0, acc[b], 0, 0, 0);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[b], b1, acc[b], 0, 0, 0);
+    }
+    if (chunk == nchunks - 1) {
+      // lane: query slot qs, candidates t0 + 16 b + 4 (lane / 16) + r; two
+      // blocks add at most 32 keys to a query, so a query with more than
+      // kKnnBuf - 32 pending keys is merged after each pair of blocks
+      offer_keys(acc[0], 0, t0, lane, snrm, tau_v, cnt + qs, buf + qs * kKnnBuf);
+      offer_keys(acc[1], 1, t0, lane, snrm, tau_v, cnt + qs, buf + qs * kKnnBuf);
+      tau = merge_full<KL, R>(list, buf, cnt, w, lane, qs, kKnnBuf - 32, tau);
+      tau_v = last_value(tau, qvalid);
+      offer_keys(acc[2], 2, t0, lane, snrm, tau_v, cnt + qs, buf + qs * kKnnBuf);
+      offer_keys(acc[3], 3, t0, lane, snrm, tau_v, cnt + qs, buf + qs * kKnnBuf);
+      tau = merge_full<KL, R>(list, buf, cnt, w, lane, qs, kKnnBuf - 32, tau);
+      tau_v = last_value(tau, qvalid);
+    }
+    if (more)
+      store_chunk<KL>(P, nchunks, step + 1, tid, pf, pfn,
+                      sc0 + ((step + 1) & 1) * (kKnnChunk * kKnnTile),
+                      snrm0 + (((step + 1) / nchunks) & 1) * kKnnTile);
+    __syncthreads();
+    if (more && nchunks > 1) {
+      // past 64 columns the query chunk changes with the step
+      stage_queries<KL>(xc, P, ((step + 1) % nchunks) * kKnnChunk, qlist, nq, tid, sq);
+      __syncthreads();
     }
   }
   // the rest of the pending keys
-  {
-    const bool need = lane < kKnnQ && cnt[w * kKnnQ + lane] > 0;
-    unsigned long long m = __ballot(need);
-    while (m) {
-      const int j = __builtin_ctzll(m);
-      m &= m - 1;
-      merge_pending<KL, R>(list, buf, cnt, w * kKnnQ + j);
-    }
-  }
+  tau = merge_full<KL, R>(list, buf, cnt, w, lane, qs, 0, tau);
   for (int j = 0; j < kKnnQ; ++j) {
     const int g = blockIdx.x * NQ + w * kKnnQ + j;
     if (g >= nq) break;
@@ -583,7 +679,7 @@ int run_select(const float* xc, const float* nrm, int n, int n_pad, int P, int n
     attr_set = true;
   }
   const int blocks = (nq + S::kQueries - 1) / S::kQueries;
-  knn_select<KL><<<blocks, 64 * S::kWaves, S::kLds, s>>>(xc, nrm, n, n_pad, P, nullptr, nq, lists);
+  knn_select<KL><<<blocks, S::kThreads, S::kLds, s>>>(xc, nrm, n, n_pad, P, nullptr, nq, lists);
   FE_LAUNCH_CHECK("knn_select");
   return 0;
 }
@@ -625,7 +721,7 @@ int fastembedr_hip_exact_knn(const float* data, int n, int p, int k, int device_
   if (p > 65535) return fail("exact KNN supports at most 65535 columns");
   if (static_cast<long long>(n) * std::max(p, k) > (1ll << 31) - 1)
     return fail("exact KNN supports at most 2^31 - 1 elements per array");
-  const int P = (p + 3) / 4 * 4;
+  const int P = (p + 7) / 8 * 8;
   const int n_pad = (n + kKnnTile - 1) / kKnnTile * kKnnTile;
   const int KL = list_length(k);
   const bool force_scan = [] {
