@@ -200,45 +200,75 @@ __global__ __launch_bounds__(kKnnThreads) void knn_norm_fold(const double* __res
 
 // ---- candidate lists ----
 
-template <int KL>
-struct SelectShape {
-  static constexpr int kWaves = KL <= 128 ? 4 : (KL == 256 ? 2 : 1);
-  static constexpr int kThreads = 64 * kWaves;
-  static constexpr int kQueries = kKnnQ * kWaves;
-  static constexpr int kSort = pow2_at_least(KL + kKnnBuf) / 64;  // keys per lane in a merge
-  // float4 loads per thread for one candidate chunk (64 rows x 64 columns)
-  static constexpr int kFetch = (kKnnTile * kKnnChunk / 4 + kThreads - 1) / kThreads;
-  // LDS: lists, pending keys, two candidate chunks, a query chunk, two norm
-  // tiles, counts
-  static constexpr size_t kListBytes = sizeof(unsigned long long) * KL * kQueries;
-  static constexpr size_t kBufBytes = sizeof(unsigned long long) * kKnnBuf * kQueries;
-  static constexpr size_t kCandBytes = 2 * sizeof(float) * kKnnChunk * kKnnTile;
-  static constexpr size_t kQueryBytes = sizeof(float) * kKnnChunk * kQueries;
-  static constexpr size_t kNormBytes = 2 * sizeof(float) * kKnnTile;
-  static constexpr size_t kCountBytes = sizeof(int) * kQueries;
-  static constexpr size_t kLds =
-      kListBytes + kBufBytes + kCandBytes + kQueryBytes + kNormBytes + kCountBytes;
-};
+constexpr int kSelWaves = 8;                       // two waves per SIMD
+constexpr int kSelThreads = 64 * kSelWaves;
+constexpr int kSelQueries = kKnnQ * kSelWaves;     // queries per workgroup
+// float4 loads per thread for one candidate chunk (64 rows x 64 columns)
+constexpr int kSelFetch = kKnnTile * kKnnChunk / 4 / kSelThreads;
+// LDS: two candidate chunks, the query chunk, two norm tiles, the pending
+// keys and their counts (the lists themselves live in the output array)
+constexpr size_t kSelCand = kKnnChunk * kKnnTile;  // floats per candidate chunk
+constexpr size_t kSelLds = sizeof(float) * (2 * kSelCand + kKnnChunk * kSelQueries + 2 * kKnnTile) +
+                           sizeof(unsigned long long) * kKnnBuf * kSelQueries +
+                           sizeof(int) * kSelQueries;
+static_assert(kSelFetch * kSelThreads == kKnnTile * kKnnChunk / 4, "whole chunk per fetch");
 
-// Merge the pending keys of query slot qs into its sorted list (whole wave).
-template <int KL, int R>
-__device__ __forceinline__ void merge_pending(unsigned long long* list, unsigned long long* buf,
-                                              int* cnt, int qs) {
+__device__ __forceinline__ unsigned long long shfl_u64(unsigned long long v, int src) {
+  const int lo = __shfl(static_cast<int>(static_cast<unsigned int>(v)), src, 64);
+  const int hi = __shfl(static_cast<int>(static_cast<unsigned int>(v >> 32)), src, 64);
+  return (static_cast<unsigned long long>(static_cast<unsigned int>(hi)) << 32) |
+         static_cast<unsigned int>(lo);
+}
+
+// Merge the pending keys of one query into its sorted list (whole wave):
+// the (at most 64) pending keys are sorted one per lane; the list (KL
+// ascending keys, KL / 64 per lane, element lane * KL / 64 + r) against the
+// pending keys in reverse order, element by element, keeps the smaller of
+// each pair -- the KL smallest keys of both, as a bitonic sequence -- which
+// a bitonic merge puts in order.  Every lane reads and writes only its own
+// list elements.  Returns the list's new last key (every lane).
+template <int KL>
+__device__ __forceinline__ unsigned long long merge_pending(unsigned long long* __restrict__ list,
+                                                            const unsigned long long* pend,
+                                                            int m) {
+  constexpr int RL = KL / 64;
   const int lane = threadIdx.x & 63;
-  const int m = cnt[qs];
-  unsigned long long e[R];
+  unsigned long long pk[1] = {lane < m ? pend[lane] : kNoKey};
+  wave_sort<1>(pk);
+  unsigned long long e[RL];
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int s = lane * R + r;
-    e[r] = s < KL ? list[qs * KL + s] : (s - KL < m ? buf[qs * kKnnBuf + (s - KL)] : kNoKey);
+  for (int r = 0; r < RL; ++r) {
+    const int i = lane * RL + r;
+    const int j = KL - 1 - i;  // the pending key paired with list[i]
+    const unsigned long long o = shfl_u64(pk[0], j < 64 ? j : 0);
+    const unsigned long long l = list[i];
+    e[r] = (j < 64 && o < l) ? o : l;
   }
-  wave_sort<R>(e);
+  constexpr int kStages = __builtin_ctz(KL);
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int s = lane * R + r;
-    if (s < KL) list[qs * KL + s] = e[r];
+  for (int t = 0; t < kStages; ++t) {
+    const int h = (KL / 2) >> t;
+    if (h >= RL) {
+#pragma unroll
+      for (int r = 0; r < RL; ++r) {
+        const unsigned long long o = shfl_xor_u64(e[r], h / RL);
+        const bool low = ((lane * RL + r) & h) == 0;
+        e[r] = low ? (o < e[r] ? o : e[r]) : (o > e[r] ? o : e[r]);
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < RL; ++r) {
+        if ((r & h) == 0 && e[r] > e[r | h]) {
+          const unsigned long long t2 = e[r];
+          e[r] = e[r | h];
+          e[r | h] = t2;
+        }
+      }
+    }
   }
-  if (lane == 0) cnt[qs] = 0;
+#pragma unroll
+  for (int r = 0; r < RL; ++r) list[lane * RL + r] = e[r];
+  return shfl_u64(e[RL - 1], 63);
 }
 
 // The keys of one MFMA block (candidates t0 + 16 b + 4 (lane / 16) + r)
@@ -270,40 +300,37 @@ __device__ __forceinline__ float last_value(unsigned long long key, bool qvalid)
                        : order_float(static_cast<unsigned int>(key >> 32));
 }
 
-// Merge every query of wave w with more than `limit` pending keys; returns
-// the lane's query's last list key (tau unchanged when nothing merged).
-template <int KL, int R>
-__device__ __forceinline__ unsigned long long merge_full(unsigned long long* list,
-                                                         unsigned long long* buf, int* cnt, int w,
-                                                         int lane, int qs, int limit,
-                                                         unsigned long long tau) {
+// Merge every query of wave w with more than `limit` pending keys; the lane's
+// tau_v follows its query's list.
+template <int KL>
+__device__ __forceinline__ void merge_full(unsigned long long* __restrict__ lists,
+                                           const unsigned long long* buf, int* cnt, int w,
+                                           int lane, int gbase, int limit, bool qvalid,
+                                           float& tau_v) {
   const bool need = lane < kKnnQ && cnt[w * kKnnQ + lane] > limit;
   unsigned long long m = __ballot(need);
-  if (!m) return tau;
   while (m) {
     const int j = __builtin_ctzll(m);
     m &= m - 1;
-    merge_pending<KL, R>(list, buf, cnt, w * kKnnQ + j);
+    const int qs = w * kKnnQ + j;
+    const unsigned long long last = merge_pending<KL>(
+        lists + static_cast<size_t>(gbase + j) * KL, buf + qs * kKnnBuf, cnt[qs]);
+    if (lane == 0) cnt[qs] = 0;
+    if ((lane & 15) == j) tau_v = last_value(last, qvalid);
   }
-  return list[qs * KL + KL - 1];
 }
 
 // One step of the walk = (candidate tile, column chunk).  While the MFMAs
-// of a step run, the next step's candidate chunk (and, past 64 columns, the
-// next query chunk) is already on its way into registers; it goes to the
-// other LDS buffer afterwards, so one barrier per step suffices.
-template <int KL>
+// of a step run, the next step's candidate chunk is already on its way into
+// registers; it goes to the other LDS buffer afterwards, so one barrier per
+// step suffices.  Slot e of a chunk: candidate e / 16, float4 e % 16.
 __device__ __forceinline__ void fetch_chunk(const float* __restrict__ xc,
-                                            const float* __restrict__ nrm, int P, int nchunks,
-                                            int step, int tid,
-                                            v4f (&f)[SelectShape<KL>::kFetch], float& fn) {
-  using S = SelectShape<KL>;
-  const int t0 = (step / nchunks) * kKnnTile, c0 = (step % nchunks) * kKnnChunk;
+                                            const float* __restrict__ nrm, int P, int t0, int c0,
+                                            int tid, v4f (&f)[kSelFetch], float& fn) {
   const int kq = min(kKnnChunk, P - c0) / 4;
-  // slot e: candidate e / 16, float4 e % 16 of the chunk
 #pragma unroll
-  for (int u = 0; u < S::kFetch; ++u) {
-    const int e = tid + u * S::kThreads;
+  for (int u = 0; u < kSelFetch; ++u) {
+    const int e = tid + u * kSelThreads;
     const int c = e >> 4, q = e & 15;
     if (q < kq)
       f[u] = *reinterpret_cast<const v4f*>(xc + static_cast<size_t>(t0 + c) * P + c0 + 4 * q);
@@ -311,16 +338,12 @@ __device__ __forceinline__ void fetch_chunk(const float* __restrict__ xc,
   if (c0 == 0 && tid < kKnnTile) fn = nrm[t0 + tid];
 }
 
-template <int KL>
-__device__ __forceinline__ void store_chunk(int P, int nchunks, int step, int tid,
-                                            const v4f (&f)[SelectShape<KL>::kFetch], float fn,
-                                            float* sc, float* snrm) {
-  using S = SelectShape<KL>;
-  const int c0 = (step % nchunks) * kKnnChunk;
+__device__ __forceinline__ void store_chunk(int P, int c0, int tid, const v4f (&f)[kSelFetch],
+                                            float fn, float* sc, float* snrm) {
   const int kq = min(kKnnChunk, P - c0) / 4;
 #pragma unroll
-  for (int u = 0; u < S::kFetch; ++u) {
-    const int e = tid + u * S::kThreads;
+  for (int u = 0; u < kSelFetch; ++u) {
+    const int e = tid + u * kSelThreads;
     const int c = e >> 4, q = e & 15;
     if (q < kq) *reinterpret_cast<v4f*>(sc + (q * kKnnTile + c) * 4) = f[u];
   }
@@ -329,79 +352,79 @@ __device__ __forceinline__ void store_chunk(int P, int nchunks, int step, int ti
 
 // The workgroup's query rows, columns c0 .. c0 + 63, into LDS as
 // [k-step][query][4] (the MFMA's B operand layout).
-template <int KL>
 __device__ __forceinline__ void stage_queries(const float* __restrict__ xc, int P, int c0,
                                               const int* __restrict__ qlist, int nq, int tid,
                                               float* sq) {
-  using S = SelectShape<KL>;
   const int kq = min(kKnnChunk, P - c0) / 4;
-  for (int e = tid; e < S::kQueries * kq; e += S::kThreads) {
+  for (int e = tid; e < kSelQueries * kq; e += kSelThreads) {
     const int q = e / kq, s = e - q * kq;
-    const int g = blockIdx.x * S::kQueries + q;
+    const int g = blockIdx.x * kSelQueries + q;
     const int row = g < nq ? (qlist ? qlist[g] : g) : 0;
-    *reinterpret_cast<v4f*>(sq + (s * S::kQueries + q) * 4) =
+    *reinterpret_cast<v4f*>(sq + (s * kSelQueries + q) * 4) =
         *reinterpret_cast<const v4f*>(xc + static_cast<size_t>(row) * P + c0 + 4 * s);
   }
 }
 
 // Stage one: each wave owns kKnnQ queries; the workgroup walks every
-// candidate tile; out: nq lists of KL keys (ascending).
-template <int KL>
-__global__ __launch_bounds__(SelectShape<KL>::kThreads) void knn_select(
-    const float* __restrict__ xc, const float* __restrict__ nrm, int n, int n_pad, int P,
-    const int* __restrict__ qlist, int nq, unsigned long long* __restrict__ out) {
-  using S = SelectShape<KL>;
-  constexpr int NQ = S::kQueries;
-  constexpr int R = S::kSort;
+// candidate tile.  lists: nq lists of KL keys (ascending), kept in place.
+// ONE: the rows fit one 64-column chunk.
+template <int KL, bool ONE>
+__global__ __launch_bounds__(kSelThreads) void knn_select(
+    const float* __restrict__ xc, const float* __restrict__ nrm, int n_pad, int P,
+    const int* __restrict__ qlist, int nq, unsigned long long* __restrict__ lists) {
+  constexpr int RL = KL / 64;
   extern __shared__ unsigned long long knn_smem[];
-  // the MFMA operands first (small LDS offsets), then the lists
+  // the MFMA operands first (small LDS offsets), then the pending keys
   float* sc0 = reinterpret_cast<float*>(knn_smem);          // [step][candidate][4], x2
-  float* sq = sc0 + 2 * kKnnChunk * kKnnTile;               // [step][query][4]
-  float* snrm0 = sq + kKnnChunk * NQ;                        // x2
-  unsigned long long* list = reinterpret_cast<unsigned long long*>(snrm0 + 2 * kKnnTile);
-  unsigned long long* buf = list + KL * NQ;
-  int* cnt = reinterpret_cast<int*>(buf + kKnnBuf * NQ);
+  float* sq = sc0 + 2 * kSelCand;                            // [step][query][4]
+  float* snrm0 = sq + kKnnChunk * kSelQueries;               // x2
+  unsigned long long* buf = reinterpret_cast<unsigned long long*>(snrm0 + 2 * kKnnTile);
+  int* cnt = reinterpret_cast<int*>(buf + kKnnBuf * kSelQueries);
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int qs = w * kKnnQ + (lane & 15);  // this lane's query slot
-  const int gq = blockIdx.x * NQ + qs;
-  const bool qvalid = gq < nq;
+  const int gbase = blockIdx.x * kSelQueries + w * kKnnQ;
+  const bool qvalid = gbase + (lane & 15) < nq;
 
-  for (int s = tid; s < KL * NQ; s += S::kThreads) list[s] = kNoKey;
-  for (int s = tid; s < NQ; s += S::kThreads) cnt[s] = 0;
-  unsigned long long tau = kNoKey;
+  // the wave's lists start empty (each lane writes the elements it merges)
+  for (int j = 0; j < kKnnQ && gbase + j < nq; ++j) {
+#pragma unroll
+    for (int r = 0; r < RL; ++r) lists[static_cast<size_t>(gbase + j) * KL + lane * RL + r] = kNoKey;
+  }
+  if (tid < kSelQueries) cnt[tid] = 0;
   float tau_v = qvalid ? __int_as_float(0x7f800000) : -__int_as_float(0x7f800000);
 
-  const int nchunks = (P + kKnnChunk - 1) / kKnnChunk;
+  const int nchunks = ONE ? 1 : (P + kKnnChunk - 1) / kKnnChunk;
   const int nsteps = (n_pad / kKnnTile) * nchunks;
-  v4f pf[S::kFetch];
+  v4f pf[kSelFetch];
   float pfn = 0.0f;
-  stage_queries<KL>(xc, P, 0, qlist, nq, tid, sq);
-  fetch_chunk<KL>(xc, nrm, P, nchunks, 0, tid, pf, pfn);
-  store_chunk<KL>(P, nchunks, 0, tid, pf, pfn, sc0, snrm0);
+  stage_queries(xc, P, 0, qlist, nq, tid, sq);
+  fetch_chunk(xc, nrm, P, 0, 0, tid, pf, pfn);
+  store_chunk(P, 0, tid, pf, pfn, sc0, snrm0);
   __syncthreads();
 
+  const float* qb = sq + (w * kKnnQ + (lane & 15)) * 4 + (lane >> 4);
   v4f acc[4];
   for (int step = 0; step < nsteps; ++step) {
-    const int tile = step / nchunks, chunk = step - tile * nchunks;
+    const int tile = ONE ? step : step / nchunks;
+    const int chunk = ONE ? 0 : step - tile * nchunks;
     const int t0 = tile * kKnnTile;
-    const float* sc = sc0 + (step & 1) * (kKnnChunk * kKnnTile);
+    const float* sc = sc0 + (step & 1) * kSelCand;
     const float* snrm = snrm0 + (tile & 1) * kKnnTile;
     const bool more = step + 1 < nsteps;
-    if (more) {
-      fetch_chunk<KL>(xc, nrm, P, nchunks, step + 1, tid, pf, pfn);
-    }
+    const int nt0 = ONE ? t0 + kKnnTile : ((step + 1) / nchunks) * kKnnTile;
+    const int nc0 = ONE ? 0 : ((step + 1) - ((step + 1) / nchunks) * nchunks) * kKnnChunk;
+    if (more) fetch_chunk(xc, nrm, P, nt0, nc0, tid, pf, pfn);
     if (chunk == 0) {
 #pragma unroll
       for (int b = 0; b < 4; ++b) acc[b] = v4f{0.f, 0.f, 0.f, 0.f};
     }
     // P is a multiple of 8: the chunk has an even number of k-steps, taken
     // two at a time (no branch between the MFMAs of a chunk)
-    const int kq = min(kKnnChunk, P - chunk * kKnnChunk) / 4;
-    const float* qb = sq + (w * kKnnQ + (lane & 15)) * 4 + (lane >> 4);
+    const int kq = ONE ? P / 4 : min(kKnnChunk, P - chunk * kKnnChunk) / 4;
     const float* cb = sc + (lane & 15) * 4 + (lane >> 4);
     for (int s = 0; s < kq; s += 2) {
-      const float b0 = qb[s * NQ * 4], b1 = qb[(s + 1) * NQ * 4];
+      const float b0 = qb[s * kSelQueries * 4], b1 = qb[(s + 1) * kSelQueries * 4];
       float a0[4], a1[4];
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
@@ -414,37 +437,49 @@ __global__ __launch_bounds__(SelectShape<KL>::kThreads) void knn_select(
       for (int b = 0; b < 4; ++b) acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[b], b1, acc[b], 0, 0, 0);
     }
     if (chunk == nchunks - 1) {
-      // lane: query slot qs, candidates t0 + 16 b + 4 (lane / 16) + r; two
-      // blocks add at most 32 keys to a query, so a query with more than
+      // two blocks add at most 32 keys to a query, so a query with more than
       // kKnnBuf - 32 pending keys is merged after each pair of blocks
-      offer_keys(acc[0], 0, t0, lane, snrm, tau_v, cnt + qs, buf + qs * kKnnBuf);
-      offer_keys(acc[1], 1, t0, lane, snrm, tau_v, cnt + qs, buf + qs * kKnnBuf);
-      tau = merge_full<KL, R>(list, buf, cnt, w, lane, qs, kKnnBuf - 32, tau);
-      tau_v = last_value(tau, qvalid);
-      offer_keys(acc[2], 2, t0, lane, snrm, tau_v, cnt + qs, buf + qs * kKnnBuf);
-      offer_keys(acc[3], 3, t0, lane, snrm, tau_v, cnt + qs, buf + qs * kKnnBuf);
-      tau = merge_full<KL, R>(list, buf, cnt, w, lane, qs, kKnnBuf - 32, tau);
-      tau_v = last_value(tau, qvalid);
+      unsigned long long* pend = buf + qs * kKnnBuf;
+      offer_keys(acc[0], 0, t0, lane, snrm, tau_v, cnt + qs, pend);
+      offer_keys(acc[1], 1, t0, lane, snrm, tau_v, cnt + qs, pend);
+      merge_full<KL>(lists, buf, cnt, w, lane, gbase, kKnnBuf - 32, qvalid, tau_v);
+      offer_keys(acc[2], 2, t0, lane, snrm, tau_v, cnt + qs, pend);
+      offer_keys(acc[3], 3, t0, lane, snrm, tau_v, cnt + qs, pend);
+      merge_full<KL>(lists, buf, cnt, w, lane, gbase, kKnnBuf - 32, qvalid, tau_v);
     }
     if (more)
-      store_chunk<KL>(P, nchunks, step + 1, tid, pf, pfn,
-                      sc0 + ((step + 1) & 1) * (kKnnChunk * kKnnTile),
-                      snrm0 + (((step + 1) / nchunks) & 1) * kKnnTile);
+      store_chunk(P, nc0, tid, pf, pfn, sc0 + ((step + 1) & 1) * kSelCand,
+                  snrm0 + ((nt0 / kKnnTile) & 1) * kKnnTile);
     __syncthreads();
-    if (more && nchunks > 1) {
+    if (!ONE && more && nchunks > 1) {
       // past 64 columns the query chunk changes with the step
-      stage_queries<KL>(xc, P, ((step + 1) % nchunks) * kKnnChunk, qlist, nq, tid, sq);
+      stage_queries(xc, P, nc0, qlist, nq, tid, sq);
       __syncthreads();
     }
   }
   // the rest of the pending keys
-  tau = merge_full<KL, R>(list, buf, cnt, w, lane, qs, 0, tau);
-  for (int j = 0; j < kKnnQ; ++j) {
-    const int g = blockIdx.x * NQ + w * kKnnQ + j;
-    if (g >= nq) break;
-    for (int s = lane; s < KL; s += 64)
-      out[static_cast<size_t>(g) * KL + s] = list[(w * kKnnQ + j) * KL + s];
+  merge_full<KL>(lists, buf, cnt, w, lane, gbase, 0, qvalid, tau_v);
+}
+
+template <int KL>
+int run_select_kl(const float* xc, const float* nrm, int n_pad, int P, int nq,
+                  unsigned long long* lists, hipStream_t s) {
+  const bool one = P <= kKnnChunk;
+  const void* fn = one ? reinterpret_cast<const void*>(&knn_select<KL, true>)
+                       : reinterpret_cast<const void*>(&knn_select<KL, false>);
+  static bool attr_set[2] = {false, false};
+  if (!attr_set[one]) {
+    FE_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 static_cast<int>(kSelLds)));
+    attr_set[one] = true;
   }
+  const int blocks = (nq + kSelQueries - 1) / kSelQueries;
+  if (one)
+    knn_select<KL, true><<<blocks, kSelThreads, kSelLds, s>>>(xc, nrm, n_pad, P, nullptr, nq, lists);
+  else
+    knn_select<KL, false><<<blocks, kSelThreads, kSelLds, s>>>(xc, nrm, n_pad, P, nullptr, nq, lists);
+  FE_LAUNCH_CHECK("knn_select");
+  return 0;
 }
 
 // ---- exact distances ----
@@ -668,23 +703,6 @@ __global__ __launch_bounds__(kKnnThreads) void knn_scan(const float* __restrict_
 }
 
 template <int KL>
-int run_select(const float* xc, const float* nrm, int n, int n_pad, int P, int nq,
-               unsigned long long* lists, hipStream_t s) {
-  using S = SelectShape<KL>;
-  static bool attr_set = false;
-  if (!attr_set) {
-    FE_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&knn_select<KL>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 static_cast<int>(S::kLds)));
-    attr_set = true;
-  }
-  const int blocks = (nq + S::kQueries - 1) / S::kQueries;
-  knn_select<KL><<<blocks, S::kThreads, S::kLds, s>>>(xc, nrm, n, n_pad, P, nullptr, nq, lists);
-  FE_LAUNCH_CHECK("knn_select");
-  return 0;
-}
-
-template <int KL>
 int run_refine(const unsigned long long* lists, const float* xr, int n, int p, int P,
                const double* qn, const double* cmax2, int k, int index_base, int force_scan,
                int* out_idx, float* out_dist, int* failed, int* failed_count, hipStream_t s) {
@@ -788,22 +806,22 @@ int fastembedr_hip_exact_knn(const float* data, int n, int p, int k, int device_
   int rc = 0;
   switch (KL) {
     case 64:
-      rc = run_select<64>(xc, nrm, n, n_pad, P, n, lists, s) ||
+      rc = run_select_kl<64>(xc, nrm, n_pad, P, n, lists, s) ||
            run_refine<64>(lists, xr, n, p, P, qn, cmax2, k, index_base, force_scan, o_idx, o_dist,
                           failed, failed_count, s);
       break;
     case 128:
-      rc = run_select<128>(xc, nrm, n, n_pad, P, n, lists, s) ||
+      rc = run_select_kl<128>(xc, nrm, n_pad, P, n, lists, s) ||
            run_refine<128>(lists, xr, n, p, P, qn, cmax2, k, index_base, force_scan, o_idx, o_dist,
                            failed, failed_count, s);
       break;
     case 256:
-      rc = run_select<256>(xc, nrm, n, n_pad, P, n, lists, s) ||
+      rc = run_select_kl<256>(xc, nrm, n_pad, P, n, lists, s) ||
            run_refine<256>(lists, xr, n, p, P, qn, cmax2, k, index_base, force_scan, o_idx, o_dist,
                            failed, failed_count, s);
       break;
     default:
-      rc = run_select<512>(xc, nrm, n, n_pad, P, n, lists, s) ||
+      rc = run_select_kl<512>(xc, nrm, n_pad, P, n, lists, s) ||
            run_refine<512>(lists, xr, n, p, P, qn, cmax2, k, index_base, force_scan, o_idx, o_dist,
                            failed, failed_count, s);
       break;
