@@ -85,6 +85,8 @@ SIGNATURES = {
     "fastembedr_hip_umap_graph_destroy": (None, [_vp]),
     "fastembedr_hip_exact_structure_metrics": (
         c_int, [_dp, c_int, c_int, _dp, c_int, _ip, c_int, _dp]),
+    "fastembedr_hip_knn_structure_score": (
+        c_int, [_dp, _ip, _ip, _ip, c_int, c_int, c_int, c_int, c_int, c_int, c_int, _dp]),
     "fastembedr_hip_exact_knn": (
         c_int, [_vp, c_int, c_int, c_int, c_int, c_int, _vp, _vp, c_int, _ip]),
 }

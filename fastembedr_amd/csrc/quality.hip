@@ -28,6 +28,12 @@ constexpr int kRankThreads = 1024;
 constexpr int kRankMaxN = 5000;
 constexpr int kRankSums = 4;  // shared, trust penalty, continuity penalty, rank error
 
+__host__ __device__ constexpr int pow2_at_least_i(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
 __host__ __device__ constexpr int next_pow2(int v) {
   int p = 1;
   while (p < v) p <<= 1;
@@ -154,6 +160,225 @@ __global__ void rank_totals(const long long* __restrict__ sums, int n, int width
   totals[s] = t;
 }
 
+// ---- KNN structure score ----
+//
+// knn_structure_score (knn_structure_score_cpp, src/embedding_utils.cpp:
+// 1386-1580; structure_score_rows_kernel, src/embedding_cuda_kernels.cpp:
+// 1827-1944): one workgroup per kept row scans every row of the layout once
+// -- FP64 squared distances, each counted against the row's sorted
+// high-dimensional neighbours (continuity ranks) and offered to the row's
+// list of nearest layout rows (LDS, merged by a bitonic sort) -- then the
+// preservation, local trustworthiness, continuity and label-majority terms;
+// the rows' terms are added in row order.
+
+constexpr int kScoreThreads = 256;
+constexpr int kScoreMaxK = 256;
+constexpr int kScoreSlots = 512;  // list (<= 256) + one chunk of candidates
+constexpr int kScoreWidth = 6;
+
+__device__ __forceinline__ double layout_d2(const double* __restrict__ y, int n, int dim, int i,
+                                            int j) {
+  double d = 0.0;
+  for (int c = 0; c < dim; ++c) {
+    const double diff = __dsub_rn(y[static_cast<size_t>(c) * n + i], y[static_cast<size_t>(c) * n + j]);
+    d = __dadd_rn(d, __dmul_rn(diff, diff));
+  }
+  return d;
+}
+
+__device__ __forceinline__ bool pair_lt(double a, int ia, double b, int ib) {
+  return a < b || (a == b && ia < ib);
+}
+
+// ascending bitonic sort of the first m (power of two) pairs, whole block
+__device__ void block_sort_pairs(double* d, int* j, int m) {
+  for (int size = 2; size <= m; size <<= 1) {
+    for (int h = size >> 1; h > 0; h >>= 1) {
+      for (int t = threadIdx.x; t < m / 2; t += kScoreThreads) {
+        const int a = 2 * h * (t / h) + (t % h), b = a + h;
+        const bool up = (a & size) == 0;
+        if (pair_lt(d[b], j[b], d[a], j[a]) == up) {
+          const double td = d[a];
+          const int tj = j[a];
+          d[a] = d[b];
+          j[a] = j[b];
+          d[b] = td;
+          j[b] = tj;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__global__ __launch_bounds__(kScoreThreads) void score_rows(
+    const double* __restrict__ y, int n, int dim, const int* __restrict__ ind, int index_rows,
+    int limit, const int* __restrict__ keep, int compact, int k, const int* __restrict__ labels,
+    int n_levels, double* __restrict__ rows) {
+  __shared__ double td[kScoreMaxK];
+  __shared__ int tj[kScoreMaxK];
+  __shared__ int diff[kScoreMaxK + 1];
+  __shared__ double sd[kScoreSlots];
+  __shared__ int sj[kScoreSlots];
+  __shared__ int ia[kScoreThreads], ib[kScoreThreads];
+  __shared__ int pending, n_targets, repeated;
+  const int kk = blockIdx.x, tid = threadIdx.x;
+  double* out = rows + static_cast<size_t>(kk) * kScoreWidth;
+  const int q = keep[kk] - 1;
+  const int row = compact ? kk : q;
+  if (q < 0 || q >= n || row < 0 || row >= index_rows) {  // not scored
+    if (tid < kScoreWidth) out[tid] = 0.0;
+    return;
+  }
+  // the targets: the first k high-dimensional neighbours, by (layout d2, index)
+  {
+    double d = INFINITY;
+    int j = INT_MAX;
+    if (tid < k) {
+      const int nb = ind[static_cast<size_t>(tid) * index_rows + row] - 1;
+      if (nb >= 0 && nb < n) {
+        d = layout_d2(y, n, dim, q, nb);
+        j = nb;
+      }
+    }
+    td[tid] = d;
+    tj[tid] = j;
+  }
+  diff[tid] = 0;
+  for (int s = tid; s < kScoreSlots; s += kScoreThreads) {
+    sd[s] = INFINITY;
+    sj[s] = INT_MAX;
+  }
+  if (tid == 0) {
+    diff[kScoreMaxK] = 0;
+    pending = 0;
+    n_targets = 0;
+    repeated = 0;
+  }
+  __syncthreads();
+  block_sort_pairs(td, tj, kScoreMaxK);
+  if (tj[tid] != INT_MAX && (tid == kScoreMaxK - 1 || tj[tid + 1] == INT_MAX)) n_targets = tid + 1;
+  // a target listed twice (adjacent after the sort) or the row itself
+  if (tj[tid] != INT_MAX && (tj[tid] == q || (tid > 0 && tj[tid] == tj[tid - 1]))) repeated = 1;
+  __syncthreads();
+  const int T = n_targets;
+  const bool bound = T == k && repeated == 0;
+  const int KF = pow2_at_least_i(k);
+  const int N = pow2_at_least_i(KF + kScoreThreads);
+  // a key at or below tau may enter the list: at first the farthest target
+  // (k distinct targets other than the row are k candidates, so the k-th
+  // nearest is no farther)
+  double tau_d = bound ? td[T - 1] : INFINITY;
+  int tau_j = bound ? tj[T - 1] : INT_MAX;
+  for (int b0 = 0; b0 < n; b0 += kScoreThreads) {
+    const int c = b0 + tid;
+    if (c < n && c != q) {
+      const double d = layout_d2(y, n, dim, q, c);
+      int lo = 0, hi = T;  // the first target after (d, c)
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (pair_lt(d, c, td[mid], tj[mid])) hi = mid;
+        else lo = mid + 1;
+      }
+      if (lo < T) atomicAdd(&diff[lo], 1);
+      if (d < tau_d || (d == tau_d && c <= tau_j)) {
+        const int pos = KF + atomicAdd(&pending, 1);
+        sd[pos] = d;
+        sj[pos] = c;
+      }
+    }
+    __syncthreads();
+    if (pending > 0) {
+      block_sort_pairs(sd, sj, N);
+      for (int s = KF + tid; s < N; s += kScoreThreads) {
+        sd[s] = INFINITY;
+        sj[s] = INT_MAX;
+      }
+      if (tid == 0) pending = 0;
+      __syncthreads();
+      if (sj[k - 1] != INT_MAX) {
+        tau_d = sd[k - 1];
+        tau_j = sj[k - 1];
+      }
+    }
+  }
+  if (sj[k - 1] == INT_MAX) {  // fewer than k other rows
+    if (tid < kScoreWidth) out[tid] = 0.0;
+    return;
+  }
+  // high ranks of the k nearest layout rows
+  if (tid < k) {
+    const int c = sj[tid];
+    int rank = limit + 1;
+    for (int s = 0; s < limit; ++s)
+      if (ind[static_cast<size_t>(s) * index_rows + row] - 1 == c) {
+        rank = s + 1;
+        break;
+      }
+    ia[tid] = rank;
+    int count = 0, lab = 0;
+    if (labels != nullptr) {
+      lab = labels[c];
+      if (lab >= 1 && lab <= n_levels)
+        for (int s = 0; s < k; ++s) count += labels[sj[s]] == lab ? 1 : 0;
+    }
+    ib[tid] = count;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int shared = 0;
+    double trust_penalty = 0.0;
+    for (int r = 0; r < k; ++r) {
+      if (ia[r] <= k) ++shared;
+      else trust_penalty += static_cast<double>(ia[r] - k);
+    }
+    double cont_penalty = 0.0;
+    int before = 0;
+    for (int t = 0; t < T; ++t) {
+      before += diff[t];
+      const int low_rank = 1 + before;
+      if (low_rank > k) cont_penalty += static_cast<double>(low_rank - k);
+    }
+    double hit = 0.0, hit_n = 0.0;
+    if (labels != nullptr && n_levels > 0) {
+      const int truth = labels[q];
+      if (truth >= 1 && truth <= n_levels) {
+        int best = 0, best_count = 0;  // the largest count, then the smallest label
+        for (int r = 0; r < k; ++r) {
+          const int lab = labels[sj[r]];
+          if (ib[r] > best_count || (ib[r] == best_count && ib[r] > 0 && lab < best)) {
+            best_count = ib[r];
+            best = lab;
+          }
+        }
+        if (best_count > 0) {
+          hit = best == truth ? 1.0 : 0.0;
+          hit_n = 1.0;
+        }
+      }
+    }
+    const double kd = static_cast<double>(k);
+    const double trust_denom = kd * static_cast<double>(max(1, limit + 1 - k));
+    const double cont_denom = kd * static_cast<double>(max(1, n - k));
+    out[0] = static_cast<double>(shared) / kd;
+    out[1] = fmax(0.0, fmin(1.0, 1.0 - trust_penalty / trust_denom));
+    out[2] = fmax(0.0, fmin(1.0, 1.0 - cont_penalty / cont_denom));
+    out[3] = hit;
+    out[4] = hit_n;
+    out[5] = 1.0;
+  }
+}
+
+// totals of the rows, in row order
+__global__ void score_totals(const double* __restrict__ rows, int keep_n,
+                             double* __restrict__ totals) {
+  const int m = threadIdx.x;
+  if (m >= kScoreWidth) return;
+  double t = 0.0;
+  for (int r = 0; r < keep_n; ++r) t = __dadd_rn(t, rows[static_cast<size_t>(r) * kScoreWidth + m]);
+  totals[m] = t;
+}
+
 }  // namespace
 }  // namespace fe
 
@@ -225,6 +450,58 @@ int fastembedr_hip_exact_structure_metrics(const double* high, int n, int high_c
     out[2 * nk + q] = shared / (static_cast<double>(n) * k);
     out[3 * nk + q] = rank_error / (static_cast<double>(n) * k);
   }
+  return 0;
+}
+
+int fastembedr_hip_knn_structure_score(const double* layout, const int* indices, const int* keep,
+                                       const int* labels, int n, int index_rows,
+                                       int high_rank_limit, int preserve_k, int keep_n,
+                                       int compact_indices, int n_label_levels, double* totals) {
+  using namespace fe;
+  clear_error();
+  // fastembedr_cuda_knn_structure_score's checks (embedding_cuda_kernels.cpp:4583-4596)
+  if (layout == nullptr || indices == nullptr || keep == nullptr || totals == nullptr)
+    return fail("null host pointer");
+  if (n < 2 || index_rows < 1 || high_rank_limit < 1 || preserve_k < 1 ||
+      preserve_k > high_rank_limit || preserve_k > kScoreMaxK || keep_n < 1)
+    return fail("invalid structure-score dimensions");
+  if (n_label_levels > 0 && labels == nullptr) return fail("labels pointer is null");
+  const int dim = 2;
+  Stream stream;
+  if (stream.init()) return 1;
+  hipStream_t s = stream.get();
+  const size_t yb = static_cast<size_t>(n) * dim;
+  const size_t ib = static_cast<size_t>(index_rows) * high_rank_limit;
+  const bool with_labels = n_label_levels > 0;
+  ArenaPlan ap;
+  ap.add<double>(yb);
+  ap.add<int>(ib);
+  ap.add<int>(keep_n);
+  if (with_labels) ap.add<int>(n);
+  ap.add<double>(static_cast<size_t>(keep_n) * kScoreWidth);
+  ap.add<double>(kScoreWidth);
+  Arena mem;
+  if (mem.init(ap.bytes, "structure score")) return 1;
+  double* d_y = mem.take<double>(yb);
+  int* d_ind = mem.take<int>(ib);
+  int* d_keep = mem.take<int>(keep_n);
+  int* d_lab = with_labels ? mem.take<int>(n) : nullptr;
+  double* d_rows = mem.take<double>(static_cast<size_t>(keep_n) * kScoreWidth);
+  double* d_tot = mem.take<double>(kScoreWidth);
+  FE_CHECK(hipMemcpyAsync(d_y, layout, yb * sizeof(double), hipMemcpyHostToDevice, s));
+  FE_CHECK(hipMemcpyAsync(d_ind, indices, ib * sizeof(int), hipMemcpyHostToDevice, s));
+  FE_CHECK(hipMemcpyAsync(d_keep, keep, keep_n * sizeof(int), hipMemcpyHostToDevice, s));
+  if (with_labels)
+    FE_CHECK(hipMemcpyAsync(d_lab, labels, static_cast<size_t>(n) * sizeof(int),
+                            hipMemcpyHostToDevice, s));
+  score_rows<<<keep_n, kScoreThreads, 0, s>>>(d_y, n, dim, d_ind, index_rows, high_rank_limit,
+                                              d_keep, compact_indices ? 1 : 0, preserve_k, d_lab,
+                                              with_labels ? n_label_levels : 0, d_rows);
+  FE_LAUNCH_CHECK("score_rows");
+  score_totals<<<1, 64, 0, s>>>(d_rows, keep_n, d_tot);
+  FE_LAUNCH_CHECK("score_totals");
+  FE_CHECK(hipMemcpyAsync(totals, d_tot, kScoreWidth * sizeof(double), hipMemcpyDeviceToHost, s));
+  FE_CHECK(hipStreamSynchronize(s));
   return 0;
 }
 

@@ -313,6 +313,21 @@ int fastembedr_hip_exact_structure_metrics(const double* high, int n, int high_c
                                            const double* low, int low_cols,
                                            const int* requested_k, int nk, double* out);
 
+/* Replaces fastembedr_cuda_knn_structure_score
+ * (src/embedding_cuda_kernels.cpp:4570-4700, the GPU form of
+ * knn_structure_score_cpp, src/embedding_utils.cpp:1386-1580), same
+ * arguments: layout column-major n x 2 FP64; indices column-major
+ * index_rows x high_rank_limit, 1-based (index_rows = n, or keep_n with
+ * compact_indices); keep: keep_n 1-based rows; labels: n factor codes or
+ * NULL (n_label_levels 0); 1 <= preserve_k <= min(high_rank_limit, 256).
+ * totals[6]: sums over the scored rows of preservation, local
+ * trustworthiness, local continuity, label hits, labelled rows, and the
+ * number of scored rows, added in row order. */
+int fastembedr_hip_knn_structure_score(const double* layout, const int* indices, const int* keep,
+                                       const int* labels, int n, int index_rows,
+                                       int high_rank_limit, int preserve_k, int keep_n,
+                                       int compact_indices, int n_label_levels, double* totals);
+
 /* ---- exact k-nearest neighbours -----------------------------------------
  * Replaces the exact route of native_cuda_knn_impl
  * (src/native_knn_cuda_impl.cpp:520-700: FAISS bfKnn or cuVS brute force

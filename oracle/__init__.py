@@ -96,6 +96,9 @@ def _declare(L):
     if hasattr(L, "or_exact_knn"):
         L.or_exact_knn.argtypes = [_fp, c_int, c_int, c_int, _ip, c_int, _ip, _fp]
         L.or_exact_knn.restype = c_int
+        L.or_knn_structure_score.argtypes = [_dp, c_int, c_int, _ip, c_int, c_int, _ip, c_int,
+                                             c_int, c_int, _ip, c_int, _dp]
+        L.or_knn_structure_score.restype = c_int
 
 
 def _declare_umap(L):
@@ -327,6 +330,30 @@ def exact_knn(data, k, queries=None):
     if rc != 0:
         raise ValueError("exact KNN needs n >= 2, p >= 1 and 1 <= k < n")
     return idx, dist
+
+
+def knn_structure_score(layout, indices, keep, preserve_k, labels=None, n_levels=0):
+    """The six totals of the KNN structure score (preservation, trust,
+    continuity, label hits, label rows, scored rows), summed in row order.
+
+    layout: n x dim; indices: index_rows x limit, 1-based (index_rows = n, or
+    len(keep) for compact rows); keep: 1-based rows; labels: n factor codes.
+    """
+    y = np.asfortranarray(np.asarray(layout, dtype=np.float64))
+    n, dim = y.shape
+    ind = np.asfortranarray(np.asarray(indices, dtype=np.int32))
+    index_rows, limit = ind.shape
+    kp = np.ascontiguousarray(keep, dtype=np.int32)
+    lab = None if labels is None else np.ascontiguousarray(labels, dtype=np.int32)
+    out = np.zeros(6)
+    yf, inf_ = y.ravel(order="F"), ind.ravel(order="F")
+    rc = lib().or_knn_structure_score(
+        _ptr(yf, _dp), n, dim, _ptr(inf_, _ip), index_rows, limit, _ptr(kp, _ip), len(kp),
+        int(index_rows == len(kp)), int(preserve_k), None if lab is None else _ptr(lab, _ip),
+        int(n_levels if lab is not None else 0), _ptr(out, _dp))
+    if rc != 0:
+        raise ValueError("invalid structure-score dimensions")
+    return out
 
 
 def port_tsne_iterate(row_ptr, col, val, y, iters, exag=12.0, learning_rate=None, momentum=0.8,

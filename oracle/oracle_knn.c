@@ -105,3 +105,135 @@ int or_exact_knn(const float* data, int n, int p, int k, const int* queries, int
   free(heap);
   return 0;
 }
+
+/* KNN structure score (knn_structure_score_cpp, src/embedding_utils.cpp:
+ * 1386-1580; the GPU form structure_score_rows_kernel,
+ * src/embedding_cuda_kernels.cpp:1827-1944).  layout: column-major n x dim
+ * double; indices: column-major index_rows x limit, 1-based; keep: keep_n
+ * 1-based rows; compact: indices row kk belongs to keep[kk] (else to the
+ * row itself); labels: n factor codes (NULL or n_levels = 0: none).  For
+ * every kept row: its first preserve_k high-dimensional neighbours (the
+ * targets) and its preserve_k nearest rows in the layout by (d2, index);
+ * preservation, local trustworthiness and continuity as the reference forms
+ * them, and the majority label of the layout neighbours (ties by the
+ * smaller label; rows whose own label is outside 1..n_levels are not
+ * counted, as the GPU form).  totals: the six sums of the GPU form
+ * (preservation, trust, continuity, label hits, label rows, scored rows),
+ * added in row order. */
+static double or_layout_d2(const double* y, int n, int dim, int i, int j) {
+  double d2 = 0.0;
+  for (int c = 0; c < dim; ++c) {
+    const double diff = y[(size_t)c * n + i] - y[(size_t)c * n + j];
+    d2 = d2 + diff * diff;
+  }
+  return d2;
+}
+
+int or_knn_structure_score(const double* layout, int n, int dim, const int* indices,
+                           int index_rows, int limit, const int* keep, int keep_n,
+                           int compact, int preserve_k, const int* labels, int n_levels,
+                           double* totals) {
+  if (n < 2 || preserve_k < 1 || preserve_k > limit) return 1;
+  for (int m = 0; m < 6; ++m) totals[m] = 0.0;
+  or_nb* low = (or_nb*)malloc(sizeof(or_nb) * (size_t)preserve_k);
+  or_nb* tg = (or_nb*)malloc(sizeof(or_nb) * (size_t)preserve_k);
+  int* cnt = (int*)malloc(sizeof(int) * (size_t)(n_levels + 1));
+  if (!low || !tg || !cnt) {
+    free(low);
+    free(tg);
+    free(cnt);
+    return 1;
+  }
+  for (int kk = 0; kk < keep_n; ++kk) {
+    const int q = keep[kk] - 1;
+    if (q < 0 || q >= n) continue;
+    const int row = compact ? kk : q;
+    if (row < 0 || row >= index_rows) continue;
+    int T = 0;
+    for (int r = 0; r < preserve_k; ++r) {
+      const int nb = indices[(size_t)r * index_rows + row] - 1;
+      if (nb < 0 || nb >= n) continue;
+      or_nb e = {or_layout_d2(layout, n, dim, q, nb), nb};
+      tg[T++] = e;
+    }
+    qsort(tg, (size_t)T, sizeof(or_nb), or_nb_cmp);
+    /* the preserve_k nearest rows of the layout */
+    int m = 0;
+    for (int c = 0; c < n; ++c) {
+      if (c == q) continue;
+      or_nb e = {or_layout_d2(layout, n, dim, q, c), c};
+      if (m < preserve_k) {
+        low[m] = e;
+        or_sift_up(low, m);
+        ++m;
+      } else if (or_nb_less(e, low[0])) {
+        low[0] = e;
+        or_sift_down(low, m, 0);
+      }
+    }
+    if (m < preserve_k) continue;
+    qsort(low, (size_t)m, sizeof(or_nb), or_nb_cmp);
+    int shared = 0;
+    double trust_penalty = 0.0;
+    for (int r = 0; r < preserve_k; ++r) {
+      int rank = limit + 1;
+      for (int s = 0; s < limit; ++s)
+        if (indices[(size_t)s * index_rows + row] - 1 == low[r].j) {
+          rank = s + 1;
+          break;
+        }
+      if (rank <= preserve_k) ++shared;
+      if (rank > preserve_k) trust_penalty += (double)(rank - preserve_k);
+    }
+    /* low rank of each target: 1 + the rows that come before it */
+    double cont_penalty = 0.0;
+    for (int t = 0; t < T; ++t) {
+      int before = 0;
+      for (int c = 0; c < n; ++c) {
+        if (c == q) continue;
+        or_nb e = {or_layout_d2(layout, n, dim, q, c), c};
+        if (or_nb_less(e, tg[t])) ++before;
+      }
+      const int low_rank = 1 + before;
+      if (low_rank > preserve_k) cont_penalty += (double)(low_rank - preserve_k);
+    }
+    double hit = 0.0, hit_n = 0.0;
+    if (labels && n_levels > 0) {
+      const int truth = labels[q];
+      if (truth >= 1 && truth <= n_levels) {
+        for (int l = 0; l <= n_levels; ++l) cnt[l] = 0;
+        for (int r = 0; r < preserve_k; ++r) {
+          const int l = labels[low[r].j];
+          if (l >= 1 && l <= n_levels) ++cnt[l];
+        }
+        int best = 0, best_count = 0;
+        for (int l = 1; l <= n_levels; ++l)
+          if (cnt[l] > best_count) {
+            best_count = cnt[l];
+            best = l;
+          }
+        if (best_count > 0) {
+          hit = best == truth ? 1.0 : 0.0;
+          hit_n = 1.0;
+        }
+      }
+    }
+    const double k = (double)preserve_k;
+    const double trust_denom = k * (double)(limit + 1 - preserve_k > 1 ? limit + 1 - preserve_k : 1);
+    const double cont_denom = k * (double)(n - preserve_k > 1 ? n - preserve_k : 1);
+    double trust = 1.0 - trust_penalty / trust_denom;
+    double cont = 1.0 - cont_penalty / cont_denom;
+    trust = trust < 0.0 ? 0.0 : (trust > 1.0 ? 1.0 : trust);
+    cont = cont < 0.0 ? 0.0 : (cont > 1.0 ? 1.0 : cont);
+    totals[0] += (double)shared / k;
+    totals[1] += trust;
+    totals[2] += cont;
+    totals[3] += hit;
+    totals[4] += hit_n;
+    totals[5] += 1.0;
+  }
+  free(low);
+  free(tg);
+  free(cnt);
+  return 0;
+}

@@ -97,3 +97,33 @@ def test_exact_knn_argument_errors_need_no_device():
         knn.exact_knn(np.zeros((300, 2), np.float32), 257)
     with pytest.raises(HipError, match="index_base must be 0 or 1"):
         knn.exact_knn(x, 3, index_base=2)
+
+
+def test_structure_score_argument_errors_need_no_device():
+    # fastembedr_cuda_knn_structure_score's checks run before any device work
+    import ctypes
+
+    import numpy as np
+
+    from fastembedr_amd import HipError, _lib, quality
+
+    y = np.zeros((10, 2))
+    ind = np.ones((10, 5), np.int32)
+    for k in (0, 6):
+        with pytest.raises(HipError, match="invalid structure-score dimensions"):
+            quality.knn_structure_totals(y, ind, [1, 2], k)
+    big = np.ones((10, 300), np.int32)
+    with pytest.raises(HipError, match="invalid structure-score dimensions"):
+        quality.knn_structure_totals(y, big, [1], 257)
+    yc = np.zeros(20)
+    ic = np.ones(50, np.int32)
+    kp = np.ones(1, np.int32)
+    tot = np.zeros(6)
+    rc = _lib.lib().fastembedr_hip_knn_structure_score(
+        yc.ctypes.data_as(_lib._dp), ic.ctypes.data_as(_lib._ip), kp.ctypes.data_as(_lib._ip),
+        None, 10, 10, 5, 3, 1, 0, 4, tot.ctypes.data_as(_lib._dp))
+    assert rc == 1 and "labels pointer is null" in _lib.last_error()
+    with pytest.raises(ValueError, match="two-dimensional"):
+        quality.knn_structure_score(np.zeros((10, 3)), ind, [1], 2)
+    assert np.isnan(quality.knn_structure_score(y, ind, [], 2)["structure_score"])
+    del ctypes
