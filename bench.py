@@ -321,6 +321,8 @@ def main():
     if args.umap_graph_k > 0:
         umap_graph = bench_umap_graph(fe, knn_i, knn_d, n, min(args.umap_graph_k, k),
                                       not args.no_cpu_baseline)
+    # high-dimensional neighbours for the structure score of the final layout
+    score_idx = (knn_i[:30].T.cpu().numpy() + 1).astype(np.int32) if args.quality_sample > 0 else None
     del knn_i, knn_d
     torch.cuda.empty_cache()
     log(f"[bench] plan: {info} (P build + init {setup_s:.2f}s)")
@@ -416,6 +418,29 @@ def main():
             quality["cpu_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
             quality["cpu_kind"] = "oracle restatement of exact_structure_metrics_cpp, 1 thread"
             quality["identical_to_cpu"] = bool(np.array_equal(np.asarray(mc), m))
+        # the KNN structure score of the whole layout for the same rows
+        # (30 high-dimensional neighbours, preserve_k 15, cluster labels)
+        lay64 = layout.astype(np.float64)
+        labs = (np.asarray(labels) + 1).astype(np.int32)
+        keep = (sidx + 1).astype(np.int32)
+        fq.knn_structure_score(lay64, score_idx, keep[:8], 15, labs, 100)  # warm-up
+        t0 = time.perf_counter()
+        sc = fq.knn_structure_score(lay64, score_idx, keep, 15, labs, 100)
+        score = {"rows": int(qs), "n": int(n), "preserve_k": 15, "neighbours": 30,
+                 **{kk: float(v) for kk, v in sc.items()},
+                 "device_ms": round(1e3 * (time.perf_counter() - t0), 3)}
+        if not args.no_cpu_baseline:
+            import oracle
+
+            few = keep[:50]
+            t0 = time.perf_counter()
+            ref = oracle.knn_structure_score(lay64, score_idx, few, 15, labs, 100)
+            score["cpu_rows"] = int(len(few))
+            score["cpu_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
+            score["cpu_kind"] = "oracle restatement of knn_structure_score_cpp, 1 thread"
+            got = fq.knn_structure_totals(lay64, score_idx, few, 15, labs, 100)
+            score["sample_matches_cpu"] = bool(np.array_equal(got, ref))
+        quality["structure_score"] = score
         log(f"[bench] quality {quality}")
     plan.close()
     torch.cuda.empty_cache()
