@@ -89,6 +89,8 @@ SIGNATURES = {
         c_int, [_dp, _ip, _ip, _ip, c_int, c_int, c_int, c_int, c_int, c_int, c_int, _dp]),
     "fastembedr_hip_exact_knn": (
         c_int, [_vp, c_int, c_int, c_int, c_int, c_int, _vp, _vp, c_int, _ip]),
+    "fastembedr_hip_exact_query_knn": (
+        c_int, [_vp, c_int, _vp, c_int, c_int, c_int, c_int, c_int, _vp, _vp, c_int, _ip]),
 }
 
 _lib = None

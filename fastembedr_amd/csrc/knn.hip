@@ -350,8 +350,8 @@ __device__ __forceinline__ void store_chunk(int P, int c0, int tid, const v4f (&
   if (c0 == 0 && tid < kKnnTile) snrm[tid] = fn;
 }
 
-// The workgroup's query rows, columns c0 .. c0 + 63, into LDS as
-// [k-step][query][4] (the MFMA's B operand layout).
+// The workgroup's query rows (centred, row-major xc), columns c0 .. c0 + 63,
+// into LDS as [k-step][query][4] (the MFMA's B operand layout).
 __device__ __forceinline__ void stage_queries(const float* __restrict__ xc, int P, int c0,
                                               const int* __restrict__ qlist, int nq, int tid,
                                               float* sq) {
@@ -371,7 +371,8 @@ __device__ __forceinline__ void stage_queries(const float* __restrict__ xc, int 
 template <int KL, bool ONE>
 __global__ __launch_bounds__(kSelThreads) void knn_select(
     const float* __restrict__ xc, const float* __restrict__ nrm, int n_pad, int P,
-    const int* __restrict__ qlist, int nq, unsigned long long* __restrict__ lists) {
+    const float* __restrict__ xqc, const int* __restrict__ qlist, int nq,
+    unsigned long long* __restrict__ lists) {
   constexpr int RL = KL / 64;
   extern __shared__ unsigned long long knn_smem[];
   // the MFMA operands first (small LDS offsets), then the pending keys
@@ -398,7 +399,7 @@ __global__ __launch_bounds__(kSelThreads) void knn_select(
   const int nsteps = (n_pad / kKnnTile) * nchunks;
   v4f pf[kSelFetch];
   float pfn = 0.0f;
-  stage_queries(xc, P, 0, qlist, nq, tid, sq);
+  stage_queries(xqc, P, 0, qlist, nq, tid, sq);
   fetch_chunk(xc, nrm, P, 0, 0, tid, pf, pfn);
   store_chunk(P, 0, tid, pf, pfn, sc0, snrm0);
   __syncthreads();
@@ -453,7 +454,7 @@ __global__ __launch_bounds__(kSelThreads) void knn_select(
     __syncthreads();
     if (!ONE && more && nchunks > 1) {
       // past 64 columns the query chunk changes with the step
-      stage_queries(xc, P, nc0, qlist, nq, tid, sq);
+      stage_queries(xqc, P, nc0, qlist, nq, tid, sq);
       __syncthreads();
     }
   }
@@ -462,7 +463,7 @@ __global__ __launch_bounds__(kSelThreads) void knn_select(
 }
 
 template <int KL>
-int run_select_kl(const float* xc, const float* nrm, int n_pad, int P, int nq,
+int run_select_kl(const float* xc, const float* nrm, int n_pad, int P, const float* xqc, int nq,
                   unsigned long long* lists, hipStream_t s) {
   const bool one = P <= kKnnChunk;
   const void* fn = one ? reinterpret_cast<const void*>(&knn_select<KL, true>)
@@ -475,17 +476,22 @@ int run_select_kl(const float* xc, const float* nrm, int n_pad, int P, int nq,
   }
   const int blocks = (nq + kSelQueries - 1) / kSelQueries;
   if (one)
-    knn_select<KL, true><<<blocks, kSelThreads, kSelLds, s>>>(xc, nrm, n_pad, P, nullptr, nq, lists);
+    knn_select<KL, true><<<blocks, kSelThreads, kSelLds, s>>>(xc, nrm, n_pad, P, xqc, nullptr, nq,
+                                                              lists);
   else
-    knn_select<KL, false><<<blocks, kSelThreads, kSelLds, s>>>(xc, nrm, n_pad, P, nullptr, nq, lists);
+    knn_select<KL, false><<<blocks, kSelThreads, kSelLds, s>>>(xc, nrm, n_pad, P, xqc, nullptr, nq,
+                                                               lists);
   FE_LAUNCH_CHECK("knn_select");
   return 0;
 }
 
 // ---- exact distances ----
 
-__device__ __forceinline__ double exact_d2(const float* __restrict__ xr, int p, int q, int j) {
-  const float* a = xr + static_cast<size_t>(q) * p;
+// FP64 squared distance of query row q (row-major xq) and candidate row j
+// (row-major xr), summed in column order
+__device__ __forceinline__ double exact_d2(const float* __restrict__ xq, const float* __restrict__ xr,
+                                          int p, int q, int j) {
+  const float* a = xq + static_cast<size_t>(q) * p;
   const float* b = xr + static_cast<size_t>(j) * p;
   double s = 0.0;
   for (int c = 0; c < p; ++c) {
@@ -549,13 +555,16 @@ __device__ __forceinline__ void wave_sort_pairs(unsigned long long (&d)[R], int 
   }
 }
 
-// Stage two: one wave per query.
+// Stage two: one wave per query.  xq / xr: original query / candidate rows;
+// self: the queries are the candidates and each drops itself; outputs are
+// column-major nq x k.
 template <int KL>
 __global__ __launch_bounds__(kKnnThreads) void knn_refine(
-    const unsigned long long* __restrict__ lists, const float* __restrict__ xr, int n, int p, int P,
-    const double* __restrict__ qn, const double* __restrict__ cmax2, const int* __restrict__ qlist,
-    int nq, int k, int index_base, int force_scan, int* __restrict__ out_idx,
-    float* __restrict__ out_dist, int* __restrict__ failed, int* __restrict__ failed_count) {
+    const unsigned long long* __restrict__ lists, const float* __restrict__ xq,
+    const float* __restrict__ xr, int p, int P, const double* __restrict__ qn,
+    const double* __restrict__ cmax2, const int* __restrict__ qlist, int nq, int self, int k,
+    int index_base, int force_scan, int* __restrict__ out_idx, float* __restrict__ out_dist,
+    int* __restrict__ failed, int* __restrict__ failed_count) {
   constexpr int R = KL / 64;
   const int lane = threadIdx.x & 63;
   const int g = blockIdx.x * (kKnnThreads / 64) + (threadIdx.x >> 6);
@@ -572,7 +581,7 @@ __global__ __launch_bounds__(kKnnThreads) void knn_refine(
       j[r] = INT_MAX;
     } else {
       j[r] = static_cast<int>(static_cast<unsigned int>(key));
-      d[r] = static_cast<unsigned long long>(__double_as_longlong(exact_d2(xr, p, q, j[r])));
+      d[r] = static_cast<unsigned long long>(__double_as_longlong(exact_d2(xq, xr, p, q, j[r])));
     }
   }
   wave_sort_pairs<R>(d, j);
@@ -580,7 +589,7 @@ __global__ __launch_bounds__(kKnnThreads) void knn_refine(
   int self_at = INT_MAX;
 #pragma unroll
   for (int r = 0; r < R; ++r)
-    if (j[r] == q) self_at = lane * R + r;
+    if (self && j[r] == q) self_at = lane * R + r;
   for (int o = 32; o > 0; o >>= 1) self_at = min(self_at, __shfl_xor(self_at, o, 64));
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -588,8 +597,8 @@ __global__ __launch_bounds__(kKnnThreads) void knn_refine(
     if (s == self_at || j[r] == INT_MAX) continue;
     const int o = s - (self_at < s ? 1 : 0);
     if (o < k) {
-      out_idx[static_cast<size_t>(o) * n + q] = j[r] + index_base;
-      out_dist[static_cast<size_t>(o) * n + q] =
+      out_idx[static_cast<size_t>(o) * nq + q] = j[r] + index_base;
+      out_dist[static_cast<size_t>(o) * nq + q] =
           static_cast<float>(__dsqrt_rn(__longlong_as_double(static_cast<long long>(d[r]))));
     }
   }
@@ -636,9 +645,11 @@ __global__ __launch_bounds__(kKnnThreads) void knn_refine(
 
 constexpr int kScanSlots = 1024;  // list (<= 256) + one chunk (256), padded
 
-__global__ __launch_bounds__(kKnnThreads) void knn_scan(const float* __restrict__ xr, int n, int p,
-                                                        const int* __restrict__ rows, int k,
-                                                        int index_base, int* __restrict__ out_idx,
+__global__ __launch_bounds__(kKnnThreads) void knn_scan(const float* __restrict__ xq,
+                                                        const float* __restrict__ xr, int n, int p,
+                                                        const int* __restrict__ rows, int nq,
+                                                        int self, int k, int index_base,
+                                                        int* __restrict__ out_idx,
                                                         float* __restrict__ out_dist) {
   __shared__ double sd[kScanSlots];
   __shared__ int sj[kScanSlots];
@@ -657,8 +668,8 @@ __global__ __launch_bounds__(kKnnThreads) void knn_scan(const float* __restrict_
   int tau_j = INT_MAX;
   for (int b0 = 0; b0 < n; b0 += kKnnThreads) {
     const int c = b0 + tid;
-    if (c < n && c != q) {
-      const double d = exact_d2(xr, p, q, c);
+    if (c < n && !(self && c == q)) {
+      const double d = exact_d2(xq, xr, p, q, c);
       if (d < tau_d || (d == tau_d && c < tau_j)) {
         const int pos = KF + atomicAdd(&pending, 1);
         sd[pos] = d;
@@ -697,19 +708,20 @@ __global__ __launch_bounds__(kKnnThreads) void knn_scan(const float* __restrict_
     __syncthreads();
   }
   for (int o = tid; o < k; o += kKnnThreads) {
-    out_idx[static_cast<size_t>(o) * n + q] = sj[o] + index_base;
-    out_dist[static_cast<size_t>(o) * n + q] = static_cast<float>(__dsqrt_rn(sd[o]));
+    out_idx[static_cast<size_t>(o) * nq + q] = sj[o] + index_base;
+    out_dist[static_cast<size_t>(o) * nq + q] = static_cast<float>(__dsqrt_rn(sd[o]));
   }
 }
 
 template <int KL>
-int run_refine(const unsigned long long* lists, const float* xr, int n, int p, int P,
-               const double* qn, const double* cmax2, int k, int index_base, int force_scan,
-               int* out_idx, float* out_dist, int* failed, int* failed_count, hipStream_t s) {
+int run_refine(const unsigned long long* lists, const float* xq, const float* xr, int p, int P,
+               const double* qn, const double* cmax2, int nq, int self, int k, int index_base,
+               int force_scan, int* out_idx, float* out_dist, int* failed, int* failed_count,
+               hipStream_t s) {
   const int per = kKnnThreads / 64;
-  knn_refine<KL><<<(n + per - 1) / per, kKnnThreads, 0, s>>>(lists, xr, n, p, P, qn, cmax2, nullptr,
-                                                            n, k, index_base, force_scan, out_idx,
-                                                            out_dist, failed, failed_count);
+  knn_refine<KL><<<(nq + per - 1) / per, kKnnThreads, 0, s>>>(
+      lists, xq, xr, p, P, qn, cmax2, nullptr, nq, self, k, index_base, force_scan, out_idx,
+      out_dist, failed, failed_count);
   FE_LAUNCH_CHECK("knn_refine");
   return 0;
 }
@@ -717,6 +729,155 @@ int run_refine(const unsigned long long* lists, const float* xr, int n, int p, i
 int list_length(int k) {
   const int need = k + 1 + kKnnMargin;
   return need <= 64 ? 64 : need <= 128 ? 128 : need <= 256 ? 256 : 512;
+}
+
+// Exact k-NN of the query rows among the candidate rows; self: the query
+// rows are the candidates and each drops itself.  Both matrices column-major
+// (rows x p) float32; outputs column-major nq x k.
+int exact_knn_run(const float* data, int n, const float* query, int nq, int p, int k, bool self,
+                  int device_inputs, int index_base, int* indices, float* distances,
+                  int device_outputs, int* scanned_rows) {
+  const int P = (p + 7) / 8 * 8;
+  const int n_pad = (n + kKnnTile - 1) / kKnnTile * kKnnTile;
+  const int KL = list_length(k);
+  const bool force_scan = [] {
+    const char* e = std::getenv("FASTEMBEDR_HIP_KNN_SCAN_ALL");
+    return e != nullptr && std::strcmp(e, "1") == 0;
+  }();
+  Stream stream;
+  if (stream.init()) return 1;
+  hipStream_t s = stream.get();
+  const size_t np = static_cast<size_t>(n) * p, nqp = static_cast<size_t>(nq) * p;
+  const size_t nk = static_cast<size_t>(nq) * k;
+  const int pack_blocks = (n_pad + kKnnThreads - 1) / kKnnThreads;
+  const int qpack_blocks = (nq + kKnnThreads - 1) / kKnnThreads;
+  ArenaPlan ap;
+  if (!device_inputs) {
+    ap.add<float>(np);
+    if (!self) ap.add<float>(nqp);
+  }
+  if (!device_outputs) {
+    ap.add<int>(nk);
+    ap.add<float>(nk);
+  }
+  ap.add<double>(static_cast<size_t>(p) * kKnnPartBlocks);
+  ap.add<double>(p);
+  ap.add<float>(static_cast<size_t>(n_pad) * P);  // candidates: centred, padded
+  ap.add<float>(np);                              // candidates: original
+  ap.add<float>(n_pad);
+  ap.add<double>(n);
+  ap.add<double>(pack_blocks);
+  ap.add<double>(1);
+  if (!self) {
+    ap.add<float>(static_cast<size_t>(nq) * P);   // queries: centred
+    ap.add<float>(nqp);                           // queries: original
+    ap.add<float>(nq);
+    ap.add<double>(nq);
+    ap.add<double>(qpack_blocks);
+  }
+  ap.add<unsigned long long>(static_cast<size_t>(nq) * KL);
+  ap.add<int>(nq);
+  ap.add<int>(1);
+  Arena mem;
+  if (mem.init(ap.bytes, "exact KNN")) return 1;
+  const float* x = data;
+  const float* xq_in = self ? data : query;
+  if (!device_inputs) {
+    float* up = mem.take<float>(np);
+    FE_CHECK(hipMemcpyAsync(up, data, np * sizeof(float), hipMemcpyHostToDevice, s));
+    x = up;
+    xq_in = up;
+    if (!self) {
+      float* uq = mem.take<float>(nqp);
+      FE_CHECK(hipMemcpyAsync(uq, query, nqp * sizeof(float), hipMemcpyHostToDevice, s));
+      xq_in = uq;
+    }
+  }
+  int* o_idx = indices;
+  float* o_dist = distances;
+  if (!device_outputs) {
+    o_idx = mem.take<int>(nk);
+    o_dist = mem.take<float>(nk);
+  }
+  double* part = mem.take<double>(static_cast<size_t>(p) * kKnnPartBlocks);
+  double* mean = mem.take<double>(p);
+  float* xc = mem.take<float>(static_cast<size_t>(n_pad) * P);
+  float* xr = mem.take<float>(np);
+  float* nrm = mem.take<float>(n_pad);
+  double* qn = mem.take<double>(n);
+  double* norm_part = mem.take<double>(pack_blocks);
+  double* cmax2 = mem.take<double>(1);
+  float* xqc = xc;
+  float* xqr = xr;
+  double* qqn = qn;
+  float* q_nrm = nullptr;
+  double* q_part = nullptr;
+  if (!self) {
+    xqc = mem.take<float>(static_cast<size_t>(nq) * P);
+    xqr = mem.take<float>(nqp);
+    q_nrm = mem.take<float>(nq);
+    qqn = mem.take<double>(nq);
+    q_part = mem.take<double>(qpack_blocks);
+  }
+  unsigned long long* lists = mem.take<unsigned long long>(static_cast<size_t>(nq) * KL);
+  int* failed = mem.take<int>(nq);
+  int* failed_count = mem.take<int>(1);
+
+  // the candidates' column means centre both matrices
+  knn_col_partials<<<dim3(kKnnPartBlocks, p), kKnnThreads, 0, s>>>(x, n, part);
+  FE_LAUNCH_CHECK("knn_col_partials");
+  knn_col_fold<<<(p + 63) / 64, 64, 0, s>>>(part, p, n, mean);
+  FE_LAUNCH_CHECK("knn_col_fold");
+  knn_pack<<<pack_blocks, kKnnThreads, 0, s>>>(x, n, n_pad, p, P, mean, xc, xr, nrm, qn, norm_part);
+  FE_LAUNCH_CHECK("knn_pack");
+  knn_norm_fold<<<1, kKnnThreads, 0, s>>>(norm_part, pack_blocks, cmax2);
+  FE_LAUNCH_CHECK("knn_norm_fold");
+  if (!self) {
+    knn_pack<<<qpack_blocks, kKnnThreads, 0, s>>>(xq_in, nq, nq, p, P, mean, xqc, xqr, q_nrm, qqn,
+                                                  q_part);
+    FE_LAUNCH_CHECK("knn_pack (queries)");
+  }
+  FE_CHECK(hipMemsetAsync(failed_count, 0, sizeof(int), s));
+  const int sf = self ? 1 : 0, fs = force_scan ? 1 : 0;
+  int rc = 0;
+  switch (KL) {
+    case 64:
+      rc = run_select_kl<64>(xc, nrm, n_pad, P, xqc, nq, lists, s) ||
+           run_refine<64>(lists, xqr, xr, p, P, qqn, cmax2, nq, sf, k, index_base, fs, o_idx,
+                          o_dist, failed, failed_count, s);
+      break;
+    case 128:
+      rc = run_select_kl<128>(xc, nrm, n_pad, P, xqc, nq, lists, s) ||
+           run_refine<128>(lists, xqr, xr, p, P, qqn, cmax2, nq, sf, k, index_base, fs, o_idx,
+                           o_dist, failed, failed_count, s);
+      break;
+    case 256:
+      rc = run_select_kl<256>(xc, nrm, n_pad, P, xqc, nq, lists, s) ||
+           run_refine<256>(lists, xqr, xr, p, P, qqn, cmax2, nq, sf, k, index_base, fs, o_idx,
+                           o_dist, failed, failed_count, s);
+      break;
+    default:
+      rc = run_select_kl<512>(xc, nrm, n_pad, P, xqc, nq, lists, s) ||
+           run_refine<512>(lists, xqr, xr, p, P, qqn, cmax2, nq, sf, k, index_base, fs, o_idx,
+                           o_dist, failed, failed_count, s);
+      break;
+  }
+  if (rc) return 1;
+  int h_failed = 0;
+  FE_CHECK(hipMemcpyAsync(&h_failed, failed_count, sizeof(int), hipMemcpyDeviceToHost, s));
+  FE_CHECK(hipStreamSynchronize(s));
+  if (h_failed > 0) {
+    knn_scan<<<h_failed, kKnnThreads, 0, s>>>(xqr, xr, n, p, failed, nq, sf, k, index_base, o_idx,
+                                              o_dist);
+    FE_LAUNCH_CHECK("knn_scan");
+  }
+  if (!device_outputs) {
+    FE_CHECK(hipMemcpyAsync(indices, o_idx, nk * sizeof(int), hipMemcpyDeviceToHost, s));
+    FE_CHECK(hipMemcpyAsync(distances, o_dist, nk * sizeof(float), hipMemcpyDeviceToHost, s));
+  }
+  FE_CHECK(hipStreamSynchronize(s));
+  if (scanned_rows) *scanned_rows = h_failed;
+  return 0;
 }
 
 }  // namespace
@@ -739,108 +900,28 @@ int fastembedr_hip_exact_knn(const float* data, int n, int p, int k, int device_
   if (p > 65535) return fail("exact KNN supports at most 65535 columns");
   if (static_cast<long long>(n) * std::max(p, k) > (1ll << 31) - 1)
     return fail("exact KNN supports at most 2^31 - 1 elements per array");
-  const int P = (p + 7) / 8 * 8;
-  const int n_pad = (n + kKnnTile - 1) / kKnnTile * kKnnTile;
-  const int KL = list_length(k);
-  const bool force_scan = [] {
-    const char* e = std::getenv("FASTEMBEDR_HIP_KNN_SCAN_ALL");
-    return e != nullptr && std::strcmp(e, "1") == 0;
-  }();
-  Stream stream;
-  if (stream.init()) return 1;
-  hipStream_t s = stream.get();
-  const size_t np = static_cast<size_t>(n) * p, nk = static_cast<size_t>(n) * k;
-  const int pack_blocks = (n_pad + kKnnThreads - 1) / kKnnThreads;
-  ArenaPlan ap;
-  if (!device_inputs) ap.add<float>(np);
-  if (!device_outputs) {
-    ap.add<int>(nk);
-    ap.add<float>(nk);
-  }
-  ap.add<double>(static_cast<size_t>(p) * kKnnPartBlocks);
-  ap.add<double>(p);
-  ap.add<float>(static_cast<size_t>(n_pad) * P);
-  ap.add<float>(np);
-  ap.add<float>(n_pad);
-  ap.add<double>(n);
-  ap.add<double>(pack_blocks);
-  ap.add<double>(1);
-  ap.add<unsigned long long>(static_cast<size_t>(n) * KL);
-  ap.add<int>(n);
-  ap.add<int>(1);
-  Arena mem;
-  if (mem.init(ap.bytes, "exact KNN")) return 1;
-  const float* x = data;
-  if (!device_inputs) {
-    float* up = mem.take<float>(np);
-    FE_CHECK(hipMemcpyAsync(up, data, np * sizeof(float), hipMemcpyHostToDevice, s));
-    x = up;
-  }
-  int* o_idx = indices;
-  float* o_dist = distances;
-  if (!device_outputs) {
-    o_idx = mem.take<int>(nk);
-    o_dist = mem.take<float>(nk);
-  }
-  double* part = mem.take<double>(static_cast<size_t>(p) * kKnnPartBlocks);
-  double* mean = mem.take<double>(p);
-  float* xc = mem.take<float>(static_cast<size_t>(n_pad) * P);
-  float* xr = mem.take<float>(np);
-  float* nrm = mem.take<float>(n_pad);
-  double* qn = mem.take<double>(n);
-  double* norm_part = mem.take<double>(pack_blocks);
-  double* cmax2 = mem.take<double>(1);
-  unsigned long long* lists = mem.take<unsigned long long>(static_cast<size_t>(n) * KL);
-  int* failed = mem.take<int>(n);
-  int* failed_count = mem.take<int>(1);
+  return exact_knn_run(data, n, data, n, p, k, true, device_inputs, index_base, indices, distances,
+                       device_outputs, scanned_rows);
+}
 
-  knn_col_partials<<<dim3(kKnnPartBlocks, p), kKnnThreads, 0, s>>>(x, n, part);
-  FE_LAUNCH_CHECK("knn_col_partials");
-  knn_col_fold<<<(p + 63) / 64, 64, 0, s>>>(part, p, n, mean);
-  FE_LAUNCH_CHECK("knn_col_fold");
-  knn_pack<<<pack_blocks, kKnnThreads, 0, s>>>(x, n, n_pad, p, P, mean, xc, xr, nrm, qn, norm_part);
-  FE_LAUNCH_CHECK("knn_pack");
-  knn_norm_fold<<<1, kKnnThreads, 0, s>>>(norm_part, pack_blocks, cmax2);
-  FE_LAUNCH_CHECK("knn_norm_fold");
-  FE_CHECK(hipMemsetAsync(failed_count, 0, sizeof(int), s));
-  int rc = 0;
-  switch (KL) {
-    case 64:
-      rc = run_select_kl<64>(xc, nrm, n_pad, P, n, lists, s) ||
-           run_refine<64>(lists, xr, n, p, P, qn, cmax2, k, index_base, force_scan, o_idx, o_dist,
-                          failed, failed_count, s);
-      break;
-    case 128:
-      rc = run_select_kl<128>(xc, nrm, n_pad, P, n, lists, s) ||
-           run_refine<128>(lists, xr, n, p, P, qn, cmax2, k, index_base, force_scan, o_idx, o_dist,
-                           failed, failed_count, s);
-      break;
-    case 256:
-      rc = run_select_kl<256>(xc, nrm, n_pad, P, n, lists, s) ||
-           run_refine<256>(lists, xr, n, p, P, qn, cmax2, k, index_base, force_scan, o_idx, o_dist,
-                           failed, failed_count, s);
-      break;
-    default:
-      rc = run_select_kl<512>(xc, nrm, n_pad, P, n, lists, s) ||
-           run_refine<512>(lists, xr, n, p, P, qn, cmax2, k, index_base, force_scan, o_idx, o_dist,
-                           failed, failed_count, s);
-      break;
-  }
-  if (rc) return 1;
-  int h_failed = 0;
-  FE_CHECK(hipMemcpyAsync(&h_failed, failed_count, sizeof(int), hipMemcpyDeviceToHost, s));
-  FE_CHECK(hipStreamSynchronize(s));
-  if (h_failed > 0) {
-    knn_scan<<<h_failed, kKnnThreads, 0, s>>>(xr, n, p, failed, k, index_base, o_idx, o_dist);
-    FE_LAUNCH_CHECK("knn_scan");
-  }
-  if (!device_outputs) {
-    FE_CHECK(hipMemcpyAsync(indices, o_idx, nk * sizeof(int), hipMemcpyDeviceToHost, s));
-    FE_CHECK(hipMemcpyAsync(distances, o_dist, nk * sizeof(float), hipMemcpyDeviceToHost, s));
-  }
-  FE_CHECK(hipStreamSynchronize(s));
-  if (scanned_rows) *scanned_rows = h_failed;
-  return 0;
+int fastembedr_hip_exact_query_knn(const float* data, int n, const float* query, int n_query,
+                                   int p, int k, int device_inputs, int index_base, int* indices,
+                                   float* distances, int device_outputs, int* scanned_rows) {
+  using namespace fe;
+  clear_error();
+  if (data == nullptr || query == nullptr || indices == nullptr || distances == nullptr)
+    return fail("null data or output pointer");
+  // native_cuda_query_knn_impl checks (src/native_knn_cuda_impl.cpp:894-906)
+  if (n < 1 || n_query < 1 || p < 1)
+    return fail("`data` and `query` must have at least one row and matching columns.");
+  if (k < 1 || k > n || k > kKnnMaxK)
+    return fail("exact query KNN requires k in [1, min(nrow(data), 256)]");
+  if (index_base != 0 && index_base != 1) return fail("index_base must be 0 or 1");
+  if (p > 65535) return fail("exact KNN supports at most 65535 columns");
+  if (static_cast<long long>(std::max(n, n_query)) * std::max(p, k) > (1ll << 31) - 1)
+    return fail("exact KNN supports at most 2^31 - 1 elements per array");
+  return exact_knn_run(data, n, query, n_query, p, k, false, device_inputs, index_base, indices,
+                       distances, device_outputs, scanned_rows);
 }
 
 }  // extern "C"

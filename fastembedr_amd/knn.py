@@ -55,3 +55,27 @@ def exact_knn_device(data, n, p, k, indices, distances, index_base=0):
         _device_ptr(data), int(n), int(p), int(k), 1, int(index_base), _device_ptr(indices),
         _device_ptr(distances), 1, ctypes.byref(scanned)), "exact KNN")
     return scanned.value
+
+
+def exact_query_knn(data, query, k, index_base=0):
+    """Exact k-NN of the rows of ``query`` (m x p) among the rows of ``data``
+    (n x p), no row dropped (the reference's exact query route).
+
+    Returns (indices, distances) as (k, m) arrays: column-major m x k,
+    indices into ``data`` with ``index_base``, float32 Euclidean distances.
+    """
+    x = np.asarray(data, dtype=np.float32)
+    q = np.asarray(query, dtype=np.float32)
+    if x.ndim != 2 or q.ndim != 2 or x.shape[1] != q.shape[1]:
+        raise ValueError("`data` and `query` must have at least one row and matching columns.")
+    n, p = x.shape
+    m = q.shape[0]
+    xc = np.asfortranarray(x).ravel(order="F")
+    qc = np.asfortranarray(q).ravel(order="F")
+    idx = np.zeros((max(k, 0), m), dtype=np.int32)
+    dist = np.zeros((max(k, 0), m), dtype=np.float32)
+    scanned = ctypes.c_int(0)
+    check(_lib.lib().fastembedr_hip_exact_query_knn(
+        xc.ctypes.data, int(n), qc.ctypes.data, int(m), int(p), int(k), 0, int(index_base),
+        idx.ctypes.data, dist.ctypes.data, 0, ctypes.byref(scanned)), "exact query KNN")
+    return idx, dist

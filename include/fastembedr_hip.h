@@ -347,6 +347,18 @@ int fastembedr_hip_exact_knn(const float* data, int n, int p, int k, int device_
                              int index_base, int* indices, float* distances, int device_outputs,
                              int* scanned_rows);
 
+/* Replaces the exact route of native_cuda_query_knn_impl
+ * (src/native_knn_cuda_impl.cpp:878-1010, FAISS bfKnn / cuVS brute force of
+ * the query rows against the data rows, then finalize_cuvs_query_knn_kernel,
+ * src/embedding_cuda_kernels.cpp:1482-1522): the k nearest data rows of
+ * every query row (no row dropped), 1 <= k <= min(n, 256).  data:
+ * column-major n x p, query: column-major n_query x p, float32; outputs
+ * column-major n_query x k, indices into data with index_base; exact as
+ * fastembedr_hip_exact_knn.  The query rows of the t-SNE transform. */
+int fastembedr_hip_exact_query_knn(const float* data, int n, const float* query, int n_query,
+                                   int p, int k, int device_inputs, int index_base, int* indices,
+                                   float* distances, int device_outputs, int* scanned_rows);
+
 #ifdef __cplusplus
 }
 #endif

@@ -99,6 +99,8 @@ def _declare(L):
         L.or_knn_structure_score.argtypes = [_dp, c_int, c_int, _ip, c_int, c_int, _ip, c_int,
                                              c_int, c_int, _ip, c_int, _dp]
         L.or_knn_structure_score.restype = c_int
+        L.or_exact_query_knn.argtypes = [_fp, c_int, _fp, c_int, c_int, c_int, _ip, _fp]
+        L.or_exact_query_knn.restype = c_int
 
 
 def _declare_umap(L):
@@ -329,6 +331,24 @@ def exact_knn(data, k, queries=None):
                             _ptr(idx, _ip), _ptr(dist, _fp))
     if rc != 0:
         raise ValueError("exact KNN needs n >= 2, p >= 1 and 1 <= k < n")
+    return idx, dist
+
+
+def exact_query_knn(data, query, k):
+    """Exact k-NN of the rows of ``query`` among the rows of ``data`` (no row
+    dropped): (indices, distances) row-major len(query) x k, 0-based."""
+    x = np.asarray(data, dtype=np.float32)
+    q = np.asarray(query, dtype=np.float32)
+    n, p = x.shape
+    m = q.shape[0]
+    xf = np.asfortranarray(x).ravel(order="F")
+    qf = np.asfortranarray(q).ravel(order="F")
+    idx = np.zeros((m, k), dtype=np.int32)
+    dist = np.zeros((m, k), dtype=np.float32)
+    rc = lib().or_exact_query_knn(_ptr(xf, _fp), n, _ptr(qf, _fp), m, p, int(k), _ptr(idx, _ip),
+                                  _ptr(dist, _fp))
+    if rc != 0:
+        raise ValueError("exact query KNN needs 1 <= k <= nrow(data)")
     return idx, dist
 
 

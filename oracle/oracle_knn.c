@@ -237,3 +237,51 @@ int or_knn_structure_score(const double* layout, int n, int dim, const int* indi
   free(cnt);
   return 0;
 }
+
+/* Exact k-NN of query rows among data rows (no row is dropped): the result
+ * the reference's exact query route approximates (native_cuda_query_knn_impl,
+ * src/native_knn_cuda_impl.cpp:878-1010, with finalize_cuvs_query_knn_kernel,
+ * src/embedding_cuda_kernels.cpp:1482-1522).  data: column-major n x p;
+ * query: column-major m x p; out: row-major m x k, 0-based. */
+int or_exact_query_knn(const float* data, int n, const float* query, int m, int p, int k,
+                       int* out_idx, float* out_dist) {
+  if (n < 1 || m < 1 || p < 1 || k < 1 || k > n) return 1;
+  double* d2 = (double*)malloc(sizeof(double) * (size_t)n);
+  or_nb* heap = (or_nb*)malloc(sizeof(or_nb) * (size_t)k);
+  if (!d2 || !heap) {
+    free(d2);
+    free(heap);
+    return 1;
+  }
+  for (int t = 0; t < m; ++t) {
+    for (int j = 0; j < n; ++j) d2[j] = 0.0;
+    for (int c = 0; c < p; ++c) {
+      const float* col = data + (size_t)c * n;
+      const double xq = (double)query[(size_t)c * m + t];
+      for (int j = 0; j < n; ++j) {
+        const double diff = (double)col[j] - xq;
+        d2[j] = d2[j] + diff * diff;
+      }
+    }
+    int h = 0;
+    for (int j = 0; j < n; ++j) {
+      or_nb e = {d2[j], j};
+      if (h < k) {
+        heap[h] = e;
+        or_sift_up(heap, h);
+        ++h;
+      } else if (or_nb_less(e, heap[0])) {
+        heap[0] = e;
+        or_sift_down(heap, h, 0);
+      }
+    }
+    qsort(heap, (size_t)h, sizeof(or_nb), or_nb_cmp);
+    for (int r = 0; r < k; ++r) {
+      out_idx[(size_t)t * k + r] = heap[r].j;
+      out_dist[(size_t)t * k + r] = (float)sqrt(heap[r].d);
+    }
+  }
+  free(d2);
+  free(heap);
+  return 0;
+}

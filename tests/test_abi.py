@@ -127,3 +127,19 @@ def test_structure_score_argument_errors_need_no_device():
         quality.knn_structure_score(np.zeros((10, 3)), ind, [1], 2)
     assert np.isnan(quality.knn_structure_score(y, ind, [], 2)["structure_score"])
     del ctypes
+
+
+def test_query_knn_argument_errors_need_no_device():
+    # native_cuda_query_knn_impl's checks (src/native_knn_cuda_impl.cpp:894-906)
+    import numpy as np
+
+    from fastembedr_amd import HipError, knn
+
+    x = np.zeros((10, 3), np.float32)
+    with pytest.raises(ValueError, match="matching columns"):
+        knn.exact_query_knn(x, np.zeros((4, 2), np.float32), 2)
+    for k in (0, 11):
+        with pytest.raises(HipError, match=r"k in \[1, min\(nrow\(data\), 256\)\]"):
+            knn.exact_query_knn(x, x[:2], k)
+    with pytest.raises(HipError, match="at least one row"):
+        knn.exact_query_knn(x, np.zeros((0, 3), np.float32), 1)

@@ -117,3 +117,35 @@ def test_knn_device_buffers():
         di.free()
         dd.free()
     assert np.array_equal(gi, hi) and np.array_equal(gd, hd)
+
+
+def test_query_knn_matches_oracle():
+    import ctypes
+
+    from fastembedr_amd import _lib, knn
+
+    rng = np.random.default_rng(908)
+    for n, m, p, k in ((2000, 700, 50, 90), (500, 333, 3, 15), (300, 40, 70, 300 - 45),
+                       (64, 10, 8, 64), (1, 3, 5, 1)):
+        data = (rng.normal(size=(n, p)) * 3.0).astype(np.float32)
+        query = (rng.normal(size=(m, p)) * 3.0).astype(np.float32)
+        print(f"[query knn] n={n} m={m} p={p} k={k}", flush=True)
+        gi, gd = knn.exact_query_knn(data, query, k, index_base=1)
+        oi, od = oracle.exact_query_knn(data, query, k)
+        assert np.array_equal(gi.T - 1, oi) and np.array_equal(gd.T, od)
+    # queries equal to data rows (distance 0) and duplicated data rows
+    data = rng.normal(size=(900, 6)).astype(np.float32)
+    data[200:500] = data[200]
+    query = np.concatenate([data[:100], data[200:210], rng.normal(size=(50, 6))]).astype(np.float32)
+    gi, gd = knn.exact_query_knn(data, query, 40)
+    oi, od = oracle.exact_query_knn(data, query, 40)
+    assert np.array_equal(gi.T, oi) and np.array_equal(gd.T, od)
+    col = np.asfortranarray(data).ravel(order="F")
+    qcol = np.asfortranarray(query).ravel(order="F")
+    idx = np.zeros((40, len(query)), np.int32)
+    dist = np.zeros((40, len(query)), np.float32)
+    scanned = ctypes.c_int(-1)
+    _lib.check(_lib.lib().fastembedr_hip_exact_query_knn(
+        col.ctypes.data, 900, qcol.ctypes.data, len(query), 6, 40, 0, 0, idx.ctypes.data,
+        dist.ctypes.data, 0, ctypes.byref(scanned)), "exact query KNN")
+    assert scanned.value >= 10 and np.array_equal(idx, gi) and np.array_equal(dist, gd)

@@ -48,3 +48,23 @@ def test_knn_oracle_known_answers():
 def test_knn_oracle_rejects_bad_sizes():
     with pytest.raises(ValueError):
         oracle.exact_knn(np.zeros((5, 2), np.float32), 5)
+
+
+def test_query_knn_oracle_matches_kdtree():
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(21)
+    data = rng.normal(size=(700, 9)).astype(np.float32)
+    query = rng.normal(size=(120, 9)).astype(np.float32)
+    idx, dist = oracle.exact_query_knn(data, query, 20)
+    d_tree, i_tree = cKDTree(data.astype(np.float64)).query(query.astype(np.float64), k=20)
+    assert np.array_equal(idx, i_tree) and np.allclose(dist, d_tree, rtol=1e-6, atol=0)
+    # queries that are data rows find themselves first, at distance 0; the
+    # rest is the self-excluded answer
+    qi, qd = oracle.exact_query_knn(data, data[:50], 11)
+    si, sd = oracle.exact_knn(data, 10, queries=np.arange(50))
+    assert np.array_equal(qi[:, 0], np.arange(50)) and np.all(qd[:, 0] == 0)
+    assert np.array_equal(qi[:, 1:], si) and np.array_equal(qd[:, 1:], sd)
+    # k = nrow(data)
+    ai, _ = oracle.exact_query_knn(data[:30], query[:5], 30)
+    assert sorted(ai[0]) == list(range(30))
