@@ -1211,32 +1211,26 @@ __device__ __forceinline__ void attract_range(const SRC& src, const BlockTables&
   bool head_open = true;       // no segment start seen yet
   int base = e0 & ~(kLaneStep - 1);
   LaneCols cur = lane_cols<SRC::kLocal>(t, base, lane);
+  // the step's values, loaded one step ahead (the segments are padded by
+  // kLanePad entries past their end, so the read past a range stays inside)
+  float4 va = stream_load(reinterpret_cast<const float4*>(t.val + base + kLaneEntries * lane));
+  float4 vb = stream_load(reinterpret_cast<const float4*>(t.val + base + kLaneEntries * lane + 4));
   for (; base < e1; base += kLaneStep) {
     const int eb = base + kLaneEntries * lane;
     const bool valid = eb >= e0 && eb < e1;
     // a lane that begins a segment closes the one below it
     const bool seen = valid && ((cur.starts >> lane) & 1ull);
     const int row = cur.row;
-    float v[kLaneEntries];
-    {
-      const float4 a = stream_load(reinterpret_cast<const float4*>(t.val + eb));
-      const float4 b = stream_load(reinterpret_cast<const float4*>(t.val + eb + 4));
-      v[0] = a.x;
-      v[1] = a.y;
-      v[2] = a.z;
-      v[3] = a.w;
-      v[4] = b.x;
-      v[5] = b.y;
-      v[6] = b.z;
-      v[7] = b.w;
-    }
+    const float v[kLaneEntries] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
     // the row's own point (adjacent rows: through the caches), the gathers,
-    // then the next step's columns and rows
+    // then the next step's columns, rows and values
     const float2 pr = src.y[row];
     typename SRC::Fetch fj[kLaneEntries];
 #pragma unroll
     for (int u = 0; u < kLaneEntries; ++u) fj[u] = src.fetch(cur.j[u]);
     cur = lane_cols<SRC::kLocal>(t, base + kLaneStep, lane);
+    va = stream_load(reinterpret_cast<const float4*>(t.val + eb + kLaneStep));
+    vb = stream_load(reinterpret_cast<const float4*>(t.val + eb + kLaneStep + 4));
     // keep every load of the step above the first wait
     __builtin_amdgcn_sched_barrier(0);
     const float xi = pr.x - mx, yi = pr.y - my;
