@@ -261,6 +261,28 @@ def bench_umap_graph(fe, knn_i, knn_d, n, kk, cpu):
     dev_s = time.perf_counter() - t0
     out = {"n": int(n), "k": int(kk), "nnz": g.nnz, "device_ms": round(1e3 * dev_s, 2),
            "device_points_per_s": round(n / dev_s, 1)}
+    # the UMAP epochs on the resident graph (U[-10, 10]^2 start, seed 301; 500
+    # epochs, 5 negatives per positive, seed 302; the deterministic mode)
+    n_epochs = 500
+    eps = g.export()["epochs_per_sample"].astype(np.float64)
+    init = np.random.default_rng(301).uniform(-10.0, 10.0, size=(n, 2)).astype(np.float32)
+    g.optimize(init, 2, negative_sample_rate=5, seed=302)  # warm-up
+    t0 = time.perf_counter()
+    y_umap = g.optimize(init, n_epochs, negative_sample_rate=5, learning_rate=1.0, seed=302,
+                        optimizer_mode=1)
+    ep_s = time.perf_counter() - t0
+    pos = float(np.floor(n_epochs / eps).sum())  # positive samples over the run
+    epoch_bytes = 16 * g.nnz + 16 * n
+    out["epochs"] = {"epochs": n_epochs, "seconds": round(ep_s, 4),
+                     "epochs_per_s": round(n_epochs / ep_s, 1),
+                     "positive_samples_per_s": round(pos / ep_s, 1),
+                     "sampled_edges_per_s": round(pos * 6 / ep_s, 1),
+                     "roofline": {"bound": "hbm", "bytes_per_epoch": int(epoch_bytes),
+                                  "achieved_gbs": round(epoch_bytes * n_epochs / ep_s / 1e9, 1),
+                                  "frac": round(epoch_bytes * n_epochs / ep_s / 1e9 / HBM_PEAK_GBS,
+                                                4)},
+                     "layout_finite": bool(np.isfinite(y_umap).all()),
+                     "note": "host start layout in and layout out included"}
     g.close()
     if cpu:
         import oracle
@@ -279,6 +301,22 @@ def bench_umap_graph(fe, knn_i, knn_d, n, kk, cpu):
                     "cpu_points_per_s": round(m / cpu_s, 1), "cpu_cores": 1,
                     "cpu_kind": "oracle restatement of umap_graph_csr_float_cpp, 1 thread",
                     "sample_matches_cpu": bool(same)})
+        # the CPU restatement of the epoch on the 100k-point slice's graph
+        off = ref["offsets"]
+        hh = np.repeat(np.arange(m, dtype=np.int32), np.diff(off))
+        tt = np.asarray(ref["neighbors"], np.int32)
+        pp = np.asarray(ref["epochs_per_sample"], np.float32)
+        ys = np.random.default_rng(301).uniform(-10.0, 10.0, size=(m, 2)).astype(np.float32)
+        cpu_epochs = 5
+        t0 = time.perf_counter()
+        oracle.umap_optimize(hh, tt, pp, ys, cpu_epochs, nsr=5, lr=1.0, seed=302, mode=1)
+        ce = time.perf_counter() - t0
+        cpos = float(np.floor(cpu_epochs / pp.astype(np.float64)).sum())
+        out["epochs"].update({"cpu_sample_points": int(m), "cpu_epochs": cpu_epochs,
+                              "cpu_s": round(ce, 3),
+                              "cpu_positive_samples_per_s": round(cpos / ce, 1), "cpu_cores": 1,
+                              "cpu_kind": "oracle restatement of the epoch (order-free delta "
+                                          "form), 1 thread"})
     log(f"[bench] umap graph {out}")
     return out
 

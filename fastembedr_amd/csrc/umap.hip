@@ -178,6 +178,99 @@ __global__ __launch_bounds__(kUThreads) void umap_epoch_kernel(float* layout,
   }
 }
 
+__device__ __forceinline__ unsigned long long shfl_up_u64(unsigned long long v, int d) {
+  const int lo = __shfl_up(static_cast<int>(static_cast<unsigned>(v)), d, 64);
+  const int hi = __shfl_up(static_cast<int>(static_cast<unsigned>(v >> 32)), d, 64);
+  return (static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) |
+         static_cast<unsigned>(lo);
+}
+
+// optimizer_mode 1 with the head updates summed before they reach memory.
+// The epoch reads its start layout, so an edge's positive samples are all the
+// same update (npos times the fixed-point value); the head's share of an edge
+// (positives and negatives) is summed in a register, and the lanes of a wave
+// holding consecutive edges of one head (COO heads in row order) add theirs
+// by a segmented scan, one atomic pair per head run.  The tail's share goes
+// straight to memory.  Integer sums: the same delta as one atomic per sample.
+__global__ __launch_bounds__(kUThreads) void umap_epoch_grouped(unsigned long long* delta,
+                                                                const float* __restrict__ layout,
+                                                                const int* __restrict__ heads,
+                                                                const int* __restrict__ tails,
+                                                                const float* __restrict__ period,
+                                                                int n_edges, UmapParams p,
+                                                                unsigned epoch) {
+  const float alpha = p.lr * (1.0f - __fdiv_rn(static_cast<float>(epoch),
+                                               fmaxf(1.0f, static_cast<float>(p.n_epochs))));
+  constexpr float eps = 1.1920928955078125e-7f;
+  const int lane = threadIdx.x & 63;
+  const unsigned long long upto_lane = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  for (int base = blockIdx.x * kUThreads + (threadIdx.x & ~63); base < n_edges;
+       base += gridDim.x * kUThreads) {
+    const int e = base + lane;
+    int h = -1;
+    long long hx = 0, hy = 0;
+    if (e < n_edges) {
+      h = heads[e];
+      const int t = tails[e];
+      if (h >= 0 && h < p.n && t >= 0 && t < p.n && h != t) {
+        const float per = period[e];
+        const int npos = n_positive(per, epoch);
+        if (npos > 0) {
+          const size_t hb = static_cast<size_t>(h) * 2, tb = static_cast<size_t>(t) * 2;
+          const float hxv = layout[hb], hyv = layout[hb + 1];
+          {
+            const float dx = hxv - layout[tb];
+            const float dy = hyv - layout[tb + 1];
+            const float d2 = fmaxf(eps, dx * dx + dy * dy);
+            const float c = attract(d2, p);
+            const float gx = clip4(c * dx) * alpha, gy = clip4(c * dy) * alpha;
+            const long long fx = npos * fixed(gx), fy = npos * fixed(gy);
+            hx += fx;
+            hy += fy;
+            atomicAdd(delta + tb, static_cast<unsigned long long>(-fx));
+            atomicAdd(delta + tb + 1, static_cast<unsigned long long>(-fy));
+          }
+          const int nneg = n_negative(per, p.nsr, epoch);
+          for (int s = 0; s < nneg; ++s) {
+            const unsigned v = negative_vertex(static_cast<unsigned>(p.n), p.seed, epoch,
+                                               static_cast<unsigned>(h), static_cast<unsigned>(t),
+                                               static_cast<unsigned>(s));
+            if (static_cast<int>(v) == h || static_cast<int>(v) == t) continue;
+            const size_t vb = static_cast<size_t>(v) * 2;
+            const float dx = hxv - layout[vb];
+            const float dy = hyv - layout[vb + 1];
+            const float d2 = fmaxf(eps, dx * dx + dy * dy);
+            const float c = repel(d2, p);
+            hx += fixed(clip4(c * dx) * alpha);
+            hy += fixed(clip4(c * dy) * alpha);
+          }
+        }
+      }
+    }
+    // segmented inclusive scan over the runs of equal heads
+    const int hprev = __shfl_up(h, 1, 64);
+    const bool start = lane == 0 || hprev != h;
+    const unsigned long long starts = __ballot(start) & upto_lane;
+    const int lim = 63 - __clzll(starts);  // first lane of this lane's run
+    unsigned long long ux = static_cast<unsigned long long>(hx);
+    unsigned long long uy = static_cast<unsigned long long>(hy);
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const unsigned long long ox = shfl_up_u64(ux, d), oy = shfl_up_u64(uy, d);
+      if (lane - d >= lim) {
+        ux += ox;
+        uy += oy;
+      }
+    }
+    const int hnext = __shfl_down(h, 1, 64);
+    const bool last = lane == 63 || hnext != h;
+    if (last && h >= 0 && h < p.n && (ux != 0ull || uy != 0ull)) {
+      atomicAdd(delta + static_cast<size_t>(h) * 2, ux);
+      atomicAdd(delta + static_cast<size_t>(h) * 2 + 1, uy);
+    }
+  }
+}
+
 // sanitise (:3227-3258); MODE 1 first adds and clears the fixed-point delta
 template <int MODE>
 __global__ __launch_bounds__(kUThreads) void umap_apply_kernel(float* layout,
@@ -271,9 +364,9 @@ static int umap_run(const int* heads, const int* tails, const float* weights,
       FE_LAUNCH_CHECK("umap_epoch_kernel");
       umap_apply_kernel<0><<<rblocks, kUThreads, 0, s>>>(layout, delta, static_cast<int>(m));
     } else {
-      umap_epoch_kernel<1><<<eblocks, kUThreads, 0, s>>>(layout, delta, dh, dt, dp, n_edges, p,
-                                                         static_cast<unsigned>(epoch));
-      FE_LAUNCH_CHECK("umap_epoch_kernel");
+      umap_epoch_grouped<<<eblocks, kUThreads, 0, s>>>(delta, layout, dh, dt, dp, n_edges, p,
+                                                       static_cast<unsigned>(epoch));
+      FE_LAUNCH_CHECK("umap_epoch_grouped");
       umap_apply_kernel<1><<<rblocks, kUThreads, 0, s>>>(layout, delta, static_cast<int>(m));
     }
     FE_LAUNCH_CHECK("umap_apply_kernel");

@@ -44,6 +44,20 @@ def test_umap_delta_mode_matches_oracle_and_is_deterministic():
     assert np.array_equal(a, b)
 
 
+def test_umap_delta_mode_independent_of_edge_order():
+    # the delta is an integer sum of per-edge updates: any edge order (heads
+    # sorted or not, so any grouping of the head sums) gives the same bits
+    from fastembedr_amd import umap
+
+    heads, tails, w, per, _ = _graph(900, 12, 14)
+    y0 = np.random.default_rng(15).uniform(-10, 10, size=(900, 2)).astype(np.float32)
+    a = umap.optimize_coo(heads, tails, w, per, y0, 25, seed=9, optimizer_mode=1)
+    perm = np.random.default_rng(16).permutation(len(heads))
+    b = umap.optimize_coo(heads[perm], tails[perm], w[perm], per[perm], y0, 25, seed=9,
+                          optimizer_mode=1)
+    assert np.array_equal(a, b)
+
+
 def test_umap_end_to_end_quality():
     from fastembedr_amd import umap
 
