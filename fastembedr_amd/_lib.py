@@ -83,6 +83,8 @@ SIGNATURES = {
     "fastembedr_hip_umap_graph_export": (c_int, [_vp, _ip, _ip, _fp, _fp]),
     "fastembedr_hip_umap_graph_device": (c_int, [_vp, _vpp, _vpp, _vpp, _vpp]),
     "fastembedr_hip_umap_graph_destroy": (None, [_vp]),
+    "fastembedr_hip_umap_graph_optimize": (
+        c_int, [_vp, _fp, c_int, c_int, c_float, c_float, c_float, c_float, c_uint, c_int, _fp]),
     "fastembedr_hip_exact_structure_metrics": (
         c_int, [_dp, c_int, c_int, _dp, c_int, _ip, c_int, _dp]),
     "fastembedr_hip_knn_structure_score": (

@@ -192,6 +192,11 @@ __device__ __forceinline__ unsigned long long shfl_up_u64(unsigned long long v, 
 // holding consecutive edges of one head (COO heads in row order) add theirs
 // by a segmented scan, one atomic pair per head run.  The tail's share goes
 // straight to memory.  Integer sums: the same delta as one atomic per sample.
+// SYM: every edge (h, t) has its reverse (t, h) with the same period (the
+// symmetric fuzzy graph); the reverse edge's positive update of t is exactly
+// this edge's tail update (the same d2, the difference negated, a symmetric
+// clip and rounding), so each edge adds twice its head update and no tail's.
+template <bool SYM>
 __global__ __launch_bounds__(kUThreads) void umap_epoch_grouped(unsigned long long* delta,
                                                                 const float* __restrict__ layout,
                                                                 const int* __restrict__ heads,
@@ -225,10 +230,15 @@ __global__ __launch_bounds__(kUThreads) void umap_epoch_grouped(unsigned long lo
             const float c = attract(d2, p);
             const float gx = clip4(c * dx) * alpha, gy = clip4(c * dy) * alpha;
             const long long fx = npos * fixed(gx), fy = npos * fixed(gy);
-            hx += fx;
-            hy += fy;
-            atomicAdd(delta + tb, static_cast<unsigned long long>(-fx));
-            atomicAdd(delta + tb + 1, static_cast<unsigned long long>(-fy));
+            if constexpr (SYM) {
+              hx += 2 * fx;
+              hy += 2 * fy;
+            } else {
+              hx += fx;
+              hy += fy;
+              atomicAdd(delta + tb, static_cast<unsigned long long>(-fx));
+              atomicAdd(delta + tb + 1, static_cast<unsigned long long>(-fy));
+            }
           }
           const int nneg = n_negative(per, p.nsr, epoch);
           for (int s = 0; s < nneg; ++s) {
@@ -312,10 +322,12 @@ __global__ void umap_schedule_kernel(const int* __restrict__ heads, const int* _
 
 }  // namespace
 
-static int umap_run(const int* heads, const int* tails, const float* weights,
-                    const float* epochs_per_sample, const float* init, int n, int n_edges,
-                    int n_epochs, int nsr, float lr, float a, float b, float gamma, unsigned seed,
-                    int mode, float* out, bool device_inputs) {
+// symmetric: the COO is a symmetric graph (see umap_epoch_grouped); only
+// the resident fuzzy graph passes it
+int umap_run(const int* heads, const int* tails, const float* weights,
+             const float* epochs_per_sample, const float* init, int n, int n_edges, int n_epochs,
+             int nsr, float lr, float a, float b, float gamma, unsigned seed, int mode, float* out,
+             bool device_inputs, bool symmetric) {
   clear_error();
   if (!heads || !tails || !weights || !epochs_per_sample || !init || !out)
     return fail("null host pointer");
@@ -364,8 +376,12 @@ static int umap_run(const int* heads, const int* tails, const float* weights,
       FE_LAUNCH_CHECK("umap_epoch_kernel");
       umap_apply_kernel<0><<<rblocks, kUThreads, 0, s>>>(layout, delta, static_cast<int>(m));
     } else {
-      umap_epoch_grouped<<<eblocks, kUThreads, 0, s>>>(delta, layout, dh, dt, dp, n_edges, p,
-                                                       static_cast<unsigned>(epoch));
+      if (symmetric)
+        umap_epoch_grouped<true><<<eblocks, kUThreads, 0, s>>>(delta, layout, dh, dt, dp, n_edges,
+                                                               p, static_cast<unsigned>(epoch));
+      else
+        umap_epoch_grouped<false><<<eblocks, kUThreads, 0, s>>>(delta, layout, dh, dt, dp, n_edges,
+                                                                p, static_cast<unsigned>(epoch));
       FE_LAUNCH_CHECK("umap_epoch_grouped");
       umap_apply_kernel<1><<<rblocks, kUThreads, 0, s>>>(layout, delta, static_cast<int>(m));
     }
@@ -413,7 +429,7 @@ int fastembedr_hip_umap_optimize_coo(const int* heads, const int* tails, const f
                                      int optimizer_mode, float* out) {
   return fe::umap_run(heads, tails, weights, epochs_per_sample, init, n, n_edges, n_epochs,
                       negative_sample_rate, learning_rate, a, b, repulsion_strength, seed,
-                      optimizer_mode, out, false);
+                      optimizer_mode, out, false, false);
 }
 
 int fastembedr_hip_umap_optimize_coo_device(const int* heads, const int* tails,
@@ -424,7 +440,7 @@ int fastembedr_hip_umap_optimize_coo_device(const int* heads, const int* tails,
                                             int optimizer_mode, float* out) {
   return fe::umap_run(heads, tails, weights, epochs_per_sample, init, n, n_edges, n_epochs,
                       negative_sample_rate, learning_rate, a, b, repulsion_strength, seed,
-                      optimizer_mode, out, true);
+                      optimizer_mode, out, true, false);
 }
 
 int fastembedr_hip_umap_schedule(const int* heads, const int* tails,

@@ -519,6 +519,13 @@ int UmapGraph::build(const int* indices, const float* distances, int n_, int k_t
 
 }  // namespace fe
 
+namespace fe {
+int umap_run(const int* heads, const int* tails, const float* weights,
+             const float* epochs_per_sample, const float* init, int n, int n_edges, int n_epochs,
+             int nsr, float lr, float a, float b, float gamma, unsigned seed, int mode, float* out,
+             bool device_inputs, bool symmetric);  // umap.hip
+}  // namespace fe
+
 struct fastembedr_hip_umap_graph {
   fe::UmapGraph g;
 };
@@ -587,6 +594,23 @@ int fastembedr_hip_umap_graph_device(fastembedr_hip_umap_graph* graph, const int
   if (weights) *weights = graph->g.weights;
   if (epochs_per_sample) *epochs_per_sample = graph->g.eps;
   return 0;
+}
+
+int fastembedr_hip_umap_graph_optimize(fastembedr_hip_umap_graph* graph, const float* init,
+                                       int n_epochs, int negative_sample_rate,
+                                       float learning_rate, float a, float b,
+                                       float repulsion_strength, unsigned int seed,
+                                       int optimizer_mode, float* out) {
+  fe::clear_error();
+  if (!graph) return fe::fail("null graph");
+  const auto& g = graph->g;
+  if (g.nnz < 1) return fe::fail("the fuzzy graph has no edges");
+  // the graph is symmetric with equal weights both ways (f + r - f*r of a
+  // row's forward and reverse memberships is the same float either way), so
+  // the deterministic mode applies each pair's attraction through its heads
+  return fe::umap_run(g.heads, g.nbrs, g.weights, g.eps, init, g.n, static_cast<int>(g.nnz),
+                      n_epochs, negative_sample_rate, learning_rate, a, b, repulsion_strength,
+                      seed, optimizer_mode, out, true, true);
 }
 
 void fastembedr_hip_umap_graph_destroy(fastembedr_hip_umap_graph* graph) { delete graph; }

@@ -149,6 +149,20 @@ class FuzzyGraph:
                  a=A_MIN_DIST_001, b=B_MIN_DIST_001, repulsion_strength=1.0, seed=0,
                  optimizer_mode=1):
         """UMAP epochs on the resident graph; returns the float32 n x 2 layout."""
+        y = np.ascontiguousarray(init, dtype=np.float32)
+        out = np.zeros_like(y)
+        rc = _lib.lib().fastembedr_hip_umap_graph_optimize(
+            self._h, ptr(y, _fp), int(n_epochs), int(negative_sample_rate), float(learning_rate),
+            float(a), float(b), float(repulsion_strength), int(seed) & 0xFFFFFFFF,
+            int(optimizer_mode), ptr(out, _fp))
+        check(rc, "HIP UMAP optimizer (resident graph)")
+        return out
+
+    def optimize_coo_device(self, init, n_epochs, negative_sample_rate=5, learning_rate=1.0,
+                            a=A_MIN_DIST_001, b=B_MIN_DIST_001, repulsion_strength=1.0, seed=0,
+                            optimizer_mode=1):
+        """The same epochs through the general device-COO entry point (no use
+        of the graph's symmetry)."""
         ptrs = [ctypes.c_void_p() for _ in range(4)]
         L = _lib.lib()
         check(L.fastembedr_hip_umap_graph_device(self._h, *[ctypes.byref(p) for p in ptrs]),

@@ -259,6 +259,19 @@ int fastembedr_hip_umap_graph_device(fastembedr_hip_umap_graph* graph, const int
                                      const int** tails, const float** weights,
                                      const float** epochs_per_sample);
 
+/* UMAP epochs on the resident graph: the arguments and result of
+ * fastembedr_hip_umap_optimize_coo on the graph's COO (its heads, neighbours,
+ * weights and epochs_per_sample).  The graph is symmetric with equal weights
+ * both ways, which the deterministic mode (optimizer_mode 1) uses to apply
+ * each pair's attraction through the heads alone (the same bits as the COO
+ * entry point).  No reference counterpart (the reference runs the epochs
+ * from a host graph, umap_cuda_optimize_csr_cpp). */
+int fastembedr_hip_umap_graph_optimize(fastembedr_hip_umap_graph* graph, const float* init,
+                                       int n_epochs, int negative_sample_rate,
+                                       float learning_rate, float a, float b,
+                                       float repulsion_strength, unsigned int seed,
+                                       int optimizer_mode, float* out);
+
 void fastembedr_hip_umap_graph_destroy(fastembedr_hip_umap_graph* graph);
 
 /* ---- fixed-reference t-SNE transform ------------------------------------

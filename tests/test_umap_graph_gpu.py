@@ -73,3 +73,7 @@ def test_fuzzy_graph_device_inputs_and_resident_epochs():
                            seed=5, optimizer_mode=1)
     assert np.array_equal(r1, r2)
     assert np.isfinite(r1).all()
+    # the symmetric form (attraction through the heads) against the general
+    # device-COO path on the same resident graph
+    r3 = dev.optimize_coo_device(y0, 30, seed=5, optimizer_mode=1)
+    assert np.array_equal(r1, r3)
