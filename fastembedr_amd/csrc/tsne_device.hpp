@@ -307,22 +307,15 @@ __device__ __forceinline__ int point_bin(const Cell& c, int ts) {
   return 4 * (ty * ts + tx) + cls;
 }
 
-// The four bin ranges that feed tile t (own, left, upper, upper-left);
-// missing neighbours give empty ranges.
-__device__ __forceinline__ void tile_ranges(const int* bs, int t, int ts, int* b0, int* b1) {
+// Bin range r of the four that feed tile t (0 own, 1 left, 2 upper,
+// 3 upper-left) as [x, y); a missing neighbour gives an empty range.
+__device__ __forceinline__ int2 tile_range(const int* bs, int t, int ts, int r) {
   const int tx = t % ts, ty = t / ts;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int dx = (r & 1) ? 1 : 0, dy = (r & 2) ? 1 : 0;
-    const int lo = r == 0 ? 0 : (r == 1 ? 1 : 2), hi = r == 0 ? 4 : (r == 2 ? 4 : 3);
-    if (tx < dx || ty < dy) {
-      b0[r] = b1[r] = 0;
-    } else {
-      const int sb = 4 * ((ty - dy) * ts + (tx - dx));
-      b0[r] = bs[sb + lo];
-      b1[r] = bs[sb + hi];
-    }
-  }
+  const int dx = r & 1, dy = r >> 1;
+  if (tx < dx || ty < dy) return make_int2(0, 0);
+  const int lo = r == 0 ? 0 : (r == 1 ? 1 : 2), hi = r == 0 ? 4 : (r == 2 ? 4 : 3);
+  const int sb = 4 * ((ty - dy) * ts + (tx - dx));
+  return make_int2(bs[sb + lo], bs[sb + hi]);
 }
 
 // points [beg, end) of block blk of nblk
@@ -434,9 +427,12 @@ __device__ __forceinline__ void splat_prep_body(SplatBuffers sb, int g) {
   __syncthreads();
   int items = 0;
   if (t < tiles) {
-    int r0[4], r1[4];
-    tile_ranges(bs, t, ts, r0, r1);
-    const int pts = (r1[0] - r0[0]) + (r1[1] - r0[1]) + (r1[2] - r0[2]) + (r1[3] - r0[3]);
+    int pts = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int2 rg = tile_range(bs, t, ts, r);
+      pts += rg.y - rg.x;
+    }
     items = max(1, (pts + kSplatItem - 1) / kSplatItem);
   }
   int all;
@@ -518,10 +514,10 @@ __device__ __forceinline__ void splat_tiles_body(const TsneStats* __restrict__ s
   }
   const int tile = lo;
   const int first = sb.tile_items[tile], nitems = sb.tile_items[tile + 1] - first;
-  int b0[4], b1[4];
-  tile_ranges(sb.bin_start, tile, ts, b0, b1);
-  const int len0 = b1[0] - b0[0], len1 = b1[1] - b0[1], len2 = b1[2] - b0[2];
-  const int pts = len0 + len1 + len2 + (b1[3] - b0[3]);
+  const int2 g0 = tile_range(sb.bin_start, tile, ts, 0), g1 = tile_range(sb.bin_start, tile, ts, 1);
+  const int2 g2 = tile_range(sb.bin_start, tile, ts, 2), g3 = tile_range(sb.bin_start, tile, ts, 3);
+  const int len0 = g0.y - g0.x, len1 = g1.y - g1.x, len2 = g2.y - g2.x;
+  const int pts = len0 + len1 + len2 + (g3.y - g3.x);
   const int per = (pts + nitems - 1) / nitems;
   const int s0 = min(pts, (item - first) * per), s1 = min(pts, s0 + per);
   const int ox = (tile % ts) * kTile, oy = (tile / ts) * kTile;
@@ -536,10 +532,10 @@ __device__ __forceinline__ void splat_tiles_body(const TsneStats* __restrict__ s
   for (int v = s0 + threadIdx.x; v < s1; v += kThreads) {
     // position v of the concatenated ranges own ++ left ++ upper ++ upper-left
     int idx;
-    if (v < len0) idx = b0[0] + v;
-    else if (v < len0 + len1) idx = b0[1] + (v - len0);
-    else if (v < len0 + len1 + len2) idx = b0[2] + (v - len0 - len1);
-    else idx = b0[3] + (v - len0 - len1 - len2);
+    if (v < len0) idx = g0.x + v;
+    else if (v < len0 + len1) idx = g1.x + (v - len0);
+    else if (v < len0 + len1 + len2) idx = g2.x + (v - len0 - len1);
+    else idx = g3.x + (v - len0 - len1 - len2);
     const float2 p = sb.binned[idx];
     const Cell c = grid_cell(p.x, p.y, lx, ly, inv, g);
     const int key = c.y0 * g + c.x0;
@@ -1521,19 +1517,12 @@ __device__ __forceinline__ int seg_slot(const int* __restrict__ group_row, int g
   return kPasses * rb + k * rows + (r - rb);
 }
 
-// entries [a[k], b[k]) of row r in window k (empty when the window is absent)
-__device__ __forceinline__ void row_windows(const int* __restrict__ col, int s, int e,
-                                            const int2* __restrict__ win, int* a, int* b) {
-#pragma unroll
-  for (int k = 0; k < kMaxWindows; ++k) {
-    const int2 w = win[k];
-    if (w.y > 0) {
-      a[k] = lower_col(col, s, e, w.x);
-      b[k] = lower_col(col, a[k], e, w.x + w.y);
-    } else {
-      a[k] = b[k] = s;
-    }
-  }
+// entries [x, y) of a row's column range [s, e) that fall in window w
+// (empty when the window is absent)
+__device__ __forceinline__ int2 row_window(const int* __restrict__ col, int s, int e, int2 w) {
+  if (w.y <= 0) return make_int2(s, s);
+  const int a = lower_col(col, s, e, w.x);
+  return make_int2(a, lower_col(col, a, e, w.x + w.y));
 }
 
 // padded length of every segment (one thread per row)
@@ -1551,13 +1540,15 @@ __global__ __launch_bounds__(kThreads) void blk_seg_lengths(const int* __restric
   }
   const int g = group_of_row(group_row, groups, r);
   const int s = row_ptr[r], e = row_ptr[r + 1];
-  int a[kMaxWindows], b[kMaxWindows];
-  row_windows(col, s, e, window + g * kMaxWindows, a, b);
+  int2 ab[kMaxWindows];
+#pragma unroll
+  for (int k = 0; k < kMaxWindows; ++k) ab[k] = row_window(col, s, e, window[g * kMaxWindows + k]);
   int rest = e - s;
+#pragma unroll
   for (int k = 0; k < kPasses; ++k) {
     int l;
     if (k < kMaxWindows) {
-      l = b[k] - a[k];
+      l = ab[k].y - ab[k].x;
       rest -= l;
     } else {
       l = rest;
@@ -1584,9 +1575,11 @@ __global__ __launch_bounds__(kThreads) void blk_fill(const int* __restrict__ row
   if (r >= n) return;
   const int g = group_of_row(group_row, groups, r);
   const int s = row_ptr[r], e = row_ptr[r + 1];
-  int a[kMaxWindows], b[kMaxWindows];
-  row_windows(col, s, e, window + g * kMaxWindows, a, b);
+  int2 ab[kMaxWindows];
+#pragma unroll
+  for (int k = 0; k < kMaxWindows; ++k) ab[k] = row_window(col, s, e, window[g * kMaxWindows + k]);
   int rest_out = 0;  // entries of the last segment written so far
+#pragma unroll
   for (int k = 0; k < kPasses; ++k) {
     const int slot = seg_slot(group_row, g, k, r);
     const int o = off[slot], plen = off[slot + 1] - o;
@@ -1594,14 +1587,14 @@ __global__ __launch_bounds__(kThreads) void blk_fill(const int* __restrict__ row
     if (lane == 0) atomicOr(starts + o / kLaneStep, 1ull << ((o / kLaneEntries) & 63));
     for (int q = lane; q < plen / kLaneEntries; q += 64) lane_row[o / kLaneEntries + q] = r;
     if (k < kMaxWindows) {
-      const int l = b[k] - a[k];
+      const int l = ab[k].y - ab[k].x;
       const int lo = window[g * kMaxWindows + k].x;
       for (int q = lane; q < plen; q += 64) {
         const bool real = q < l;
-        const int j = real ? col[a[k] + q] : r;
+        const int j = real ? col[ab[k].x + q] : r;
         pcol[o + q] = j;
         pcol16[o + q] = static_cast<unsigned short>(real ? j - lo : 0);
-        pval[o + q] = real ? val[a[k] + q] : 0.0f;
+        pval[o + q] = real ? val[ab[k].x + q] : 0.0f;
       }
     } else {
       // the entries outside every window, in column order
@@ -1610,7 +1603,7 @@ __global__ __launch_bounds__(kThreads) void blk_fill(const int* __restrict__ row
         bool out = q < e;
         if (out) {
 #pragma unroll
-          for (int w = 0; w < kMaxWindows; ++w) out = out && (q < a[w] || q >= b[w]);
+          for (int w = 0; w < kMaxWindows; ++w) out = out && (q < ab[w].x || q >= ab[w].y);
         }
         const unsigned long long m = __ballot(out);
         if (out) {
