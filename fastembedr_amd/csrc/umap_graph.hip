@@ -13,7 +13,10 @@
 //                     every kept directed membership w = exp(-(d - rho)/sigma)
 //                     (1 at d <= rho; float arithmetic, :1608-1634) goes into
 //                     both endpoint rows, tagged forward or reverse
-//   ug_sort_union     one block per row: LDS bitonic sort of (neighbour,
+//   ug_sort_union_short / ug_sort_union
+//                     one wave per row of at most 64 entries (a bitonic sort
+//                     across the lanes), one block per longer row (LDS
+//                     bitonic sort) of (neighbour,
 //                     direction, weight bits); per neighbour the largest
 //                     forward and reverse weights (the reference's max-dedupe
 //                     of :1291-1364 and its incoming lists) give
@@ -245,15 +248,30 @@ __global__ __launch_bounds__(kUgThreads) void ug_fill(const int* __restrict__ id
   }
 }
 
-// One block per row: sort, then one union weight per neighbour, written to
-// the front of the row's raw range as (neighbour << 32 | weight bits).
-__global__ __launch_bounds__(kUgThreads) void ug_sort_union(unsigned long long* __restrict__ raw,
-                                                            const int* __restrict__ raw_ptr,
-                                                            int* __restrict__ union_len,
-                                                            unsigned* error_flags) {
+__device__ __forceinline__ bool ug_kept(float w) { return isfinite(w) && w > 1.0e-6f; }
+
+// the union weight f + r - f*r of the neighbour group that starts at sorted
+// entry t (f, r: the largest forward and reverse weights of the group)
+__device__ __forceinline__ float ug_group_weight(const unsigned long long* keys, int t, int len) {
+  const unsigned long long nb = keys[t] >> 32;
+  float f = 0.0f, r = 0.0f;
+  for (int u = t; u < len && (keys[u] >> 32) == nb; ++u) {
+    const float w = __uint_as_float(static_cast<unsigned>(keys[u] & 0x7fffffffu));
+    if (keys[u] & kUgReverse) r = fmaxf(r, w);
+    else f = fmaxf(f, w);
+  }
+  return __fsub_rn(__fadd_rn(f, r), __fmul_rn(f, r));
+}
+
+// One row by one block: LDS bitonic sort of up to kSortCap entries, then
+// one union weight per neighbour, written to the front of the row's raw range
+// as (neighbour << 32 | weight bits).
+__device__ __forceinline__ void sort_union_long(unsigned long long* __restrict__ raw,
+                                                const int* __restrict__ raw_ptr, int row,
+                                                int* __restrict__ union_len,
+                                                unsigned* error_flags) {
   __shared__ unsigned long long keys[kSortCap];
   __shared__ int flag_scan[kUgThreads];
-  const int row = blockIdx.x;
   const int beg = raw_ptr[row], len = raw_ptr[row + 1] - beg;
   if (len > kSortCap) {
     if (threadIdx.x == 0) {
@@ -285,20 +303,9 @@ __global__ __launch_bounds__(kUgThreads) void ug_sort_union(unsigned long long* 
   constexpr int CH = kSortCap / kUgThreads;
   const int c0 = threadIdx.x * CH;
   auto starts_group = [&](int t) { return t == 0 || (keys[t] >> 32) != (keys[t - 1] >> 32); };
-  auto group_weight = [&](int t) {
-    const unsigned long long nb = keys[t] >> 32;
-    float f = 0.0f, r = 0.0f;
-    for (int u = t; u < len && (keys[u] >> 32) == nb; ++u) {
-      const float w = __uint_as_float(static_cast<unsigned>(keys[u] & 0x7fffffffu));
-      if (keys[u] & kUgReverse) r = fmaxf(r, w);
-      else f = fmaxf(f, w);
-    }
-    return __fsub_rn(__fadd_rn(f, r), __fmul_rn(f, r));
-  };
-  auto kept = [](float w) { return isfinite(w) && w > 1.0e-6f; };
   int mine = 0;
   for (int t = c0; t < min(c0 + CH, len); ++t)
-    if (starts_group(t) && kept(group_weight(t))) ++mine;
+    if (starts_group(t) && ug_kept(ug_group_weight(keys, t, len))) ++mine;
   flag_scan[threadIdx.x] = mine;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -314,10 +321,72 @@ __global__ __launch_bounds__(kUgThreads) void ug_sort_union(unsigned long long* 
   int out = flag_scan[threadIdx.x];
   for (int t = c0; t < min(c0 + CH, len); ++t) {
     if (!starts_group(t)) continue;
-    const float w = group_weight(t);
-    if (!kept(w)) continue;
+    const float w = ug_group_weight(keys, t, len);
+    if (!ug_kept(w)) continue;
     raw[beg + out] = (keys[t] >> 32 << 32) | __float_as_uint(w);
     ++out;
+  }
+}
+
+// Rows of at most 64 raw entries, one wave each: a bitonic sort across the
+// wave's lanes (shuffles), the groups' union weights from the sorted keys in
+// LDS, the kept ones written to the front of the row's raw range as
+// (neighbour << 32 | weight bits).  Longer rows are queued for ug_sort_union.
+__global__ __launch_bounds__(kUgThreads) void ug_sort_union_short(
+    unsigned long long* __restrict__ raw, const int* __restrict__ raw_ptr, int n,
+    int* __restrict__ union_len, int* __restrict__ long_rows, int* __restrict__ long_count) {
+  __shared__ unsigned long long sk[kUgThreads];
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * (kUgThreads / 64) + static_cast<int>(threadIdx.x >> 6);
+  int beg = 0, len = 0;
+  bool mine = false;  // this wave writes the row
+  if (row < n) {
+    beg = raw_ptr[row];
+    len = raw_ptr[row + 1] - beg;
+    if (len > 64) {
+      if (lane == 0) long_rows[atomicAdd(long_count, 1)] = row;
+      len = 0;
+    } else {
+      mine = true;
+    }
+  }
+  unsigned long long key = lane < len ? raw[beg + lane] : ~0ull;
+#pragma unroll
+  for (int size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const unsigned long long other = __shfl_xor(key, stride, 64);
+      const bool keep_min = ((lane & stride) == 0) == ((lane & size) == 0);
+      key = keep_min ? (other < key ? other : key) : (other > key ? other : key);
+    }
+  }
+  sk[threadIdx.x] = key;
+  __syncthreads();
+  const unsigned long long* wk = sk + (threadIdx.x & ~63u);
+  bool keep = false;
+  unsigned long long out = 0;
+  if (lane < len && (lane == 0 || (key >> 32) != (wk[lane - 1] >> 32))) {
+    const float w = ug_group_weight(wk, lane, len);
+    keep = ug_kept(w);
+    out = (key >> 32 << 32) | __float_as_uint(w);
+  }
+  const unsigned long long m = __ballot(keep);
+  if (keep) raw[beg + __popcll(m & ((1ull << lane) - 1ull))] = out;
+  if (mine && lane == 0) union_len[row] = __popcll(m);
+}
+
+// The queued rows of more than 64 entries, one block per row: the same
+// result through an LDS bitonic sort of up to kSortCap entries.
+__global__ __launch_bounds__(kUgThreads) void ug_sort_union(unsigned long long* __restrict__ raw,
+                                                            const int* __restrict__ raw_ptr,
+                                                            const int* __restrict__ long_rows,
+                                                            const int* __restrict__ long_count,
+                                                            int* __restrict__ union_len,
+                                                            unsigned* error_flags) {
+  const int rows = *long_count;
+  for (int q = blockIdx.x; q < rows; q += gridDim.x) {
+    __syncthreads();  // the previous row's keys are read
+    sort_union_long(raw, raw_ptr, long_rows[q], union_len, error_flags);
   }
 }
 
@@ -413,6 +482,7 @@ int UmapGraph::build(const int* indices, const float* distances, int n_, int k_t
   wp.add<int>(n + 1);   // final offsets
   wp.add<int>(nbs);
   wp.add<int>(1);
+  wp.add<int>(1);       // rows queued for the block sort
   wp.add<unsigned>(2);  // error flags, max weight bits
   if (work.init(wp.bytes, "UMAP fuzzy graph workspace")) return 1;
   const int* d_idx = indices;
@@ -437,6 +507,7 @@ int UmapGraph::build(const int* indices, const float* distances, int n_, int k_t
   int* offs = work.take<int>(n + 1);
   int* sums = work.take<int>(nbs);
   int* tot = work.take<int>(1);
+  int* long_count = work.take<int>(1);
   unsigned* flags = work.take<unsigned>(2);
   FE_CHECK(hipMemsetAsync(flags, 0, 2 * sizeof(unsigned), s));
   if (index_offset < 0) {
@@ -478,7 +549,15 @@ int UmapGraph::build(const int* indices, const float* distances, int n_, int k_t
   FE_LAUNCH_CHECK("ug_fill");
   // cursors become the union lengths (the last slot stays 0 for the scan)
   FE_CHECK(hipMemsetAsync(cursor, 0, (n + 1) * sizeof(int), s));
-  ug_sort_union<<<n, kUgThreads, 0, s>>>(raw, raw_ptr, cursor, flags);
+  // rows of at most 64 entries by one wave each; the degrees are no longer
+  // needed and hold the queue of longer rows
+  FE_CHECK(hipMemsetAsync(long_count, 0, sizeof(int), s));
+  constexpr int kRowsPerBlock = kUgThreads / 64;
+  ug_sort_union_short<<<(n + kRowsPerBlock - 1) / kRowsPerBlock, kUgThreads, 0, s>>>(
+      raw, raw_ptr, n, cursor, deg, long_count);
+  FE_LAUNCH_CHECK("ug_sort_union_short");
+  ug_sort_union<<<std::min(n, 2048), kUgThreads, 0, s>>>(raw, raw_ptr, deg, long_count, cursor,
+                                                         flags);
   FE_LAUNCH_CHECK("ug_sort_union");
   if (ug_scan(cursor, n + 1, offs, sums, tot, s)) return 1;
   int h_nnz = 0;

@@ -35,6 +35,32 @@ def test_fuzzy_graph_matches_oracle():
     fg.close()
 
 
+def test_fuzzy_graph_hub_rows():
+    """Points in many KNN lists: union rows of more than 64 entries (the
+    block sort) beside rows of at most 64 (the wave sort), duplicate entries
+    of one neighbour in a list, and the symmetric epochs on such a graph."""
+    from fastembedr_amd import umap
+
+    x, _ = gaussian_mixture(2000, 6, 4, seed=904)
+    idx, dist = exact_knn(x, 15)
+    for hub, rows in ((0, range(1, 301)), (1, range(400, 450)), (2, range(500, 530)),
+                      (3, range(600, 1400, 2))):
+        for r in rows:
+            idx[r, 14] = hub  # keeps the distance of the entry it replaces
+    idx[700, 13] = idx[700, 12]  # a duplicated neighbour
+    print("[umap graph] hub rows n=2000 k=15", flush=True)
+    fg = umap.FuzzyGraph(idx, dist)
+    g = fg.export()
+    lengths = np.diff(g["offsets"])
+    assert lengths.max() > 64 and (lengths <= 64).sum() > 1000
+    _compare(g, oracle.umap_fuzzy_graph(idx, dist, index_offset=0))
+    y0 = np.random.default_rng(905).uniform(-10, 10, size=(2000, 2)).astype(np.float32)
+    r1 = fg.optimize(y0, 20, seed=6, optimizer_mode=1)
+    r2 = fg.optimize_coo_device(y0, 20, seed=6, optimizer_mode=1)
+    assert np.array_equal(r1, r2) and np.isfinite(r1).all()
+    fg.close()
+
+
 def test_fuzzy_graph_columns_offsets_and_duplicates():
     from fastembedr_amd import umap
 
