@@ -13,10 +13,10 @@
 //                     every kept directed membership w = exp(-(d - rho)/sigma)
 //                     (1 at d <= rho; float arithmetic, :1608-1634) goes into
 //                     both endpoint rows, tagged forward or reverse
-//   ug_sort_union_short / ug_sort_union
-//                     one wave per row of at most 64 entries (a bitonic sort
-//                     across the lanes), one block per longer row (LDS
-//                     bitonic sort) of (neighbour,
+//   ug_sort_union_wave / ug_sort_union
+//                     one wave per row of at most 512 entries (a bitonic sort
+//                     across lanes and registers), one block per longer row
+//                     (LDS bitonic sort) of (neighbour,
 //                     direction, weight bits); per neighbour the largest
 //                     forward and reverse weights (the reference's max-dedupe
 //                     of :1291-1364 and its incoming lists) give
@@ -50,6 +50,7 @@ namespace {
 constexpr int kUgThreads = 256;
 constexpr int kUgMeanBlocks = 1024;
 constexpr unsigned long long kUgReverse = 1ull << 31;
+constexpr int kUgWaveMaxR = 8;  // union rows of up to 512 entries are sorted by one wave
 
 // double-double accumulation (Knuth two-sum)
 struct DD {
@@ -271,7 +272,6 @@ __device__ __forceinline__ void sort_union_long(unsigned long long* __restrict__
                                                 int* __restrict__ union_len,
                                                 unsigned* error_flags) {
   __shared__ unsigned long long keys[kSortCap];
-  __shared__ int flag_scan[kUgThreads];
   const int beg = raw_ptr[row], len = raw_ptr[row + 1] - beg;
   if (len > kSortCap) {
     if (threadIdx.x == 0) {
@@ -306,19 +306,9 @@ __device__ __forceinline__ void sort_union_long(unsigned long long* __restrict__
   int mine = 0;
   for (int t = c0; t < min(c0 + CH, len); ++t)
     if (starts_group(t) && ug_kept(ug_group_weight(keys, t, len))) ++mine;
-  flag_scan[threadIdx.x] = mine;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int run = 0;
-    for (int w = 0; w < kUgThreads; ++w) {
-      const int v = flag_scan[w];
-      flag_scan[w] = run;
-      run += v;
-    }
-    union_len[row] = run;
-  }
-  __syncthreads();
-  int out = flag_scan[threadIdx.x];
+  int kept_total;
+  int out = block_exclusive_scan_nt<kUgThreads>(mine, &kept_total);
+  if (threadIdx.x == 0) union_len[row] = kept_total;
   for (int t = c0; t < min(c0 + CH, len); ++t) {
     if (!starts_group(t)) continue;
     const float w = ug_group_weight(keys, t, len);
@@ -328,54 +318,88 @@ __device__ __forceinline__ void sort_union_long(unsigned long long* __restrict__
   }
 }
 
-// Rows of at most 64 raw entries, one wave each: a bitonic sort across the
-// wave's lanes (shuffles), the groups' union weights from the sorted keys in
-// LDS, the kept ones written to the front of the row's raw range as
-// (neighbour << 32 | weight bits).  Longer rows are queued for ug_sort_union.
-__global__ __launch_bounds__(kUgThreads) void ug_sort_union_short(
+// Union rows of (64 R/2, 64 R] raw entries (R = 1: at most 64), one wave
+// each: entry e = 64 r + lane sits in register r of its lane; a bitonic sort
+// exchanges across registers for strides of 64 and more and across lanes
+// (shuffles) below.  The groups' union weights come from the sorted keys in
+// LDS; the kept ones are written to the front of the row's raw range as
+// (neighbour << 32 | weight bits).  The widest class queues the rows longer
+// than 64 R for ug_sort_union.
+template <int R>
+__global__ __launch_bounds__(kUgThreads) void ug_sort_union_wave(
     unsigned long long* __restrict__ raw, const int* __restrict__ raw_ptr, int n,
     int* __restrict__ union_len, int* __restrict__ long_rows, int* __restrict__ long_count) {
-  __shared__ unsigned long long sk[kUgThreads];
-  const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * (kUgThreads / 64) + static_cast<int>(threadIdx.x >> 6);
+  constexpr int kCap = 64 * R;
+  __shared__ unsigned long long sk[(kUgThreads / 64) * kCap];
+  const int lane = threadIdx.x & 63, wave = static_cast<int>(threadIdx.x >> 6);
+  const int row = blockIdx.x * (kUgThreads / 64) + wave;
   int beg = 0, len = 0;
   bool mine = false;  // this wave writes the row
   if (row < n) {
     beg = raw_ptr[row];
     len = raw_ptr[row + 1] - beg;
-    if (len > 64) {
+    if (len > kCap && R == kUgWaveMaxR) {
       if (lane == 0) long_rows[atomicAdd(long_count, 1)] = row;
-      len = 0;
-    } else {
-      mine = true;
     }
+    mine = len <= kCap && (R == 1 || len > kCap / 2);
   }
-  unsigned long long key = lane < len ? raw[beg + lane] : ~0ull;
+  if (!mine) return;  // wave-uniform: the wave owns no row of this class
+  unsigned long long key[R];
 #pragma unroll
-  for (int size = 2; size <= 64; size <<= 1) {
+  for (int r = 0; r < R; ++r) key[r] = 64 * r + lane < len ? raw[beg + 64 * r + lane] : ~0ull;
+#pragma unroll
+  for (int size = 2; size <= kCap; size <<= 1) {
 #pragma unroll
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      const unsigned long long other = __shfl_xor(key, stride, 64);
-      const bool keep_min = ((lane & stride) == 0) == ((lane & size) == 0);
-      key = keep_min ? (other < key ? other : key) : (other > key ? other : key);
+      if (stride >= 64) {
+        const int rs = stride / 64;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if (r & rs) continue;
+          const bool up = ((64 * r) & size) == 0;
+          const unsigned long long a = key[r], b = key[r | rs];
+          const bool swap = (a > b) == up;
+          key[r] = swap ? b : a;
+          key[r | rs] = swap ? a : b;
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const unsigned long long other = __shfl_xor(key[r], stride, 64);
+          const bool up = ((64 * r + lane) & size) == 0;
+          const bool keep_min = ((lane & stride) == 0) == up;
+          key[r] = keep_min ? (other < key[r] ? other : key[r]) : (other > key[r] ? other : key[r]);
+        }
+      }
     }
   }
-  sk[threadIdx.x] = key;
-  __syncthreads();
-  const unsigned long long* wk = sk + (threadIdx.x & ~63u);
-  bool keep = false;
-  unsigned long long out = 0;
-  if (lane < len && (lane == 0 || (key >> 32) != (wk[lane - 1] >> 32))) {
-    const float w = ug_group_weight(wk, lane, len);
-    keep = ug_kept(w);
-    out = (key >> 32 << 32) | __float_as_uint(w);
+  unsigned long long* wk = sk + wave * kCap;
+#pragma unroll
+  for (int r = 0; r < R; ++r) wk[64 * r + lane] = key[r];
+  // the wave reads only its own keys: a wave-scope barrier orders the LDS
+  // stores before the loads of other lanes' entries
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  int base = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int e = 64 * r + lane;
+    bool keep = false;
+    unsigned long long out = 0;
+    if (e < len && (e == 0 || (key[r] >> 32) != (wk[e - 1] >> 32))) {
+      const float w = ug_group_weight(wk, e, len);
+      keep = ug_kept(w);
+      out = (key[r] >> 32 << 32) | __float_as_uint(w);
+    }
+    const unsigned long long m = __ballot(keep);
+    if (keep) raw[beg + base + __popcll(m & ((1ull << lane) - 1ull))] = out;
+    base += __popcll(m);
   }
-  const unsigned long long m = __ballot(keep);
-  if (keep) raw[beg + __popcll(m & ((1ull << lane) - 1ull))] = out;
-  if (mine && lane == 0) union_len[row] = __popcll(m);
+  if (lane == 0) union_len[row] = base;
 }
 
-// The queued rows of more than 64 entries, one block per row: the same
+// The queued rows of more than 64 kUgWaveMaxR entries, one block per row: the same
 // result through an LDS bitonic sort of up to kSortCap entries.
 __global__ __launch_bounds__(kUgThreads) void ug_sort_union(unsigned long long* __restrict__ raw,
                                                             const int* __restrict__ raw_ptr,
@@ -397,10 +421,13 @@ __global__ __launch_bounds__(kUgThreads) void ug_compact(const unsigned long lon
                                                          int* __restrict__ nbrs,
                                                          float* __restrict__ weights,
                                                          unsigned* max_bits) {
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  __shared__ unsigned wave_max[kUgThreads / 64];
   const int lane = threadIdx.x & 63;
   unsigned mb = 0u;
-  if (row < n) {
+  // rows strided over the grid, so the largest weight takes one atomic per
+  // block (same-address atomics serialise)
+  for (int row = blockIdx.x * (kUgThreads / 64) + static_cast<int>(threadIdx.x >> 6); row < n;
+       row += gridDim.x * (kUgThreads / 64)) {
     const int src = raw_ptr[row], dst = offsets[row], len = offsets[row + 1] - dst;
     for (int t = lane; t < len; t += 64) {
       const unsigned long long e = raw[src + t];
@@ -412,7 +439,13 @@ __global__ __launch_bounds__(kUgThreads) void ug_compact(const unsigned long lon
     }
   }
   for (int o = 32; o > 0; o >>= 1) mb = max(mb, static_cast<unsigned>(__shfl_xor(static_cast<int>(mb), o, 64)));
-  if (lane == 0 && mb) atomicMax(max_bits, mb);
+  if (lane == 0) wave_max[threadIdx.x >> 6] = mb;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned m = 0u;
+    for (int w = 0; w < kUgThreads / 64; ++w) m = max(m, wave_max[w]);
+    if (m) atomicMax(max_bits, m);
+  }
 }
 
 __global__ __launch_bounds__(kUgThreads) void ug_epochs(const float* __restrict__ weights,
@@ -549,13 +582,17 @@ int UmapGraph::build(const int* indices, const float* distances, int n_, int k_t
   FE_LAUNCH_CHECK("ug_fill");
   // cursors become the union lengths (the last slot stays 0 for the scan)
   FE_CHECK(hipMemsetAsync(cursor, 0, (n + 1) * sizeof(int), s));
-  // rows of at most 64 entries by one wave each; the degrees are no longer
-  // needed and hold the queue of longer rows
+  // rows of up to 512 entries by one wave each, one launch per length class;
+  // the degrees are no longer needed and hold the queue of longer rows
   FE_CHECK(hipMemsetAsync(long_count, 0, sizeof(int), s));
   constexpr int kRowsPerBlock = kUgThreads / 64;
-  ug_sort_union_short<<<(n + kRowsPerBlock - 1) / kRowsPerBlock, kUgThreads, 0, s>>>(
-      raw, raw_ptr, n, cursor, deg, long_count);
-  FE_LAUNCH_CHECK("ug_sort_union_short");
+  const int wblocks = (n + kRowsPerBlock - 1) / kRowsPerBlock;
+  ug_sort_union_wave<1><<<wblocks, kUgThreads, 0, s>>>(raw, raw_ptr, n, cursor, deg, long_count);
+  ug_sort_union_wave<2><<<wblocks, kUgThreads, 0, s>>>(raw, raw_ptr, n, cursor, deg, long_count);
+  ug_sort_union_wave<4><<<wblocks, kUgThreads, 0, s>>>(raw, raw_ptr, n, cursor, deg, long_count);
+  ug_sort_union_wave<kUgWaveMaxR><<<wblocks, kUgThreads, 0, s>>>(raw, raw_ptr, n, cursor, deg,
+                                                                 long_count);
+  FE_LAUNCH_CHECK("ug_sort_union_wave");
   ug_sort_union<<<std::min(n, 2048), kUgThreads, 0, s>>>(raw, raw_ptr, deg, long_count, cursor,
                                                          flags);
   FE_LAUNCH_CHECK("ug_sort_union");
@@ -581,8 +618,8 @@ int UmapGraph::build(const int* indices, const float* distances, int n_, int k_t
   weights = store.take<float>(std::max<long long>(nnz, 1));
   eps = store.take<float>(std::max<long long>(nnz, 1));
   FE_CHECK(hipMemcpyAsync(offsets, offs, (n + 1) * sizeof(int), hipMemcpyDeviceToDevice, s));
-  ug_compact<<<(n + 3) / 4, kUgThreads, 0, s>>>(raw, raw_ptr, offs, n, heads, nbrs, weights,
-                                                flags + 1);
+  ug_compact<<<std::min((n + 3) / 4, 2048), kUgThreads, 0, s>>>(raw, raw_ptr, offs, n, heads,
+                                                                nbrs, weights, flags + 1);
   FE_LAUNCH_CHECK("ug_compact");
   if (nnz > 0) {
     const int pblocks = static_cast<int>(std::min<long long>((nnz + kUgThreads - 1) / kUgThreads, 8192));

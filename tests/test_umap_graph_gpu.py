@@ -36,15 +36,16 @@ def test_fuzzy_graph_matches_oracle():
 
 
 def test_fuzzy_graph_hub_rows():
-    """Points in many KNN lists: union rows of more than 64 entries (the
-    block sort) beside rows of at most 64 (the wave sort), duplicate entries
-    of one neighbour in a list, and the symmetric epochs on such a graph."""
+    """Points in many KNN lists: union rows in every length class of the
+    sort (at most 64, 128, 256 and 512 entries by one wave, longer by one
+    block), duplicate entries of one neighbour in a list, and the symmetric
+    epochs on such a graph."""
     from fastembedr_amd import umap
 
     x, _ = gaussian_mixture(2000, 6, 4, seed=904)
     idx, dist = exact_knn(x, 15)
-    for hub, rows in ((0, range(1, 301)), (1, range(400, 450)), (2, range(500, 530)),
-                      (3, range(600, 1400, 2))):
+    for hub, rows in ((0, range(1000, 2000)), (1, range(10, 60)), (2, range(100, 250)),
+                      (3, range(300, 600)), (4, range(700, 740))):
         for r in rows:
             idx[r, 14] = hub  # keeps the distance of the entry it replaces
     idx[700, 13] = idx[700, 12]  # a duplicated neighbour
@@ -52,7 +53,9 @@ def test_fuzzy_graph_hub_rows():
     fg = umap.FuzzyGraph(idx, dist)
     g = fg.export()
     lengths = np.diff(g["offsets"])
-    assert lengths.max() > 64 and (lengths <= 64).sum() > 1000
+    assert lengths.max() > 512 and (lengths <= 64).sum() > 1000
+    for lo, hi in ((64, 128), (128, 256), (256, 512)):
+        assert ((lengths > lo) & (lengths <= hi)).any()
     _compare(g, oracle.umap_fuzzy_graph(idx, dist, index_offset=0))
     y0 = np.random.default_rng(905).uniform(-10, 10, size=(2000, 2)).astype(np.float32)
     r1 = fg.optimize(y0, 20, seed=6, optimizer_mode=1)
