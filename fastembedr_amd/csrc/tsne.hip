@@ -146,7 +146,18 @@ static int build_affinities(const int* idx, const DistT* dist, int n, int k, int
   FE_CHECK(hipMemsetAsync(cursor, 0, (n + 1) * sizeof(int), s));
   sym_fill<<<count_blocks, 256, 0, s>>>(nbr, pdir, total, k, raw_ptr, cursor, raw);
   FE_LAUNCH_CHECK("sym_fill");
-  sym_sort_merge<<<n, 256, 0, s>>>(raw, raw_ptr, merged_len, flags);
+  // rows of up to 512 entries by one wave each, one launch per length class;
+  // the cursors are no longer needed and hold the queue of longer rows
+  int* long_count = misc + 3;  // cleared with misc
+  const int wblocks = (n + 3) / 4;
+  sym_sort_merge_wave<1><<<wblocks, 256, 0, s>>>(raw, raw_ptr, n, merged_len, cursor, long_count);
+  sym_sort_merge_wave<2><<<wblocks, 256, 0, s>>>(raw, raw_ptr, n, merged_len, cursor, long_count);
+  sym_sort_merge_wave<4><<<wblocks, 256, 0, s>>>(raw, raw_ptr, n, merged_len, cursor, long_count);
+  sym_sort_merge_wave<kWaveSortMaxR><<<wblocks, 256, 0, s>>>(raw, raw_ptr, n, merged_len, cursor,
+                                                             long_count);
+  FE_LAUNCH_CHECK("sym_sort_merge_wave");
+  sym_sort_merge<<<std::min(n, 2048), 256, 0, s>>>(raw, raw_ptr, cursor, long_count, merged_len,
+                                                   flags);
   FE_LAUNCH_CHECK("sym_sort_merge");
   FE_CHECK(hipMemsetAsync(merged_len + n, 0, sizeof(int), s));
   std::vector<double> mp(count_blocks);

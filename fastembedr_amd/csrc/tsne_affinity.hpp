@@ -12,7 +12,9 @@
 //                     the symmetrisation of build_tsne_probabilities_float
 //                     (tsne_neighbors.cpp:616-724): every directed entry is
 //                     placed in both endpoint rows, each row is sorted by
-//                     (column, value bits) so the result does not depend on
+//                     (column, value bits) — by one wave for rows of up to
+//                     512 entries, by one block in LDS for longer ones — so
+//                     the result does not depend on
 //                     atomic arrival order, equal columns are summed in FP64,
 //                     and P_ij = float(sum) * float(0.5 / total_mass).  Rows
 //                     come out with ascending columns, exactly as the
@@ -32,6 +34,56 @@ namespace {
 constexpr int kAffRowsPerBlock = 64;  // rows staged per 256-thread block
 constexpr int kMaxK = 256;            // embedding_cuda_kernels.cpp:5661
 constexpr int kSortCap = 8192;        // longest symmetric row sorted in LDS
+
+constexpr int kWaveSortMaxR = 8;      // rows of up to 64 * 8 entries are sorted by one wave
+
+// Ascending bitonic sort of the 64 R keys of one wave: entry e = 64 r + lane
+// is register r of its lane.  Strides of 64 and more exchange registers
+// within a lane; shorter strides exchange across lanes with shuffles.
+template <int R>
+__device__ __forceinline__ void wave_sort_keys(unsigned long long (&key)[R], int lane) {
+#pragma unroll
+  for (int size = 2; size <= 64 * R; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      if (stride >= 64) {
+        const int rs = stride / 64;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if (r & rs) continue;
+          const bool up = ((64 * r) & size) == 0;
+          const unsigned long long a = key[r], b = key[r | rs];
+          const bool swap = (a > b) == up;
+          key[r] = swap ? b : a;
+          key[r | rs] = swap ? a : b;
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const unsigned long long other = __shfl_xor(key[r], stride, 64);
+          const bool up = ((64 * r + lane) & size) == 0;
+          const bool keep_min = ((lane & stride) == 0) == up;
+          key[r] = keep_min ? (other < key[r] ? other : key[r]) : (other > key[r] ? other : key[r]);
+        }
+      }
+    }
+  }
+}
+
+// Orders a wave's LDS stores before its lanes read each other's entries
+// (the wave touches no other wave's LDS, so no block barrier is needed).
+__device__ __forceinline__ void wave_lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// the length class of a row sorted by one wave with R keys per lane:
+// (32 R, 64 R] entries, and every row of at most 64 for R = 1
+template <int R>
+__device__ __forceinline__ bool wave_sort_class(int len) {
+  return len <= 64 * R && (R == 1 || len > 32 * R);
+}
 
 enum AffinityError : unsigned { kErrIndexRange = 1u, kErrDistance = 2u, kErrRowTooLong = 4u };
 
@@ -193,16 +245,64 @@ __global__ __launch_bounds__(256) void sym_fill(const int* __restrict__ nbr,
   }
 }
 
-// One block per row: LDS bitonic sort of (col << 32 | value bits), then a
-// merge of equal columns with FP64 sums.  Merged entries are written back to
-// the front of the row's raw range as (col << 32 | float(sum) bits).
-__global__ __launch_bounds__(256) void sym_sort_merge(unsigned long long* __restrict__ raw,
-                                                      const int* __restrict__ raw_ptr,
-                                                      int* __restrict__ merged_len,
-                                                      unsigned* error_flags) {
+// The sorted row's merge: equal columns summed in FP64 in sorted order,
+// written back to the front of the row's raw range as
+// (col << 32 | float(sum) bits).
+__device__ __forceinline__ unsigned long long merged_entry(const unsigned long long* keys, int t,
+                                                           int len) {
+  const unsigned long long cidx = keys[t] >> 32;
+  double s = 0.0;
+  for (int u = t; u < len && (keys[u] >> 32) == cidx; ++u)
+    s += static_cast<double>(__uint_as_float(static_cast<unsigned>(keys[u] & 0xffffffffu)));
+  return (cidx << 32) | __float_as_uint(static_cast<float>(s));
+}
+
+// Symmetric rows of length class R (wave_sort_class), one wave each: the
+// keys (col << 32 | value bits) sorted across the wave, then merged.  The
+// widest class queues the rows longer than 64 kWaveSortMaxR entries for
+// sym_sort_merge.
+template <int R>
+__global__ __launch_bounds__(256) void sym_sort_merge_wave(unsigned long long* __restrict__ raw,
+                                                           const int* __restrict__ raw_ptr, int n,
+                                                           int* __restrict__ merged_len,
+                                                           int* __restrict__ long_rows,
+                                                           int* __restrict__ long_count) {
+  constexpr int kCap = 64 * R;
+  __shared__ unsigned long long sk[4 * kCap];
+  const int lane = threadIdx.x & 63, wave = static_cast<int>(threadIdx.x >> 6);
+  const int row = blockIdx.x * 4 + wave;
+  if (row >= n) return;
+  const int beg = raw_ptr[row], len = raw_ptr[row + 1] - beg;
+  if (R == kWaveSortMaxR && len > kCap && lane == 0) long_rows[atomicAdd(long_count, 1)] = row;
+  if (!wave_sort_class<R>(len)) return;  // wave-uniform
+  unsigned long long key[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) key[r] = 64 * r + lane < len ? raw[beg + 64 * r + lane] : ~0ull;
+  wave_sort_keys<R>(key, lane);
+  unsigned long long* wk = sk + wave * kCap;
+#pragma unroll
+  for (int r = 0; r < R; ++r) wk[64 * r + lane] = key[r];
+  wave_lds_barrier();
+  int base = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int e = 64 * r + lane;
+    const bool head = e < len && (e == 0 || (key[r] >> 32) != (wk[e - 1] >> 32));
+    const unsigned long long out = head ? merged_entry(wk, e, len) : 0ull;
+    const unsigned long long m = __ballot(head);
+    if (head) raw[beg + base + __popcll(m & ((1ull << lane) - 1ull))] = out;
+    base += __popcll(m);
+  }
+  if (lane == 0) merged_len[row] = base;
+}
+
+// One queued long row by one block: LDS bitonic sort of up to kSortCap
+// entries, then the merge.
+__device__ __forceinline__ void sort_merge_long(unsigned long long* __restrict__ raw,
+                                                const int* __restrict__ raw_ptr, int row,
+                                                int* __restrict__ merged_len,
+                                                unsigned* error_flags) {
   __shared__ unsigned long long keys[kSortCap];
-  __shared__ int flag_scan[256];
-  const int row = blockIdx.x;
   const int beg = raw_ptr[row], len = raw_ptr[row + 1] - beg;
   if (len > kSortCap) {
     if (threadIdx.x == 0) {
@@ -236,28 +336,28 @@ __global__ __launch_bounds__(256) void sym_sort_merge(unsigned long long* __rest
   int heads = 0;
   for (int t = c0; t < min(c0 + CH, len); ++t)
     if (t == 0 || (keys[t] >> 32) != (keys[t - 1] >> 32)) ++heads;
-  flag_scan[threadIdx.x] = heads;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int run = 0;
-    for (int w = 0; w < 256; ++w) {
-      const int v = flag_scan[w];
-      flag_scan[w] = run;
-      run += v;
-    }
-    merged_len[row] = run;
-  }
-  __syncthreads();
-  int out = flag_scan[threadIdx.x];
+  int total;
+  int out = block_exclusive_scan_nt<256>(heads, &total);
+  if (threadIdx.x == 0) merged_len[row] = total;
   for (int t = c0; t < min(c0 + CH, len); ++t) {
     if (t == 0 || (keys[t] >> 32) != (keys[t - 1] >> 32)) {
-      const unsigned long long cidx = keys[t] >> 32;
-      double s = 0.0;
-      for (int u = t; u < len && (keys[u] >> 32) == cidx; ++u)
-        s += static_cast<double>(__uint_as_float(static_cast<unsigned>(keys[u] & 0xffffffffu)));
-      raw[beg + out] = (cidx << 32) | __float_as_uint(static_cast<float>(s));
+      raw[beg + out] = merged_entry(keys, t, len);
       ++out;
     }
+  }
+}
+
+// The rows queued by sym_sort_merge_wave, strided over the grid.
+__global__ __launch_bounds__(256) void sym_sort_merge(unsigned long long* __restrict__ raw,
+                                                      const int* __restrict__ raw_ptr,
+                                                      const int* __restrict__ long_rows,
+                                                      const int* __restrict__ long_count,
+                                                      int* __restrict__ merged_len,
+                                                      unsigned* error_flags) {
+  const int rows = *long_count;
+  for (int q = blockIdx.x; q < rows; q += gridDim.x) {
+    __syncthreads();  // the previous row's keys are read
+    sort_merge_long(raw, raw_ptr, long_rows[q], merged_len, error_flags);
   }
 }
 

@@ -50,7 +50,6 @@ namespace {
 constexpr int kUgThreads = 256;
 constexpr int kUgMeanBlocks = 1024;
 constexpr unsigned long long kUgReverse = 1ull << 31;
-constexpr int kUgWaveMaxR = 8;  // union rows of up to 512 entries are sorted by one wave
 
 // double-double accumulation (Knuth two-sum)
 struct DD {
@@ -338,49 +337,20 @@ __global__ __launch_bounds__(kUgThreads) void ug_sort_union_wave(
   if (row < n) {
     beg = raw_ptr[row];
     len = raw_ptr[row + 1] - beg;
-    if (len > kCap && R == kUgWaveMaxR) {
+    if (len > kCap && R == kWaveSortMaxR) {
       if (lane == 0) long_rows[atomicAdd(long_count, 1)] = row;
     }
-    mine = len <= kCap && (R == 1 || len > kCap / 2);
+    mine = wave_sort_class<R>(len);
   }
   if (!mine) return;  // wave-uniform: the wave owns no row of this class
   unsigned long long key[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) key[r] = 64 * r + lane < len ? raw[beg + 64 * r + lane] : ~0ull;
-#pragma unroll
-  for (int size = 2; size <= kCap; size <<= 1) {
-#pragma unroll
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      if (stride >= 64) {
-        const int rs = stride / 64;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          if (r & rs) continue;
-          const bool up = ((64 * r) & size) == 0;
-          const unsigned long long a = key[r], b = key[r | rs];
-          const bool swap = (a > b) == up;
-          key[r] = swap ? b : a;
-          key[r | rs] = swap ? a : b;
-        }
-      } else {
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const unsigned long long other = __shfl_xor(key[r], stride, 64);
-          const bool up = ((64 * r + lane) & size) == 0;
-          const bool keep_min = ((lane & stride) == 0) == up;
-          key[r] = keep_min ? (other < key[r] ? other : key[r]) : (other > key[r] ? other : key[r]);
-        }
-      }
-    }
-  }
+  wave_sort_keys<R>(key, lane);
   unsigned long long* wk = sk + wave * kCap;
 #pragma unroll
   for (int r = 0; r < R; ++r) wk[64 * r + lane] = key[r];
-  // the wave reads only its own keys: a wave-scope barrier orders the LDS
-  // stores before the loads of other lanes' entries
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  wave_lds_barrier();
   int base = 0;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -399,7 +369,7 @@ __global__ __launch_bounds__(kUgThreads) void ug_sort_union_wave(
   if (lane == 0) union_len[row] = base;
 }
 
-// The queued rows of more than 64 kUgWaveMaxR entries, one block per row: the same
+// The queued rows of more than 64 kWaveSortMaxR entries, one block per row: the same
 // result through an LDS bitonic sort of up to kSortCap entries.
 __global__ __launch_bounds__(kUgThreads) void ug_sort_union(unsigned long long* __restrict__ raw,
                                                             const int* __restrict__ raw_ptr,
@@ -590,7 +560,7 @@ int UmapGraph::build(const int* indices, const float* distances, int n_, int k_t
   ug_sort_union_wave<1><<<wblocks, kUgThreads, 0, s>>>(raw, raw_ptr, n, cursor, deg, long_count);
   ug_sort_union_wave<2><<<wblocks, kUgThreads, 0, s>>>(raw, raw_ptr, n, cursor, deg, long_count);
   ug_sort_union_wave<4><<<wblocks, kUgThreads, 0, s>>>(raw, raw_ptr, n, cursor, deg, long_count);
-  ug_sort_union_wave<kUgWaveMaxR><<<wblocks, kUgThreads, 0, s>>>(raw, raw_ptr, n, cursor, deg,
+  ug_sort_union_wave<kWaveSortMaxR><<<wblocks, kUgThreads, 0, s>>>(raw, raw_ptr, n, cursor, deg,
                                                                  long_count);
   FE_LAUNCH_CHECK("ug_sort_union_wave");
   ug_sort_union<<<std::min(n, 2048), kUgThreads, 0, s>>>(raw, raw_ptr, deg, long_count, cursor,

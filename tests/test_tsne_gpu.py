@@ -242,6 +242,11 @@ def test_attraction_kernels_agree_on_hub_rows(monkeypatch):
         got[kind] = fe().gradient_diagnostic(idx, dist.astype(np.float32), y, 10, 4.0, 128)
         assert relative_l2(got[kind].attractive_force, ref) < 5e-5, kind
     lanes = got["lanes"]
+    # the symmetric P itself: its rows are sorted and merged by one wave up to
+    # 512 entries and by one block above (the hub row)
+    assert np.array_equal(lanes.row_ptr, csr.row_ptr)
+    assert np.array_equal(lanes.col, csr.col)
+    assert np.abs(lanes.val - csr.val).max() <= 2e-6 * csr.val.max()
     # the lane-padded kernels sum in one order; the staged window changes
     # only where the points are read from
     assert np.array_equal(got["staged"].attractive_force, lanes.attractive_force)
