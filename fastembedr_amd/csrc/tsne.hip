@@ -140,11 +140,14 @@ static int build_affinities(const int* idx, const DistT* dist, int n, int k, int
   if (h_flags & kErrDistance) return fail("KNN distances must be finite and non-negative.");
 
   FE_CHECK(hipMemsetAsync(deg, 0, (n + 1) * sizeof(int), s));
+  const int row_blocks = (n + 3) / 4;  // one wave per row
+  sym_row_counts<<<row_blocks, 256, 0, s>>>(nbr, n, k, deg, cursor);
+  FE_LAUNCH_CHECK("sym_row_counts");
   sym_count<<<count_blocks, 256, 0, s>>>(nbr, pdir, total, k, deg, mass_part);
   FE_LAUNCH_CHECK("sym_count");
   if (exclusive_scan(deg, n + 1, raw_ptr, sums, misc, s)) return 1;
-  FE_CHECK(hipMemsetAsync(cursor, 0, (n + 1) * sizeof(int), s));
-  sym_fill<<<count_blocks, 256, 0, s>>>(nbr, pdir, total, k, raw_ptr, cursor, raw);
+  // cursor[i] holds row i's forward count: its reverse entries follow them
+  sym_fill<<<row_blocks, 256, 0, s>>>(nbr, pdir, n, k, raw_ptr, cursor, raw);
   FE_LAUNCH_CHECK("sym_fill");
   // rows of up to 512 entries by one wave each, one launch per length class;
   // the cursors are no longer needed and hold the queue of longer rows

@@ -206,8 +206,28 @@ __global__ __launch_bounds__(256) void perplexity_rows(const int* __restrict__ i
   }
 }
 
-// Count both endpoints of every non-self directed entry; block partial sums of
-// the directed mass (FP64) for the global normaliser.
+// One wave per row: its forward (non-self) entries, added to its degree with
+// one atomic and kept in `cursor` as the start of its reverse entries.
+__global__ __launch_bounds__(256) void sym_row_counts(const int* __restrict__ nbr, int n, int k,
+                                                      int* __restrict__ deg,
+                                                      int* __restrict__ cursor) {
+  const int row = blockIdx.x * 4 + static_cast<int>(threadIdx.x >> 6);
+  if (row >= n) return;
+  const int lane = threadIdx.x & 63;
+  int cnt = 0;
+  for (int c0 = 0; c0 < k; c0 += 64) {
+    const int c = c0 + lane;
+    cnt += __popcll(__ballot(c < k && nbr[static_cast<long long>(row) * k + c] != row));
+  }
+  if (lane == 0) {
+    cursor[row] = cnt;
+    if (cnt) atomicAdd(deg + row, cnt);
+  }
+}
+
+// Count the reverse endpoint of every non-self directed entry (the forward
+// ones are counted per row); block partial sums of the directed mass (FP64)
+// for the global normaliser.
 __global__ __launch_bounds__(256) void sym_count(const int* __restrict__ nbr,
                                                  const float* __restrict__ pdir, long long total,
                                                  int k, int* __restrict__ deg,
@@ -217,7 +237,6 @@ __global__ __launch_bounds__(256) void sym_count(const int* __restrict__ nbr,
     const int i = static_cast<int>(e / k);
     const int j = nbr[e];
     if (j == i) continue;
-    atomicAdd(deg + i, 1);
     atomicAdd(deg + j, 1);
     local += static_cast<double>(pdir[e]);
   }
@@ -228,20 +247,38 @@ __global__ __launch_bounds__(256) void sym_count(const int* __restrict__ nbr,
   if (threadIdx.x == 0) mass_partials[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
+// One wave per row: its forward entries go to the front of its raw range in
+// KNN order (no atomics), each one's reverse entry to the neighbour's range
+// after that row's forward entries (one atomic on its cursor).
 __global__ __launch_bounds__(256) void sym_fill(const int* __restrict__ nbr,
-                                                const float* __restrict__ pdir, long long total,
-                                                int k, const int* __restrict__ raw_ptr,
+                                                const float* __restrict__ pdir, int n, int k,
+                                                const int* __restrict__ raw_ptr,
                                                 int* __restrict__ cursor,
                                                 unsigned long long* __restrict__ raw) {
-  for (long long e = blockIdx.x * 256ll + threadIdx.x; e < total; e += gridDim.x * 256ll) {
-    const int i = static_cast<int>(e / k);
-    const int j = nbr[e];
-    if (j == i) continue;
-    const unsigned long long bits = __float_as_uint(pdir[e]);
-    int pos = raw_ptr[i] + atomicAdd(cursor + i, 1);
-    raw[pos] = (static_cast<unsigned long long>(static_cast<unsigned>(j)) << 32) | bits;
-    pos = raw_ptr[j] + atomicAdd(cursor + j, 1);
-    raw[pos] = (static_cast<unsigned long long>(static_cast<unsigned>(i)) << 32) | bits;
+  const int row = blockIdx.x * 4 + static_cast<int>(threadIdx.x >> 6);
+  if (row >= n) return;
+  const int lane = threadIdx.x & 63;
+  const int base = raw_ptr[row];
+  int fwd = 0;
+  for (int c0 = 0; c0 < k; c0 += 64) {
+    const int c = c0 + lane;
+    int j = row;
+    float p = 0.0f;
+    if (c < k) {
+      const long long e = static_cast<long long>(row) * k + c;
+      j = nbr[e];
+      p = pdir[e];
+    }
+    const bool fw = j != row;
+    const unsigned long long m = __ballot(fw);
+    if (fw) {
+      const unsigned long long bits = __float_as_uint(p);
+      raw[base + fwd + __popcll(m & ((1ull << lane) - 1ull))] =
+          (static_cast<unsigned long long>(static_cast<unsigned>(j)) << 32) | bits;
+      const int pos = raw_ptr[j] + atomicAdd(cursor + j, 1);
+      raw[pos] = (static_cast<unsigned long long>(static_cast<unsigned>(row)) << 32) | bits;
+    }
+    fwd += __popcll(m);
   }
 }
 
