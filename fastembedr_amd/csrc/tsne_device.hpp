@@ -279,7 +279,7 @@ constexpr int kTileNodes = kTile * kTile;
 constexpr int kMaxTilesSide = 32;  // grids up to 1024 x 1024 nodes
 constexpr int kMaxTiles = kMaxTilesSide * kMaxTilesSide;
 constexpr int kMaxBins = 4 * kMaxTiles;
-constexpr int kSplatItem = 4096;  // points per tile work item
+constexpr int kSplatItem = 1024;  // points per tile work item
 
 struct SplatBuffers {
   int* bin_count;            // bins; zero between passes
