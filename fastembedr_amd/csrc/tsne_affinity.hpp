@@ -8,17 +8,20 @@
 //                     452-530).  Reads the reference's column-major k x n
 //                     KNN layout through an LDS transpose; writes row-major
 //                     0-based neighbour ids and float p_{j|i}.
-//   sym_count / sym_fill / sym_sort_merge / sym_compact
+//   sym_row_counts / sym_count / sym_fill / sym_sort_merge_wave /
+//   sym_sort_merge / sym_compact
 //                     the symmetrisation of build_tsne_probabilities_float
 //                     (tsne_neighbors.cpp:616-724): every directed entry is
-//                     placed in both endpoint rows, each row is sorted by
-//                     (column, value bits) — by one wave for rows of up to
-//                     512 entries, by one block in LDS for longer ones — so
-//                     the result does not depend on
-//                     atomic arrival order, equal columns are summed in FP64,
-//                     and P_ij = float(sum) * float(0.5 / total_mass).  Rows
-//                     come out with ascending columns, exactly as the
-//                     reference's key-sorted fill lays them out.
+//                     placed in both endpoint rows (the forward ones per row
+//                     without atomics, the reverse ones with one atomic
+//                     each), each row is sorted by (column, value bits) —
+//                     by one wave for rows of up to 512 entries, by one
+//                     block in LDS for longer ones — so the result does not
+//                     depend on atomic arrival order, equal columns are
+//                     summed in FP64, and P_ij = float(sum) *
+//                     float(0.5 / total_mass).  Rows come out with ascending
+//                     columns, exactly as the reference's key-sorted fill
+//                     lays them out.
 #pragma once
 
 #include <hip/hip_runtime.h>
