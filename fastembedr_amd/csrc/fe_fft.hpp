@@ -1,4 +1,4 @@
-// In-LDS Stockham FFT for one 256-thread workgroup (gfx950, wave64).
+// In-LDS Stockham FFT for one workgroup of NT threads (gfx950, wave64).
 //
 // A workgroup transforms NB independent length-N complex arrays held in LDS
 // with one padding slot after every 32 elements: element i of array b sits at
