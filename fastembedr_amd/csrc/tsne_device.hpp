@@ -681,7 +681,11 @@ __device__ __forceinline__ void conv_rows_fwd_body(SplatBuffers sb,
       if (x <= G) o2[x] = buf[fft_at<F>(1, x)];
     }
   } else {
-    const int dy0 = b - G;
+    // kernel rows dy0 and dy0 + G/2; block G + j runs on XCD j % 8, which
+    // gets a contiguous range of dy0, so the 8-byte stores of neighbouring
+    // rows into one 128-byte line of the transposed spectrum meet in one L2
+    const int j = b - G;
+    const int dy0 = (j & 7) * (G / 16) + (j >> 3);
     const float h = st->spacing;
     for (int t = threadIdx.x; t < 2 * F; t += kFftThreads) {
       const int which = t / F, xx = t - which * F;
