@@ -44,6 +44,8 @@ SIGNATURES = {
         c_int, [_vp, _vp, c_int, c_int, c_int, c_int, c_float, c_int, _fp, c_uint, _vpp]),
     "fastembedr_hip_tsne_plan_run": (
         c_int, [_vp, c_int, c_float, c_float, c_float, c_float, c_float]),
+    "fastembedr_hip_tsne_plan_prepare": (
+        c_int, [_vp, c_int, c_float, c_float, c_float, c_float, c_float, c_ll]),
     "fastembedr_hip_tsne_plan_sync": (c_int, [_vp]),
     "fastembedr_hip_tsne_plan_layout": (c_int, [_vp, _fp]),
     "fastembedr_hip_tsne_plan_info": (c_int, [_vp, _llp]),

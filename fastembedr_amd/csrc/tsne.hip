@@ -251,7 +251,13 @@ class TsnePlan {
   int setup_attraction(hipStream_t s);
   int enqueue_iteration(const StepParams& sp);
   int enqueue_diag(const StepParams& sp);
+  GraphExec* chunk_graph(int len, const StepParams& sp);
   int run(int iters, const StepParams& sp);
+  int prepare(int iters, const StepParams& sp, long long after);
+  long long device_bytes() const {
+    return static_cast<long long>(state.capacity() + csr_store.capacity() +
+                                  group_store.capacity() + attr_store.capacity());
+  }
   int layout(float* out_host);
   int profile(int iters, const StepParams& sp, double* ms);
 };
@@ -637,43 +643,74 @@ static bool same_params(const StepParams& a, const StepParams& b) {
   return std::memcmp(&a, &b, sizeof(StepParams)) == 0;
 }
 
-int TsnePlan::run(int iters, const StepParams& sp) {
-  constexpr int kChunk = 50;  // even, so a captured chunk preserves buffer parity
+// Runs are cut into chunks of at most kGraphChunk iterations; each chunk of
+// at least kGraphMin iterations replays a captured graph of that length,
+// constants and starting buffer parity (an odd chunk flips the parity).
+constexpr int kGraphChunk = 50;
+constexpr int kGraphMin = 4;
+constexpr size_t kGraphCache = 16;
+
+GraphExec* TsnePlan::chunk_graph(int len, const StepParams& sp) {
+  for (auto& c : graphs)
+    if (c.iters == len && c.parity == cur && same_params(c.sp, sp)) return c.exec.get();
   hipStream_t s = stream.get();
-  int left = iters;
-  if (graph_capture_enabled() && iters >= kChunk) {
-    GraphExec* exec = nullptr;
-    for (auto& c : graphs)
-      if (c.iters == kChunk && c.parity == cur && same_params(c.sp, sp)) exec = c.exec.get();
-    if (!exec) {
-      hipGraph_t graph = nullptr;
-      bool ok = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
-      const int cur0 = cur;
-      for (int it = 0; ok && it < kChunk; ++it) ok = enqueue_iteration(sp) == 0;
-      const hipError_t end = hipStreamEndCapture(s, &graph);
-      cur = cur0;
-      if (ok && end == hipSuccess && graph) {
-        CachedGraph c{sp, kChunk, cur0, std::make_unique<GraphExec>()};
-        if (c.exec->instantiate(graph) == 0) {
-          exec = c.exec.get();
-          graphs.push_back(std::move(c));
-        }
-      } else if (graph) {
-        (void)hipGraphDestroy(graph);
-      }
-      (void)hipGetLastError();
-      clear_error();
+  hipGraph_t graph = nullptr;
+  bool ok = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+  const int cur0 = cur;
+  for (int it = 0; ok && it < len; ++it) ok = enqueue_iteration(sp) == 0;
+  const hipError_t end = hipStreamEndCapture(s, &graph);
+  cur = cur0;
+  GraphExec* exec = nullptr;
+  if (ok && end == hipSuccess && graph) {
+    CachedGraph c{sp, len, cur0, std::make_unique<GraphExec>()};
+    if (c.exec->instantiate(graph) == 0) {
+      if (graphs.size() >= kGraphCache) graphs.erase(graphs.begin());
+      exec = c.exec.get();
+      graphs.push_back(std::move(c));
     }
-    if (exec) {
-      while (left >= kChunk) {
-        FE_CHECK(hipGraphLaunch(exec->get(), s));
-        left -= kChunk;
-      }
-    }
+  } else if (graph) {
+    (void)hipGraphDestroy(graph);
   }
-  for (; left > 0; --left)
-    if (enqueue_iteration(sp)) return 1;
+  (void)hipGetLastError();
+  clear_error();
+  return exec;
+}
+
+int TsnePlan::run(int iters, const StepParams& sp) {
+  hipStream_t s = stream.get();
+  const bool capture = graph_capture_enabled();
+  for (int left = iters; left > 0;) {
+    const int len = std::min(left, kGraphChunk);
+    GraphExec* exec = capture && len >= kGraphMin ? chunk_graph(len, sp) : nullptr;
+    if (exec) {
+      FE_CHECK(hipGraphLaunch(exec->get(), s));
+      cur ^= len & 1;
+    } else {
+      for (int it = 0; it < len; ++it)
+        if (enqueue_iteration(sp)) return 1;
+    }
+    left -= len;
+  }
   iters_run += iters;
+  return 0;
+}
+
+// Captures the graphs that run(iters, sp) will replay when it starts after
+// `after` more iterations, without running anything.
+int TsnePlan::prepare(int iters, const StepParams& sp, long long after) {
+  if (!graph_capture_enabled()) return 0;
+  const int cur0 = cur;
+  cur ^= static_cast<int>(after & 1);
+  for (int left = iters; left > 0;) {
+    const int len = std::min(left, kGraphChunk);
+    if (len >= kGraphMin && !chunk_graph(len, sp)) {
+      cur = cur0;
+      return fail("iteration graph capture failed");
+    }
+    cur ^= len & 1;
+    left -= len;
+  }
+  cur = cur0;
   return 0;
 }
 
@@ -949,7 +986,7 @@ class TsneBatch {
     FE_CHECK(hipMemcpyAsync(d_desc + slot, &desc[slot], sizeof(JobDesc), hipMemcpyHostToDevice,
                             stream.get()));
     FE_CHECK(hipStreamSynchronize(stream.get()));
-    slot_bytes[slot] = static_cast<long long>(p.state.capacity() + p.csr_store.capacity());
+    slot_bytes[slot] = p.device_bytes();
     used += slot_bytes[slot];
     slots[slot] = std::move(plan);
     dirty = true;
@@ -1324,6 +1361,18 @@ int fastembedr_hip_tsne_plan_run(fastembedr_hip_tsne_plan* plan, int n_iter, flo
                         fe::StepParams{exaggeration, learning_rate, momentum, min_gain, max_step_norm});
 }
 
+int fastembedr_hip_tsne_plan_prepare(fastembedr_hip_tsne_plan* plan, int n_iter,
+                                     float exaggeration, float momentum, float learning_rate,
+                                     float min_gain, float max_step_norm, long long after) {
+  fe::clear_error();
+  if (!plan) return fail("null plan");
+  if (n_iter < 0 || after < 0) return fail("iteration counts must be non-negative.");
+  if (!(exaggeration > 0.0f) || !(learning_rate > 0.0f) || momentum < 0.0f || !(min_gain > 0.0f))
+    return fail("invalid t-SNE phase parameters");
+  return plan->impl.prepare(
+      n_iter, fe::StepParams{exaggeration, learning_rate, momentum, min_gain, max_step_norm}, after);
+}
+
 int fastembedr_hip_tsne_plan_sync(fastembedr_hip_tsne_plan* plan) {
   if (!plan) return fail("null plan");
   return fe::check(hipStreamSynchronize(plan->impl.stream.get()), "hipStreamSynchronize");
@@ -1343,7 +1392,7 @@ int fastembedr_hip_tsne_plan_info(fastembedr_hip_tsne_plan* plan, long long* inf
   info[2] = p.g;
   info[3] = p.csr.nnz;
   info[4] = p.attr_kind == fe::kAttrRows ? p.tpr : 0;
-  info[5] = static_cast<long long>(p.state.capacity() + p.csr_store.capacity());
+  info[5] = p.device_bytes();
   info[6] = p.iters_run;
   info[7] = static_cast<long long>(p.setup_us);
   return 0;

@@ -39,17 +39,19 @@ class Request:
     finished: float = -1.0
     result: np.ndarray | None = None
     slot: int = -1
+    error: str | None = None
 
 
 @dataclass
 class ServeStats:
     latencies: dict = field(default_factory=dict)
     completed: int = 0
+    failed: int = 0
     wall: float = 0.0
 
     def summary(self):
         lat = np.array(sorted(self.latencies.values())) if self.latencies else np.zeros(1)
-        return {"completed": self.completed, "wall_s": round(self.wall, 4),
+        return {"completed": self.completed, "failed": self.failed, "wall_s": round(self.wall, 4),
                 "jobs_per_s": round(self.completed / max(self.wall, 1e-9), 3),
                 "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
                 "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3)}
@@ -93,8 +95,15 @@ class Server:
                 p = r.payload
                 try:
                     r.slot = self.batch.add(**p)
-                except NotAdmitted:
-                    break
+                except NotAdmitted as exc:
+                    if running:
+                        break  # retried when a running job frees its slot and budget
+                    # nothing holds the device: this job never fits the budget
+                    waiting_embed.popleft()
+                    r.error = f"not admissible: {exc}"
+                    r.finished = now()
+                    stats.failed += 1
+                    continue
                 r.started = now()
                 running[r.slot] = r
                 waiting_embed.popleft()

@@ -263,6 +263,15 @@ class TsnePlan:
                                                       float(min_gain), float(max_step_norm)),
               "HIP t-SNE plan run")
 
+    def prepare(self, n_iter, exaggeration, momentum, learning_rate, min_gain=0.01,
+                max_step_norm=5.0, after=0):
+        """Capture the iteration graphs a later ``run`` with these arguments
+        replays (started after ``after`` further iterations) without running
+        them."""
+        check(_lib.lib().fastembedr_hip_tsne_plan_prepare(
+            self._h, int(n_iter), float(exaggeration), float(momentum), float(learning_rate),
+            float(min_gain), float(max_step_norm), int(after)), "HIP t-SNE plan prepare")
+
     def sync(self):
         check(_lib.lib().fastembedr_hip_tsne_plan_sync(self._h), "HIP t-SNE plan sync")
 

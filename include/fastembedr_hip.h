@@ -103,12 +103,23 @@ int fastembedr_hip_tsne_plan_create(const int* indices, const float* distances,
                                     float perplexity, int grid_size, const float* init,
                                     unsigned int seed, fastembedr_hip_tsne_plan** plan);
 
-/* Enqueue n_iter optimizer iterations with one phase's constants (captured
- * into hipGraphs of up to 50 iterations unless FASTEMBEDR_HIP_GRAPH_CAPTURE=0).
+/* Enqueue n_iter optimizer iterations with one phase's constants (chunks of
+ * up to 50 iterations, each of 4 or more replayed from a captured hipGraph
+ * unless FASTEMBEDR_HIP_GRAPH_CAPTURE=0).
  * Asynchronous: call _plan_sync or _plan_layout to wait. */
 int fastembedr_hip_tsne_plan_run(fastembedr_hip_tsne_plan* plan, int n_iter, float exaggeration,
                                  float momentum, float learning_rate, float min_gain,
                                  float max_step_norm);
+
+/* Captures, without running anything, the iteration graphs that
+ * _plan_run(plan, n_iter, <same constants>) will replay when it is called
+ * after `after` further iterations (runs are cut into chunks of at most 50
+ * iterations; every chunk of 4 or more replays a graph keyed by its length,
+ * constants and starting buffer parity).  Lets a caller move graph
+ * instantiation out of a latency- or time-critical region. */
+int fastembedr_hip_tsne_plan_prepare(fastembedr_hip_tsne_plan* plan, int n_iter,
+                                     float exaggeration, float momentum, float learning_rate,
+                                     float min_gain, float max_step_norm, long long after);
 
 int fastembedr_hip_tsne_plan_sync(fastembedr_hip_tsne_plan* plan);
 
