@@ -67,6 +67,9 @@ SIGNATURES = {
     "fastembedr_hip_umap_optimize_coo_device": (
         c_int, [_vp, _vp, _vp, _vp, _fp, c_int, c_int, c_int, c_int, c_float, c_float, c_float,
                 c_float, c_uint, c_int, _fp]),
+    "fastembedr_hip_umap_optimize_coo_device_noalloc": (
+        c_int, [_vp, _vp, _vp, _vp, _vp, c_int, c_int, c_int, c_int, c_float, c_float, c_float,
+                c_float, c_uint, c_int, _vp]),
     "fastembedr_hip_umap_schedule": (
         c_int, [_ip, _ip, _fp, c_int, c_int, c_int, c_int, c_uint, c_int, _ip, _ip, _ip]),
     "fastembedr_hip_tsne_transform": (

@@ -18,6 +18,8 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "../../include/fastembedr_hip.h"
@@ -417,6 +419,89 @@ int umap_run(const int* heads, const int* tails, const float* weights,
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Epochs that allocate nothing: the fixed-point delta of optimizer_mode 1
+// lives in a file-scope device array instead of a per-call allocation.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocMaxPoints = 1 << 21;  // 2,097,152 points: 32 MiB per device
+constexpr int kMaxDevices = 64;
+
+// Every GPU the library runs on holds its own copy until the process ends.
+// Host code must never pass `scratch` itself to a kernel (in host code the
+// name is a stand-in, not a device address): it uses the per-device address
+// from hipGetSymbolAddress below.
+__device__ __align__(256) unsigned long long scratch[2 * static_cast<size_t>(kNoAllocMaxPoints)];
+
+namespace {
+std::mutex g_scratch_mutex;
+unsigned long long* g_scratch_ptr[kMaxDevices] = {};
+
+__global__ void clear_u64_kernel(unsigned long long* p, int m) {
+  const int i = blockIdx.x * kUThreads + threadIdx.x;
+  if (i < m) p[i] = 0ull;
+}
+}  // namespace
+
+// The current device's address of `scratch`, looked up once per device.
+static int scratch_address(unsigned long long** out) {
+  int dev = 0;
+  FE_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail("device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess || !p)
+      return check(rc != hipSuccess ? rc : hipErrorInvalidSymbol,
+                   "hipGetSymbolAddress(UMAP scratch)");
+    g_scratch_ptr[dev] = static_cast<unsigned long long*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+int umap_run_noalloc(const int* heads, const int* tails, const float* epochs_per_sample,
+                     float* layout, int n, int n_edges, int n_epochs, int nsr, float lr, float a,
+                     float b, float gamma, unsigned seed, int mode, hipStream_t s) {
+  clear_error();
+  if (!heads || !tails || !epochs_per_sample || !layout) return fail("null device pointer");
+  if (n < 2 || n_edges < 1 || n_epochs < 1 || nsr < 0 || !(lr > 0.0f) || !(gamma > 0.0f))
+    return fail("invalid HIP COO UMAP optimization parameters");
+  if (mode != 0 && mode != 1) return fail("optimizer_mode must be 0 (atomic) or 1 (delta)");
+  if (mode == 1 && n > kNoAllocMaxPoints)
+    return fail("the allocation-free UMAP entry point takes at most " +
+                std::to_string(kNoAllocMaxPoints) + " points in optimizer_mode 1 (got " +
+                std::to_string(n) + ")");
+  unsigned long long* delta = nullptr;
+  if (mode == 1 && scratch_address(&delta)) return 1;
+  const int m = 2 * n;
+  const int rblocks = (m + kUThreads - 1) / kUThreads;
+  if (mode == 1) {
+    // the previous call on this device may have stopped part way
+    clear_u64_kernel<<<rblocks, kUThreads, 0, s>>>(delta, m);
+    FE_LAUNCH_CHECK("clear_u64_kernel");
+  }
+  const UmapParams p{n, n_epochs, nsr, seed, lr, a, b, gamma};
+  const int eblocks = static_cast<int>(std::min<long long>((n_edges + kUThreads - 1) / kUThreads, 8192));
+  for (int e = 0; e < n_epochs; ++e) {
+    if (mode == 0) {
+      umap_epoch_kernel<0><<<eblocks, kUThreads, 0, s>>>(layout, delta, heads, tails,
+                                                         epochs_per_sample, n_edges, p,
+                                                         static_cast<unsigned>(e));
+      FE_LAUNCH_CHECK("umap_epoch_kernel");
+      umap_apply_kernel<0><<<rblocks, kUThreads, 0, s>>>(layout, delta, m);
+    } else {
+      umap_epoch_grouped<false><<<eblocks, kUThreads, 0, s>>>(delta, layout, heads, tails,
+                                                              epochs_per_sample, n_edges, p,
+                                                              static_cast<unsigned>(e));
+      FE_LAUNCH_CHECK("umap_epoch_grouped");
+      umap_apply_kernel<1><<<rblocks, kUThreads, 0, s>>>(layout, delta, m);
+    }
+    FE_LAUNCH_CHECK("umap_apply_kernel");
+  }
+  return 0;
+}
+
 }  // namespace fe
 
 extern "C" {
@@ -441,6 +526,17 @@ int fastembedr_hip_umap_optimize_coo_device(const int* heads, const int* tails,
   return fe::umap_run(heads, tails, weights, epochs_per_sample, init, n, n_edges, n_epochs,
                       negative_sample_rate, learning_rate, a, b, repulsion_strength, seed,
                       optimizer_mode, out, true, false);
+}
+
+int fastembedr_hip_umap_optimize_coo_device_noalloc(
+    const int* heads, const int* tails, const float* weights, const float* epochs_per_sample,
+    float* layout, int n, int n_edges, int n_epochs, int negative_sample_rate,
+    float learning_rate, float a, float b, float repulsion_strength, unsigned int seed,
+    int optimizer_mode, void* stream) {
+  (void)weights;
+  return fe::umap_run_noalloc(heads, tails, epochs_per_sample, layout, n, n_edges, n_epochs,
+                              negative_sample_rate, learning_rate, a, b, repulsion_strength, seed,
+                              optimizer_mode, static_cast<hipStream_t>(stream));
 }
 
 int fastembedr_hip_umap_schedule(const int* heads, const int* tails,

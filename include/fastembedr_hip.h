@@ -225,6 +225,25 @@ int fastembedr_hip_umap_optimize_coo_device(const int* heads, const int* tails,
                                             float b, float repulsion_strength, unsigned int seed,
                                             int optimizer_mode, float* out);
 
+/* The epochs of fastembedr_hip_umap_optimize_coo_device with no device
+ * allocation, for callers that cannot allocate (code under hipGraph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  No counterpart in
+ * the reference.  Every pointer is device memory; `layout` (row-major n x 2)
+ * holds the start layout and receives the result in place.  The work is
+ * enqueued on `stream` (a hipStream_t; NULL = the null stream) and the call
+ * returns without waiting.  optimizer_mode 1 keeps its fixed-point delta in
+ * a file-scope device array of 2 x 2,097,152 64-bit words (32 MiB per GPU,
+ * held until the process ends), so it takes n <= 2,097,152 and rejects larger
+ * inputs; mode 0 needs no scratch.  The array is shared by every call on a
+ * GPU and cleared at the start of each call: calls that overlap on one GPU
+ * (concurrent streams or threads) race and must not be issued.  The result
+ * equals fastembedr_hip_umap_optimize_coo_device's bit for bit in mode 1. */
+int fastembedr_hip_umap_optimize_coo_device_noalloc(
+    const int* heads, const int* tails, const float* weights, const float* epochs_per_sample,
+    float* layout, int n, int n_edges, int n_epochs, int negative_sample_rate,
+    float learning_rate, float a, float b, float repulsion_strength, unsigned int seed,
+    int optimizer_mode, void* stream);
+
 /* The sampling schedule of one epoch as the epoch kernel computes it
  * (positive_samples_this_epoch_umap_schedule, negative_samples_..._schedule,
  * deterministic_vertex; embedding_cuda_kernels.cpp:262-274, 3044-3084):
