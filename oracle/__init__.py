@@ -12,6 +12,9 @@ It wraps ``liboracle.so`` built from:
   ``src/embedding_utils.cpp:1582-1752``;
 * ``oracle_umap.c`` -- the UMAP edge-sampled SGD epoch and its schedule/hash
   (``src/embedding_cuda_kernels.cpp:253-274,364-407,3044-3154``);
+* ``oracle_umap_cpu.c`` -- the reference CPU UMAP optimizer
+  (``optimize_layout_csr``, ``src/fast_knn_umap.cpp:3088-3558``): hogwild,
+  delta and serial forms, used as the UMAP CPU baseline;
 * ``oracle_transform.c`` -- the fixed-reference transform
   (``src/tsne_neighbors.cpp:2124-2373``);
 * ``oracle_knn.c`` -- exact Euclidean k-nearest neighbours by FP64 brute
@@ -19,6 +22,10 @@ It wraps ``liboracle.so`` built from:
   with ``finalize_cuvs_knn_kernel``, approximates in float32);
 * ``cpu_port.c`` -- the float32, multi-threaded restatement of the reference
   CPU optimiser used as the timed CPU baseline.
+
+``oracle.ref`` loads ``_ref/libref.so``, the reference's own C++ for the
+functions that need only the standard library (``build_ref.sh``), which pins
+the restatements above where it covers them.
 
 Every array crossing this boundary is C-contiguous; neighbour indices are
 0-based and row-major (n x k).
@@ -93,6 +100,19 @@ def _declare(L):
         _declare_umap_graph(L)
     if hasattr(L, "or_transform"):
         _declare_transform(L)
+    if hasattr(L, "or_umap_cpu_optimize"):
+        ll = ctypes.c_longlong
+        L.or_umap_cpu_active.argtypes = [c_int, _ip, _ip, _fp, c_int, c_int, _ip, _ip, _fp]
+        L.or_umap_cpu_active.restype = ll
+        L.or_umap_cpu_optimize.argtypes = [ll, _ip, _ip, _fp, c_int, _fp, c_int, c_int,
+                                           c_double, c_double, c_double, c_double, c_int, c_int,
+                                           c_int, _dp]
+        L.or_umap_cpu_optimize.restype = c_int
+        L.or_umap_cpu_det_vertex.argtypes = [c_int, c_int, c_int, ll, c_int]
+        L.or_umap_cpu_tau_vertices.argtypes = [c_int, c_int, ll, c_int, c_int, c_int, _ip]
+        L.or_umap_cpu_pow.argtypes = [c_double, c_double]
+        L.or_umap_cpu_pow.restype = c_double
+        L.port_row_probabilities_f.argtypes = [_fp, c_int, c_int, c_double, _fp]
     if hasattr(L, "or_exact_knn"):
         L.or_exact_knn.argtypes = [_fp, c_int, c_int, c_int, _ip, c_int, _ip, _fp]
         L.or_exact_knn.restype = c_int
@@ -489,6 +509,49 @@ def umap_epoch(heads, tails, period, y, epoch, n_epochs, nsr=5, lr=1.0, a=UMAP_A
                         len(yy), int(epoch), int(n_epochs), int(nsr), float(lr), float(a),
                         float(b), float(gamma), int(seed) & 0xFFFFFFFF, int(mode), _ptr(yy, _fp))
     return yy
+
+
+# ------------------------- reference CPU UMAP (A16) ------------------------
+def umap_cpu_active(offsets, neighbors, weights, n_epochs, index_offset=0):
+    """The active edges of optimize_layout_csr: (rows, tails, epochs_per_sample)."""
+    off, nb = _i(offsets), _i(neighbors)
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    n = len(off) - 1
+    m = len(nb)
+    rows, tails = np.zeros(m, np.int32), np.zeros(m, np.int32)
+    eps = np.zeros(m, np.float32)
+    cnt = lib().or_umap_cpu_active(n, _ptr(off, _ip), _ptr(nb, _ip), _ptr(w, _fp),
+                                   int(index_offset), int(n_epochs), _ptr(rows, _ip),
+                                   _ptr(tails, _ip), _ptr(eps, _fp))
+    if cnt < 0:
+        raise ValueError("The CSR graph has no usable UMAP edges." if cnt == -1 else
+                         "The CSR graph has no edges sampled by n_epochs.")
+    return rows[:cnt], tails[:cnt], eps[:cnt]
+
+
+def umap_cpu_optimize(rows, tails, eps, y_init, n_epochs, nsr=5, lr=1.0, a=UMAP_A, b=UMAP_B,
+                      gamma=1.0, seed=1, n_threads=1, form=-1):
+    """The reference CPU UMAP epochs; form -1 picks as the reference does
+    (hogwild / delta / serial).  Returns (layout, form, seconds)."""
+    r, t = _i(rows), _i(tails)
+    e = np.ascontiguousarray(eps, dtype=np.float32)
+    y = np.ascontiguousarray(y_init, dtype=np.float32).copy()
+    sec = ctypes.c_double()
+    used = lib().or_umap_cpu_optimize(len(r), _ptr(r, _ip), _ptr(t, _ip), _ptr(e, _fp), len(y),
+                                      _ptr(y, _fp), int(n_epochs), int(nsr), float(a), float(b),
+                                      float(gamma), float(lr), int(seed), int(n_threads),
+                                      int(form), ctypes.byref(sec))
+    if used < 0:
+        raise ValueError("invalid UMAP CPU optimizer arguments")
+    return y, int(used), sec.value
+
+
+def port_row_probabilities_f(dist, perplexity):
+    d = np.ascontiguousarray(dist, dtype=np.float32)
+    out = np.zeros_like(d)
+    lib().port_row_probabilities_f(_ptr(d, _fp), d.shape[0], d.shape[1], float(perplexity),
+                                   _ptr(out, _fp))
+    return out
 
 
 # ----------------------------- transform ---------------------------------

@@ -836,3 +836,11 @@ int port_tsne_run(int n, int k, const int* idx, const float* dist, const float* 
   pool_destroy(pool);
   return 0;
 }
+
+/* compute_row_probabilities_float (tsne_neighbors.cpp:532-614) over the rows
+ * of a row-major n x k distance matrix, exposed for its pin against the
+ * reference's own function */
+void port_row_probabilities_f(const float* dist, int n, int k, double perplexity, float* out) {
+  for (int i = 0; i < n; ++i)
+    row_probabilities_f(dist + (size_t)i * k, k, perplexity, out + (size_t)i * k);
+}

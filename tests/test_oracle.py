@@ -209,3 +209,43 @@ def test_cpu_port_first_step_matches_fp64_oracle():
                                          early_exag=1, exag=1, learning_rate=1.0, mom0=0,
                                          mom1=0, max_step=1e6, grid=grid, n_threads=2)
         assert relative_l2(got, ref) < tol
+
+
+def _lex_ranks(d2):
+    """1-based ranks of every column in each row, self excluded, ties by index
+    (np.lexsort: the last key is primary)."""
+    n = d2.shape[0]
+    ranks = np.zeros((n, n), np.int64)
+    cols = np.arange(n)
+    for i in range(n):
+        order = np.lexsort((cols, d2[i]))
+        order = order[order != i]
+        ranks[i, order] = np.arange(1, n)
+    return ranks, None
+
+
+def test_structure_metrics_against_sklearn_and_brute_force():
+    # trustworthiness is scikit-learn's (the same definition, tie-free data);
+    # continuity is trustworthiness with the two spaces swapped; the mean
+    # neighbour rank error, |rank in high space of the r-th low neighbour - r|
+    # averaged over n k, is recounted from full lexicographic rank tables
+    from sklearn.manifold import trustworthiness
+
+    rng = np.random.default_rng(31)
+    for n, k in ((120, 5), (300, 10), (300, 15), (61, 20)):
+        high = rng.normal(size=(n, 7))
+        low = high[:, :2] + rng.normal(scale=0.7, size=(n, 2))
+        m = oracle.structure_metrics(high, low, k)
+        assert abs(m[0] - trustworthiness(high, low, n_neighbors=k)) < 1e-12
+        assert abs(m[1] - trustworthiness(low, high, n_neighbors=k)) < 1e-12
+        d_hi = ((high[:, None] - high[None]) ** 2).sum(-1)
+        d_lo = ((low[:, None] - low[None]) ** 2).sum(-1)
+        r_hi, _ = _lex_ranks(d_hi)
+        r_lo, _ = _lex_ranks(d_lo)
+        err = 0
+        for i in range(n):
+            low_nb = np.argsort(r_lo[i] + (np.arange(n) == i) * n)[:k]  # ranks 1..k in order
+            err += np.abs(r_hi[i, low_nb] - np.arange(1, k + 1)).sum()
+        assert abs(m[3] - err / (n * k)) < 1e-12
+    # a wrong normaliser or a dropped term would move these values
+    assert not np.isclose(trustworthiness(high, low, n_neighbors=k), 1.0)
