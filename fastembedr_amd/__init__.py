@@ -6,7 +6,7 @@ Public surface (mirrors the reference's ``backend = "hip"`` entry points):
 fixed layout), ``TsnePlan`` (resident jobs), ``available``.
 """
 from ._lib import HipError, available  # noqa: F401
-from .tsne import (Embedding, TsnePlan, auto_parameters, gradient_diagnostic,  # noqa: F401
-                   grid_size, tsne_knn)
+from .tsne import (Embedding, TsnePlan, auto_parameters, force_diagnostic,  # noqa: F401
+                   gradient_diagnostic, grid_size, tsne_knn)
 
 __version__ = "0.1.0"

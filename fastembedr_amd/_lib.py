@@ -40,6 +40,10 @@ SIGNATURES = {
     "fastembedr_hip_tsne_gradient_diagnostic": (
         c_int, [_ip, _fp, c_int, c_int, c_int, c_float, c_float, c_int, _fp, _fp, _fp, _fp, _dp,
                 _ip, _ip, _fp, _llp]),
+    "fastembedr_hip_opentsne_force_diagnostic": (
+        c_int, [_ip, _fp, c_int, c_int, c_int, c_float, c_float, c_int, _fp, _fp, _fp, _fp, _fp,
+                _fp, _dp, _dp, _ip, _ip, _fp, _llp]),
+    "fastembedr_hip_tsne_plan_kl": (c_int, [_vp, _dp, _dp]),
     "fastembedr_hip_tsne_plan_create": (
         c_int, [_vp, _vp, c_int, c_int, c_int, c_int, c_float, c_int, _fp, c_uint, _vpp]),
     "fastembedr_hip_tsne_plan_run": (

@@ -232,6 +232,14 @@ class TsnePlan {
   LayoutPartial* lpart = nullptr;
   double* zpart = nullptr;
   float2* diag = nullptr;  // 2n scratch for diagnostics / output
+  // exact route (grid_size < 0) and exact Z / KL: per-slice force and Z
+  // partials of exact_rep_kernel, KL block partials (allocated on first use)
+  bool exact = false;
+  Arena exact_store;
+  double2* ex_part = nullptr;
+  double* ex_zpart = nullptr;
+  double* kl_part = nullptr;
+  int ex_bx = 0, ex_splits = 0, ex_chunk = 0;
   int cur = 0;
   // blocks per launch: attraction (TPR lanes per point), per-point kernels,
   // splat binning (point ranges) and splat tiles
@@ -249,6 +257,9 @@ class TsnePlan {
   int create(const int* idx, const DistT* dist, bool device_inputs, int n_, int k_, int offset,
              float perplexity, int grid_size, const float* init, unsigned seed, bool centre);
   int setup_attraction(hipStream_t s);
+  int ensure_exact();
+  int launch_exact(bool forces);
+  int kl(double* kl_out, double* sum_q_out);
   int enqueue_iteration(const StepParams& sp);
   int enqueue_diag(const StepParams& sp);
   GraphExec* chunk_graph(int len, const StepParams& sp);
@@ -398,7 +409,10 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
                      bool centre) {
   n = n_;
   k = k_;
-  g = resolve_grid_size(n, grid_size);
+  // grid_size < 0: exact repulsion; the grid buffers keep their smallest size
+  if (grid_size < -1) return fail("grid_size must be -1 (exact), 0 (the grid rule) or a power of two");
+  exact = grid_size < 0;
+  g = exact ? 16 : resolve_grid_size(n, grid_size);
   f = 2 * g;
   if (g < 16 || g > 1024 || (g & (g - 1)) != 0)
     return fail("grid_size must be a power of two between 16 and 1024");
@@ -530,6 +544,7 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
                                hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (check(rc, "hipFuncSetAttribute(conv_cols)")) return 1;
   }
+  if (exact && ensure_exact()) return 1;
   FE_CHECK(hipStreamSynchronize(s));
   setup_us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
   cur = 0;
@@ -621,8 +636,75 @@ static int launch_update(TsnePlan& p, const StepParams& sp, hipStream_t s) {
   return 0;
 }
 
+// Exact-route buffers: the j range of every point is cut into `splits`
+// slices of `chunk` points (enough blocks to fill the GPU at small n).
+int TsnePlan::ensure_exact() {
+  if (ex_part) return 0;
+  ex_bx = (n + kThreads - 1) / kThreads;
+  const int want = std::max(1, (4 * compute_units() + ex_bx - 1) / ex_bx);
+  ex_splits = std::min({want, 64, std::max(1, n / 256)});
+  ex_chunk = (n + ex_splits - 1) / ex_splits;
+  ArenaPlan ep;
+  ep.add<double2>(static_cast<size_t>(ex_splits) * n);
+  ep.add<double>(static_cast<size_t>(ex_splits) * ex_bx);
+  ep.add<double>(ex_bx);
+  if (exact_store.init(ep.bytes, "t-SNE exact repulsion partials")) return 1;
+  ex_part = exact_store.take<double2>(static_cast<size_t>(ex_splits) * n);
+  ex_zpart = exact_store.take<double>(static_cast<size_t>(ex_splits) * ex_bx);
+  kl_part = exact_store.take<double>(ex_bx);
+  return 0;
+}
+
+int TsnePlan::launch_exact(bool forces) {
+  hipStream_t s = stream.get();
+  const dim3 grid(ex_bx, ex_splits);
+  if (forces)
+    exact_rep_kernel<true><<<grid, kThreads, 0, s>>>(y[cur], st, ex_part, ex_zpart, n, ex_chunk);
+  else
+    exact_rep_kernel<false><<<grid, kThreads, 0, s>>>(y[cur], st, ex_part, ex_zpart, n, ex_chunk);
+  FE_LAUNCH_CHECK("exact_rep_kernel");
+  return 0;
+}
+
+// KL of the current layout with the exact Z (evaluate_kl_f): the Z partials,
+// folded in their slice-major block order on the host and rounded to float
+// as compute_sum_q_f returns Z, then the row sums.
+int TsnePlan::kl(double* kl_out, double* sum_q_out) {
+  if (ensure_exact() || launch_exact(false)) return 1;
+  hipStream_t s = stream.get();
+  const size_t nz = static_cast<size_t>(ex_splits) * ex_bx;
+  std::vector<double> zp(nz), kp(ex_bx);
+  FE_CHECK(hipMemcpyAsync(zp.data(), ex_zpart, nz * sizeof(double), hipMemcpyDeviceToHost, s));
+  FE_CHECK(hipStreamSynchronize(s));
+  double z = 0.0;
+  for (double v : zp) z += v;
+  const double zf = static_cast<double>(static_cast<float>(std::fmax(z, static_cast<double>(FLT_MIN))));
+  kl_rows_kernel<<<ex_bx, kThreads, 0, s>>>(y[cur], st, csr.row_ptr, csr.col, csr.val, n, zf,
+                                            kl_part);
+  FE_LAUNCH_CHECK("kl_rows_kernel");
+  FE_CHECK(hipMemcpyAsync(kp.data(), kl_part, ex_bx * sizeof(double), hipMemcpyDeviceToHost, s));
+  FE_CHECK(hipStreamSynchronize(s));
+  double total = 0.0;
+  for (double v : kp) total += v;
+  if (kl_out) *kl_out = total;
+  if (sum_q_out) *sum_q_out = zf;
+  return 0;
+}
+
 int TsnePlan::enqueue_iteration(const StepParams& sp) {
   hipStream_t s = stream.get();
+  if (exact) {
+    if (ensure_exact() || launch_exact(true)) return 1;
+    if (launch_attract(*this, sp.exaggeration, s)) return 1;
+    update_exact_kernel<0><<<stat_blocks, kThreads, 0, s>>>(
+        y[cur], y[cur ^ 1], gains, upd, diag, ex_part, ex_splits, ex_zpart, ex_splits * ex_bx, st,
+        lpart, n, sp, nullptr);
+    FE_LAUNCH_CHECK("update_exact_kernel");
+    layout_finalize<<<1, kThreads, 0, s>>>(lpart, stat_blocks, st, n, g, 1);
+    FE_LAUNCH_CHECK("layout_finalize");
+    cur ^= 1;
+    return 0;
+  }
   if (launch_splat(*this, s)) return 1;
   if (launch_conv_any(*this, s)) return 1;
   if (launch_attract(*this, sp.exaggeration, s)) return 1;
@@ -728,6 +810,7 @@ int TsnePlan::layout(float* out_host) {
 // attract, update; kernel_ms[6]: whole iteration
 int TsnePlan::profile(int iters, const StepParams& sp, double* ms) {
   constexpr int K = 6;
+  if (exact) return fail("kernel profile covers the FFT-grid route only");
   hipStream_t s = stream.get();
   std::vector<hipEvent_t> ev((K + 1) * iters);
   for (auto& e : ev) FE_CHECK(hipEventCreate(&e));
@@ -927,6 +1010,7 @@ class TsneBatch {
       set_error("no free batch slot");
       return 2;
     }
+    if (j.grid_size < 0) return fail("batched jobs run the FFT-grid route (grid_size >= 0)");
     const int g = resolve_grid_size(j.n, j.grid_size);
     const long long need = job_resident_bytes(j.n, j.k, g);
     if (budget > 0 && used + need + job_setup_bytes(j.n, j.k) > budget) {
@@ -1328,6 +1412,70 @@ int fastembedr_hip_tsne_gradient_diagnostic(const int* indices, const float* dis
   return 0;
 }
 
+int fastembedr_hip_opentsne_force_diagnostic(
+    const int* indices, const float* distances, int n, int k, int index_offset, float perplexity,
+    float exaggeration, int grid_size, const float* layout, float* attractive,
+    float* repulsive_exact, float* gradient_exact, float* repulsive_fft, float* gradient_fft,
+    double* sum_q, double* kl, int* row_ptr, int* col, float* val, long long* nnz) {
+  fe::clear_error();
+  if (fe::validate_tsne(indices, distances, n, k, 2, perplexity, 0, 1, 1.0f, 1.0f, 1.0f, 1, 0.0f,
+                        0.0f, 1.0f, index_offset))
+    return 1;
+  if (!layout) return fail("`layout` must have one row per KNN row and exactly two columns.");
+  if (!(perplexity <= static_cast<float>(k)))
+    return fail("`perplexity` must be positive and no larger than the KNN width.");
+  if (!(exaggeration > 0.0f) || !std::isfinite(exaggeration))
+    return fail("`exaggeration` must be positive and finite.");
+  if (grid_size < 16 || grid_size > 1024 || (grid_size & (grid_size - 1)) != 0)
+    return fail("`grid_size` must be a power of two between 16 and 1024.");
+  for (size_t t = 0; t < 2 * static_cast<size_t>(n); ++t)
+    if (!std::isfinite(layout[t])) return fail("`layout` must contain only finite coordinates.");
+  fe::TsnePlan p;
+  if (p.create<float>(indices, distances, false, n, k, index_offset, perplexity, grid_size, layout,
+                      0u, false))
+    return 1;
+  hipStream_t s = p.stream.get();
+  const size_t m = 2 * static_cast<size_t>(n);
+  std::vector<float> a(m), rf(m), re(m);
+  // FFT route: attraction into diag[0, n), grid repulsion into diag[n, 2n)
+  if (p.enqueue_diag(fe::StepParams{exaggeration, 0.f, 0.f, 0.01f, 0.f})) return 1;
+  FE_CHECK(hipMemcpyAsync(a.data(), p.diag, m * sizeof(float), hipMemcpyDeviceToHost, s));
+  FE_CHECK(hipMemcpyAsync(rf.data(), p.diag + n, m * sizeof(float), hipMemcpyDeviceToHost, s));
+  FE_CHECK(hipStreamSynchronize(s));
+  // exact route: pair repulsion into diag[n, 2n), then Z and KL
+  if (p.ensure_exact() || p.launch_exact(true)) return 1;
+  fe::update_exact_kernel<1><<<p.stat_blocks, fe::kThreads, 0, s>>>(
+      p.y[p.cur], nullptr, nullptr, nullptr, p.diag, p.ex_part, p.ex_splits, p.ex_zpart,
+      p.ex_splits * p.ex_bx, p.st, p.lpart, n, fe::StepParams{}, p.diag + n);
+  FE_LAUNCH_CHECK("update_exact_kernel");
+  FE_CHECK(hipMemcpyAsync(re.data(), p.diag + n, m * sizeof(float), hipMemcpyDeviceToHost, s));
+  FE_CHECK(hipStreamSynchronize(s));
+  double h_kl = 0.0, h_sq = 0.0;
+  if (p.kl(&h_kl, &h_sq)) return 1;
+  if (attractive) std::memcpy(attractive, a.data(), m * sizeof(float));
+  if (repulsive_fft) std::memcpy(repulsive_fft, rf.data(), m * sizeof(float));
+  if (repulsive_exact) std::memcpy(repulsive_exact, re.data(), m * sizeof(float));
+  if (gradient_fft)
+    for (size_t t = 0; t < m; ++t) gradient_fft[t] = a[t] + rf[t];
+  if (gradient_exact)
+    for (size_t t = 0; t < m; ++t) gradient_exact[t] = a[t] + re[t];
+  if (sum_q) *sum_q = h_sq;
+  if (kl) *kl = h_kl;
+  if (nnz) *nnz = p.csr.nnz;
+  if (row_ptr)
+    FE_CHECK(hipMemcpy(row_ptr, p.csr.row_ptr, (n + 1) * sizeof(int), hipMemcpyDeviceToHost));
+  if (col) FE_CHECK(hipMemcpy(col, p.csr.col, p.csr.nnz * sizeof(int), hipMemcpyDeviceToHost));
+  if (val) FE_CHECK(hipMemcpy(val, p.csr.val, p.csr.nnz * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int fastembedr_hip_tsne_plan_kl(fastembedr_hip_tsne_plan* plan, double* kl, double* sum_q) {
+  fe::clear_error();
+  if (!plan) return fail("null plan");
+  FE_CHECK(hipStreamSynchronize(plan->impl.stream.get()));
+  return plan->impl.kl(kl, sum_q);
+}
+
 int fastembedr_hip_tsne_plan_create(const int* indices, const float* distances,
                                     int device_inputs, int n, int k, int index_offset,
                                     float perplexity, int grid_size, const float* init,
@@ -1389,7 +1537,7 @@ int fastembedr_hip_tsne_plan_info(fastembedr_hip_tsne_plan* plan, long long* inf
   const fe::TsnePlan& p = plan->impl;
   info[0] = p.n;
   info[1] = p.k;
-  info[2] = p.g;
+  info[2] = p.exact ? -1 : p.g;
   info[3] = p.csr.nnz;
   info[4] = p.attr_kind == fe::kAttrRows ? p.tpr : 0;
   info[5] = p.device_bytes();

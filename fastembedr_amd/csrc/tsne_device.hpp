@@ -1649,6 +1649,45 @@ __global__ __launch_bounds__(kThreads) void blk_pass_ranges(const int* __restric
 //         (layout_finalize folds them into the next iteration's geometry).
 // MODE 1: diagnostic; writes the repulsive force to diag_rep.
 // ----------------------------------------------------------------------------
+// The optimiser step of one point from its centred position and repulsive
+// force: gains, momentum, step clip (apply_open_tsne_update_f,
+// tsne_neighbors.cpp:1297-1336), the new position into y_out and into this
+// thread's layout statistics.
+__device__ __forceinline__ void opt_step(int i, float xi, float yi, float rx, float ry,
+                                         const float2* __restrict__ attr,
+                                         float2* __restrict__ gains, float2* __restrict__ upd,
+                                         float2* __restrict__ y_out, const StepParams& sp,
+                                         LayoutPartial& m) {
+  const float2 a = attr[i];
+  const float gx = rx + a.x, gy = ry + a.y;
+  float2 gn = gains[i];
+  const float2 old = upd[i];
+  gn.x = sign_tsne(old.x) != sign_tsne(gx) ? gn.x + 0.2f : gn.x * 0.8f + sp.min_gain;
+  gn.y = sign_tsne(old.y) != sign_tsne(gy) ? gn.y + 0.2f : gn.y * 0.8f + sp.min_gain;
+  gn.x = fmaxf(gn.x, sp.min_gain);
+  gn.y = fmaxf(gn.y, sp.min_gain);
+  float sx = sp.momentum * old.x - sp.learning_rate * gn.x * gx;
+  float sy = sp.momentum * old.y - sp.learning_rate * gn.y * gy;
+  if (isfinite(sp.max_step_norm) && sp.max_step_norm > 0.0f) {
+    const float norm2 = sx * sx + sy * sy;
+    if (norm2 > sp.max_step_norm * sp.max_step_norm) {
+      const float scale = sp.max_step_norm / sqrtf(fmaxf(norm2, FLT_MIN));
+      sx *= scale;
+      sy *= scale;
+    }
+  }
+  const float nx = xi + sx, ny = yi + sy;
+  gains[i] = gn;
+  upd[i] = make_float2(sx, sy);
+  y_out[i] = make_float2(nx, ny);
+  m.sx += nx;
+  m.sy += ny;
+  m.mnx = fminf(m.mnx, nx);
+  m.mxx = fmaxf(m.mxx, nx);
+  m.mny = fminf(m.mny, ny);
+  m.mxy = fmaxf(m.mxy, ny);
+}
+
 template <int MODE>
 __device__ __forceinline__ void update_body(const float2* __restrict__ y_in,
                                             float2* __restrict__ y_out,
@@ -1682,34 +1721,7 @@ __device__ __forceinline__ void update_body(const float2* __restrict__ y_in,
     if constexpr (MODE == 1) {
       diag_rep[i] = make_float2(rx, ry);
     } else {
-      const float2 a = attr[i];
-      const float gx = rx + a.x, gy = ry + a.y;
-      float2 gn = gains[i];
-      const float2 old = upd[i];
-      gn.x = sign_tsne(old.x) != sign_tsne(gx) ? gn.x + 0.2f : gn.x * 0.8f + sp.min_gain;
-      gn.y = sign_tsne(old.y) != sign_tsne(gy) ? gn.y + 0.2f : gn.y * 0.8f + sp.min_gain;
-      gn.x = fmaxf(gn.x, sp.min_gain);
-      gn.y = fmaxf(gn.y, sp.min_gain);
-      float sx = sp.momentum * old.x - sp.learning_rate * gn.x * gx;
-      float sy = sp.momentum * old.y - sp.learning_rate * gn.y * gy;
-      if (isfinite(sp.max_step_norm) && sp.max_step_norm > 0.0f) {
-        const float norm2 = sx * sx + sy * sy;
-        if (norm2 > sp.max_step_norm * sp.max_step_norm) {
-          const float scale = sp.max_step_norm / sqrtf(fmaxf(norm2, FLT_MIN));
-          sx *= scale;
-          sy *= scale;
-        }
-      }
-      const float nx = xi + sx, ny = yi + sy;
-      gains[i] = gn;
-      upd[i] = make_float2(sx, sy);
-      y_out[i] = make_float2(nx, ny);
-      m.sx += nx;
-      m.sy += ny;
-      m.mnx = fminf(m.mnx, nx);
-      m.mxx = fmaxf(m.mxx, nx);
-      m.mny = fminf(m.mny, ny);
-      m.mxy = fmaxf(m.mxy, ny);
+      opt_step(i, xi, yi, rx, ry, attr, gains, upd, y_out, sp, m);
     }
   }
   if constexpr (MODE == 0) {
@@ -1735,6 +1747,168 @@ __global__ void centred_copy_kernel(const float2* __restrict__ y, const TsneStat
   if (i < n) {
     const float2 p = y[i];
     out[i] = make_float2(p.x - st->mean_xf, p.y - st->mean_yf);
+  }
+}
+
+// ----------------------------------------------------------------------------
+// 6. Exact repulsion: the pair form of compute_gradient_pair_symmetric_f
+//    (tsne_neighbors.cpp:1090-1150), the reference CPU route for n <= 3000.
+//    For every point i the unnormalised force sum_{j != i} -q_ij^2 (y_i - y_j)
+//    and Z = sum_{i != j} q_ij with q_ij = 1 / (1 + |y_i - y_j|^2).
+//    One thread per point; blockIdx.y splits the j range into `splits`
+//    contiguous slices so small layouts still fill the GPU.  A slice streams
+//    the centred layout through LDS in tiles of kExactTile points; each tile's
+//    float partial joins FP64 running sums, so every sum has a fixed order.
+//    Partials: part[s * n + i] (force of slice s) and zpart[s * gridDim.x + b]
+//    (Z of block b's points over slice s); the readers fold them in that order.
+// ----------------------------------------------------------------------------
+constexpr int kExactTile = 512;
+
+template <bool FORCES>
+__global__ __launch_bounds__(kThreads) void exact_rep_kernel(const float2* __restrict__ y,
+                                                             const TsneStats* __restrict__ st,
+                                                             double2* __restrict__ part,
+                                                             double* __restrict__ zpart, int n,
+                                                             int chunk) {
+  __shared__ float2 tile[kExactTile];
+  __shared__ double zred[kThreads / 64];
+  const float mxf = st->mean_xf, myf = st->mean_yf;
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  const int j0 = blockIdx.y * chunk, j1 = min(n, j0 + chunk);
+  float xi = 0.0f, yi = 0.0f;
+  if (i < n) {
+    const float2 p = y[i];
+    xi = p.x - mxf;
+    yi = p.y - myf;
+  }
+  double fx = 0.0, fy = 0.0, z = 0.0;
+  for (int base = j0; base < j1; base += kExactTile) {
+    const int m = min(kExactTile, j1 - base);
+    __syncthreads();
+    for (int t = threadIdx.x; t < m; t += kThreads) {
+      const float2 p = y[base + t];
+      tile[t] = make_float2(p.x - mxf, p.y - myf);
+    }
+    __syncthreads();
+    float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+    for (int t = 0; t < m; ++t) {
+      const float dx = xi - tile[t].x, dy = yi - tile[t].y;
+      const float q = base + t == i ? 0.0f : 1.0f / (1.0f + dx * dx + dy * dy);
+      tz += q;
+      if constexpr (FORCES) {
+        const float c = -(q * q);
+        tx += c * dx;
+        ty += c * dy;
+      }
+    }
+    z += tz;
+    if constexpr (FORCES) {
+      fx += tx;
+      fy += ty;
+    }
+  }
+  if constexpr (FORCES) {
+    if (i < n) part[static_cast<size_t>(blockIdx.y) * n + i] = make_double2(fx, fy);
+  }
+  if (i >= n) z = 0.0;
+  z = wave_sum(z);
+  if ((threadIdx.x & 63) == 0) zred[threadIdx.x >> 6] = z;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kThreads / 64; ++w) t += zred[w];
+    zpart[static_cast<size_t>(blockIdx.y) * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// Z from the exact partials, folded in slice-major block order.
+__device__ __forceinline__ double fold_exact_z(const double* __restrict__ zpart, int nz) {
+  __shared__ double zs;
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int b = 0; b < nz; ++b) t += zpart[b];
+    zs = t;
+  }
+  __syncthreads();
+  return zs;
+}
+
+// Update of the exact route.  MODE 0: optimiser step from the normalised exact
+// repulsion (value * 1 / max(Z, FLT_MIN) in float, as the reference scales its
+// summed float force) plus the attraction, the layout partials; block 0 also
+// keeps Z + n in st->sum_q_total (the quantity the FFT route keeps there).
+// MODE 1: diagnostic; the normalised force into diag_rep, Z into st.
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void update_exact_kernel(
+    const float2* __restrict__ y_in, float2* __restrict__ y_out, float2* __restrict__ gains,
+    float2* __restrict__ upd, const float2* __restrict__ attr, const double2* __restrict__ part,
+    int splits, const double* __restrict__ zpart, int nz, TsneStats* st,
+    LayoutPartial* partials, int n, StepParams sp, float2* __restrict__ diag_rep) {
+  const double zsum = fold_exact_z(zpart, nz);
+  const float inv_z = static_cast<float>(1.0 / fmax(zsum, static_cast<double>(FLT_MIN)));
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->sum_q_total = zsum + static_cast<double>(n);
+  const float mxf = st->mean_xf, myf = st->mean_yf;
+  LayoutPartial m{0.0, 0.0, FLT_MAX, -FLT_MAX, FLT_MAX, -FLT_MAX};
+  int beg, end;
+  point_range(n, xcd_remap(blockIdx.x, gridDim.x), gridDim.x, &beg, &end);
+  for (int i = beg + threadIdx.x; i < end; i += kThreads) {
+    double sx = 0.0, sy = 0.0;
+    for (int s = 0; s < splits; ++s) {
+      const double2 v = part[static_cast<size_t>(s) * n + i];
+      sx += v.x;
+      sy += v.y;
+    }
+    const float rx = static_cast<float>(sx) * inv_z, ry = static_cast<float>(sy) * inv_z;
+    if constexpr (MODE == 1) {
+      diag_rep[i] = make_float2(rx, ry);
+    } else {
+      const float2 pi = y_in[i];
+      opt_step(i, pi.x - mxf, pi.y - myf, rx, ry, attr, gains, upd, y_out, sp, m);
+    }
+  }
+  if constexpr (MODE == 0) {
+    const LayoutPartial b = block_layout(m);
+    if (threadIdx.x == 0) partials[blockIdx.x] = b;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// 7. KL divergence of P against Q = q / Z (evaluate_kl_f, tsne_neighbors.cpp:
+//    1338-1365): per CSR row the FP64 sum of P_ij log((P_ij + 1e-9) /
+//    (q_ij / Z + 1e-9)), q_ij = 1 / (1 + d2) in FP64 from the float squared
+//    distance of the centred layout, Z the exact sum rounded to float as
+//    compute_sum_q_f returns it.  One thread per row; block partials in row
+//    order (the host adds them in block order).
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void kl_rows_kernel(const float2* __restrict__ y,
+                                                           const TsneStats* __restrict__ st,
+                                                           const int* __restrict__ row_ptr,
+                                                           const int* __restrict__ col,
+                                                           const float* __restrict__ val, int n,
+                                                           double zf, double* __restrict__ part) {
+  __shared__ double red[kThreads / 64];
+  const float mxf = st->mean_xf, myf = st->mean_yf;
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  double row = 0.0;
+  if (i < n) {
+    const float2 pi = y[i];
+    const float xi = pi.x - mxf, yi = pi.y - myf;
+    for (int e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
+      const float2 pj = y[col[e]];
+      const float dx = xi - (pj.x - mxf), dy = yi - (pj.y - myf);
+      const float d2 = dx * dx + dy * dy;
+      const double p = static_cast<double>(val[e]);
+      const double q = (1.0 / (1.0 + static_cast<double>(d2))) / zf;
+      row += p * log((p + 1e-9) / (q + 1e-9));
+    }
+  }
+  row = wave_sum(row);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = row;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kThreads / 64; ++w) t += red[w];
+    part[blockIdx.x] = t;
   }
 }
 

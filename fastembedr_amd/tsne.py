@@ -19,8 +19,8 @@ import numpy as np
 from . import _lib
 from ._lib import _dp, _fp, _ip, _llp, check, ptr
 
-__all__ = ["tsne_knn", "auto_parameters", "grid_size", "gradient_diagnostic", "TsnePlan",
-           "Embedding"]
+__all__ = ["tsne_knn", "auto_parameters", "grid_size", "gradient_diagnostic",
+           "force_diagnostic", "TsnePlan", "Embedding"]
 
 
 class Embedding(np.ndarray):
@@ -100,9 +100,20 @@ def tsne_knn(indices, distances=None, n_neighbors=None, perplexity=None,
     """
     if backend != "hip":
         raise ValueError("this package implements backend = \"hip\" only")
+    # resolve_opentsne_gradient_method (R/tsne_optimizer_helpers.R:61-75) with
+    # the reference's method names (tsne_repulsion_mode,
+    # src/tsne_neighbors.cpp:365-400): "auto" is the FFT grid on a GPU backend
+    # (as for backend = "cuda"); "exact" is the pair form every iteration
     method = str(negative_gradient_method).lower()
-    if method not in ("auto", "fft", "fitsne", "fit_sne", "interpolation"):
-        raise ValueError("HIP openTSNE supports `negative_gradient_method = \"fft\"` only.")
+    if method in ("bh", "barnes_hut", "barnes-hut"):
+        raise ValueError("Barnes-Hut openTSNE has been removed from fastEmbedR. Use "
+                         "`negative_gradient_method = \"fft\"` or `\"exact\"`.")
+    if method in ("exact", "pair", "pair_symmetric"):
+        exact = True
+    elif method in ("auto", "fft", "fitsne", "fit_sne", "interpolation"):
+        exact = False
+    else:
+        raise ValueError(f"unknown `negative_gradient_method` \"{negative_gradient_method}\".")
     if affinity_support not in ("standard", "compact"):
         raise ValueError("`affinity_support` must be \"standard\" or \"compact\".")
     if n_components != 2:
@@ -143,6 +154,8 @@ def tsne_knn(indices, distances=None, n_neighbors=None, perplexity=None,
             raise ValueError("`Y_init` must contain only finite coordinates.")
     offset = _index_offset(idx)
     out = np.zeros((n, 2), np.float32)
+    if exact:
+        grid_size = -1
     L = _lib.lib()
     if dist.dtype == np.float32:
         ci, cd = _column_major(idx, dist, np.float32)
@@ -157,9 +170,11 @@ def tsne_knn(indices, distances=None, n_neighbors=None, perplexity=None,
             normal_iter, ee, ex, lr, int(lr_auto), initial_momentum, final_momentum, min_gain,
             step, seed & 0xFFFFFFFF, offset, int(grid_size), ptr(out, _fp))
     check(rc, "HIP openTSNE FFT-grid")
-    g = int(grid_size) or globals()["grid_size"](n)
-    config = dict(backend="hip", optimizer="opentsne_fitsne_fft_grid_native_hip",
-                  repulsion="fft_grid_hip", fft_grid_size=g,
+    g = None if exact else (int(grid_size) or globals()["grid_size"](n))
+    config = dict(backend="hip",
+                  optimizer=("opentsne_exact_sparse_knn_hip" if exact
+                             else "opentsne_fitsne_fft_grid_native_hip"),
+                  repulsion="pair_symmetric_hip" if exact else "fft_grid_hip", fft_grid_size=g,
                   probabilities="symmetric_sparse_knn_hip_float32", precision="float32",
                   perplexity=perp, n_neighbors=nn, affinity_support=affinity_support,
                   early_exaggeration_iter=early_iter, n_iter=normal_iter,
@@ -210,6 +225,53 @@ def gradient_diagnostic(indices, distances, layout, perplexity, exaggeration=1.0
     m = int(nnz[0])
     return Diagnostic(att, rep, grad, 4.0 * grad, float(sq[0]), row_ptr, col[:m].copy(),
                       val[:m].copy())
+
+
+def force_diagnostic(indices, distances, layout, perplexity, exaggeration=1.0, grid_size=64):
+    """HIP counterpart of ``opentsne_force_diagnostic_cpp``
+    (src/tsne_neighbors.cpp:1698-1817): the whole bundle from a fixed layout --
+    attraction, exact and FFT-grid repulsion, both optimizer and objective
+    gradients, the exact ``sum_q``, ``kl``, the norms and the affinities (with
+    1-based columns, as ``affinity_col1``).  Returns a dict with the
+    reference's field names."""
+    idx, dist = _knn_arrays(indices, distances)
+    n, k = idx.shape
+    if not (0.0 < float(perplexity) <= k):
+        raise ValueError("`perplexity` must be positive and no larger than the KNN width.")
+    y = np.ascontiguousarray(layout, dtype=np.float32)
+    if y.shape != (n, 2):
+        raise ValueError("`layout` must have one row per KNN row and exactly two columns.")
+    ci, cd = _column_major(idx, dist, np.float32)
+    out = {key: np.zeros((n, 2), np.float32) for key in
+           ("attractive_force", "repulsive_force_exact", "optimizer_gradient_exact",
+            "repulsive_force_fft", "optimizer_gradient_fft")}
+    sq, kl = np.zeros(1), np.zeros(1)
+    row_ptr = np.zeros(n + 1, np.int32)
+    col = np.zeros(2 * n * k, np.int32)
+    val = np.zeros(2 * n * k, np.float32)
+    nnz = np.zeros(1, np.int64)
+    rc = _lib.lib().fastembedr_hip_opentsne_force_diagnostic(
+        ptr(ci, _ip), ptr(cd, _fp), n, k, _index_offset(idx), float(perplexity),
+        float(exaggeration), int(grid_size), ptr(y, _fp), ptr(out["attractive_force"], _fp),
+        ptr(out["repulsive_force_exact"], _fp), ptr(out["optimizer_gradient_exact"], _fp),
+        ptr(out["repulsive_force_fft"], _fp), ptr(out["optimizer_gradient_fft"], _fp),
+        ptr(sq, _dp), ptr(kl, _dp), ptr(row_ptr, _ip), ptr(col, _ip), ptr(val, _fp),
+        ptr(nnz, _llp))
+    check(rc, "HIP t-SNE force diagnostic")
+    m = int(nnz[0])
+    out["objective_gradient_exact"] = 4.0 * out["optimizer_gradient_exact"]
+    out["objective_gradient_fft"] = 4.0 * out["optimizer_gradient_fft"]
+    out.update(sum_q=float(sq[0]), kl=float(kl[0]),
+               attractive_norm=float(np.linalg.norm(out["attractive_force"].astype(np.float64))),
+               repulsive_exact_norm=float(np.linalg.norm(
+                   out["repulsive_force_exact"].astype(np.float64))),
+               repulsive_fft_norm=float(np.linalg.norm(
+                   out["repulsive_force_fft"].astype(np.float64))),
+               affinity_row_ptr0=row_ptr, affinity_col1=col[:m] + 1,
+               affinity_weight=val[:m].astype(np.float64), grid_size=int(grid_size),
+               perplexity=float(perplexity), exaggeration=float(exaggeration),
+               precision="float32", objective_gradient_scale=4.0)
+    return out
 
 
 def _device_inputs_ready(*arrays):
@@ -274,6 +336,13 @@ class TsnePlan:
 
     def sync(self):
         check(_lib.lib().fastembedr_hip_tsne_plan_sync(self._h), "HIP t-SNE plan sync")
+
+    def kl(self):
+        """(KL divergence, exact Z) of the current layout (evaluate_kl_f)."""
+        kl, sq = np.zeros(1), np.zeros(1)
+        check(_lib.lib().fastembedr_hip_tsne_plan_kl(self._h, ptr(kl, _dp), ptr(sq, _dp)),
+              "HIP t-SNE plan KL")
+        return float(kl[0]), float(sq[0])
 
     def layout(self):
         out = np.zeros((self.n, 2), np.float32)

@@ -36,7 +36,8 @@ int fastembedr_hip_opentsne_fft_grid_size(int n);
 
 /* ---- one-shot t-SNE (reference: embedding_cuda_kernels.cpp:6178-6264) ---
  * Same argument order as the CUDA functions plus `grid_size` (0 = the rule
- * above).  n_components must be 2; k <= 256.  learning_rate_auto != 0 uses
+ * above, -1 = exact pair repulsion, the reference CPU route for small n).
+ * n_components must be 2; k <= 256.  learning_rate_auto != 0 uses
  * n / exaggeration of each phase (:6077-6082).  Without init the layout is
  * N(0, 1e-4^2) from std::mt19937(seed) (tsne_neighbors.cpp:1944-1950).      */
 
@@ -88,6 +89,23 @@ int fastembedr_hip_tsne_gradient_diagnostic(const int* indices, const float* dis
                                             int* row_ptr, int* col, float* val,
                                             long long* nnz);
 
+/* Replaces opentsne_force_diagnostic_cpp (tsne_neighbors.cpp:1698-1817), the
+ * whole bundle: from a fixed row-major n x 2 layout (used as given, not
+ * centred), the attractive force (exaggeration included), the exact pair
+ * repulsion (compute_gradient_pair_symmetric_f, :1090-1150) and the FFT-grid
+ * repulsion at `grid_size` (16..1024, a power of two), the two optimizer
+ * gradients (attraction + repulsion; the objective gradient is 4 times
+ * these), the exact Z = sum_{i != j} q_ij rounded to float (compute_sum_q_f,
+ * :1034-1059), the KL divergence of P against q / Z (evaluate_kl_f,
+ * :1338-1365), and optionally the symmetric P as 0-based CSR (row_ptr n+1,
+ * col/val capacity 2*n*k).  P is the FP64-searched device P.  Host pointers;
+ * any output may be NULL.  The exact terms cost O(n^2). */
+int fastembedr_hip_opentsne_force_diagnostic(
+    const int* indices, const float* distances, int n, int k, int index_offset, float perplexity,
+    float exaggeration, int grid_size, const float* layout, float* attractive,
+    float* repulsive_exact, float* gradient_exact, float* repulsive_fft, float* gradient_fft,
+    double* sum_q, double* kl, int* row_ptr, int* col, float* val, long long* nnz);
+
 /* ---- resident plans ----------------------------------------------------
  * No counterpart in the reference (which frees everything per call,
  * embedding_cuda_kernels.cpp:88-127): a plan keeps P, the layout, the
@@ -97,7 +115,10 @@ int fastembedr_hip_tsne_gradient_diagnostic(const int* indices, const float* dis
 typedef struct fastembedr_hip_tsne_plan fastembedr_hip_tsne_plan;
 
 /* indices/distances host (device_inputs = 0) or device (1) pointers in the
- * column-major layout above; init is host row-major n x 2 or NULL. */
+ * column-major layout above; init is host row-major n x 2 or NULL.
+ * grid_size: 0 = the grid rule, a power of two 16..1024, or -1 = exact pair
+ * repulsion every iteration (the reference CPU route for n <= 3000,
+ * compute_gradient_pair_symmetric_f, tsne_neighbors.cpp:1090-1150; O(n^2)). */
 int fastembedr_hip_tsne_plan_create(const int* indices, const float* distances,
                                     int device_inputs, int n, int k, int index_offset,
                                     float perplexity, int grid_size, const float* init,
@@ -136,6 +157,13 @@ int fastembedr_hip_tsne_plan_info(fastembedr_hip_tsne_plan* plan, long long* inf
  * (tsne_neighbors.cpp:1793-1808), with 0-based columns. */
 int fastembedr_hip_tsne_plan_affinities(fastembedr_hip_tsne_plan* plan, int* row_ptr, int* col,
                                         float* val);
+
+/* KL divergence of the plan's P against the current layout with the exact Z
+ * (evaluate_kl_f, tsne_neighbors.cpp:1338-1365, the cost the reference records
+ * every 50 iterations with record_costs; opentsne_kl_diagnostic_cpp
+ * :1656-1692), and that Z rounded to float.  Waits for queued iterations;
+ * O(n^2) for Z. */
+int fastembedr_hip_tsne_plan_kl(fastembedr_hip_tsne_plan* plan, double* kl, double* sum_q);
 
 /* Runs n_iter eager iterations with hipEvents between the six kernels on
  * the plan's stream; kernel_ms[0..5] = mean ms of splat, conv_rows_fwd,

@@ -143,3 +143,25 @@ def test_query_knn_argument_errors_need_no_device():
             knn.exact_query_knn(x, x[:2], k)
     with pytest.raises(HipError, match="at least one row"):
         knn.exact_query_knn(x, np.zeros((0, 3), np.float32), 1)
+
+
+def test_gradient_method_and_grid_arguments_need_no_device():
+    # tsne_repulsion_mode (src/tsne_neighbors.cpp:365-400): Barnes-Hut is
+    # refused; the C-ABI takes grid_size -1 (exact), 0 (rule) or a power of two
+    import numpy as np
+
+    from fastembedr_amd import _lib, tsne_knn
+    from fastembedr_amd._lib import _fp, _ip, ptr
+
+    idx = np.tile(np.arange(1, 4, dtype=np.int32), (40, 1)) % 40
+    dist = np.ones((40, 3), np.float32)
+    with pytest.raises(ValueError, match="Barnes-Hut"):
+        tsne_knn(idx, dist, negative_gradient_method="bh")
+    with pytest.raises(ValueError, match="unknown"):
+        tsne_knn(idx, dist, negative_gradient_method="spectral")
+    out = np.zeros((40, 2), np.float32)
+    ci, cd = np.asfortranarray(idx), np.asfortranarray(dist)
+    rc = _lib.lib().fastembedr_hip_opentsne_fft_from_knn_float(
+        ptr(ci, _ip), ptr(cd, _fp), None, 0, 40, 3, 2, 1.0, 0, 1, 12.0, 1.0, 1.0, 1, 0.5, 0.8,
+        0.01, 5.0, 1, 0, -2, ptr(out, _fp))
+    assert rc == 1 and "grid_size must be -1" in _lib.last_error()
