@@ -150,7 +150,7 @@ def bench_served(fe, workloads, n_requests, k=90):
     n_ref = 500_000
     xr, lr_, centres = workloads.gaussian_mixture(n_ref, 50, 50, seed=402, dirichlet=True)
     weights = np.bincount(lr_, minlength=len(centres)) / n_ref
-    ki, kd = workloads.mixture_knn_device(xr, lr_, centres, k)
+    ki, kd = workloads.exact_knn_tensors(xr, k)
     ref_layout = run_plan_alone(fe, ki, kd, n_ref, k, workloads.centred_normal_init(n_ref, 403))
     del ki, kd
     torch.cuda.empty_cache()
@@ -171,7 +171,7 @@ def bench_served(fe, workloads, n_requests, k=90):
         else:
             m = int(workloads.log_uniform_sizes(1, 1_000, 50_000, rng)[0])
             xq, lq = workloads.sample_mixture(m, centres, weights, rng)
-            qi, qd = workloads.knn_to_reference(xq, lq, xr, lr_, centres, k)
+            qi, qd = workloads.knn_to_reference(xq, xr, k)
             reqs.append(serve.Request(rid, "transform", t, dict(indices=qi, distances=qd)))
     torch.cuda.synchronize()
     n_embed = sum(r.kind == "embed" for r in reqs)
