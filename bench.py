@@ -355,6 +355,8 @@ def main():
                        n=n, k=k, index_offset=0)
     setup_s = time.perf_counter() - t0
     info = plan.info()
+    attr_info = plan.attraction_info()
+    log(f"[bench] attraction {attr_info}")
     umap_graph = None
     if args.umap_graph_k > 0:
         umap_graph = bench_umap_graph(fe, knn_i, knn_d, n, min(args.umap_graph_k, k),
@@ -509,6 +511,7 @@ def main():
                                f"(Dirichlet(1) sizes), exact k={k} KNN, perplexity 30, 2D, "
                                f"FFT grid {g}x{g}",
                    "n": n, "k": k, "grid": g, "nnz_sym": info["nnz"],
+                   "attraction": attr_info,
                    "schedule": "250 early (12) + 500 normal", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

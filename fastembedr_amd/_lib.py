@@ -54,6 +54,7 @@ SIGNATURES = {
     "fastembedr_hip_tsne_plan_layout": (c_int, [_vp, _fp]),
     "fastembedr_hip_tsne_plan_info": (c_int, [_vp, _llp]),
     "fastembedr_hip_tsne_plan_affinities": (c_int, [_vp, _ip, _ip, _fp]),
+    "fastembedr_hip_tsne_plan_attraction_info": (c_int, [_vp, _llp]),
     "fastembedr_hip_tsne_plan_profile": (
         c_int, [_vp, c_int, c_float, c_float, c_float, c_float, c_float, _dp]),
     "fastembedr_hip_tsne_plan_destroy": (None, [_vp]),

@@ -219,6 +219,11 @@ class TsnePlan {
   long long seg_entries = 0;
   int groups = 0;
   int attr_kind = 0;  // AttractionKind
+  // pair-once attraction: the staged and rest streams, chunks, slots
+  Arena pair_store;
+  PairTables pairs{};
+  int pair_ranges = 0, pair_nchunks = 0, pair_nslots = 0;
+  long long pair_entries = 0, rest_entries = 0;
   float2* y[2] = {nullptr, nullptr};
   float2* gains = nullptr;
   float2* upd = nullptr;
@@ -257,6 +262,7 @@ class TsnePlan {
   int create(const int* idx, const DistT* dist, bool device_inputs, int n_, int k_, int offset,
              float perplexity, int grid_size, const float* init, unsigned seed, bool centre);
   int setup_attraction(hipStream_t s);
+  int setup_pairs(hipStream_t s, bool* built);
   int ensure_exact();
   int launch_exact(bool forces);
   int kl(double* kl_out, double* sum_q_out);
@@ -267,7 +273,8 @@ class TsnePlan {
   int prepare(int iters, const StepParams& sp, long long after);
   long long device_bytes() const {
     return static_cast<long long>(state.capacity() + csr_store.capacity() +
-                                  group_store.capacity() + attr_store.capacity());
+                                  group_store.capacity() + attr_store.capacity() +
+                                  pair_store.capacity() + exact_store.capacity());
   }
   int layout(float* out_host);
   int profile(int iters, const StepParams& sp, double* ms);
@@ -283,21 +290,271 @@ static int choose_tpr(double avg_row) {
 
 // FASTEMBEDR_HIP_ATTRACTION selects the attraction kernel: rows (one lane
 // group per CSR row), lanes (segment-blocked, points read from the layout),
-// staged (segment-blocked, window passes read the LDS windows; the default).
-// lanes and staged sum the same terms in the same order and agree bit for
-// bit.
+// staged (segment-blocked, window passes read the LDS windows), pairs (every
+// pair once, tiles staged in LDS; the default where most pairs fall in staged
+// tiles, else staged).  lanes and staged sum the same terms in the same order
+// and agree bit for bit.
 static int attraction_kind() {
   const char* v = std::getenv("FASTEMBEDR_HIP_ATTRACTION");
-  if (!v || !*v) return kAttrStaged;
+  if (!v || !*v) return kAttrPairs;
   if (std::strcmp(v, "rows") == 0) return kAttrRows;
   if (std::strcmp(v, "lanes") == 0) return kAttrLanes;
-  return kAttrStaged;
+  if (std::strcmp(v, "staged") == 0) return kAttrStaged;
+  return kAttrPairs;
+}
+
+// The pair-once tables (tsne_pairs.hpp).  *built stays false when P's tiles
+// do not suit them (too few pairs in staged tiles, or sizes beyond the
+// tables' 32-bit offsets): the caller then sets up the segment-blocked kernel.
+int TsnePlan::setup_pairs(hipStream_t s, bool* built) {
+  *built = false;
+  const int nb = (n + kPairBlock - 1) / kPairBlock;
+  if (nb > kPairMaxBlocks || csr.nnz <= 0) return 0;
+  const size_t nt = static_cast<size_t>(nb) * nb;
+  Arena scratch;
+  ArenaPlan sp;
+  sp.add<unsigned>(nt);   // pairs per tile
+  sp.add<unsigned>(nt);   // padded pairs per tile
+  sp.add<int>(nt);        // dense tile ids
+  sp.add<unsigned>(4);    // largest row sum
+  sp.add<int>(n);         // rest entries per row
+  sp.add<unsigned>(nb);   // padded rest entries per row block
+  sp.add<unsigned>(nt);   // tile cursors
+  sp.add<unsigned>(nb);   // rest cursors
+  if (scratch.init(sp.bytes, "t-SNE pair tables set-up")) return 1;
+  unsigned* tile_cnt = scratch.take<unsigned>(nt);
+  unsigned* tile_pad = scratch.take<unsigned>(nt);
+  int* tile_id = scratch.take<int>(nt);
+  unsigned* maxbits = scratch.take<unsigned>(4);
+  int* rest_len = scratch.take<int>(n);
+  unsigned* rest_pad = scratch.take<unsigned>(nb);
+  unsigned* tile_cursor = scratch.take<unsigned>(nt);
+  unsigned* rest_cursor = scratch.take<unsigned>(nb);
+  FE_CHECK(hipMemsetAsync(tile_cnt, 0, nt * sizeof(unsigned), s));
+  FE_CHECK(hipMemsetAsync(tile_pad, 0, nt * sizeof(unsigned), s));
+  FE_CHECK(hipMemsetAsync(maxbits, 0, 4 * sizeof(unsigned), s));
+  FE_CHECK(hipMemsetAsync(rest_pad, 0, nb * sizeof(unsigned), s));
+  const int rb = (n + kThreads - 1) / kThreads;
+  pair_count_kernel<<<rb, kThreads, 0, s>>>(csr.row_ptr, csr.col, csr.val, n, nb, tile_cnt, tile_pad,
+                                            maxbits);
+  FE_LAUNCH_CHECK("pair_count_kernel");
+  std::vector<unsigned> hcnt(nt), hpad(nt);
+  unsigned hmax = 0;
+  FE_CHECK(hipMemcpyAsync(hcnt.data(), tile_cnt, nt * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  FE_CHECK(hipMemcpyAsync(hpad.data(), tile_pad, nt * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  FE_CHECK(hipMemcpyAsync(&hmax, maxbits, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  FE_CHECK(hipStreamSynchronize(s));
+  // staged tiles: every diagonal tile with pairs, off-diagonal ones with
+  // enough pairs to pay for staging their second block; their stream regions
+  // in (I, J) order
+  struct Tile {
+    int I, J;
+    long long base, len;
+  };
+  std::vector<Tile> tiles;
+  std::vector<int> hid(nt, -1);
+  long long total = 0, dense_pairs = 0, all_pairs = 0;
+  // FASTEMBEDR_HIP_PAIR_DENSE_MIN overrides the staging threshold
+  long long dense_min = kPairDenseMin;
+  if (const char* v = std::getenv("FASTEMBEDR_HIP_PAIR_DENSE_MIN"))
+    if (std::atoll(v) > 0) dense_min = std::atoll(v);
+  for (int I = 0; I < nb; ++I)
+    for (int J = I; J < nb; ++J) {
+      const size_t t = static_cast<size_t>(I) * nb + J;
+      all_pairs += hcnt[t];
+      if (hcnt[t] == 0 || (I != J && hcnt[t] < dense_min)) continue;
+      hid[t] = static_cast<int>(tiles.size());
+      tiles.push_back(Tile{I, J, total, static_cast<long long>(hpad[t])});
+      total += hpad[t];
+      dense_pairs += hcnt[t];
+    }
+  // the rest in both directions costs about 2.6 times a staged pair: below
+  // half the pairs staged the segment-blocked kernel is as good
+  if (2 * dense_pairs < all_pairs || total + kPairPad >= (1ll << 31)) return 0;
+  FE_CHECK(hipMemcpyAsync(tile_id, hid.data(), nt * sizeof(int), hipMemcpyHostToDevice, s));
+  pair_rest_count_kernel<<<rb, kThreads, 0, s>>>(csr.row_ptr, csr.col, n, nb, tile_id, rest_len,
+                                                 rest_pad);
+  FE_LAUNCH_CHECK("pair_rest_count_kernel");
+  std::vector<unsigned> hrest(nb);
+  FE_CHECK(hipMemcpyAsync(hrest.data(), rest_pad, nb * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  FE_CHECK(hipStreamSynchronize(s));
+  std::vector<long long> rbase(nb + 1, 0);
+  for (int I = 0; I < nb; ++I) rbase[I + 1] = rbase[I] + hrest[I];
+  const long long rtotal = rbase[nb];
+  if (rtotal + kPairPad >= (1ll << 31)) return 0;
+  // chunks: each region cut into pieces of at most kPairChunkEntries, row
+  // block by row block (its tiles in J order, then its rest)
+  std::vector<PairChunk> chunks;
+  auto cut = [&](int I, int J, long long b0, long long len, int rest) {
+    for (long long o = 0; o < len; o += kPairChunkEntries) {
+      PairChunk c{};
+      c.I = I;
+      c.J = J;
+      c.e0 = static_cast<int>(b0 + o);
+      c.e1 = static_cast<int>(b0 + std::min<long long>(len, o + kPairChunkEntries));
+      c.rest = rest;
+      chunks.push_back(c);
+    }
+  };
+  for (size_t t = 0; t < tiles.size();) {
+    const int I = tiles[t].I;
+    for (; t < tiles.size() && tiles[t].I == I; ++t) cut(I, tiles[t].J, tiles[t].base, tiles[t].len, 0);
+  }
+  // rest regions go right after their row block's tiles
+  {
+    std::vector<PairChunk> ordered;
+    size_t k = 0;
+    for (int I = 0; I < nb; ++I) {
+      for (; k < chunks.size() && chunks[k].I == I; ++k) ordered.push_back(chunks[k]);
+      const size_t before = chunks.size();
+      if (hrest[I]) cut(I, I, rbase[I], hrest[I], 1);
+      for (size_t r = before; r < chunks.size(); ++r) ordered.push_back(chunks[r]);
+      chunks.resize(before);
+    }
+    chunks.swap(ordered);
+  }
+  // workgroups: contiguous ranges of about equal entries, two per compute
+  // unit (or one per chunk when there are fewer)
+  const int nc = static_cast<int>(chunks.size());
+  const long long all_entries = total + rtotal;
+  const int ranges = std::max(1, std::min(nc, 2 * compute_units()));
+  std::vector<int> range_first(ranges + 1, nc);
+  range_first[0] = 0;
+  {
+    long long acc = 0;
+    int r = 1;
+    for (int c = 0; c < nc && r < ranges; ++c) {
+      acc += chunks[c].e1 - chunks[c].e0;
+      while (r < ranges && acc * ranges >= all_entries * r) range_first[r++] = c + 1;
+    }
+    for (; r < ranges; ++r) range_first[r] = nc;
+  }
+  // slots: one per run of chunks of a range sharing a row block (row side)
+  // and per run sharing a staged off-diagonal tile (column side)
+  std::vector<std::vector<int>> block_slots(nb);
+  int nslots = 0;
+  for (int r = 0; r < ranges; ++r)
+    for (int c = range_first[r]; c < range_first[r + 1]; ++c) {
+      PairChunk& ch = chunks[c];
+      const bool first = c == range_first[r];
+      const PairChunk* prev = first ? nullptr : &chunks[c - 1];
+      if (first || prev->I != ch.I) {
+        ch.slot_i = nslots++;
+        block_slots[ch.I].push_back(ch.slot_i);
+      } else {
+        ch.slot_i = prev->slot_i;
+      }
+      const bool cross = !ch.rest && ch.J != ch.I;
+      ch.slot_j = -1;
+      if (cross) {
+        if (!first && !prev->rest && prev->I == ch.I && prev->J == ch.J) {
+          ch.slot_j = prev->slot_j;
+        } else {
+          ch.slot_j = nslots++;
+          block_slots[ch.J].push_back(ch.slot_j);
+        }
+      }
+      const bool last = c + 1 == range_first[r + 1];
+      const PairChunk* next = last ? nullptr : &chunks[c + 1];
+      ch.flush = 0;
+      if (last || next->I != ch.I) ch.flush |= 1;
+      if (cross && (last || next->rest || next->I != ch.I || next->J != ch.J)) ch.flush |= 2;
+    }
+  std::vector<int> hslot_ptr(nb + 1, 0), hslot_list;
+  for (int b = 0; b < nb; ++b) {
+    hslot_ptr[b + 1] = hslot_ptr[b] + static_cast<int>(block_slots[b].size());
+    hslot_list.insert(hslot_list.end(), block_slots[b].begin(), block_slots[b].end());
+  }
+  const size_t ent = static_cast<size_t>(total) + kPairPad;
+  const size_t rent = static_cast<size_t>(rtotal) + kPairPad;
+  ArenaPlan pp;
+  pp.add<unsigned short>(ent);
+  pp.add<float>(ent);
+  pp.add<unsigned short>(ent / kLaneEntries);
+  pp.add<int>(rent);
+  pp.add<float>(rent);
+  pp.add<unsigned short>(rent / kLaneEntries);
+  pp.add<PairChunk>(chunks.size());
+  pp.add<int>(ranges + 1);
+  pp.add<int>(nb + 1);
+  pp.add<int>(std::max<size_t>(1, hslot_list.size()));
+  pp.add<unsigned long long>(static_cast<size_t>(std::max(nslots, 1)) * kPairBlock);
+  if (pair_store.init(pp.bytes, "t-SNE pair attraction tables")) return 1;
+  unsigned short* col16 = pair_store.take<unsigned short>(ent);
+  float* pval = pair_store.take<float>(ent);
+  unsigned short* row16 = pair_store.take<unsigned short>(ent / kLaneEntries);
+  int* rcol = pair_store.take<int>(rent);
+  float* rval = pair_store.take<float>(rent);
+  unsigned short* rrow16 = pair_store.take<unsigned short>(rent / kLaneEntries);
+  PairChunk* dchunk = pair_store.take<PairChunk>(chunks.size());
+  int* drange = pair_store.take<int>(ranges + 1);
+  int* dslot_ptr = pair_store.take<int>(nb + 1);
+  int* dslot_list = pair_store.take<int>(std::max<size_t>(1, hslot_list.size()));
+  unsigned long long* part =
+      pair_store.take<unsigned long long>(static_cast<size_t>(std::max(nslots, 1)) * kPairBlock);
+  // whole streams zeroed: padding past every run and past the ends reads as
+  // value 0 at in-range columns
+  FE_CHECK(hipMemsetAsync(col16, 0, ent * sizeof(unsigned short), s));
+  FE_CHECK(hipMemsetAsync(pval, 0, ent * sizeof(float), s));
+  FE_CHECK(hipMemsetAsync(row16, 0, ent / kLaneEntries * sizeof(unsigned short), s));
+  FE_CHECK(hipMemsetAsync(rcol, 0, rent * sizeof(int), s));
+  FE_CHECK(hipMemsetAsync(rval, 0, rent * sizeof(float), s));
+  FE_CHECK(hipMemsetAsync(rrow16, 0, rent / kLaneEntries * sizeof(unsigned short), s));
+  std::vector<unsigned> hcur(std::max<size_t>(1, tiles.size())), hrcur(nb);
+  for (size_t t = 0; t < tiles.size(); ++t) hcur[t] = static_cast<unsigned>(tiles[t].base);
+  for (int I = 0; I < nb; ++I) hrcur[I] = static_cast<unsigned>(rbase[I]);
+  if (!tiles.empty())
+    FE_CHECK(hipMemcpyAsync(tile_cursor, hcur.data(), tiles.size() * sizeof(unsigned),
+                            hipMemcpyHostToDevice, s));
+  FE_CHECK(hipMemcpyAsync(rest_cursor, hrcur.data(), nb * sizeof(unsigned), hipMemcpyHostToDevice, s));
+  pair_fill_kernel<<<rb, kThreads, 0, s>>>(csr.row_ptr, csr.col, csr.val, n, nb, tile_id, rest_len,
+                                           tile_cursor, rest_cursor, col16, pval, row16, rcol, rval,
+                                           rrow16);
+  FE_LAUNCH_CHECK("pair_fill_kernel");
+  FE_CHECK(hipMemcpyAsync(dchunk, chunks.data(), chunks.size() * sizeof(PairChunk),
+                          hipMemcpyHostToDevice, s));
+  FE_CHECK(hipMemcpyAsync(drange, range_first.data(), (ranges + 1) * sizeof(int),
+                          hipMemcpyHostToDevice, s));
+  FE_CHECK(hipMemcpyAsync(dslot_ptr, hslot_ptr.data(), (nb + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+  if (!hslot_list.empty())
+    FE_CHECK(hipMemcpyAsync(dslot_list, hslot_list.data(), hslot_list.size() * sizeof(int),
+                            hipMemcpyHostToDevice, s));
+  FE_CHECK(hipStreamSynchronize(s));
+  // the largest row sum of P (float sums; 1% headroom) bounds the sums
+  float maxrow = 0.0f;
+  std::memcpy(&maxrow, &hmax, sizeof(float));
+  if (!std::isfinite(maxrow) || maxrow <= 0.0f) return 0;
+  pairs = PairTables{col16, pval, row16, rcol, rval, rrow16, dchunk, drange, part, dslot_ptr,
+                     dslot_list, 1.01 * static_cast<double>(maxrow)};
+  pair_ranges = ranges;
+  pair_nchunks = nc;
+  pair_nslots = nslots;
+  pair_entries = total;
+  rest_entries = rtotal;
+  static bool attr_set = false;
+  if (!attr_set) {
+    FE_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&attract_pairs),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 static_cast<int>(kPairLds)));
+    attr_set = true;
+  }
+  *built = true;
+  return 0;
 }
 
 int TsnePlan::setup_attraction(hipStream_t s) {
   const long long nnz = csr.nnz;
   attr_kind = kAttrRows;
-  const int kind = attraction_kind();
+  int kind = attraction_kind();
+  if (kind == kAttrPairs) {
+    bool built = false;
+    if (setup_pairs(s, &built)) return 1;
+    if (built) {
+      attr_kind = kAttrPairs;
+      return 0;
+    }
+    kind = kAttrStaged;
+  }
   // the padded segments must keep 32-bit offsets (at most kPasses segments
   // per row, each padded by fewer than 8 entries)
   const long long bound = nnz + static_cast<long long>(kPasses) * kLaneEntries * n + kLanePad;
@@ -595,6 +852,13 @@ static int launch_conv_any(TsnePlan& p, hipStream_t s) {
 // the repulsive force into diag[n, 2n) (MODE 1)
 static int launch_attract(TsnePlan& p, float exaggeration, hipStream_t s) {
   const float2* yin = p.y[p.cur];
+  if (p.attr_kind == kAttrPairs) {
+    attract_pairs<<<p.pair_ranges, kPairThreads, kPairLds, s>>>(yin, p.pairs, p.st, p.n);
+    FE_LAUNCH_CHECK("attract_pairs");
+    pairs_reduce<<<p.stat_blocks, kThreads, 0, s>>>(p.pairs, p.st, exaggeration, p.diag, p.n);
+    FE_LAUNCH_CHECK("pairs_reduce");
+    return 0;
+  }
   if (p.attr_kind == kAttrStaged) {
     attract_blocked<true><<<p.groups, kStagedThreads, kStagedLds, s>>>(yin, p.blk, p.st,
                                                                        exaggeration, p.diag, p.n);
@@ -936,8 +1200,20 @@ static long long job_resident_bytes(int n, int k, int g) {
   const long long segs = std::min<long long>(2 * nk, 1ll * kPasses * n);
   const long long seg_nnz = 2 * nk + 1ll * kLaneEntries * segs + kLanePad;
   const long long groups = std::max<long long>(32, seg_nnz / (16 * 4096) + 1);
-  return 8ll * 2 * nk + 4ll * (n + 1) + 10ll * seg_nnz + seg_nnz / 2 + 8ll * (seg_nnz / kLaneStep + 4) +
-         4ll * groups * (1 + 2 * kMaxWindows + 2 * kPasses) +
+  const long long blocked = 10ll * seg_nnz + seg_nnz / 2 + 8ll * (seg_nnz / kLaneStep + 4) +
+                            4ll * groups * (1 + 2 * kMaxWindows + 2 * kPasses);
+  // or the pair streams: at most nk pairs staged in runs padded per row and
+  // tile, a rest of at most nk entries both ways (else the blocked form is
+  // chosen), the chunk and slot tables and 32 KB per partial slot
+  const long long nb = (n + kPairBlock - 1) / kPairBlock;
+  const long long tiles = nb * (nb + 1) / 2;
+  const long long staged = nk + 1ll * kLaneEntries * (n + tiles) + kPairPad;
+  const long long rest = nk + 1ll * kLaneEntries * n + kPairPad;
+  const long long chunks = staged / kPairChunkEntries + rest / kPairChunkEntries + tiles + nb;
+  const long long pairs = 6ll * staged + staged / 4 + 8ll * rest + rest / 4 +
+                          static_cast<long long>(sizeof(PairChunk)) * chunks + 8ll * chunks +
+                          4ll * (nb + 1) + 2ll * chunks * kPairBlock * 8;
+  return 8ll * 2 * nk + 4ll * (n + 1) + std::max(blocked, pairs) +
          8ll * 7 * n + 16ll * (splat_bins(g) + 1) + 40ll * gg +
          8ll * 4 * g * f + 8ll * (g + 1) * g + 16ll * gg + 8ll * fft_tw_size(2 * g) + 32ll * 2048 +
          8ll * (g + 1) +
@@ -954,6 +1230,7 @@ static long long job_setup_bytes(int n, int k) {
 // row-group kernel's launches carry their lanes per row)
 constexpr int kStagedLaunch = -1000;
 constexpr int kLanesLaunch = -1001;
+constexpr int kPairsLaunch = -1002;
 
 class TsneBatch {
  public:
@@ -970,7 +1247,8 @@ class TsneBatch {
   bool dirty = true;
   struct Launch {
     int kind;   // 0 splat_count, 8 splat_prep, 6 splat_scatter, 7 splat_tiles,
-                // 1 rows_fwd, 2 cols, 3 rows_inv, 4 attract, 5 update, 9 finalize
+                // 1 rows_fwd, 2 cols, 3 rows_inv, 4 attract, 10 pair reduce,
+                // 5 update, 9 finalize
     int param;  // F or TPR
     int off;    // offset of slot[] in d_tab; first[] follows it
     int m;      // entries
@@ -987,7 +1265,7 @@ class TsneBatch {
     slot_bytes.assign(max_jobs, 0);
     desc.resize(max_jobs);
     std::memset(desc.data(), 0, desc.size() * sizeof(JobDesc));
-    tab_cap = 2 * (10 * max_jobs + 56) + 64;  // every job is in 10 launches
+    tab_cap = 2 * (11 * max_jobs + 64) + 64;  // every job is in at most 11 launches
     ArenaPlan p;
     p.add<JobDesc>(max_jobs);
     p.add<int>(tab_cap);
@@ -1026,6 +1304,11 @@ class TsneBatch {
     if (plan->create<float>(j.indices, j.distances, device_inputs, j.n, j.k, j.index_offset,
                             j.perplexity, j.grid_size, j.init, j.seed, true))
       return 1;
+    // the estimate admitted it; what it actually holds decides
+    if (budget > 0 && used + plan->device_bytes() > budget) {
+      set_error("HBM budget does not admit this job yet");
+      return 2;
+    }
     JobDesc d;
     std::memset(&d, 0, sizeof(d));
     TsnePlan& p = *plan;
@@ -1038,6 +1321,8 @@ class TsneBatch {
     d.val = p.csr.val;
     d.blk = p.blk;
     d.groups = p.groups;
+    d.pairs = p.pairs;
+    d.pair_ranges = p.pair_ranges;
     d.splat = p.splat;
     d.z1 = p.z1;
     d.z2 = p.z2;
@@ -1142,6 +1427,16 @@ class TsneBatch {
         if (slots[s] && slots[s]->attr_kind == kAttrLanes) ent.push_back({s, slots[s]->groups});
       add_launch(4, kLanesLaunch, ent);
     }
+    std::vector<std::pair<int, int>> pair_reduce;
+    {
+      std::vector<std::pair<int, int>> ent;
+      for (int s = 0; s < max_jobs; ++s)
+        if (slots[s] && slots[s]->attr_kind == kAttrPairs) {
+          ent.push_back({s, slots[s]->pair_ranges});
+          pair_reduce.push_back({s, slots[s]->stat_blocks});
+        }
+      add_launch(4, kPairsLaunch, ent);
+    }
     for (int tpr = 4; tpr <= 64; tpr *= 2) {
       std::vector<std::pair<int, int>> ent;
       for (int s = 0; s < max_jobs; ++s)
@@ -1149,6 +1444,7 @@ class TsneBatch {
           ent.push_back({s, slots[s]->upd_blocks});
       add_launch(4, tpr, ent);
     }
+    add_launch(10, 0, pair_reduce);
     std::vector<std::pair<int, int>> upd;
     for (int s = 0; s < max_jobs; ++s)
       if (slots[s]) upd.push_back({s, slots[s]->stat_blocks});
@@ -1216,6 +1512,9 @@ class TsneBatch {
             case kLanesLaunch:
               batch_attract_blocked<false><<<L.blocks, kStagedThreads, 0, s>>>(d_desc, sl, fi, L.m);
               break;
+            case kPairsLaunch:
+              batch_attract_pairs<<<L.blocks, kPairThreads, kPairLds, s>>>(d_desc, sl, fi, L.m);
+              break;
             case 4: batch_attract<4><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
             case 8: batch_attract<8><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
             case 16: batch_attract<16><<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m); break;
@@ -1225,6 +1524,9 @@ class TsneBatch {
           break;
         case 5:
           batch_update<<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
+          break;
+        case 10:
+          batch_pairs_reduce<<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
           break;
         default:
           return fail("unknown batch stage");
@@ -1248,6 +1550,9 @@ class TsneBatch {
       FE_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&batch_attract_blocked<true>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize,
                                    static_cast<int>(kStagedLds)));
+      FE_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&batch_attract_pairs),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   static_cast<int>(kPairLds)));
       attr_done = true;
     }
     // Captured chunks of kChunk iterations replay the current launch table;
@@ -1543,6 +1848,18 @@ int fastembedr_hip_tsne_plan_info(fastembedr_hip_tsne_plan* plan, long long* inf
   info[5] = p.device_bytes();
   info[6] = p.iters_run;
   info[7] = static_cast<long long>(p.setup_us);
+  return 0;
+}
+
+int fastembedr_hip_tsne_plan_attraction_info(fastembedr_hip_tsne_plan* plan, long long* info) {
+  if (!plan || !info) return fail("null plan or output");
+  const fe::TsnePlan& p = plan->impl;
+  info[0] = p.attr_kind;
+  info[1] = p.attr_kind == fe::kAttrPairs ? p.pair_entries : p.seg_entries;
+  info[2] = p.attr_kind == fe::kAttrPairs ? p.rest_entries : 0;
+  info[3] = p.attr_kind == fe::kAttrPairs ? p.pair_ranges : p.groups;
+  info[4] = p.attr_kind == fe::kAttrPairs ? p.pair_nchunks : 0;
+  info[5] = p.attr_kind == fe::kAttrPairs ? p.pair_nslots : 0;
   return 0;
 }
 

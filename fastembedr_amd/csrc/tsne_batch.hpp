@@ -14,6 +14,7 @@
 #pragma once
 
 #include "tsne_device.hpp"
+#include "tsne_pairs.hpp"
 
 namespace fe {
 namespace {
@@ -27,6 +28,8 @@ struct JobDesc {
   const float* val;
   BlockTables blk;  // segment-blocked attraction tables
   int groups;
+  PairTables pairs;  // pair-once attraction tables
+  int pair_ranges;
   SplatBuffers splat;
   float2* z1;
   float2* z2;
@@ -145,6 +148,32 @@ __global__ __launch_bounds__(kStagedThreads) void batch_attract_blocked(JobDesc*
   const int g = xcd_remap(static_cast<int>(blockIdx.x) - first[e], first[e + 1] - first[e]);
   attract_group<STAGE>(J->y[J->cur], J->blk, J->st->mean_xf, J->st->mean_yf, exag, J->attr, g,
                        sy);
+}
+
+// pair-once attraction: one workgroup per chunk range of a job, then the reduce of
+// its partial slots (integer sums: the block order within a job is free)
+__global__ __launch_bounds__(kPairThreads) void batch_attract_pairs(JobDesc* jobs,
+                                                                    const int* __restrict__ slot,
+                                                                    const int* __restrict__ first,
+                                                                    int m) {
+  extern __shared__ float2 lds[];
+  const int e = find_entry(first, m);
+  JobDesc* J = jobs + slot[e];
+  if (!J->active) return;
+  const int w = xcd_remap(static_cast<int>(blockIdx.x) - first[e], first[e + 1] - first[e]);
+  attract_pairs_range(J->y[J->cur], J->st, J->pairs, J->n, w, lds);
+}
+
+__global__ __launch_bounds__(kThreads) void batch_pairs_reduce(JobDesc* jobs,
+                                                               const int* __restrict__ slot,
+                                                               const int* __restrict__ first,
+                                                               int m) {
+  const int e = find_entry(first, m);
+  JobDesc* J = jobs + slot[e];
+  if (!J->active) return;
+  const float exag = J->iter < J->early_iter ? J->early_exag : J->exag;
+  pairs_reduce_body(J->pairs, J->st, exag, J->attr, J->n, blockIdx.x - first[e],
+                    first[e + 1] - first[e]);
 }
 
 template <int TPR>

@@ -1063,9 +1063,10 @@ constexpr int kPasses = kMaxWindows + 1;      // the last pass: entries outside 
 constexpr int kWindowMinEntries = 2048;       // fewer entries do not pay for a window
 constexpr int kWindowBins = 1024;             // candidate windows counted per group
 
-// attraction kernels: one lane group per row of P (any P), and the
-// segment-blocked kernel reading the layout directly or through the windows
-enum AttractionKind { kAttrRows = 0, kAttrLanes = 1, kAttrStaged = 2 };
+// attraction kernels: one lane group per row of P (any P), the
+// segment-blocked kernel reading the layout directly or through the windows,
+// and the pair-once kernel of tsne_pairs.hpp
+enum AttractionKind { kAttrRows = 0, kAttrLanes = 1, kAttrStaged = 2, kAttrPairs = 3 };
 
 // the segments and their tables
 struct BlockTables {

@@ -357,6 +357,15 @@ class TsnePlan:
                 "setup_us")
         return dict(zip(keys, (int(x) for x in v)))
 
+    def attraction_info(self):
+        """The attraction kernel the plan chose and the sizes of its tables."""
+        v = np.zeros(6, np.int64)
+        check(_lib.lib().fastembedr_hip_tsne_plan_attraction_info(self._h, ptr(v, _llp)),
+              "plan attraction info")
+        kinds = ("rows", "lanes", "staged", "pairs")
+        return dict(kind=kinds[int(v[0])], entries=int(v[1]), rest_entries=int(v[2]),
+                    workgroups=int(v[3]), chunks=int(v[4]), slots=int(v[5]))
+
     def affinities(self):
         """(row_ptr, col, val) of the symmetric P, 0-based CSR."""
         nnz = self.info()["nnz"]

@@ -151,6 +151,14 @@ int fastembedr_hip_tsne_plan_layout(fastembedr_hip_tsne_plan* plan, float* out);
  * iterations run, CSR setup microseconds. */
 int fastembedr_hip_tsne_plan_info(fastembedr_hip_tsne_plan* plan, long long* info);
 
+/* The attraction kernel the plan chose: info[0] = 0 row groups, 1 segment-
+ * blocked (layout reads), 2 segment-blocked (LDS windows), 3 every pair once
+ * (staged tiles); info[1] = entries streamed per iteration (padded segments,
+ * or staged pairs), info[2] = rest-stream entries (pair form), info[3] =
+ * workgroups (groups, or chunk ranges), info[4] = chunks, info[5] = partial
+ * slots (pair form; 0 otherwise).  info holds 6 values. */
+int fastembedr_hip_tsne_plan_attraction_info(fastembedr_hip_tsne_plan* plan, long long* info);
+
 /* Copies the plan's symmetric P to host: row_ptr (n+1), col (nnz, 0-based,
  * ascending per row), val (nnz) -- the affinity_row_ptr0 / affinity_col1 /
  * affinity_weight fields of opentsne_force_diagnostic_cpp

@@ -237,10 +237,11 @@ def test_attraction_kernels_agree_on_hub_rows(monkeypatch):
     assert np.diff(csr.row_ptr).max() >= 500
     ref = oracle.attractive(csr, y.astype(np.float64), 4.0)
     got = {}
-    for kind in ("lanes", "staged", "rows"):
+    for kind in ("lanes", "staged", "rows", "pairs"):
         monkeypatch.setenv("FASTEMBEDR_HIP_ATTRACTION", kind)
         got[kind] = fe().gradient_diagnostic(idx, dist.astype(np.float32), y, 10, 4.0, 128)
         assert relative_l2(got[kind].attractive_force, ref) < 5e-5, kind
+    assert relative_l2(got["pairs"].attractive_force, got["lanes"].attractive_force) < 1e-5
     lanes = got["lanes"]
     # the symmetric P itself: its rows are sorted and merged by one wave up to
     # 512 entries and by one block above (the hub row)
@@ -270,3 +271,30 @@ def test_collapsed_layout_splits_heavy_tiles():
     assert abs(dg.sum_q - z) / z < 1e-5
     again = fe().gradient_diagnostic(idx, dist, y, 10, 1.0, 256)
     assert np.array_equal(dg.repulsive_force_fft, again.repulsive_force_fft)
+
+
+@pytest.mark.parametrize("dense_min", [None, "1000000000"])
+def test_pair_attraction_staged_and_rest_streams(monkeypatch, dense_min):
+    # every pair once: tiles of 4096-point blocks staged in LDS, integer
+    # sums.  With the staging threshold raised only the diagonal tiles are
+    # staged and the pairs between blocks go through the rest stream (both
+    # directions, global columns).  Sums are integers: repeatable bit for bit.
+    n = 30000
+    x, _ = gaussian_mixture(n, 12, 8, seed=721)
+    idx, dist = exact_knn_clusters(x, 30)
+    rng = np.random.default_rng(722)
+    y = rng.normal(scale=8.0, size=(n, 2)).astype(np.float32)
+    monkeypatch.setenv("FASTEMBEDR_HIP_ATTRACTION", "pairs")
+    if dense_min:
+        monkeypatch.setenv("FASTEMBEDR_HIP_PAIR_DENSE_MIN", dense_min)
+    plan = fe().TsnePlan(idx, dist, perplexity=10, grid_size=256, Y_init=y)
+    info = plan.attraction_info()
+    plan.close()
+    assert info["kind"] == "pairs"
+    assert (info["rest_entries"] > 0) == bool(dense_min)
+    a = fe().gradient_diagnostic(idx, dist, y, 10, 12.0, 256)
+    b = fe().gradient_diagnostic(idx, dist, y, 10, 12.0, 256)
+    csr = oracle.build_p(idx, dist.astype(np.float64), 10)
+    ref = oracle.attractive(csr, y.astype(np.float64), 12.0)
+    assert relative_l2(a.attractive_force, ref) < 5e-5
+    assert np.array_equal(a.attractive_force, b.attractive_force)
