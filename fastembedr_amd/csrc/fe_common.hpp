@@ -119,6 +119,39 @@ class Stream {
   hipStream_t s_ = nullptr;
 };
 
+// A ring of timing-free events for fork/join points between two streams.  An
+// enqueue that forks takes the next pair; a wait refers to the record made
+// just before it, so reusing the ring after a full turn is safe on a stream
+// and inside one capture of up to size() / 2 forks.
+class EventRing {
+ public:
+  EventRing() = default;
+  EventRing(const EventRing&) = delete;
+  EventRing& operator=(const EventRing&) = delete;
+  ~EventRing() {
+    for (hipEvent_t e : ev_) (void)hipEventDestroy(e);
+  }
+  int init(int count) {
+    for (int i = 0; i < count; ++i) {
+      hipEvent_t e = nullptr;
+      if (check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate")) return 1;
+      ev_.push_back(e);
+    }
+    return 0;
+  }
+  bool ready() const { return !ev_.empty(); }
+  hipEvent_t next() {
+    hipEvent_t e = ev_[at_];
+    at_ = (at_ + 1) % ev_.size();
+    return e;
+  }
+  std::size_t size() const { return ev_.size(); }
+
+ private:
+  std::vector<hipEvent_t> ev_;
+  std::size_t at_ = 0;
+};
+
 class GraphExec {
  public:
   GraphExec() = default;

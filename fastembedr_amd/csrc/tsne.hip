@@ -60,6 +60,15 @@ static bool graph_capture_enabled() {
   return !(env && env[0] == '0');
 }
 
+static bool overlap_enabled() {
+  const char* env = std::getenv("FASTEMBEDR_HIP_OVERLAP");
+  return !(env && env[0] == '0');
+}
+
+// fork/join events of one plan: two per iteration, enough that a captured
+// chunk of iterations never reuses one
+constexpr int kForkEvents = 128;
+
 // ---------------------------------------------------------------------------
 // P on the device
 // ---------------------------------------------------------------------------
@@ -250,6 +259,12 @@ class TsnePlan {
   // splat binning (point ranges) and splat tiles
   int upd_blocks = 0, splat_blocks = 0, stat_blocks = 0, item_blocks = 0;
   Stream stream;
+  // the attraction runs on `side`, beside the splat and the convolution on
+  // `stream` (both read only the layout and its statistics); the update waits
+  // for both.  FASTEMBEDR_HIP_OVERLAP=0 keeps every launch on `stream`.
+  bool overlap = false;
+  Stream side;
+  EventRing forks;
   struct CachedGraph {
     StepParams sp;
     int iters;
@@ -674,6 +689,8 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
   if (g < 16 || g > 1024 || (g & (g - 1)) != 0)
     return fail("grid_size must be a power of two between 16 and 1024");
   if (stream.init()) return 1;
+  overlap = !exact && overlap_enabled();
+  if (overlap && (side.init() || forks.init(kForkEvents))) return 1;
   hipStream_t s = stream.get();
   const auto t0 = std::chrono::steady_clock::now();
 
@@ -969,9 +986,21 @@ int TsnePlan::enqueue_iteration(const StepParams& sp) {
     cur ^= 1;
     return 0;
   }
-  if (launch_splat(*this, s)) return 1;
-  if (launch_conv_any(*this, s)) return 1;
-  if (launch_attract(*this, sp.exaggeration, s)) return 1;
+  if (overlap) {
+    hipStream_t a = side.get();
+    hipEvent_t fork = forks.next(), join = forks.next();
+    FE_CHECK(hipEventRecord(fork, s));
+    FE_CHECK(hipStreamWaitEvent(a, fork, 0));
+    if (launch_attract(*this, sp.exaggeration, a)) return 1;
+    FE_CHECK(hipEventRecord(join, a));
+    if (launch_splat(*this, s)) return 1;
+    if (launch_conv_any(*this, s)) return 1;
+    FE_CHECK(hipStreamWaitEvent(s, join, 0));
+  } else {
+    if (launch_splat(*this, s)) return 1;
+    if (launch_conv_any(*this, s)) return 1;
+    if (launch_attract(*this, sp.exaggeration, s)) return 1;
+  }
   if (launch_update<0>(*this, sp, s)) return 1;
   cur ^= 1;
   return 0;
@@ -993,6 +1022,7 @@ static bool same_params(const StepParams& a, const StepParams& b) {
 // at least kGraphMin iterations replays a captured graph of that length,
 // constants and starting buffer parity (an odd chunk flips the parity).
 constexpr int kGraphChunk = 50;
+static_assert(2 * kGraphChunk <= kForkEvents, "a captured chunk reuses no fork event");
 constexpr int kGraphMin = 4;
 constexpr size_t kGraphCache = 16;
 
