@@ -733,7 +733,7 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
   const size_t gf = static_cast<size_t>(g) * f;
   upd_blocks = static_cast<int>(std::min<long long>(
       (static_cast<long long>(n) * tpr + kThreads - 1) / kThreads, 2048));
-  splat_blocks = static_cast<int>(std::max<long long>(1, std::min<long long>((n + 2047) / 2048, 1024)));
+  splat_blocks = splat_bin_blocks(n);
   item_blocks = splat_item_blocks(n, g);
   const int nbins = splat_bins(g), ntiles = nbins / 4;
   stat_blocks = static_cast<int>(std::min<long long>((n + kThreads - 1) / kThreads, 2048));
@@ -743,9 +743,9 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
   sp.add<float2>(n);  // gains
   sp.add<float2>(n);  // upd
   sp.add<float2>(2 * static_cast<size_t>(n));  // diag
+  sp.add<int>(static_cast<size_t>(splat_blocks) * nbins);  // rows
   sp.add<int>(nbins);                     // bin_count
   sp.add<int>(nbins + 1);                 // bin_start
-  sp.add<int>(nbins);                     // bin_cursor
   sp.add<int>(ntiles + 1);                // tile_items
   sp.add<float2>(n);                      // binned
   sp.add<unsigned long long>(3 * gg);     // wide
@@ -766,9 +766,9 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
   gains = state.take<float2>(n);
   upd = state.take<float2>(n);
   diag = state.take<float2>(2 * static_cast<size_t>(n));
+  splat.rows = state.take<int>(static_cast<size_t>(splat_blocks) * nbins);
   splat.bin_count = state.take<int>(nbins);
   splat.bin_start = state.take<int>(nbins + 1);
-  splat.bin_cursor = state.take<int>(nbins);
   splat.tile_items = state.take<int>(ntiles + 1);
   splat.binned = state.take<float2>(n);
   splat.wide = state.take<unsigned long long>(3 * gg);
@@ -802,7 +802,6 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
   std::vector<float2> ones(n, make_float2(1.0f, 1.0f));
   FE_CHECK(hipMemcpyAsync(gains, ones.data(), n * sizeof(float2), hipMemcpyHostToDevice, s));
   FE_CHECK(hipMemsetAsync(upd, 0, n * sizeof(float2), s));
-  FE_CHECK(hipMemsetAsync(splat.bin_count, 0, nbins * sizeof(int), s));
   FE_CHECK(hipMemsetAsync(splat.wide, 0, 3 * gg * sizeof(unsigned long long), s));
   FE_CHECK(hipMemsetAsync(st, 0, sizeof(TsneStats), s));
   layout_stats_kernel<<<stat_blocks, kThreads, 0, s>>>(y[0], lpart, n);
@@ -841,11 +840,11 @@ static int launch_conv(TsnePlan& p, hipStream_t s) {
 
 static int launch_splat(TsnePlan& p, hipStream_t s) {
   const float2* yin = p.y[p.cur];
-  splat_count<<<p.splat_blocks, kThreads, 0, s>>>(yin, p.st, p.splat, p.n, p.g);
+  splat_count<<<p.splat_blocks, kSplatThreads, 0, s>>>(yin, p.st, p.splat, p.n, p.g);
   FE_LAUNCH_CHECK("splat_count");
-  splat_prep<<<1, kPrepThreads, 0, s>>>(p.splat, p.g);
-  FE_LAUNCH_CHECK("splat_prep");
-  splat_scatter<<<p.splat_blocks, kThreads, 0, s>>>(yin, p.st, p.splat, p.n, p.g);
+  splat_colscan<<<splat_colscan_blocks(p.g), kSplatThreads, 0, s>>>(p.splat, p.g, p.splat_blocks);
+  FE_LAUNCH_CHECK("splat_colscan");
+  splat_scatter<<<p.splat_blocks, kSplatThreads, 0, s>>>(yin, p.st, p.splat, p.n, p.g);
   FE_LAUNCH_CHECK("splat_scatter");
   splat_tiles<<<p.item_blocks, kThreads, 0, s>>>(p.st, p.splat, p.g);
   FE_LAUNCH_CHECK("splat_tiles");
@@ -1244,7 +1243,8 @@ static long long job_resident_bytes(int n, int k, int g) {
                           static_cast<long long>(sizeof(PairChunk)) * chunks + 8ll * chunks +
                           4ll * (nb + 1) + 2ll * chunks * kPairBlock * 8;
   return 8ll * 2 * nk + 4ll * (n + 1) + std::max(blocked, pairs) +
-         8ll * 7 * n + 16ll * (splat_bins(g) + 1) + 40ll * gg +
+         8ll * 7 * n + 16ll * (splat_bins(g) + 1) + 4ll * splat_bin_blocks(n) * splat_bins(g) +
+         40ll * gg +
          8ll * 4 * g * f + 8ll * (g + 1) * g + 16ll * gg + 8ll * fft_tw_size(2 * g) + 32ll * 2048 +
          8ll * (g + 1) +
          4096;
@@ -1276,7 +1276,7 @@ class TsneBatch {
   int tab_cap = 0;
   bool dirty = true;
   struct Launch {
-    int kind;   // 0 splat_count, 8 splat_prep, 6 splat_scatter, 7 splat_tiles,
+    int kind;   // 0 splat_count, 11 splat_colscan, 6 splat_scatter, 7 splat_tiles,
                 // 1 rows_fwd, 2 cols, 3 rows_inv, 4 attract, 10 pair reduce,
                 // 5 update, 9 finalize
     int param;  // F or TPR
@@ -1430,8 +1430,11 @@ class TsneBatch {
         tiles.push_back({s, slots[s]->item_blocks});
         single.push_back({s, 1});
       }
+    std::vector<std::pair<int, int>> colscan;
+    for (int s = 0; s < max_jobs; ++s)
+      if (slots[s]) colscan.push_back({s, splat_colscan_blocks(slots[s]->g)});
     add_launch(0, 0, splat);
-    add_launch(8, 0, single);
+    add_launch(11, 0, colscan);
     add_launch(6, 0, splat);
     add_launch(7, 0, tiles);
     for (int kind = 1; kind <= 3; ++kind) {
@@ -1494,16 +1497,16 @@ class TsneBatch {
       const int* fi = sl + L.m;
       switch (L.kind) {
         case 0:
-          batch_splat_count<<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
+          batch_splat_count<<<L.blocks, kSplatThreads, 0, s>>>(d_desc, sl, fi, L.m);
+          break;
+        case 11:
+          batch_splat_colscan<<<L.blocks, kSplatThreads, 0, s>>>(d_desc, sl, fi, L.m);
           break;
         case 6:
-          batch_splat_scatter<<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
+          batch_splat_scatter<<<L.blocks, kSplatThreads, 0, s>>>(d_desc, sl, fi, L.m);
           break;
         case 7:
           batch_splat_tiles<<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
-          break;
-        case 8:
-          batch_splat_prep<<<L.blocks, kPrepThreads, 0, s>>>(d_desc, sl, fi, L.m);
           break;
         case 9:
           batch_finalize<<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);

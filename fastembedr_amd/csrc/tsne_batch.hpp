@@ -5,7 +5,7 @@
 // would use), finds its descriptor by a binary search of blockIdx.x over the
 // prefix table, and runs the single-job stage body with its job-local block
 // index.  Jobs therefore give bit-identical results alone and in a batch,
-// and a batch of S jobs costs 6 + 3 * (#grid sizes) + (#attraction variants)
+// and a batch of S jobs costs 6 + 3 * (#grid sizes) + (#attraction launches)
 // launches per iteration instead of 10 * S.  The descriptor's iteration
 // counter, the phase constants and the layout-buffer parity live on the
 // device: batch_finalize advances them after the job's update, and a job
@@ -60,7 +60,7 @@ __device__ __forceinline__ int find_entry(const int* __restrict__ first, int m) 
   return lo;
 }
 
-__global__ __launch_bounds__(kThreads) void batch_splat_count(JobDesc* jobs,
+__global__ __launch_bounds__(kSplatThreads) void batch_splat_count(JobDesc* jobs,
                                                               const int* __restrict__ slot,
                                                               const int* __restrict__ first, int m) {
   const int e = find_entry(first, m);
@@ -70,17 +70,17 @@ __global__ __launch_bounds__(kThreads) void batch_splat_count(JobDesc* jobs,
                    first[e + 1] - first[e]);
 }
 
-// one block per job
-__global__ __launch_bounds__(kPrepThreads) void batch_splat_prep(JobDesc* jobs,
-                                                             const int* __restrict__ slot,
-                                                             const int* __restrict__ first, int m) {
+__global__ __launch_bounds__(kSplatThreads) void batch_splat_colscan(JobDesc* jobs,
+                                                                    const int* __restrict__ slot,
+                                                                    const int* __restrict__ first,
+                                                                    int m) {
   const int e = find_entry(first, m);
   JobDesc* J = jobs + slot[e];
   if (!J->active) return;
-  splat_prep_body(J->splat, J->g);
+  splat_colscan_body(J->splat, J->g, splat_bin_blocks(J->n), blockIdx.x - first[e]);
 }
 
-__global__ __launch_bounds__(kThreads) void batch_splat_scatter(JobDesc* jobs,
+__global__ __launch_bounds__(kSplatThreads) void batch_splat_scatter(JobDesc* jobs,
                                                                 const int* __restrict__ slot,
                                                                 const int* __restrict__ first,
                                                                 int m) {
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(kThreads) void batch_splat_tiles(JobDesc* jobs,
   const int e = find_entry(first, m);
   JobDesc* J = jobs + slot[e];
   if (!J->active) return;
-  splat_tiles_body(J->st, J->splat, J->g, blockIdx.x - first[e]);
+  splat_tiles_body(J->st, J->splat, J->g, blockIdx.x - first[e], first[e + 1] - first[e]);
 }
 
 template <int F>

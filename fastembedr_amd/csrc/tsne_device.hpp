@@ -1,12 +1,12 @@
 // t-SNE device kernels for gfx950.
 //
-// Per-iteration pipeline (eight launches, all state resident in HBM):
+// Per-iteration pipeline (all state resident in HBM):
 //   1. splat            bilinear spread of every point onto the G x G grid
 //                       (mass, mass*x, mass*y): points are binned by grid
-//                       tile (count, scatter), then one block per tile sums
-//                       its points in 64-bit fixed point in LDS and writes its
-//                       nodes once, so the grid does not depend on arrival
-//                       order and no global atomics touch it;
+//                       tile (count, column scan, scatter), then blocks sum
+//                       every tile's points in 64-bit fixed point in LDS and
+//                       write its nodes once, so the grid does not depend on
+//                       arrival order;
 //   2. conv_rows_fwd    row FFTs of z1 = mass + i*mass_x/x_scale and
 //                       z2 = mass_y (zero padded to F = 2G), and of the G
 //                       kernel rows Kq + i*Kq^2 (both real and even, so the
@@ -16,7 +16,9 @@
 //                       pointwise products with the kernel spectra, inverse
 //                       column FFTs, and Z by Parseval in FP64;
 //   4. conv_rows_inv    inverse row FFTs; writes the (q2, xq2, yq2, q) field;
-//   5. attract          CSR attraction over the symmetric P (no atomics);
+//   5. attract          attraction over the symmetric P (tsne_pairs.hpp:
+//                       every pair once; or the CSR kernels below), on a
+//                       second stream beside 1-4;
 //   6. update           per point: bilinear gather of the field, the
 //                       gains/momentum/clip update, and the FP64 layout
 //                       statistics (mean, bounds) for the next iteration.
@@ -253,26 +255,30 @@ __global__ __launch_bounds__(kThreads) void layout_stats_kernel(const float2* __
 //     0 interior, 1 right edge, 2 right and bottom edge, 3 bottom edge,
 // so that what a tile owner needs from each neighbour is one contiguous range
 // of bins: own tile [0, 4), left neighbour [1, 3), upper neighbour [2, 4),
-// upper-left neighbour [2, 3).
-//   a. splat_count    per-block LDS histogram of the bins, added to the
-//                     global counts;
-//   b. splat_prep     one block turns the counts into bin starts and cursors
-//                     (clearing the counts) and cuts every tile's points into
-//                     work items of at most kSplatItem;
-//   c. splat_scatter  each point takes a slot in its bin (one global atomic
-//                     per block and non-empty bin) and stores its centred
-//                     coordinates there;
-//   d. splat_tiles    one block per work item accumulates its points in an
-//                     LDS window of the tile (a lane sums consecutive points of
-//                     one cell in registers before touching LDS, which keeps
-//                     collapsed clusters cheap).  A tile of one item writes its
-//                     nodes once with plain stores as the input of the row FFTs,
+// upper-left neighbour [2, 3).  Four launches, none with a global atomic on
+// a shared address (those serialise across the XCDs):
+//   a. splat_count    a block of kSplatThreads takes a range of kSplatChunk
+//                     points (loads issued together), builds their bin
+//                     histogram in LDS and stores it as its row of `rows`;
+//   b. splat_colscan  per bin, the exclusive prefix of the rows down the
+//                     blocks (in place) and the bin's total;
+//   c. splat_scatter  every block scans the totals into bin starts in LDS
+//                     (block 0 also stores them, with the first work item of
+//                     every tile, for the tile pass); a point's slot is its
+//                     bin start + its block's prefix + its rank among the
+//                     block's points of that bin (an LDS atomic);
+//   d. splat_tiles    a block takes work items of at most kSplatItem points
+//                     of one tile in turn and accumulates each in an LDS window
+//                     of the tile (a lane sums consecutive points of one cell
+//                     in registers before touching LDS, which keeps collapsed
+//                     clusters cheap).  A tile of one item writes its nodes
+//                     with plain stores as the input of the row FFTs,
 //                       sgrid(node) = (mass, mass*x / x_scale, mass*y, 0);
 //                     the items of a larger tile add their windows to the
 //                     64-bit `wide` grid, which the row FFT pass converts and
 //                     clears.
-// Slots inside a bin are taken in atomic order, but all sums are integer
-// sums, so the grid is the same on every run and matches a direct splat.
+// Ranks inside a bin follow LDS atomic order, but all sums are integer sums,
+// so the grid is the same on every run and matches a direct splat.
 // ----------------------------------------------------------------------------
 constexpr int kTile = 32;
 constexpr int kTileNodes = kTile * kTile;
@@ -280,11 +286,27 @@ constexpr int kMaxTilesSide = 32;  // grids up to 1024 x 1024 nodes
 constexpr int kMaxTiles = kMaxTilesSide * kMaxTilesSide;
 constexpr int kMaxBins = 4 * kMaxTiles;
 constexpr int kSplatItem = 1024;  // points per tile work item
+constexpr int kItemPer = kSplatItem / kThreads;
+// binning passes: blocks of kSplatThreads take kSplatChunk points; in the
+// scans a thread owns kScanBins consecutive bins and kScanTiles tiles
+constexpr int kSplatThreads = 1024;
+constexpr int kSplatPerThread = 4;
+constexpr int kSplatChunk = kSplatThreads * kSplatPerThread;
+constexpr int kScanBins = kMaxBins / kSplatThreads;
+constexpr int kScanTiles = kMaxTiles / kSplatThreads;
+static_assert(kScanBins % 4 == 0 && kScanBins * kSplatThreads == kMaxBins &&
+                  kScanTiles * kSplatThreads == kMaxTiles,
+              "the binning scans split bins and tiles evenly over the threads");
+static_assert(kSplatItem % kThreads == 0, "whole points per lane in the tile pass");
+// column scan: a block takes kColCols bins, its waves a share of the rows
+constexpr int kColCols = 64;
+constexpr int kColWaves = kSplatThreads / 64;
 
 struct SplatBuffers {
-  int* bin_count;            // bins; zero between passes
+  int* rows;                 // splat_bin_blocks x bins: block histograms, then
+                             // their exclusive prefixes down the blocks
+  int* bin_count;            // bins: totals
   int* bin_start;            // bins + 1
-  int* bin_cursor;           // bins
   int* tile_items;           // tiles + 1: first work item of every tile
   float2* binned;            // n centred points in bin order
   unsigned long long* wide;  // 3 x g*g sums of multi-item tiles; zero between passes
@@ -293,11 +315,22 @@ struct SplatBuffers {
 
 __host__ __device__ __forceinline__ int tiles_side(int g) { return (g + kTile - 1) / kTile; }
 __host__ __device__ __forceinline__ int splat_bins(int g) { return 4 * tiles_side(g) * tiles_side(g); }
-// launch size of splat_tiles: every tile has at most 1 + points/kSplatItem
-// items and a point is counted by at most four tiles
+// launch size of splat_count and splat_scatter, and the rows of `rows`
+__host__ __device__ __forceinline__ int splat_bin_blocks(int n) {
+  return n > kSplatChunk ? (n + kSplatChunk - 1) / kSplatChunk : 1;
+}
+__host__ __device__ __forceinline__ int splat_colscan_blocks(int g) {
+  return (splat_bins(g) + kColCols - 1) / kColCols;
+}
+// launch size of splat_tiles: a block per tile and kTileExtraBlocks more for
+// tiles of several items, never more than the items can be (every tile has
+// at most 1 + points/kSplatItem items and a point is counted by at most four
+// tiles); blocks loop over further items
+constexpr int kTileExtraBlocks = 256;
 __host__ __device__ __forceinline__ int splat_item_blocks(int n, int g) {
-  return tiles_side(g) * tiles_side(g) +
-         static_cast<int>((4ll * n + kSplatItem - 1) / kSplatItem);
+  const long long tiles = static_cast<long long>(tiles_side(g)) * tiles_side(g);
+  const long long bound = tiles + (4ll * n + kSplatItem - 1) / kSplatItem;
+  return static_cast<int>(bound < tiles + kTileExtraBlocks ? bound : tiles + kTileExtraBlocks);
 }
 
 __device__ __forceinline__ int point_bin(const Cell& c, int ts) {
@@ -325,52 +358,6 @@ __device__ __forceinline__ void point_range(int n, int blk, int nblk, int* beg, 
   *end = min(n, *beg + chunk);
 }
 
-// exclusive prefix sum of v over the block in thread order; *total = block sum
-__device__ __forceinline__ int block_exclusive_scan(int v, int* total) {
-  __shared__ int wsum[kThreads / 64];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  int incl = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int u = __shfl_up(incl, o, 64);
-    if (lane >= o) incl += u;
-  }
-  if (lane == 63) wsum[wid] = incl;
-  __syncthreads();
-  int off = 0, tot = 0;
-#pragma unroll
-  for (int w = 0; w < kThreads / 64; ++w) {
-    const int s = wsum[w];
-    off += w < wid ? s : 0;
-    tot += s;
-  }
-  __syncthreads();
-  *total = tot;
-  return off + incl - v;
-}
-
-__device__ __forceinline__ void splat_count_body(const float2* __restrict__ y,
-                                                 const TsneStats* __restrict__ st,
-                                                 SplatBuffers sb, int n, int g, int blk,
-                                                 int nblk) {
-  __shared__ int hist[kMaxBins];
-  const int ts = tiles_side(g), nbins = 4 * ts * ts;
-  const float mx = st->mean_xf, my = st->mean_yf;
-  const float lx = st->lower_x, ly = st->lower_y, inv = st->inv_spacing;
-  for (int t = threadIdx.x; t < nbins; t += kThreads) hist[t] = 0;
-  __syncthreads();
-  int beg, end;
-  point_range(n, xcd_remap(blk, nblk), nblk, &beg, &end);
-  for (int i = beg + threadIdx.x; i < end; i += kThreads) {
-    const float2 p = y[i];
-    const Cell c = grid_cell(p.x - mx, p.y - my, lx, ly, inv, g);
-    atomicAdd(&hist[point_bin(c, ts)], 1);
-  }
-  __syncthreads();
-  for (int t = threadIdx.x; t < nbins; t += kThreads)
-    if (hist[t]) atomicAdd(sb.bin_count + t, hist[t]);
-}
-
 // exclusive prefix sum of v over a block of NT threads in thread order
 template <int NT>
 __device__ __forceinline__ int block_exclusive_scan_nt(int v, int* total) {
@@ -396,85 +383,194 @@ __device__ __forceinline__ int block_exclusive_scan_nt(int v, int* total) {
   return off + incl - v;
 }
 
-// One block of kPrepThreads, after splat_count: bin starts and cursors
-// (clearing the counts for the next pass) and the work items of every tile.
-// Thread t owns bins 4t .. 4t+3 (one 16-byte load and store each) and tile t.
-constexpr int kPrepThreads = kMaxTiles;
-static_assert(kMaxBins == 4 * kPrepThreads, "four bins per thread");
+// Centred coordinates and bins of the points i0 + threadIdx.x + u *
+// kSplatThreads below `end` (bin -1 past it); all loads are issued first.
+struct SplatLoad {
+  float2 c[kSplatPerThread];
+  int bin[kSplatPerThread];
+};
 
-__device__ __forceinline__ void splat_prep_body(SplatBuffers sb, int g) {
-  __shared__ int bs[kMaxBins + 1];
-  const int ts = tiles_side(g), tiles = ts * ts, nbins = 4 * tiles;
-  const int t = threadIdx.x, b0 = 4 * t;
-  int4 cnt = make_int4(0, 0, 0, 0);
-  if (b0 < nbins) cnt = *reinterpret_cast<const int4*>(sb.bin_count + b0);
-  int total;
-  const int run = block_exclusive_scan_nt<kPrepThreads>(cnt.x + cnt.y + cnt.z + cnt.w, &total);
-  if (b0 < nbins) {
-    const int4 st = make_int4(run, run + cnt.x, run + cnt.x + cnt.y, run + cnt.x + cnt.y + cnt.z);
-    *reinterpret_cast<int4*>(sb.bin_start + b0) = st;
-    *reinterpret_cast<int4*>(sb.bin_cursor + b0) = st;
-    *reinterpret_cast<int4*>(sb.bin_count + b0) = make_int4(0, 0, 0, 0);
-    bs[b0] = st.x;
-    bs[b0 + 1] = st.y;
-    bs[b0 + 2] = st.z;
-    bs[b0 + 3] = st.w;
+__device__ __forceinline__ SplatLoad splat_load(const float2* __restrict__ y,
+                                                const TsneStats* __restrict__ st, int i0, int end,
+                                                int g) {
+  const int ts = tiles_side(g);
+  const float mx = st->mean_xf, my = st->mean_yf;
+  const float lx = st->lower_x, ly = st->lower_y, inv = st->inv_spacing;
+  SplatLoad L;
+#pragma unroll
+  for (int u = 0; u < kSplatPerThread; ++u) {
+    const int i = i0 + threadIdx.x + u * kSplatThreads;
+    L.c[u] = i < end ? y[i] : make_float2(0.f, 0.f);
   }
-  if (t == 0) {
-    sb.bin_start[nbins] = total;
-    bs[nbins] = total;
+#pragma unroll
+  for (int u = 0; u < kSplatPerThread; ++u) {
+    const int i = i0 + threadIdx.x + u * kSplatThreads;
+    L.c[u] = make_float2(L.c[u].x - mx, L.c[u].y - my);
+    L.bin[u] = i < end ? point_bin(grid_cell(L.c[u].x, L.c[u].y, lx, ly, inv, g), ts) : -1;
+  }
+  return L;
+}
+
+__device__ __forceinline__ void splat_count_body(const float2* __restrict__ y,
+                                                 const TsneStats* __restrict__ st,
+                                                 SplatBuffers sb, int n, int g, int blk,
+                                                 int nblk) {
+  __shared__ int hist[kMaxBins];
+  const int nbins = splat_bins(g);
+  for (int t = threadIdx.x; t < nbins; t += kSplatThreads) hist[t] = 0;
+  __syncthreads();
+  const int rb = xcd_remap(blk, nblk);
+  int beg, end;
+  point_range(n, rb, nblk, &beg, &end);
+  for (int i0 = beg; i0 < end; i0 += kSplatChunk) {
+    const SplatLoad L = splat_load(y, st, i0, end, g);
+#pragma unroll
+    for (int u = 0; u < kSplatPerThread; ++u)
+      if (L.bin[u] >= 0) atomicAdd(&hist[L.bin[u]], 1);
   }
   __syncthreads();
-  int items = 0;
-  if (t < tiles) {
-    int pts = 0;
+  int* row = sb.rows + static_cast<size_t>(rb) * nbins;
+  for (int t = threadIdx.x; t < nbins; t += kSplatThreads) row[t] = hist[t];
+}
+
+// Block blk owns bins [kColCols * blk, + kColCols); lane = bin, wave w =
+// rows [w * per, (w + 1) * per) of the `nrows` block rows.
+__device__ __forceinline__ void splat_colscan_body(SplatBuffers sb, int g, int nrows, int blk) {
+  __shared__ int part[kColWaves][kColCols];
+  constexpr int kU = 8;
+  const int nbins = splat_bins(g);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int col = blk * kColCols + lane;
+  const int per = (nrows + kColWaves - 1) / kColWaves;
+  const int r0 = min(nrows, w * per), r1 = min(nrows, r0 + per);
+  int* base = sb.rows + col;
+  int s = 0;
+  if (col < nbins) {
+    for (int r = r0; r < r1; r += kU) {
+      int v[kU];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int2 rg = tile_range(bs, t, ts, r);
-      pts += rg.y - rg.x;
+      for (int u = 0; u < kU; ++u) v[u] = r + u < r1 ? base[static_cast<size_t>(r + u) * nbins] : 0;
+#pragma unroll
+      for (int u = 0; u < kU; ++u) s += v[u];
     }
-    items = max(1, (pts + kSplatItem - 1) / kSplatItem);
   }
-  int all;
-  const int first = block_exclusive_scan_nt<kPrepThreads>(items, &all);
-  if (t < tiles) sb.tile_items[t] = first;
-  if (t == 0) sb.tile_items[tiles] = all;
+  part[w][lane] = s;
+  __syncthreads();
+  int run = 0, total = 0;
+#pragma unroll
+  for (int q = 0; q < kColWaves; ++q) {
+    const int v = part[q][lane];
+    run += q < w ? v : 0;
+    total += v;
+  }
+  if (col >= nbins) return;
+  if (w == 0) sb.bin_count[col] = total;
+  for (int r = r0; r < r1; r += kU) {
+    int v[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) v[u] = r + u < r1 ? base[static_cast<size_t>(r + u) * nbins] : 0;
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      if (r + u < r1) base[static_cast<size_t>(r + u) * nbins] = run;
+      run += v[u];
+    }
+  }
 }
 
 __device__ __forceinline__ void splat_scatter_body(const float2* __restrict__ y,
                                                    const TsneStats* __restrict__ st,
                                                    SplatBuffers sb, int n, int g, int blk,
                                                    int nblk) {
+  __shared__ int start[kMaxBins + 1];
   __shared__ int hist[kMaxBins];
-  __shared__ int base[kMaxBins];
-  const int ts = tiles_side(g), nbins = 4 * ts * ts;
-  const float mx = st->mean_xf, my = st->mean_yf;
-  const float lx = st->lower_x, ly = st->lower_y, inv = st->inv_spacing;
-  for (int t = threadIdx.x; t < nbins; t += kThreads) hist[t] = 0;
-  __syncthreads();
-  int beg, end;
-  point_range(n, xcd_remap(blk, nblk), nblk, &beg, &end);
-  for (int i = beg + threadIdx.x; i < end; i += kThreads) {
-    const float2 p = y[i];
-    const Cell c = grid_cell(p.x - mx, p.y - my, lx, ly, inv, g);
-    atomicAdd(&hist[point_bin(c, ts)], 1);
+  __shared__ int off[kMaxBins];
+  const int ts = tiles_side(g), tiles = ts * ts, nbins = 4 * tiles;
+  const int t = threadIdx.x, b0 = kScanBins * t;
+  const int rb = xcd_remap(blk, nblk);
+  const int* row = sb.rows + static_cast<size_t>(rb) * nbins;
+  // bin starts: thread t scans bins b0 .. b0 + kScanBins - 1 (nbins is a
+  // multiple of four, so a group of four is in or out as a whole); the
+  // block's own prefixes are loaded with the totals
+  int cnt[kScanBins], pre[kScanBins];
+#pragma unroll
+  for (int q = 0; q < kScanBins; q += 4) {
+    int4 c4 = make_int4(0, 0, 0, 0), p4 = make_int4(0, 0, 0, 0);
+    if (b0 + q < nbins) {
+      c4 = *reinterpret_cast<const int4*>(sb.bin_count + b0 + q);
+      p4 = *reinterpret_cast<const int4*>(row + b0 + q);
+    }
+    cnt[q] = c4.x;
+    cnt[q + 1] = c4.y;
+    cnt[q + 2] = c4.z;
+    cnt[q + 3] = c4.w;
+    pre[q] = p4.x;
+    pre[q + 1] = p4.y;
+    pre[q + 2] = p4.z;
+    pre[q + 3] = p4.w;
   }
-  __syncthreads();
-  for (int t = threadIdx.x; t < nbins; t += kThreads) {
-    const int c = hist[t];
-    if (c) {
-      base[t] = atomicAdd(sb.bin_cursor + t, c);
-      hist[t] = 0;
+  int mine = 0;
+#pragma unroll
+  for (int q = 0; q < kScanBins; ++q) mine += cnt[q];
+  int total;
+  int run = block_exclusive_scan_nt<kSplatThreads>(mine, &total);
+  int sv[kScanBins];
+#pragma unroll
+  for (int q = 0; q < kScanBins; ++q) {
+    sv[q] = run;
+    run += cnt[q];
+    if (b0 + q < nbins) {
+      start[b0 + q] = sv[q];
+      off[b0 + q] = sv[q] + pre[q];
+      hist[b0 + q] = 0;
     }
   }
+  if (t == 0) start[nbins] = total;
   __syncthreads();
-  for (int i = beg + threadIdx.x; i < end; i += kThreads) {
-    const float2 p = y[i];
-    const float xc = p.x - mx, yc = p.y - my;
-    const Cell c = grid_cell(xc, yc, lx, ly, inv, g);
-    const int b = point_bin(c, ts);
-    const int r = atomicAdd(&hist[b], 1);
-    sb.binned[base[b] + r] = make_float2(xc, yc);
+  if (blk == 0) {
+    // the bin starts and the first work item of every tile, for the tile pass
+#pragma unroll
+    for (int q = 0; q < kScanBins; q += 4)
+      if (b0 + q < nbins)
+        *reinterpret_cast<int4*>(sb.bin_start + b0 + q) =
+            make_int4(sv[q], sv[q + 1], sv[q + 2], sv[q + 3]);
+    if (t == 0) sb.bin_start[nbins] = total;
+    int items[kScanTiles];
+    int my_items = 0;
+#pragma unroll
+    for (int q = 0; q < kScanTiles; ++q) {
+      const int tt = kScanTiles * t + q;
+      items[q] = 0;
+      if (tt < tiles) {
+        int pts = 0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int2 rg = tile_range(start, tt, ts, r);
+          pts += rg.y - rg.x;
+        }
+        items[q] = max(1, (pts + kSplatItem - 1) / kSplatItem);
+      }
+      my_items += items[q];
+    }
+    int all;
+    int first = block_exclusive_scan_nt<kSplatThreads>(my_items, &all);
+#pragma unroll
+    for (int q = 0; q < kScanTiles; ++q) {
+      const int tt = kScanTiles * t + q;
+      if (tt < tiles) sb.tile_items[tt] = first;
+      first += items[q];
+    }
+    if (t == 0) sb.tile_items[tiles] = all;
+  }
+  int beg, end;
+  point_range(n, rb, nblk, &beg, &end);
+  for (int i0 = beg; i0 < end; i0 += kSplatChunk) {
+    const SplatLoad L = splat_load(y, st, i0, end, g);
+#pragma unroll
+    for (int u = 0; u < kSplatPerThread; ++u)
+      if (L.bin[u] >= 0) {
+        const int r = atomicAdd(&hist[L.bin[u]], 1);
+        sb.binned[off[L.bin[u]] + r] = L.c[u];
+      }
   }
 }
 
@@ -501,43 +597,54 @@ __device__ __forceinline__ float4 node_value(long long m, long long mxv, long lo
                      static_cast<float>(static_cast<double>(myv) * inv_xy), 0.f);
 }
 
-__device__ __forceinline__ void splat_tiles_body(const TsneStats* __restrict__ st, SplatBuffers sb,
-                                                 int g, int item) {
-  __shared__ unsigned long long win[3 * kTileNodes];
+// One work item: window cleared, points summed, nodes written.  Starts and
+// ends with a barrier (the window and titems are shared with the block's
+// previous item).
+__device__ __forceinline__ void splat_tile_item(const TsneStats* __restrict__ st, SplatBuffers sb,
+                                                int g, int item, unsigned long long* win,
+                                                const int* titems) {
   const int ts = tiles_side(g), tiles = ts * ts;
-  if (item >= sb.tile_items[tiles]) return;
-  int lo = 0, hi = tiles - 1;  // tile: last t with tile_items[t] <= item
+  __syncthreads();
+  for (int t = threadIdx.x; t < 3 * kTileNodes; t += kThreads) win[t] = 0ull;
+  __syncthreads();
+  int lo = 0, hi = tiles - 1;  // tile: last t with titems[t] <= item
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (sb.tile_items[mid] <= item) lo = mid;
+    if (titems[mid] <= item) lo = mid;
     else hi = mid - 1;
   }
   const int tile = lo;
-  const int first = sb.tile_items[tile], nitems = sb.tile_items[tile + 1] - first;
+  const int first = titems[tile], nitems = titems[tile + 1] - first;
   const int2 g0 = tile_range(sb.bin_start, tile, ts, 0), g1 = tile_range(sb.bin_start, tile, ts, 1);
   const int2 g2 = tile_range(sb.bin_start, tile, ts, 2), g3 = tile_range(sb.bin_start, tile, ts, 3);
   const int len0 = g0.y - g0.x, len1 = g1.y - g1.x, len2 = g2.y - g2.x;
   const int pts = len0 + len1 + len2 + (g3.y - g3.x);
-  const int per = (pts + nitems - 1) / nitems;
+  const int per = (pts + nitems - 1) / nitems;  // <= kSplatItem
   const int s0 = min(pts, (item - first) * per), s1 = min(pts, s0 + per);
   const int ox = (tile % ts) * kTile, oy = (tile / ts) * kTile;
   const float lx = st->lower_x, ly = st->lower_y, inv = st->inv_spacing;
   const double sm = st->fx_mass, sxy = st->fx_xy;
-  for (int t = threadIdx.x; t < 3 * kTileNodes; t += kThreads) win[t] = 0ull;
-  __syncthreads();
-  int pend = -1;
-  unsigned long long acc[12];
+  // the item's points, all loads issued together
+  float2 p[kItemPer];
 #pragma unroll
-  for (int q = 0; q < 12; ++q) acc[q] = 0ull;
-  for (int v = s0 + threadIdx.x; v < s1; v += kThreads) {
+  for (int u = 0; u < kItemPer; ++u) {
     // position v of the concatenated ranges own ++ left ++ upper ++ upper-left
+    const int v = s0 + threadIdx.x + u * kThreads;
     int idx;
     if (v < len0) idx = g0.x + v;
     else if (v < len0 + len1) idx = g1.x + (v - len0);
     else if (v < len0 + len1 + len2) idx = g2.x + (v - len0 - len1);
     else idx = g3.x + (v - len0 - len1 - len2);
-    const float2 p = sb.binned[idx];
-    const Cell c = grid_cell(p.x, p.y, lx, ly, inv, g);
+    p[u] = v < s1 ? sb.binned[idx] : make_float2(0.f, 0.f);
+  }
+  int pend = -1;
+  unsigned long long acc[12];
+#pragma unroll
+  for (int q = 0; q < 12; ++q) acc[q] = 0ull;
+#pragma unroll
+  for (int u = 0; u < kItemPer; ++u) {
+    if (s0 + threadIdx.x + u * kThreads >= s1) break;
+    const Cell c = grid_cell(p[u].x, p[u].y, lx, ly, inv, g);
     const int key = c.y0 * g + c.x0;
     if (key != pend) {
       if (pend >= 0) flush_cell(win, pend, g, ox, oy, acc);
@@ -550,8 +657,8 @@ __device__ __forceinline__ void splat_tiles_body(const TsneStats* __restrict__ s
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       acc[3 * q] += to_fixed(w[q], sm);
-      acc[3 * q + 1] += to_fixed(w[q] * p.x, sxy);
-      acc[3 * q + 2] += to_fixed(w[q] * p.y, sxy);
+      acc[3 * q + 1] += to_fixed(w[q] * p[u].x, sxy);
+      acc[3 * q + 2] += to_fixed(w[q] * p[u].y, sxy);
     }
   }
   if (pend >= 0) flush_cell(win, pend, g, ox, oy, acc);
@@ -584,25 +691,39 @@ __device__ __forceinline__ void splat_tiles_body(const TsneStats* __restrict__ s
   }
 }
 
-__global__ __launch_bounds__(kThreads) void splat_count(const float2* __restrict__ y,
-                                                        const TsneStats* __restrict__ st,
-                                                        SplatBuffers sb, int n, int g) {
+// Block blk of nblk takes the work items blk, blk + nblk, ...
+__device__ __forceinline__ void splat_tiles_body(const TsneStats* __restrict__ st, SplatBuffers sb,
+                                                 int g, int blk, int nblk) {
+  __shared__ unsigned long long win[3 * kTileNodes];
+  __shared__ int titems[kMaxTiles + 1];
+  const int ts = tiles_side(g), tiles = ts * ts;
+  const int total = sb.tile_items[tiles];
+  if (blk >= total) return;
+  for (int t = threadIdx.x; t <= tiles; t += kThreads) titems[t] = sb.tile_items[t];
+  for (int item = blk; item < total; item += nblk) {
+    splat_tile_item(st, sb, g, item, win, titems);
+  }
+}
+
+__global__ __launch_bounds__(kSplatThreads) void splat_count(const float2* __restrict__ y,
+                                                             const TsneStats* __restrict__ st,
+                                                             SplatBuffers sb, int n, int g) {
   splat_count_body(y, st, sb, n, g, blockIdx.x, gridDim.x);
 }
 
-__global__ __launch_bounds__(kPrepThreads) void splat_prep(SplatBuffers sb, int g) {
-  splat_prep_body(sb, g);
+__global__ __launch_bounds__(kSplatThreads) void splat_colscan(SplatBuffers sb, int g, int nrows) {
+  splat_colscan_body(sb, g, nrows, blockIdx.x);
 }
 
-__global__ __launch_bounds__(kThreads) void splat_scatter(const float2* __restrict__ y,
-                                                          const TsneStats* __restrict__ st,
-                                                          SplatBuffers sb, int n, int g) {
+__global__ __launch_bounds__(kSplatThreads) void splat_scatter(const float2* __restrict__ y,
+                                                               const TsneStats* __restrict__ st,
+                                                               SplatBuffers sb, int n, int g) {
   splat_scatter_body(y, st, sb, n, g, blockIdx.x, gridDim.x);
 }
 
 __global__ __launch_bounds__(kThreads) void splat_tiles(const TsneStats* __restrict__ st,
                                                         SplatBuffers sb, int g) {
-  splat_tiles_body(st, sb, g, blockIdx.x);
+  splat_tiles_body(st, sb, g, blockIdx.x, gridDim.x);
 }
 
 // ----------------------------------------------------------------------------
