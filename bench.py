@@ -15,8 +15,15 @@ optimizer state are resident in HBM before timing starts; the timed region is
 exactly --steps iterations (the first 250 with early exaggeration), bracketed
 by stream synchronisation.  After timing, the per-kernel durations are read
 with hipEvents on the plan's own stream (fastembedr_hip_tsne_plan_profile)
-for the roofline figure, and the reference CPU iteration (float32 port,
-oracle/cpu_port.c) is timed on a bounded sample on the host cores.
+for the roofline figure, and the reference CPU iteration (the reference's own
+functions from oracle/_ref, else the float32 port oracle/cpu_port.c) is timed
+on bounded samples at the host's thread count and at 4 threads.
+
+Quality gate: a second plan from the same start runs the full schedule
+(250 + 500) whatever --steps is, the CPU port runs the same job end to end
+(its own float32 P build, the CPU grid rule G = 256, 750 iterations), and
+trustworthiness and kNN preservation (k = 15) of both layouts on the same
+5000-row sample must agree within 0.01.
 """
 from __future__ import annotations
 
@@ -40,8 +47,13 @@ def parse():
     ap.add_argument("--k", type=int, default=90)
     ap.add_argument("--profile-iters", type=int, default=20)
     ap.add_argument("--cpu-iters", type=int, default=160,
-                    help="iterations of the CPU reference algorithm in the baseline sample")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu_count)")
+                    help="iterations of the CPU reference algorithm in the baseline sample at "
+                         "the host's thread count (a quarter of them at 4 threads)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="0: the host's thread count (nproc: the CPUs this process may run on)")
+    ap.add_argument("--no-quality-gate", action="store_true",
+                    help="skip the quality leg: the full 250 + 500 schedule on the device and the "
+                         "CPU port run end to end at the same size, compared on a 5000-row sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-json", default=None,
                     help="rocprofv3-derived HBM bytes per launch of the dominant kernel")
@@ -49,6 +61,9 @@ def parse():
                     help="build the UMAP fuzzy graph of the workload's first k neighbours on the "
                          "device (timed beside the CPU restatement on a 100k-point slice); 0 "
                          "skips it")
+    ap.add_argument("--umap-quality-n", type=int, default=50_000,
+                    help="points of the UMAP quality gate (device mode 0 against the reference "
+                         "CPU optimizer, 500 epochs); 0 skips it")
     ap.add_argument("--knn-check-rows", type=int, default=32,
                     help="query rows of the workload's exact KNN recomputed by the FP64 brute "
                          "force on the CPU (timed, compared); 0 skips it")
@@ -198,6 +213,72 @@ def bench_served(fe, workloads, n_requests, k=90):
 MFMA_F32_PEAK_TFLOPS = 157.3
 
 
+def host_cpus():
+    """The host's CPUs as this process sees them: nproc (the CPUs it may run
+    on), the cgroup CPU quota where one is set, and lscpu's summary."""
+    import subprocess
+
+    info = {"nproc": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count()}
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as fh:
+            quota, period = fh.read().split()
+        if quota != "max":
+            info["cgroup_cpus"] = round(int(quota) / int(period), 2)
+    except (OSError, ValueError):
+        pass
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        keep = ("Architecture", "CPU(s)", "Model name", "Socket(s)", "Core(s) per socket",
+                "Thread(s) per core", "NUMA node(s)", "CPU max MHz")
+        for line in out.splitlines():
+            key, _, val = line.partition(":")
+            if key.strip() in keep:
+                info["lscpu " + key.strip()] = val.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_baseline(row_ptr, col, val, layout, iters, threads, lr, n):
+    """The reference CPU optimizer's iteration rate on the same P and layout:
+    the reference's own functions (oracle/_ref: compute_gradient_fft_grid_f,
+    apply_open_tsne_update_f, zero_mean_f at its CPU grid rule) where they are
+    built, else the float32 port (oracle/cpu_port.c); timed at the host's
+    thread count and at 4 threads, on bounded samples of iterations."""
+    import oracle
+    from oracle import ref as oref
+
+    host = host_cpus()
+    # nproc, or the cgroup's CPU quota where that is smaller
+    nthreads = threads or max(1, min(host["nproc"], int(host.get("cgroup_cpus", host["nproc"]))))
+    use_ref = oref.lib() is not None
+
+    def rate(its, th):
+        if use_ref:
+            _, _, _, sec = oref.tsne_iterate(row_ptr, col, val, layout, its, 1.0, lr, 0.8,
+                                             grid=0, threads=th)
+        else:
+            _, sec = oracle.port_tsne_iterate(row_ptr, col, val, layout, its, exag=1.0,
+                                              learning_rate=lr, momentum=0.8, grid=0,
+                                              n_threads=th)
+        return its / sec, sec
+
+    g_cpu = 256 if n >= 50000 else 128
+    what = ("the reference's own CPU iteration (oracle/_ref, built from its sources)" if use_ref
+            else "the float32 port of the reference CPU iteration (oracle/cpu_port.c)")
+    v, sec = rate(iters, nthreads)
+    q = max(1, iters // 4)
+    v4, sec4 = rate(q, 4)
+    return {"value": round(v, 4), "unit": "iterations/s", "cores": nthreads,
+            "kind": "reference" if use_ref else "port",
+            "sample": f"{iters} normal-phase iterations of {what}: FFT grid at the CPU rule "
+                      f"G={g_cpu}, attraction, update, on the same {n}-point P and layout, "
+                      f"{nthreads} threads, {sec:.1f}s",
+            "at_4_threads": {"value": round(v4, 4), "cores": 4, "iterations": q,
+                             "seconds": round(sec4, 2)},
+            "host": host}
+
+
 def bench_knn(x, k, cpu, check_rows):
     """Exact KNN of the workload on the device (the producer feeding the
     t-SNE plan), with the FP64 brute force (oracle) timed on a sample of
@@ -267,22 +348,28 @@ def bench_umap_graph(fe, knn_i, knn_d, n, kk, cpu):
     eps = g.export()["epochs_per_sample"].astype(np.float64)
     init = np.random.default_rng(301).uniform(-10.0, 10.0, size=(n, 2)).astype(np.float32)
     g.optimize(init, 2, negative_sample_rate=5, seed=302)  # warm-up
-    t0 = time.perf_counter()
-    y_umap = g.optimize(init, n_epochs, negative_sample_rate=5, learning_rate=1.0, seed=302,
-                        optimizer_mode=1)
-    ep_s = time.perf_counter() - t0
     pos = float(np.floor(n_epochs / eps).sum())  # positive samples over the run
     epoch_bytes = 16 * g.nnz + 16 * n
-    out["epochs"] = {"epochs": n_epochs, "seconds": round(ep_s, 4),
-                     "epochs_per_s": round(n_epochs / ep_s, 1),
-                     "positive_samples_per_s": round(pos / ep_s, 1),
-                     "sampled_edges_per_s": round(pos * 6 / ep_s, 1),
-                     "roofline": {"bound": "hbm", "bytes_per_epoch": int(epoch_bytes),
-                                  "achieved_gbs": round(epoch_bytes * n_epochs / ep_s / 1e9, 1),
-                                  "frac": round(epoch_bytes * n_epochs / ep_s / 1e9 / HBM_PEAK_GBS,
-                                                4)},
-                     "layout_finite": bool(np.isfinite(y_umap).all()),
-                     "note": "host start layout in and layout out included"}
+    modes = {}
+    # mode 0: the reference's default for the GPU path (per-edge atomic
+    # updates); mode 1: the deterministic order-free delta form
+    for mode in (0, 1):
+        t0 = time.perf_counter()
+        y_umap = g.optimize(init, n_epochs, negative_sample_rate=5, learning_rate=1.0, seed=302,
+                            optimizer_mode=mode)
+        ep_s = time.perf_counter() - t0
+        modes[mode] = {"optimizer_mode": mode, "epochs": n_epochs, "seconds": round(ep_s, 4),
+                       "epochs_per_s": round(n_epochs / ep_s, 1),
+                       "positive_samples_per_s": round(pos / ep_s, 1),
+                       "sampled_edges_per_s": round(pos * 6 / ep_s, 1),
+                       "roofline": {"bound": "hbm", "bytes_per_epoch": int(epoch_bytes),
+                                    "achieved_gbs": round(epoch_bytes * n_epochs / ep_s / 1e9, 1),
+                                    "frac": round(epoch_bytes * n_epochs / ep_s / 1e9 /
+                                                  HBM_PEAK_GBS, 4)},
+                       "layout_finite": bool(np.isfinite(y_umap).all()),
+                       "note": "host start layout in and layout out included"}
+    out["epochs"] = modes[1]
+    out["epochs_mode0"] = modes[0]
     g.close()
     if cpu:
         import oracle
@@ -301,23 +388,89 @@ def bench_umap_graph(fe, knn_i, knn_d, n, kk, cpu):
                     "cpu_points_per_s": round(m / cpu_s, 1), "cpu_cores": 1,
                     "cpu_kind": "oracle restatement of umap_graph_csr_float_cpp, 1 thread",
                     "sample_matches_cpu": bool(same)})
-        # the CPU restatement of the epoch on the 100k-point slice's graph
-        off = ref["offsets"]
-        hh = np.repeat(np.arange(m, dtype=np.int32), np.diff(off))
-        tt = np.asarray(ref["neighbors"], np.int32)
-        pp = np.asarray(ref["epochs_per_sample"], np.float32)
-        ys = np.random.default_rng(301).uniform(-10.0, 10.0, size=(m, 2)).astype(np.float32)
-        cpu_epochs = 5
-        t0 = time.perf_counter()
-        oracle.umap_optimize(hh, tt, pp, ys, cpu_epochs, nsr=5, lr=1.0, seed=302, mode=1)
-        ce = time.perf_counter() - t0
-        cpos = float(np.floor(cpu_epochs / pp.astype(np.float64)).sum())
-        out["epochs"].update({"cpu_sample_points": int(m), "cpu_epochs": cpu_epochs,
-                              "cpu_s": round(ce, 3),
-                              "cpu_positive_samples_per_s": round(cpos / ce, 1), "cpu_cores": 1,
-                              "cpu_kind": "oracle restatement of the epoch (order-free delta "
-                                          "form), 1 thread"})
+        # the reference CPU optimizer (optimize_layout_csr) on the 100k-point
+        # slice's graph: its hogwild edge loop at the reference's thread cap
+        # (effective_cpu_threads: at most 4), and its serial form on 1 thread
+        out["epochs_cpu"] = umap_cpu_baseline(ref["offsets"], ref["neighbors"], ref["weights"], m)
     log(f"[bench] umap graph {out}")
+    return out
+
+
+def umap_cpu_baseline(offsets, neighbors, weights, m, n_epochs=500, sample_epochs=20):
+    """Positive samples/s of the reference CPU UMAP optimizer on a graph of m
+    points over a bounded number of epochs of the 500-epoch schedule: the
+    reference's own hogwild edge loop (oracle/_ref) where it is built, else its
+    restatement (oracle/oracle_umap_cpu.c), at the reference's cap of 4
+    threads; and the serial form (restatement) on 1 thread."""
+    import oracle
+    from oracle import ref as oref
+
+    rows, tails, eps = oracle.umap_cpu_active(offsets, neighbors, weights, n_epochs)
+    ys = np.random.default_rng(301).uniform(-10.0, 10.0, size=(m, 2)).astype(np.float32)
+    pos = float(np.floor(sample_epochs / eps.astype(np.float64)).sum())
+    threads = min(4, len(os.sched_getaffinity(0)))
+    if oref.lib() is not None:
+        _, sec = oref.umap_hogwild(rows, tails, eps, ys, sample_epochs, nsr=5, seed=302,
+                                   threads=threads)
+        kind = "reference"
+    else:
+        _, _, sec = oracle.umap_cpu_optimize(rows, tails, eps, ys, sample_epochs, nsr=5, seed=302,
+                                             n_threads=threads, form=2)
+        kind = "port"
+    _, _, sec1 = oracle.umap_cpu_optimize(rows, tails, eps, ys, sample_epochs, nsr=5, seed=302,
+                                          n_threads=1, form=0)
+    return {"value": round(pos / sec, 1), "unit": "positive samples/s", "cores": threads,
+            "kind": kind,
+            "sample": f"{sample_epochs} epochs (schedule of {n_epochs}) of the reference CPU "
+                      f"optimizer's hogwild form on the {m}-point graph, {threads} threads "
+                      f"(the reference caps its UMAP threads at 4), {sec:.2f}s",
+            "serial_1_thread": {"value": round(pos / sec1, 1), "seconds": round(sec1, 2),
+                                "kind": "port (serial form)"}}
+
+
+def umap_quality_gate(fe, workloads, n, k=15, n_epochs=500, sample=5000):
+    """GPU UMAP in the reference's GPU default (optimizer mode 0) against the
+    reference CPU optimizer (hogwild edge loop at its 4-thread cap) on one
+    n-point draw of the single-job generator: same fuzzy graph, same start;
+    trustworthiness and kNN preservation (k = 15) on a shared sample."""
+    import oracle
+    from oracle import ref as oref
+    from fastembedr_amd import knn, quality as fq, umap
+
+    x, _, _ = workloads.gaussian_mixture(n, 50, 100, seed=311, dirichlet=True)
+    ki, kd = knn.exact_knn(x, k)
+    ki, kd = np.ascontiguousarray(ki.T), np.ascontiguousarray(kd.T)
+    g = umap.FuzzyGraph(ki, kd, index_offset=0)
+    gr = g.export()
+    y0 = np.random.default_rng(301).uniform(-10, 10, size=(n, 2)).astype(np.float32)
+    t0 = time.perf_counter()
+    dev = g.optimize(y0, n_epochs, negative_sample_rate=5, seed=302, optimizer_mode=0)
+    dev_s = time.perf_counter() - t0
+    g.close()
+    rows, tails, eps = oracle.umap_cpu_active(gr["offsets"], gr["neighbors"], gr["weights"],
+                                              n_epochs)
+    threads = min(4, len(os.sched_getaffinity(0)))
+    if oref.lib() is not None:
+        cpu, cpu_s = oref.umap_hogwild(rows, tails, eps, y0, n_epochs, nsr=5, seed=302,
+                                       threads=threads)
+        kind = "reference hogwild edge loop (oracle/_ref)"
+    else:
+        cpu, _, cpu_s = oracle.umap_cpu_optimize(rows, tails, eps, y0, n_epochs, nsr=5, seed=302,
+                                                 n_threads=threads, form=2)
+        kind = "restated hogwild form (oracle/oracle_umap_cpu.c)"
+    s = np.sort(np.random.default_rng(7).choice(n, min(sample, n), replace=False))
+    hx = x[s].astype(np.float64)
+    mg = fq.exact_structure_metrics(hx, dev[s].astype(np.float64), [15])[0]
+    mc = fq.exact_structure_metrics(hx, cpu[s].astype(np.float64), [15])[0]
+    out = {"n": n, "k": k, "epochs": n_epochs, "sample": int(len(s)), "metric_k": 15,
+           "device": {"optimizer_mode": 0, "seconds": round(dev_s, 3),
+                      "trustworthiness": float(mg[0]), "knn_preservation": float(mg[2])},
+           "cpu": {"kind": kind, "threads": threads, "seconds": round(cpu_s, 3),
+                   "trustworthiness": float(mc[0]), "knn_preservation": float(mc[2])},
+           "delta_trustworthiness": float(mg[0] - mc[0]),
+           "delta_knn_preservation": float(mg[2] - mc[2]), "tolerance": 0.01,
+           "pass": bool(abs(mg[0] - mc[0]) <= 0.01 and abs(mg[2] - mc[2]) <= 0.01)}
+    log(f"[bench] umap quality gate {out}")
     return out
 
 
@@ -354,6 +507,15 @@ def main():
     plan = fe.TsnePlan(knn_i, knn_d, perplexity=30.0, grid_size=0, Y_init=y0, device=True,
                        n=n, k=k, index_offset=0)
     setup_s = time.perf_counter() - t0
+    gate_plan = host_knn = None
+    if not args.no_quality_gate:
+        # the quality leg's own plan: the full schedule from the same start,
+        # whatever --steps and --warmup are
+        gate_plan = fe.TsnePlan(knn_i, knn_d, perplexity=30.0, grid_size=0, Y_init=y0,
+                                device=True, n=n, k=k, index_offset=0)
+        if not args.no_cpu_baseline:
+            host_knn = (np.ascontiguousarray(knn_i.T.cpu().numpy()),
+                        np.ascontiguousarray(knn_d.T.cpu().numpy()))
     info = plan.info()
     attr_info = plan.attraction_info()
     log(f"[bench] attraction {attr_info}")
@@ -426,20 +588,40 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        import oracle
-
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         row_ptr, col, val = plan.affinities()
-        _, sec = oracle.port_tsne_iterate(row_ptr, col, val, layout, args.cpu_iters, exag=1.0,
-                                          learning_rate=lr, momentum=0.8, grid=0,
-                                          n_threads=threads)
-        cpu = {"value": round(args.cpu_iters / sec, 4), "unit": "iterations/s", "cores": threads,
-               "kind": "port",
-               "sample": f"{args.cpu_iters} iterations of the float32 reference CPU algorithm "
-                         f"(oracle/cpu_port.c: serial splat, radix-2 FFT at the CPU grid rule "
-                         f"G={256 if n >= 50000 else 128}, CSR attraction, update) on the same "
-                         f"1M-point P and layout, {threads} threads, {sec:.1f}s"}
+        cpu = cpu_baseline(row_ptr, col, val, layout, args.cpu_iters, args.cpu_threads, lr, n)
+        del row_ptr, col, val
         log(f"[bench] cpu baseline {cpu}")
+    gate = None
+    if gate_plan is not None:
+        t0 = time.perf_counter()
+        gate_plan.run(SCHEDULE["early"], SCHEDULE["early_exaggeration"], SCHEDULE["momentum"], lr)
+        gate_plan.run(SCHEDULE["normal"], 1.0, SCHEDULE["momentum"], lr)
+        gate_plan.sync()
+        gate = {"schedule": "250 early (12) + 500 normal, learning rate n/12, momentum 0.8",
+                "device_s": round(time.perf_counter() - t0, 3)}
+        # the quality figures below describe the layout of the full schedule
+        layout = gate_plan.layout()
+        gate_plan.close()
+        finite = finite and bool(np.all(np.isfinite(layout)))
+        if host_knn is not None:
+            import oracle
+
+            threads = cpu["cores"]
+            t0 = time.perf_counter()
+            y_cpu, aff_s, opt_s = oracle.port_tsne_run(
+                host_knn[0], host_knn[1], y0, perplexity=30.0, early_iter=SCHEDULE["early"],
+                n_iter=SCHEDULE["normal"], early_exag=SCHEDULE["early_exaggeration"], exag=1.0,
+                learning_rate=lr, mom0=SCHEDULE["momentum"], mom1=SCHEDULE["momentum"],
+                max_step=5.0, grid=0, n_threads=threads)
+            gate["cpu"] = {"kind": "port (oracle/cpu_port.c: its own float32 P build, then the "
+                                   "reference CPU optimizer at its grid rule G="
+                                   f"{256 if n >= 50000 else 128})",
+                           "threads": threads, "affinity_s": round(aff_s, 3),
+                           "optimise_s": round(opt_s, 3),
+                           "wall_s": round(time.perf_counter() - t0, 3)}
+            host_knn = None
+            log(f"[bench] quality gate CPU run {gate['cpu']}")
     quality = None
     if args.quality_sample > 0:
         from fastembedr_amd import quality as fq
@@ -455,6 +637,17 @@ def main():
         quality = {"sample": int(qs), "k": 15, "trustworthiness": float(m[0]),
                    "continuity": float(m[1]), "knn_preservation": float(m[2]),
                    "mean_neighbor_rank_error": float(m[3]), "device_ms": round(1e3 * dev_s, 3)}
+        if gate is not None and "cpu" in gate:
+            mc = fq.exact_structure_metrics(hs, y_cpu[sidx].astype(np.float64), [15])[0]
+            gate["sample"] = int(qs)
+            gate["k"] = 15
+            gate["device"] = {"trustworthiness": float(m[0]), "knn_preservation": float(m[2])}
+            gate["cpu"].update({"trustworthiness": float(mc[0]), "knn_preservation": float(mc[2])})
+            gate["delta_trustworthiness"] = float(m[0] - mc[0])
+            gate["delta_knn_preservation"] = float(m[2] - mc[2])
+            gate["tolerance"] = 0.01
+            gate["pass"] = bool(abs(m[0] - mc[0]) <= 0.01 and abs(m[2] - mc[2]) <= 0.01)
+            log(f"[bench] quality gate {gate}")
         if not args.no_cpu_baseline:
             import oracle
 
@@ -490,6 +683,9 @@ def main():
     plan.close()
     torch.cuda.empty_cache()
 
+    umap_gate = None
+    if args.umap_quality_n > 0 and not args.no_cpu_baseline:
+        umap_gate = umap_quality_gate(fe, workloads, args.umap_quality_n)
     batched = bench_batched(fe, workloads, args.batched_jobs) if args.batched_jobs > 0 else None
     torch.cuda.empty_cache()
     served = bench_served(fe, workloads, args.served_requests) if args.served_requests > 0 else None
@@ -525,12 +721,15 @@ def main():
         "setup_s": round(setup_s, 3),
         "layout_finite": finite,
         "cpu_baseline": cpu,
+        "quality_gate": gate,
     }
     out["knn"] = knn_report
     if quality:
         out["quality"] = quality
     if umap_graph:
         out["umap_graph"] = umap_graph
+    if umap_gate:
+        out["umap_quality_gate"] = umap_gate
     out["batched"] = batched
     out["served"] = served
     print(json.dumps(out), flush=True)

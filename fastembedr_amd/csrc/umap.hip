@@ -9,7 +9,8 @@
 // head, alpha = lr (1 - e / E), then coordinates sanitised to [-16, 16].
 //
 // optimizer_mode 0 updates the layout in place with float atomics (the
-// reference's hogwild form; not bit-reproducible).  optimizer_mode 1 reads the
+// reference's hogwild form; not bit-reproducible; see umap_epoch_hogwild for
+// how the atomics are grouped).  optimizer_mode 1 reads the
 // epoch-start layout and accumulates every update into a 64-bit fixed-point
 // delta (scale 2^40), so the epoch result does not depend on arrival order;
 // the apply kernel adds, sanitises and clears the delta in one pass, which
@@ -121,65 +122,6 @@ __device__ __forceinline__ int n_negative(float period, int nsr, unsigned epoch)
 
 __device__ __forceinline__ long long fixed(float v) { return __double2ll_rn(static_cast<double>(v) * kDeltaScale); }
 
-template <int MODE>
-__global__ __launch_bounds__(kUThreads) void umap_epoch_kernel(float* layout,
-                                                               unsigned long long* delta,
-                                                               const int* __restrict__ heads,
-                                                               const int* __restrict__ tails,
-                                                               const float* __restrict__ period,
-                                                               int n_edges, UmapParams p,
-                                                               unsigned epoch) {
-  const float alpha = p.lr * (1.0f - __fdiv_rn(static_cast<float>(epoch),
-                                               fmaxf(1.0f, static_cast<float>(p.n_epochs))));
-  constexpr float eps = 1.1920928955078125e-7f;
-  for (int e = blockIdx.x * kUThreads + threadIdx.x; e < n_edges; e += gridDim.x * kUThreads) {
-    const int h = heads[e], t = tails[e];
-    if (h < 0 || h >= p.n || t < 0 || t >= p.n || h == t) continue;
-    const float per = period[e];
-    const int npos = n_positive(per, epoch);
-    if (npos <= 0) continue;
-    const size_t hb = static_cast<size_t>(h) * 2, tb = static_cast<size_t>(t) * 2;
-    for (int s = 0; s < npos; ++s) {
-      const float dx = layout[hb] - layout[tb];
-      const float dy = layout[hb + 1] - layout[tb + 1];
-      const float d2 = fmaxf(eps, dx * dx + dy * dy);
-      const float c = attract(d2, p);
-      const float gx = clip4(c * dx) * alpha, gy = clip4(c * dy) * alpha;
-      if constexpr (MODE == 0) {
-        atomicAdd(layout + hb, gx);
-        atomicAdd(layout + hb + 1, gy);
-        atomicAdd(layout + tb, -gx);
-        atomicAdd(layout + tb + 1, -gy);
-      } else {
-        atomicAdd(delta + hb, static_cast<unsigned long long>(fixed(gx)));
-        atomicAdd(delta + hb + 1, static_cast<unsigned long long>(fixed(gy)));
-        atomicAdd(delta + tb, static_cast<unsigned long long>(-fixed(gx)));
-        atomicAdd(delta + tb + 1, static_cast<unsigned long long>(-fixed(gy)));
-      }
-    }
-    const int nneg = n_negative(per, p.nsr, epoch);
-    for (int s = 0; s < nneg; ++s) {
-      const unsigned v = negative_vertex(static_cast<unsigned>(p.n), p.seed, epoch,
-                                         static_cast<unsigned>(h), static_cast<unsigned>(t),
-                                         static_cast<unsigned>(s));
-      if (static_cast<int>(v) == h || static_cast<int>(v) == t) continue;
-      const size_t vb = static_cast<size_t>(v) * 2;
-      const float dx = layout[hb] - layout[vb];
-      const float dy = layout[hb + 1] - layout[vb + 1];
-      const float d2 = fmaxf(eps, dx * dx + dy * dy);
-      const float c = repel(d2, p);
-      const float gx = clip4(c * dx) * alpha, gy = clip4(c * dy) * alpha;
-      if constexpr (MODE == 0) {
-        atomicAdd(layout + hb, gx);
-        atomicAdd(layout + hb + 1, gy);
-      } else {
-        atomicAdd(delta + hb, static_cast<unsigned long long>(fixed(gx)));
-        atomicAdd(delta + hb + 1, static_cast<unsigned long long>(fixed(gy)));
-      }
-    }
-  }
-}
-
 __device__ __forceinline__ unsigned long long shfl_up_u64(unsigned long long v, int d) {
   const int lo = __shfl_up(static_cast<int>(static_cast<unsigned>(v)), d, 64);
   const int hi = __shfl_up(static_cast<int>(static_cast<unsigned>(v >> 32)), d, 64);
@@ -283,6 +225,112 @@ __global__ __launch_bounds__(kUThreads) void umap_epoch_grouped(unsigned long lo
   }
 }
 
+// optimizer_mode 0 (hogwild): every read sees the live layout and every
+// update is a float atomic on it, as in the reference's in-place kernel; an
+// edge's positive samples run one after another in registers (each seeing the
+// previous ones), its tail receives their sum in one atomic pair, and the
+// head's share of the edges of one head run (positives and negatives) is
+// summed across the lanes holding the run, one atomic pair per run.  Which
+// concurrent update a read sees is a race, in the reference too.  SYM (the
+// symmetric fuzzy graph, where the reverse edge (t, h) moves t as this edge
+// would): the head takes both endpoints' share of a positive sample, twice its
+// own, and the tail none, so the pair's distance shrinks by the same amount
+// per sample and no tail atomics are issued.
+template <bool SYM>
+__global__ __launch_bounds__(kUThreads) void umap_epoch_hogwild(float* layout,
+                                                                const int* __restrict__ heads,
+                                                                const int* __restrict__ tails,
+                                                                const float* __restrict__ period,
+                                                                int n_edges, UmapParams p,
+                                                                unsigned epoch) {
+  const float alpha = p.lr * (1.0f - __fdiv_rn(static_cast<float>(epoch),
+                                               fmaxf(1.0f, static_cast<float>(p.n_epochs))));
+  constexpr float eps = 1.1920928955078125e-7f;
+  const int lane = threadIdx.x & 63;
+  const unsigned long long upto_lane = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  for (int base = blockIdx.x * kUThreads + (threadIdx.x & ~63); base < n_edges;
+       base += gridDim.x * kUThreads) {
+    const int e = base + lane;
+    int h = -1;
+    float ux = 0.0f, uy = 0.0f;
+    if (e < n_edges) {
+      h = heads[e];
+      const int t = tails[e];
+      if (h >= 0 && h < p.n && t >= 0 && t < p.n && h != t) {
+        const float per = period[e];
+        const int npos = n_positive(per, epoch);
+        if (npos > 0) {
+          const size_t hb = static_cast<size_t>(h) * 2, tb = static_cast<size_t>(t) * 2;
+          float hx = layout[hb], hy = layout[hb + 1];
+          float tx = layout[tb], ty = layout[tb + 1];
+          float sx = 0.0f, sy = 0.0f;
+          for (int s = 0; s < npos; ++s) {
+            const float dx = hx - tx, dy = hy - ty;
+            const float d2 = fmaxf(eps, dx * dx + dy * dy);
+            const float c = attract(d2, p);
+            const float gx = clip4(c * dx) * alpha, gy = clip4(c * dy) * alpha;
+            if constexpr (SYM) {
+              hx += 2.0f * gx;
+              hy += 2.0f * gy;
+              sx += 2.0f * gx;
+              sy += 2.0f * gy;
+            } else {
+              hx += gx;
+              hy += gy;
+              tx -= gx;
+              ty -= gy;
+              sx += gx;
+              sy += gy;
+            }
+          }
+          if constexpr (!SYM) {
+            atomicAdd(layout + tb, -sx);
+            atomicAdd(layout + tb + 1, -sy);
+          }
+          ux = sx;
+          uy = sy;
+          const int nneg = n_negative(per, p.nsr, epoch);
+          for (int s = 0; s < nneg; ++s) {
+            const unsigned v = negative_vertex(static_cast<unsigned>(p.n), p.seed, epoch,
+                                               static_cast<unsigned>(h), static_cast<unsigned>(t),
+                                               static_cast<unsigned>(s));
+            if (static_cast<int>(v) == h || static_cast<int>(v) == t) continue;
+            const size_t vb = static_cast<size_t>(v) * 2;
+            const float dx = hx - layout[vb];
+            const float dy = hy - layout[vb + 1];
+            const float d2 = fmaxf(eps, dx * dx + dy * dy);
+            const float c = repel(d2, p);
+            const float gx = clip4(c * dx) * alpha, gy = clip4(c * dy) * alpha;
+            hx += gx;
+            hy += gy;
+            ux += gx;
+            uy += gy;
+          }
+        }
+      }
+    }
+    // segmented inclusive scan over the runs of equal heads
+    const int hprev = __shfl_up(h, 1, 64);
+    const bool start = lane == 0 || hprev != h;
+    const unsigned long long starts = __ballot(start) & upto_lane;
+    const int lim = 63 - __clzll(starts);  // first lane of this lane's run
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const float ox = __shfl_up(ux, d, 64), oy = __shfl_up(uy, d, 64);
+      if (lane - d >= lim) {
+        ux += ox;
+        uy += oy;
+      }
+    }
+    const int hnext = __shfl_down(h, 1, 64);
+    const bool last = lane == 63 || hnext != h;
+    if (last && h >= 0 && h < p.n && (ux != 0.0f || uy != 0.0f)) {
+      atomicAdd(layout + static_cast<size_t>(h) * 2, ux);
+      atomicAdd(layout + static_cast<size_t>(h) * 2 + 1, uy);
+    }
+  }
+}
+
 // sanitise (:3227-3258); MODE 1 first adds and clears the fixed-point delta
 template <int MODE>
 __global__ __launch_bounds__(kUThreads) void umap_apply_kernel(float* layout,
@@ -373,9 +421,13 @@ int umap_run(const int* heads, const int* tails, const float* weights,
   const int rblocks = static_cast<int>((m + kUThreads - 1) / kUThreads);
   auto enqueue = [&](int epoch) -> int {
     if (mode == 0) {
-      umap_epoch_kernel<0><<<eblocks, kUThreads, 0, s>>>(layout, delta, dh, dt, dp, n_edges, p,
-                                                         static_cast<unsigned>(epoch));
-      FE_LAUNCH_CHECK("umap_epoch_kernel");
+      if (symmetric)
+        umap_epoch_hogwild<true><<<eblocks, kUThreads, 0, s>>>(layout, dh, dt, dp, n_edges, p,
+                                                               static_cast<unsigned>(epoch));
+      else
+        umap_epoch_hogwild<false><<<eblocks, kUThreads, 0, s>>>(layout, dh, dt, dp, n_edges, p,
+                                                                static_cast<unsigned>(epoch));
+      FE_LAUNCH_CHECK("umap_epoch_hogwild");
       umap_apply_kernel<0><<<rblocks, kUThreads, 0, s>>>(layout, delta, static_cast<int>(m));
     } else {
       if (symmetric)
@@ -485,10 +537,10 @@ int umap_run_noalloc(const int* heads, const int* tails, const float* epochs_per
   const int eblocks = static_cast<int>(std::min<long long>((n_edges + kUThreads - 1) / kUThreads, 8192));
   for (int e = 0; e < n_epochs; ++e) {
     if (mode == 0) {
-      umap_epoch_kernel<0><<<eblocks, kUThreads, 0, s>>>(layout, delta, heads, tails,
-                                                         epochs_per_sample, n_edges, p,
-                                                         static_cast<unsigned>(e));
-      FE_LAUNCH_CHECK("umap_epoch_kernel");
+      umap_epoch_hogwild<false><<<eblocks, kUThreads, 0, s>>>(layout, heads, tails,
+                                                              epochs_per_sample, n_edges, p,
+                                                              static_cast<unsigned>(e));
+      FE_LAUNCH_CHECK("umap_epoch_hogwild");
       umap_apply_kernel<0><<<rblocks, kUThreads, 0, s>>>(layout, delta, m);
     } else {
       umap_epoch_grouped<false><<<eblocks, kUThreads, 0, s>>>(delta, layout, heads, tails,
