@@ -177,9 +177,18 @@ static int build_affinities(const int* idx, const DistT* dist, int n, int k, int
                           hipMemcpyDeviceToHost, s));
   FE_CHECK(hipMemcpyAsync(&h_flags, flags, sizeof(unsigned), hipMemcpyDeviceToHost, s));
   FE_CHECK(hipStreamSynchronize(s));
-  if (h_flags & kErrRowTooLong)
-    return fail("symmetric affinity row longer than " + std::to_string(kSortCap) +
-                " entries (a point is the neighbour of too many others)");
+  Arena long_scratch;
+  if (h_flags & kLongRow) {
+    // rows longer than kSortCap: sorted in global memory, one block each
+    ArenaPlan lp;
+    lp.add<unsigned long long>(2 * total);
+    if (long_scratch.init(lp.bytes, "t-SNE affinity long-row sort")) return 1;
+    unsigned long long* scratch = long_scratch.take<unsigned long long>(2 * total);
+    sym_sort_merge_global<<<64, kLongThreads, 0, s>>>(raw, scratch, raw_ptr, cursor, long_count,
+                                                     merged_len);
+    FE_LAUNCH_CHECK("sym_sort_merge_global");
+    FE_CHECK(hipStreamSynchronize(s));
+  }
   double mass = 0.0;
   for (double v : mp) mass += v;
   if (!std::isfinite(mass) || mass <= 0.0) {

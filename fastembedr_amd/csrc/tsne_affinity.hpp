@@ -16,7 +16,9 @@
 //                     without atomics, the reverse ones with one atomic
 //                     each), each row is sorted by (column, value bits) —
 //                     by one wave for rows of up to 512 entries, by one
-//                     block in LDS for longer ones — so the result does not
+//                     block in LDS up to kSortCap, by one block in global
+//                     memory (LDS-sorted chunks merged pairwise) beyond —
+//                     so the result does not
 //                     depend on atomic arrival order, equal columns are
 //                     summed in FP64, and P_ij = float(sum) *
 //                     float(0.5 / total_mass).  Rows come out with ascending
@@ -88,7 +90,96 @@ __device__ __forceinline__ bool wave_sort_class(int len) {
   return len <= 64 * R && (R == 1 || len > 32 * R);
 }
 
-enum AffinityError : unsigned { kErrIndexRange = 1u, kErrDistance = 2u, kErrRowTooLong = 4u };
+// kLongRow: some row is longer than kSortCap and is left to the global sort
+enum AffinityError : unsigned { kErrIndexRange = 1u, kErrDistance = 2u, kLongRow = 4u };
+
+// ---------------------------------------------------------------------------
+// Rows longer than kSortCap (a point in very many KNN lists), one block of
+// kLongThreads per row: chunks of kSortCap keys are sorted in LDS into
+// `scratch`, then sorted runs are merged pairwise between `scratch` and the
+// row itself (merge path: every thread writes an equal share of the output,
+// its start found by a binary search on its diagonal; ties take the left
+// run first) until one run remains, which ends in `scratch`.  The caller's
+// group pass then reads `scratch` and writes its compacted result over the
+// row.  The sorted order is a function of the keys only.
+// ---------------------------------------------------------------------------
+constexpr int kLongThreads = 256;
+
+// ascending bitonic sort of keys[0, pw) (pw a power of two) by the block
+__device__ __forceinline__ void block_bitonic(unsigned long long* keys, int pw) {
+  for (int size = 2; size <= pw; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = threadIdx.x; t < pw / 2; t += kLongThreads) {
+        const int lo = 2 * stride * (t / stride) + (t % stride);
+        const int hi = lo + stride;
+        const bool up = ((lo & size) == 0);
+        const unsigned long long a = keys[lo], b = keys[hi];
+        if ((a > b) == up) {
+          keys[lo] = b;
+          keys[hi] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// entries of a among the first d outputs of the merge of a and b
+__device__ __forceinline__ int merge_split(const unsigned long long* a, int la,
+                                           const unsigned long long* b, int lb, int d) {
+  int lo = max(0, d - lb), hi = min(d, la);
+  while (lo < hi) {
+    const int m = (lo + hi) >> 1;
+    if (a[m] <= b[d - 1 - m]) lo = m + 1;
+    else hi = m;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ void merge_runs(const unsigned long long* a, int la,
+                                           const unsigned long long* b, int lb,
+                                           unsigned long long* out) {
+  const long long total = static_cast<long long>(la) + lb;
+  const int d0 = static_cast<int>(total * threadIdx.x / kLongThreads);
+  const int d1 = static_cast<int>(total * (threadIdx.x + 1) / kLongThreads);
+  int i = merge_split(a, la, b, lb, d0), j = d0 - i;
+  for (int o = d0; o < d1; ++o) {
+    const bool take_a = j >= lb || (i < la && a[i] <= b[j]);
+    out[o] = take_a ? a[i++] : b[j++];
+  }
+}
+
+// Sorts row[0, len) into scratch[0, len); keys: kSortCap entries of LDS.
+__device__ inline void sort_row_global(unsigned long long* row, unsigned long long* scratch,
+                                       int len, unsigned long long* keys) {
+  for (int c0 = 0; c0 < len; c0 += kSortCap) {
+    const int cl = min(kSortCap, len - c0);
+    int pw = 1;
+    while (pw < cl) pw <<= 1;
+    __syncthreads();  // the previous chunk's keys are stored
+    for (int t = threadIdx.x; t < pw; t += kLongThreads) keys[t] = t < cl ? row[c0 + t] : ~0ull;
+    __syncthreads();
+    block_bitonic(keys, pw);
+    for (int t = threadIdx.x; t < cl; t += kLongThreads) scratch[c0 + t] = keys[t];
+  }
+  __syncthreads();
+  unsigned long long* src = scratch;
+  unsigned long long* dst = row;
+  for (int w = kSortCap; w < len; w *= 2) {
+    for (int lo = 0; lo < len; lo += 2 * w) {
+      const int mid = min(lo + w, len), hi = min(lo + 2 * w, len);
+      merge_runs(src + lo, mid - lo, src + mid, hi - mid, dst + lo);
+    }
+    __syncthreads();
+    unsigned long long* t = src;
+    src = dst;
+    dst = t;
+  }
+  if (src != scratch) {
+    for (int t = threadIdx.x; t < len; t += kLongThreads) scratch[t] = src[t];
+    __syncthreads();
+  }
+}
 
 template <typename DistT>
 constexpr int aff_rows_per_block() {
@@ -345,10 +436,8 @@ __device__ __forceinline__ void sort_merge_long(unsigned long long* __restrict__
   __shared__ unsigned long long keys[kSortCap];
   const int beg = raw_ptr[row], len = raw_ptr[row + 1] - beg;
   if (len > kSortCap) {
-    if (threadIdx.x == 0) {
-      atomicOr(error_flags, kErrRowTooLong);
-      merged_len[row] = 0;
-    }
+    // sym_sort_merge_global takes it
+    if (threadIdx.x == 0) atomicOr(error_flags, kLongRow);
     return;
   }
   int pw = 1;
@@ -398,6 +487,38 @@ __global__ __launch_bounds__(256) void sym_sort_merge(unsigned long long* __rest
   for (int q = blockIdx.x; q < rows; q += gridDim.x) {
     __syncthreads();  // the previous row's keys are read
     sort_merge_long(raw, raw_ptr, long_rows[q], merged_len, error_flags);
+  }
+}
+
+// The queued rows longer than kSortCap: global sort, then the merge of
+// equal columns written over the row.
+__global__ __launch_bounds__(kLongThreads) void sym_sort_merge_global(
+    unsigned long long* __restrict__ raw, unsigned long long* __restrict__ scratch,
+    const int* __restrict__ raw_ptr, const int* __restrict__ long_rows,
+    const int* __restrict__ long_count, int* __restrict__ merged_len) {
+  __shared__ unsigned long long keys[kSortCap];
+  const int rows = *long_count;
+  for (int q = blockIdx.x; q < rows; q += gridDim.x) {
+    const int row = long_rows[q];
+    const int beg = raw_ptr[row], len = raw_ptr[row + 1] - beg;
+    if (len <= kSortCap) continue;  // block-uniform
+    __syncthreads();
+    sort_row_global(raw + beg, scratch + beg, len, keys);
+    const unsigned long long* sorted = scratch + beg;
+    const int ch = (len + kLongThreads - 1) / kLongThreads;
+    const int c0 = threadIdx.x * ch, c1 = min(c0 + ch, len);
+    int heads = 0;
+    for (int t = c0; t < c1; ++t)
+      if (t == 0 || (sorted[t] >> 32) != (sorted[t - 1] >> 32)) ++heads;
+    int total;
+    int out = block_exclusive_scan_nt<kLongThreads>(heads, &total);
+    if (threadIdx.x == 0) merged_len[row] = total;
+    for (int t = c0; t < c1; ++t) {
+      if (t == 0 || (sorted[t] >> 32) != (sorted[t - 1] >> 32)) {
+        raw[beg + out] = merged_entry(sorted, t, len);
+        ++out;
+      }
+    }
   }
 }
 

@@ -273,10 +273,8 @@ __device__ __forceinline__ void sort_union_long(unsigned long long* __restrict__
   __shared__ unsigned long long keys[kSortCap];
   const int beg = raw_ptr[row], len = raw_ptr[row + 1] - beg;
   if (len > kSortCap) {
-    if (threadIdx.x == 0) {
-      atomicOr(error_flags, kErrRowTooLong);
-      union_len[row] = 0;
-    }
+    // ug_sort_union_global takes it
+    if (threadIdx.x == 0) atomicOr(error_flags, kLongRow);
     return;
   }
   int pw = 1;
@@ -381,6 +379,40 @@ __global__ __launch_bounds__(kUgThreads) void ug_sort_union(unsigned long long* 
   for (int q = blockIdx.x; q < rows; q += gridDim.x) {
     __syncthreads();  // the previous row's keys are read
     sort_union_long(raw, raw_ptr, long_rows[q], union_len, error_flags);
+  }
+}
+
+// The queued rows longer than kSortCap: global sort (sort_row_global), then
+// one union weight per neighbour written over the row.
+__global__ __launch_bounds__(kLongThreads) void ug_sort_union_global(
+    unsigned long long* __restrict__ raw, unsigned long long* __restrict__ scratch,
+    const int* __restrict__ raw_ptr, const int* __restrict__ long_rows,
+    const int* __restrict__ long_count, int* __restrict__ union_len) {
+  __shared__ unsigned long long keys[kSortCap];
+  const int rows = *long_count;
+  for (int q = blockIdx.x; q < rows; q += gridDim.x) {
+    const int row = long_rows[q];
+    const int beg = raw_ptr[row], len = raw_ptr[row + 1] - beg;
+    if (len <= kSortCap) continue;  // block-uniform
+    __syncthreads();
+    sort_row_global(raw + beg, scratch + beg, len, keys);
+    const unsigned long long* sorted = scratch + beg;
+    auto starts_group = [&](int t) { return t == 0 || (sorted[t] >> 32) != (sorted[t - 1] >> 32); };
+    const int ch = (len + kLongThreads - 1) / kLongThreads;
+    const int c0 = threadIdx.x * ch, c1 = min(c0 + ch, len);
+    int mine = 0;
+    for (int t = c0; t < c1; ++t)
+      if (starts_group(t) && ug_kept(ug_group_weight(sorted, t, len))) ++mine;
+    int kept_total;
+    int out = block_exclusive_scan_nt<kLongThreads>(mine, &kept_total);
+    if (threadIdx.x == 0) union_len[row] = kept_total;
+    for (int t = c0; t < c1; ++t) {
+      if (!starts_group(t)) continue;
+      const float w = ug_group_weight(sorted, t, len);
+      if (!ug_kept(w)) continue;
+      raw[beg + out] = (sorted[t] >> 32 << 32) | __float_as_uint(w);
+      ++out;
+    }
   }
 }
 
@@ -572,8 +604,21 @@ int UmapGraph::build(const int* indices, const float* distances, int n_, int k_t
   FE_CHECK(hipMemcpyAsync(&h_nnz, offs + n, sizeof(int), hipMemcpyDeviceToHost, s));
   FE_CHECK(hipMemcpyAsync(&h_flags, flags, sizeof(unsigned), hipMemcpyDeviceToHost, s));
   FE_CHECK(hipStreamSynchronize(s));
-  if (h_flags & kErrRowTooLong)
-    return fail("UMAP fuzzy graph: a point takes part in more than 8192 KNN edges");
+  Arena long_scratch;
+  if (h_flags & kLongRow) {
+    // rows longer than kSortCap: sorted in global memory, one block each,
+    // then the union lengths scanned again
+    ArenaPlan lp;
+    lp.add<unsigned long long>(std::max(h_raw, 1));
+    if (long_scratch.init(lp.bytes, "UMAP fuzzy graph long-row sort")) return 1;
+    unsigned long long* scratch = long_scratch.take<unsigned long long>(std::max(h_raw, 1));
+    ug_sort_union_global<<<64, kLongThreads, 0, s>>>(raw, scratch, raw_ptr, deg, long_count,
+                                                    cursor);
+    FE_LAUNCH_CHECK("ug_sort_union_global");
+    if (ug_scan(cursor, n + 1, offs, sums, tot, s)) return 1;
+    FE_CHECK(hipMemcpyAsync(&h_nnz, offs + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    FE_CHECK(hipStreamSynchronize(s));
+  }
   nnz = h_nnz;
   ArenaPlan sp;
   sp.add<int>(n + 1);

@@ -298,3 +298,36 @@ def test_pair_attraction_staged_and_rest_streams(monkeypatch, dense_min):
     ref = oracle.attractive(csr, y.astype(np.float64), 12.0)
     assert relative_l2(a.attractive_force, ref) < 5e-5
     assert np.array_equal(a.attractive_force, b.attractive_force)
+
+
+def _hub_knn(n=30000, k=15):
+    """A KNN-shaped input with hub points: row i's neighbours are i+1 .. i+k
+    (mod n) at increasing distances; the last neighbour of 20000 rows is
+    replaced by point 0 and of 9000 rows by point 1, and the next-to-last
+    neighbour of 8200 rows by point 2, so the symmetric rows of the hubs
+    exceed the 8192 entries sorted in LDS (point 0's needs two merge passes)."""
+    i = np.arange(n)[:, None]
+    idx = ((i + np.arange(1, k + 1)[None, :]) % n).astype(np.int32)
+    dist = ((np.arange(k)[None, :] + 1.0) * (1.0 + 0.01 * (i % 7))).astype(np.float32)
+    rows = np.arange(100, n - 100)
+    np.random.default_rng(731).shuffle(rows)
+    for hub, col, take in ((0, k - 1, rows[:20000]), (1, k - 1, rows[20000:29000]),
+                           (2, k - 2, rows[:8200])):
+        idx[take, col] = hub
+    return idx, dist
+
+
+def test_hub_rows_beyond_lds_sort_cap():
+    idx, dist = _hub_knn()
+    csr = oracle.build_p(idx, dist.astype(np.float64), 5.0)
+    lengths = np.diff(csr.row_ptr)
+    assert lengths[0] > 16384 and lengths[1] > 8192 and lengths[2] > 8192
+    y = np.random.default_rng(732).normal(scale=5.0, size=(len(idx), 2)).astype(np.float32)
+    print("[tsne] hub rows beyond the LDS sort", flush=True)
+    dg = fe().gradient_diagnostic(idx, dist, y, 5.0, 1.0, 128)
+    assert np.array_equal(dg.row_ptr, csr.row_ptr)
+    assert np.array_equal(dg.col, csr.col)
+    assert np.abs(dg.val - csr.val).max() <= 2e-6 * csr.val.max()
+    again = fe().gradient_diagnostic(idx, dist, y, 5.0, 1.0, 128)
+    assert np.array_equal(dg.val, again.val)
+    assert relative_l2(dg.attractive_force, oracle.attractive(csr, y.astype(np.float64))) < 5e-5

@@ -106,3 +106,19 @@ def test_fuzzy_graph_device_inputs_and_resident_epochs():
     # device-COO path on the same resident graph
     r3 = dev.optimize_coo_device(y0, 30, seed=5, optimizer_mode=1)
     assert np.array_equal(r1, r3)
+
+
+def test_fuzzy_graph_rows_beyond_lds_sort_cap():
+    """Union rows of more than 8192 raw entries (a point in over 8192 KNN
+    lists) take the global sort: the graph is still the oracle's."""
+    from fastembedr_amd import umap
+    from tests.test_tsne_gpu import _hub_knn
+
+    idx, dist = _hub_knn()
+    print("[umap graph] rows beyond the LDS sort", flush=True)
+    fg = umap.FuzzyGraph(idx, dist, index_offset=0)
+    g = fg.export()
+    lengths = np.diff(g["offsets"])
+    assert lengths[0] > 16384 and lengths[1] > 8192
+    _compare(g, oracle.umap_fuzzy_graph(idx, dist.astype(np.float64), index_offset=0))
+    fg.close()
