@@ -179,22 +179,27 @@ def bench_served(fe, workloads, n_requests, k=90):
                       d=int(rng.integers(30, 51)), clusters=int(rng.integers(5, 51)),
                       seed=int(rng.integers(1 << 30)))
             ji, jd = workloads.job_knn_device(sp, k)
-            payload = dict(indices=ji, distances=jd, device=True, n=sp["n"], k=k, index_offset=0,
-                           perplexity=30.0, Y_init=workloads.centred_normal_init(sp["n"], sp["seed"]),
+            # host n x k views of the column-major KNN (Fortran order: passed as is)
+            payload = dict(indices=ji.cpu().numpy().T, distances=jd.cpu().numpy().T,
+                           index_offset=0, perplexity=30.0,
+                           Y_init=workloads.centred_normal_init(sp["n"], sp["seed"]),
                            early_exaggeration_iter=SCHEDULE["early"], n_iter=SCHEDULE["normal"])
+            del ji, jd
             reqs.append(serve.Request(rid, "embed", t, payload))
         else:
             m = int(workloads.log_uniform_sizes(1, 1_000, 50_000, rng)[0])
             xq, lq = workloads.sample_mixture(m, centres, weights, rng)
             qi, qd = workloads.knn_to_reference(xq, xr, k)
-            reqs.append(serve.Request(rid, "transform", t, dict(indices=qi, distances=qd)))
+            reqs.append(serve.Request(rid, "transform", t, dict(indices=np.asfortranarray(qi),
+                                                                distances=np.asfortranarray(qd))))
     torch.cuda.synchronize()
     n_embed = sum(r.kind == "embed" for r in reqs)
     log(f"[served] {n_requests} requests ({n_embed} embed), trace span {t:.2f}s")
     free, _ = torch.cuda.mem_get_info()
     budget = int(0.5 * free)
-    srv = serve.Server(reference_layout=ref_layout, max_jobs=64, hbm_budget_bytes=budget, tick=25)
+    srv = serve.Server(reference_layout=ref_layout, max_jobs=64, hbm_budget_bytes=budget)
     stats = srv.run(reqs, realtime=True)
+    sess = srv.session.stats()
     srv.close()
     summ = stats.summary()
     lat_e = [stats.latencies[r.rid] for r in reqs if r.kind == "embed"]
@@ -203,6 +208,10 @@ def bench_served(fe, workloads, n_requests, k=90):
     return {"workload": f"{n_requests} Poisson requests at {rate}/s: transform_tsne of 1k-50k "
                         f"queries against a resident {n_ref}-point reference (k={k}) and "
                         f"10k-200k-point t-SNE jobs (750 iterations), HBM budget {budget >> 20} MiB",
+            "engine": "native session (fastembedr_hip_session_*): one engine thread, 25-iteration "
+                      "batch ticks, transforms on 2 streams with pooled workspace "
+                      f"({sess['pool_bytes'] >> 20} MiB of the budget)",
+            "engine_ticks": sess["ticks"],
             **summ,
             "embed_p50_ms": round(1e3 * float(np.percentile(lat_e, 50)), 3) if lat_e else None,
             "embed_p99_ms": round(1e3 * float(np.percentile(lat_e, 99)), 3) if lat_e else None,

@@ -66,6 +66,15 @@ SIGNATURES = {
     "fastembedr_hip_tsne_batch_layout": (c_int, [_vp, c_int, _fp]),
     "fastembedr_hip_tsne_batch_remove": (c_int, [_vp, c_int]),
     "fastembedr_hip_tsne_batch_destroy": (None, [_vp]),
+    "fastembedr_hip_session_create": (c_int, [_fp, c_int, c_int, c_ll, c_ll, _vpp]),
+    "fastembedr_hip_session_submit_embed": (c_int, [_vp, _vp, _llp]),
+    "fastembedr_hip_session_submit_transform": (
+        c_int, [_vp, _ip, _fp, c_int, c_int, c_int, _fp, c_int, c_double, c_int, c_int, c_double,
+                c_double, c_double, c_double, c_double, c_double, c_double, c_int, c_int, c_int,
+                _llp]),
+    "fastembedr_hip_session_poll": (c_int, [_vp, c_ll, c_int, _ip, _fp, _dp]),
+    "fastembedr_hip_session_stats": (c_int, [_vp, _llp]),
+    "fastembedr_hip_session_destroy": (None, [_vp]),
     "fastembedr_hip_umap_optimize_coo": (
         c_int, [_ip, _ip, _fp, _fp, _fp, c_int, c_int, c_int, c_int, c_float, c_float, c_float,
                 c_float, c_uint, c_int, _fp]),

@@ -10,7 +10,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfastembedr_hip.so")
-SOURCES = ["tsne.hip", "umap.hip", "umap_graph.hip", "transform.hip", "quality.hip", "knn.hip"]
+SOURCES = ["tsne.hip", "umap.hip", "umap_graph.hip", "transform.hip", "quality.hip", "knn.hip",
+           "serve.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950",

@@ -20,14 +20,17 @@
 // per-query sums reduced in a fixed butterfly order (deterministic).
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <random>
 #include <vector>
 
 #include "../../include/fastembedr_hip.h"
 #include "fe_common.hpp"
+#include "serve_iface.hpp"
 #include "tsne_device.hpp"
 #include "tsne_affinity.hpp"
 
@@ -282,15 +285,29 @@ struct TransformReference {
   float2* layout = nullptr;
 };
 
+// A transform request with its arguments checked and its host-side inputs
+// built: the start layout when the host makes it (explicit or random), the
+// weighted start's row-major distances, the jump matrices.
+struct TransformTask {
+  TransformParams tp{};
+  int index_offset = 0;
+  double perplexity = 0.0;
+  int start = 0;  // 0 median, 1 weighted (both by the init kernel), 2 host start
+  bool lds_ref = false;
+  std::size_t dist_size = 0;
+  std::vector<float> host_start;
+  std::vector<float> dist_rm;
+  std::vector<uint32_t> jump;
+};
+
 template <typename DistT>
-static int transform_run(const TransformReference& R, const int* indices, const DistT* distances,
-                         int n_query, int k, int index_offset, const float* init,
-                         int initialization, double perplexity, int n_iter, int early_iter,
-                         double lr, double early_exag, double exag, double mom0, double mom1,
-                         double max_grad, double max_step, int n_negatives, int threshold,
-                         int seed, float* out, hipStream_t s) {
-  const int n_ref = R.n_ref;
-  if (!indices || !distances || !out) return fail("null transform input pointer");
+static int transform_prepare(int n_ref, const int* indices, const DistT* distances, int n_query,
+                             int k, int index_offset, const float* init, int initialization,
+                             double perplexity, int n_iter, int early_iter, double lr,
+                             double early_exag, double exag, double mom0, double mom1,
+                             double max_grad, double max_step, int n_negatives, int threshold,
+                             int seed, TransformTask* t) {
+  if (!indices || !distances) return fail("null transform input pointer");
   if (n_query < 1 || k < 1) return fail("KNN input must have at least one query row and one neighbor column.");
   if (k > kMaxK) return fail("HIP transform supports at most 256 neighbors.");
   if (!(perplexity > 0.0)) return fail("`perplexity` must be positive.");
@@ -308,79 +325,50 @@ static int transform_run(const TransformReference& R, const int* indices, const 
   if (n_negatives > n_ref) n_negatives = n_ref;
   const bool exact = n_ref <= threshold || n_negatives >= n_ref;
 
-  const size_t total = static_cast<size_t>(n_query) * k;
-  ArenaPlan plan;
-  plan.add<int>(total);          // ids upload
-  plan.add<DistT>(total);        // distances upload
-  plan.add<int>(total);          // nbr
-  plan.add<float>(total);        // p rows
-  plan.add<float>(total);        // float distances (weighted init)
-  plan.add<float2>(n_query);     // start
-  plan.add<float2>(n_query);     // result
-  plan.add<uint32_t>(kQL * 32);  // jump matrices
-  plan.add<unsigned>(4);
-  Arena a;
-  if (a.init(plan.bytes, "HIP transform workspace")) return 1;
-  int* d_idx = a.take<int>(total);
-  DistT* d_dist = a.take<DistT>(total);
-  int* nbr = a.take<int>(total);
-  float* prow = a.take<float>(total);
-  float* dist_f = a.take<float>(total);
-  float2* y0 = a.take<float2>(n_query);
-  float2* yo = a.take<float2>(n_query);
-  uint32_t* jump = a.take<uint32_t>(kQL * 32);
-  unsigned* flags = a.take<unsigned>(4);
-  FE_CHECK(hipMemcpyAsync(d_idx, indices, total * sizeof(int), hipMemcpyHostToDevice, s));
-  FE_CHECK(hipMemcpyAsync(d_dist, distances, total * sizeof(DistT), hipMemcpyHostToDevice, s));
-  FE_CHECK(hipMemsetAsync(flags, 0, 4 * sizeof(unsigned), s));
-  constexpr int rpb = aff_rows_per_block<DistT>();
-  const size_t lds = static_cast<size_t>(rpb) * k * (sizeof(DistT) + sizeof(int));
-  if (lds > 65536)
-    FE_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&perplexity_rows<DistT>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-  perplexity_rows<DistT><<<(n_query + rpb - 1) / rpb, 256, lds, s>>>(
-      d_idx, d_dist, n_query, n_ref, k, index_offset, perplexity, nbr, prow, flags);
-  FE_LAUNCH_CHECK("perplexity_rows(transform)");
-  unsigned h_flags = 0;
-  FE_CHECK(hipMemcpyAsync(&h_flags, flags, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-  FE_CHECK(hipStreamSynchronize(s));
-  if (h_flags & kErrIndexRange) return fail("KNN indices are out of range for `reference_layout`.");
-  if (h_flags & kErrDistance) return fail("KNN distances must be finite and non-negative.");
+  // the KNN checks of the P-row kernel, on the host before anything is
+  // enqueued (the kernel still flags them)
+  const std::size_t total = static_cast<std::size_t>(n_query) * k;
+  for (std::size_t e = 0; e < total; ++e) {
+    const long long nb = static_cast<long long>(indices[e]) - index_offset;
+    if (nb < 0 || nb >= n_ref) return fail("KNN indices are out of range for `reference_layout`.");
+  }
+  for (std::size_t e = 0; e < total; ++e) {
+    const double d = static_cast<double>(distances[e]);
+    if (!std::isfinite(d) || d < 0.0) return fail("KNN distances must be finite and non-negative.");
+  }
 
-  // start positions
+  t->index_offset = index_offset;
+  t->perplexity = perplexity;
+  t->dist_size = sizeof(DistT);
   if (init) {
-    for (size_t t = 0; t < 2 * static_cast<size_t>(n_query); ++t)
-      if (!std::isfinite(init[t])) return fail("`Y_init` must contain only finite coordinates.");
-    FE_CHECK(hipMemcpyAsync(y0, init, n_query * sizeof(float2), hipMemcpyHostToDevice, s));
+    for (std::size_t e = 0; e < 2 * static_cast<std::size_t>(n_query); ++e)
+      if (!std::isfinite(init[e])) return fail("`Y_init` must contain only finite coordinates.");
+    t->start = 2;
+    t->host_start.assign(init, init + 2 * static_cast<std::size_t>(n_query));
   } else if (initialization == 2) {
     // tsne_neighbors.cpp:1397-1414: per-query std::mt19937 and N(0, 1e-4)
-    std::vector<float> h(2 * static_cast<size_t>(n_query));
+    t->start = 2;
+    t->host_start.resize(2 * static_cast<std::size_t>(n_query));
     const uint32_t rs = static_cast<uint32_t>(seed);
     for (int i = 0; i < n_query; ++i) {
       std::mt19937 rng(mix_uint32(rs ^ (static_cast<uint32_t>(i + 1) * 0x9e3779b9u)));
       std::normal_distribution<double> normal(0.0, 1.0e-4);
-      h[2 * i] = static_cast<float>(normal(rng));
-      h[2 * i + 1] = static_cast<float>(normal(rng));
+      t->host_start[2 * i] = static_cast<float>(normal(rng));
+      t->host_start[2 * i + 1] = static_cast<float>(normal(rng));
     }
-    FE_CHECK(hipMemcpyAsync(y0, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, s));
-    FE_CHECK(hipStreamSynchronize(s));
   } else {
+    t->start = initialization;
     if (initialization == 1) {
-      std::vector<float> hd(total);
-      for (size_t t = 0; t < total; ++t) {
-        // column-major input -> row-major float copy for the init kernel
-        const size_t i = t / k, j = t % k;
-        hd[t] = static_cast<float>(distances[j * n_query + i]);
+      // column-major input -> row-major float copy for the init kernel
+      t->dist_rm.resize(total);
+      for (std::size_t e = 0; e < total; ++e) {
+        const std::size_t i = e / k, j = e % k;
+        t->dist_rm[e] = static_cast<float>(distances[j * n_query + i]);
       }
-      FE_CHECK(hipMemcpyAsync(dist_f, hd.data(), total * sizeof(float), hipMemcpyHostToDevice, s));
-      FE_CHECK(hipStreamSynchronize(s));
     }
-    transform_init_kernel<<<(n_query + kQPerBlock - 1) / kQPerBlock, 256, 0, s>>>(
-        R.layout, nbr, dist_f, n_query, k, initialization == 1 ? 1 : 0, y0);
-    FE_LAUNCH_CHECK("transform_init_kernel");
   }
 
-  TransformParams tp;
+  TransformParams& tp = t->tp;
   tp.n_ref = n_ref;
   tp.n_query = n_query;
   tp.k = k;
@@ -397,23 +385,134 @@ static int transform_run(const TransformReference& R, const int* indices, const 
   tp.max_grad = static_cast<float>(max_grad);
   tp.max_step = static_cast<float>(max_step);
   tp.neg_chunk = (n_negatives + kQL - 1) / kQL;
-  std::vector<uint32_t> hj(kQL * 32);
+  t->lds_ref = exact && n_ref <= kLdsRef;
+  t->jump.resize(kQL * 32);
   for (int l = 0; l < kQL; ++l) {
     const Gf2 m = gf2_pow_xorshift(static_cast<long long>(l) * tp.neg_chunk);
-    std::memcpy(hj.data() + 32 * l, m.c, sizeof(m.c));
+    std::memcpy(t->jump.data() + 32 * l, m.c, sizeof(m.c));
   }
-  FE_CHECK(hipMemcpyAsync(jump, hj.data(), hj.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-  const int blocks = (n_query + kQPerBlock - 1) / kQPerBlock;
-  if (exact && n_ref <= kLdsRef) {
-    transform_kernel<1><<<blocks, 256, n_ref * sizeof(float2), s>>>(R.layout, nbr, prow, y0, jump,
-                                                                    tp, yo);
+  return 0;
+}
+
+// Device workspace of a request, carved in this order by transform_enqueue.
+static std::size_t transform_device_bytes(const TransformTask& t) {
+  const std::size_t q = t.tp.n_query, total = q * t.tp.k;
+  ArenaPlan plan;
+  plan.add<int>(total);                              // ids upload
+  plan.add<unsigned char>(total * t.dist_size);      // distances upload
+  plan.add<int>(total);                              // nbr
+  plan.add<float>(total);                            // p rows
+  plan.add<float>(t.start == 1 ? total : 0);         // float distances (weighted start)
+  plan.add<float2>(q);                               // start
+  plan.add<float2>(q);                               // result
+  plan.add<uint32_t>(kQL * 32);                      // jump matrices
+  plan.add<unsigned>(4);
+  return plan.bytes;
+}
+
+// Host-side pointers of one enqueue: where the uploads come from and the
+// downloads go.
+struct TransformIO {
+  const int* idx;
+  const void* dist;
+  const float* start;    // host start (start == 2)
+  const float* dist_rm;  // weighted start
+  const uint32_t* jump;
+  float* out;
+  unsigned* flags;
+};
+
+// Enqueues the request on s without waiting: uploads, P rows, start, the
+// optimisation, and the downloads of the result and the error flags.
+template <typename DistT>
+static int transform_enqueue(const TransformReference& R, const TransformTask& t,
+                             const TransformIO& io, void* dev, hipStream_t s) {
+  const int n_query = t.tp.n_query, k = t.tp.k;
+  const std::size_t total = static_cast<std::size_t>(n_query) * k;
+  unsigned char* base = static_cast<unsigned char*>(dev);
+  std::size_t off = 0;
+  auto take = [&](std::size_t bytes) -> void* {
+    if (!bytes) return nullptr;
+    void* p = base + off;
+    off += align_up(bytes);
+    return p;
+  };
+  int* d_idx = static_cast<int*>(take(total * sizeof(int)));
+  DistT* d_dist = static_cast<DistT*>(take(total * sizeof(DistT)));
+  int* nbr = static_cast<int*>(take(total * sizeof(int)));
+  float* prow = static_cast<float*>(take(total * sizeof(float)));
+  float* dist_f = static_cast<float*>(take(t.start == 1 ? total * sizeof(float) : 0));
+  float2* y0 = static_cast<float2*>(take(n_query * sizeof(float2)));
+  float2* yo = static_cast<float2*>(take(n_query * sizeof(float2)));
+  uint32_t* jump = static_cast<uint32_t*>(take(kQL * 32 * sizeof(uint32_t)));
+  unsigned* flags = static_cast<unsigned*>(take(4 * sizeof(unsigned)));
+  FE_CHECK(hipMemcpyAsync(d_idx, io.idx, total * sizeof(int), hipMemcpyHostToDevice, s));
+  FE_CHECK(hipMemcpyAsync(d_dist, io.dist, total * sizeof(DistT), hipMemcpyHostToDevice, s));
+  FE_CHECK(hipMemsetAsync(flags, 0, 4 * sizeof(unsigned), s));
+  constexpr int rpb = aff_rows_per_block<DistT>();
+  const std::size_t lds = static_cast<std::size_t>(rpb) * k * (sizeof(DistT) + sizeof(int));
+  if (lds > 65536)
+    FE_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&perplexity_rows<DistT>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+  perplexity_rows<DistT><<<(n_query + rpb - 1) / rpb, 256, lds, s>>>(
+      d_idx, d_dist, n_query, R.n_ref, k, t.index_offset, t.perplexity, nbr, prow, flags);
+  FE_LAUNCH_CHECK("perplexity_rows(transform)");
+  if (t.start == 2) {
+    FE_CHECK(hipMemcpyAsync(y0, io.start, n_query * sizeof(float2), hipMemcpyHostToDevice, s));
   } else {
-    transform_kernel<0><<<blocks, 256, 0, s>>>(R.layout, nbr, prow, y0, jump, tp, yo);
+    if (t.start == 1)
+      FE_CHECK(hipMemcpyAsync(dist_f, io.dist_rm, total * sizeof(float), hipMemcpyHostToDevice, s));
+    transform_init_kernel<<<(n_query + kQPerBlock - 1) / kQPerBlock, 256, 0, s>>>(
+        R.layout, nbr, dist_f, n_query, k, t.start == 1 ? 1 : 0, y0);
+    FE_LAUNCH_CHECK("transform_init_kernel");
+  }
+  FE_CHECK(hipMemcpyAsync(jump, io.jump, kQL * 32 * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  const int blocks = (n_query + kQPerBlock - 1) / kQPerBlock;
+  if (t.lds_ref) {
+    transform_kernel<1><<<blocks, 256, R.n_ref * sizeof(float2), s>>>(R.layout, nbr, prow, y0,
+                                                                      jump, t.tp, yo);
+  } else {
+    transform_kernel<0><<<blocks, 256, 0, s>>>(R.layout, nbr, prow, y0, jump, t.tp, yo);
   }
   FE_LAUNCH_CHECK("transform_kernel");
-  FE_CHECK(hipMemcpyAsync(out, yo, n_query * sizeof(float2), hipMemcpyDeviceToHost, s));
-  FE_CHECK(hipStreamSynchronize(s));
+  FE_CHECK(hipMemcpyAsync(io.out, yo, n_query * sizeof(float2), hipMemcpyDeviceToHost, s));
+  FE_CHECK(hipMemcpyAsync(io.flags, flags, sizeof(unsigned), hipMemcpyDeviceToHost, s));
   return 0;
+}
+
+static int transform_flags_error(unsigned flags) {
+  if (flags & kErrIndexRange) return fail("KNN indices are out of range for `reference_layout`.");
+  if (flags & kErrDistance) return fail("KNN distances must be finite and non-negative.");
+  return 0;
+}
+
+// One request, synchronously: its own workspace, enqueue, wait.
+template <typename DistT>
+static int transform_run(const TransformReference& R, const int* indices, const DistT* distances,
+                         int n_query, int k, int index_offset, const float* init,
+                         int initialization, double perplexity, int n_iter, int early_iter,
+                         double lr, double early_exag, double exag, double mom0, double mom1,
+                         double max_grad, double max_step, int n_negatives, int threshold,
+                         int seed, float* out, hipStream_t s) {
+  if (!out) return fail("null transform input pointer");
+  TransformTask t;
+  if (transform_prepare<DistT>(R.n_ref, indices, distances, n_query, k, index_offset, init,
+                               initialization, perplexity, n_iter, early_iter, lr, early_exag,
+                               exag, mom0, mom1, max_grad, max_step, n_negatives, threshold, seed,
+                               &t))
+    return 1;
+  Arena a;
+  if (a.init(transform_device_bytes(t), "HIP transform workspace")) return 1;
+  unsigned h_flags = 0;
+  const TransformIO io{indices, distances, t.host_start.data(), t.dist_rm.data(), t.jump.data(),
+                       out, &h_flags};
+  void* dev = a.take<unsigned char>(a.capacity());
+  if (transform_enqueue<DistT>(R, t, io, dev, s)) {
+    (void)hipStreamSynchronize(s);
+    return 1;
+  }
+  FE_CHECK(hipStreamSynchronize(s));
+  return transform_flags_error(h_flags);
 }
 
 static int make_reference(const float* layout, int n_ref, TransformReference* R, hipStream_t s) {
@@ -430,12 +529,101 @@ static int make_reference(const float* layout, int n_ref, TransformReference* R,
   return check(hipStreamSynchronize(s), "reference upload");
 }
 
+// ---- the serving session's interface (serve_iface.hpp) ----
+
+int transform_task_create(const TransformReference& R, const int* indices, const float* distances,
+                          int n_query, int k, int index_offset, const float* init,
+                          int initialization, double perplexity, int n_iter,
+                          int early_exaggeration_iter, double learning_rate,
+                          double early_exaggeration, double exaggeration,
+                          double initial_momentum, double final_momentum, double max_grad_norm,
+                          double max_step_norm, int n_negatives, int exact_repulsion_threshold,
+                          int seed, TransformTaskPtr* out) {
+  TransformTaskPtr t(new TransformTask);
+  if (transform_prepare<float>(R.n_ref, indices, distances, n_query, k, index_offset, init,
+                               initialization, perplexity, n_iter, early_exaggeration_iter,
+                               learning_rate, early_exaggeration, exaggeration, initial_momentum,
+                               final_momentum, max_grad_norm, max_step_norm, n_negatives,
+                               exact_repulsion_threshold, seed, t.get()))
+    return 1;
+  *out = std::move(t);
+  return 0;
+}
+
+void transform_task_destroy(TransformTask* t) { delete t; }
+
+std::size_t transform_task_device_bytes(const TransformTask& t) { return transform_device_bytes(t); }
+
+int transform_task_queries(const TransformTask& t) { return t.tp.n_query; }
+
+// pinned staging, in this order: ids, distances, host start, weighted-start
+// distances, jump matrices, result, flags
+namespace {
+struct PinnedLayout {
+  std::size_t idx, dist, start, dist_rm, jump, out, flags, bytes;
+};
+PinnedLayout pinned_layout(const TransformTask& t) {
+  const std::size_t q = t.tp.n_query, total = q * t.tp.k;
+  PinnedLayout L{};
+  std::size_t off = 0;
+  auto put = [&](std::size_t bytes) {
+    const std::size_t at = off;
+    off += align_up(bytes);
+    return at;
+  };
+  L.idx = put(total * sizeof(int));
+  L.dist = put(total * sizeof(float));
+  L.start = put(t.host_start.size() * sizeof(float));
+  L.dist_rm = put(t.dist_rm.size() * sizeof(float));
+  L.jump = put(t.jump.size() * sizeof(uint32_t));
+  L.out = put(q * sizeof(float2));
+  L.flags = put(4 * sizeof(unsigned));
+  L.bytes = off;
+  return L;
+}
+}  // namespace
+
+std::size_t transform_task_pinned_bytes(const TransformTask& t) { return pinned_layout(t).bytes; }
+
+int transform_task_enqueue(const TransformReference& R, const TransformTask& t, const int* indices,
+                           const float* distances, void* dev, void* pinned, hipStream_t s) {
+  const PinnedLayout L = pinned_layout(t);
+  unsigned char* h = static_cast<unsigned char*>(pinned);
+  const std::size_t total = static_cast<std::size_t>(t.tp.n_query) * t.tp.k;
+  std::memcpy(h + L.idx, indices, total * sizeof(int));
+  std::memcpy(h + L.dist, distances, total * sizeof(float));
+  if (!t.host_start.empty())
+    std::memcpy(h + L.start, t.host_start.data(), t.host_start.size() * sizeof(float));
+  if (!t.dist_rm.empty())
+    std::memcpy(h + L.dist_rm, t.dist_rm.data(), t.dist_rm.size() * sizeof(float));
+  std::memcpy(h + L.jump, t.jump.data(), t.jump.size() * sizeof(uint32_t));
+  const TransformIO io{reinterpret_cast<const int*>(h + L.idx), h + L.dist,
+                       reinterpret_cast<const float*>(h + L.start),
+                       reinterpret_cast<const float*>(h + L.dist_rm),
+                       reinterpret_cast<const uint32_t*>(h + L.jump),
+                       reinterpret_cast<float*>(h + L.out), reinterpret_cast<unsigned*>(h + L.flags)};
+  return transform_enqueue<float>(R, t, io, dev, s);
+}
+
+int transform_task_result(const TransformTask& t, const void* pinned, const float** layout) {
+  const PinnedLayout L = pinned_layout(t);
+  const unsigned char* h = static_cast<const unsigned char*>(pinned);
+  if (transform_flags_error(*reinterpret_cast<const unsigned*>(h + L.flags))) return 1;
+  *layout = reinterpret_cast<const float*>(h + L.out);
+  return 0;
+}
+
 }  // namespace fe
 
 struct fastembedr_hip_transform_reference {
   fe::TransformReference impl;
   fe::Stream stream;
 };
+
+const fe::TransformReference* fe::transform_reference_of(
+    const fastembedr_hip_transform_reference* ref) {
+  return ref ? &ref->impl : nullptr;
+}
 
 extern "C" {
 

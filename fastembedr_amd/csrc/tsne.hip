@@ -18,6 +18,7 @@
 
 #include "../../include/fastembedr_hip.h"
 #include "fe_common.hpp"
+#include "serve_iface.hpp"
 #include "tsne_device.hpp"
 #include "tsne_affinity.hpp"
 #include "tsne_batch.hpp"
@@ -1199,6 +1200,18 @@ static int validate_tsne(const void* idx, const void* dist, int n, int k, int n_
   return 0;
 }
 
+int tsne_job_validate(const fastembedr_hip_tsne_job& j) {
+  if (j.grid_size < 0) return fail("batched jobs run the FFT-grid route (grid_size >= 0)");
+  if (j.grid_size > 0 && (j.grid_size < 16 || j.grid_size > 1024 || (j.grid_size & (j.grid_size - 1))))
+    return fail("grid_size must be a power of two between 16 and 1024");
+  if (validate_tsne(j.indices, j.distances, j.n, j.k, 2, j.perplexity, j.early_exaggeration_iter,
+                    j.n_iter, j.early_exaggeration, j.exaggeration, j.learning_rate_early, 0,
+                    j.initial_momentum, j.final_momentum, j.min_gain, j.index_offset))
+    return 1;
+  if (!(j.learning_rate_normal > 0.0f)) return fail("HIP openTSNE FFT-grid learning rate must be positive.");
+  return 0;
+}
+
 template <typename DistT>
 static int tsne_one_shot(const int* indices, const DistT* distances, bool device_inputs,
                          const float* init, int has_init, int n, int k, int n_components,
@@ -1334,11 +1347,7 @@ class TsneBatch {
       set_error("HBM budget does not admit this job yet");
       return 2;
     }
-    if (validate_tsne(j.indices, j.distances, j.n, j.k, 2, j.perplexity, j.early_exaggeration_iter,
-                      j.n_iter, j.early_exaggeration, j.exaggeration, j.learning_rate_early, 0,
-                      j.initial_momentum, j.final_momentum, j.min_gain, j.index_offset))
-      return 1;
-    if (!(j.learning_rate_normal > 0.0f)) return fail("HIP openTSNE FFT-grid learning rate must be positive.");
+    if (tsne_job_validate(j)) return 1;
     auto plan = std::make_unique<TsnePlan>();
     if (plan->create<float>(j.indices, j.distances, device_inputs, j.n, j.k, j.index_offset,
                             j.perplexity, j.grid_size, j.init, j.seed, true))
@@ -1940,6 +1949,10 @@ void fastembedr_hip_tsne_plan_destroy(fastembedr_hip_tsne_plan* plan) {
 struct fastembedr_hip_tsne_batch {
   fe::TsneBatch impl;
 };
+
+hipStream_t fe::tsne_batch_stream(const fastembedr_hip_tsne_batch* batch) {
+  return batch ? batch->impl.stream.get() : nullptr;
+}
 
 extern "C" {
 

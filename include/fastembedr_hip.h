@@ -236,6 +236,62 @@ int fastembedr_hip_tsne_batch_remove(fastembedr_hip_tsne_batch* batch, int slot)
 
 void fastembedr_hip_tsne_batch_destroy(fastembedr_hip_tsne_batch* batch);
 
+/* ---- serving session -------------------------------------------------------
+ * No reference counterpart (the reference runs one job per call and frees
+ * everything before it returns, embedding_cuda_kernels.cpp:88-127).  A
+ * session keeps a t-SNE job batch, a fixed reference layout and the
+ * transforms' workspace pools resident on the current device and serves
+ * requests continuously: embed jobs join the batch when the HBM budget and
+ * a slot allow and advance in 25-iteration ticks with every other resident
+ * job; transforms run on their own streams beside the ticks, with device
+ * workspace and pinned staging taken from pools carved out of the budget.
+ * One engine thread inside the session makes every device call; submit and
+ * poll may be called from any thread.  Inputs are copied at submit. */
+typedef struct fastembedr_hip_session fastembedr_hip_session;
+
+/* reference_layout: row-major n_reference x 2 (NULL / 0: no transforms).
+ * hbm_budget_bytes <= 0: no budget.  transform_pool_bytes <= 0: one eighth
+ * of the budget (1 GiB without one); the batch gets the rest of the budget.
+ * The pool is allocated on the device and as pinned host memory. */
+int fastembedr_hip_session_create(const float* reference_layout, int n_reference, int max_jobs,
+                                  long long hbm_budget_bytes, long long transform_pool_bytes,
+                                  fastembedr_hip_session** session);
+
+/* Queues a t-SNE job (host inputs, fastembedr_hip_tsne_batch_add's rules).
+ * Returns 1 with the message when the job is invalid; *request receives its
+ * id. */
+int fastembedr_hip_session_submit_embed(fastembedr_hip_session* session,
+                                        const fastembedr_hip_tsne_job* job, long long* request);
+
+/* Queues a transform against the session's reference: the arguments of
+ * fastembedr_hip_transform_reference_run.  Returns 1 with the message when
+ * the request is invalid or larger than the transform pool. */
+int fastembedr_hip_session_submit_transform(
+    fastembedr_hip_session* session, const int* indices, const float* distances, int n_query,
+    int k, int index_offset, const float* init, int initialization, double perplexity, int n_iter,
+    int early_exaggeration_iter, double learning_rate, double early_exaggeration,
+    double exaggeration, double initial_momentum, double final_momentum, double max_grad_norm,
+    double max_step_norm, int n_negatives, int exact_repulsion_threshold, int seed,
+    long long* request);
+
+/* *state: 0 queued, 1 running, 2 done, 3 failed.  wait_ms > 0 waits up to
+ * that long for the request to end.  Done: the layout (row-major, the job's
+ * n or the query count x 2) is copied to out (may be NULL) and the request
+ * is released; failed: the request's message becomes the last error and the
+ * request is released.  latency_ms: submit to end (-1 while it runs).
+ * Returns 1 for an unknown or already released id. */
+int fastembedr_hip_session_poll(fastembedr_hip_session* session, long long request, int wait_ms,
+                                int* state, float* out, double* latency_ms);
+
+/* stats[0..9]: submitted, completed, failed, queued embeds, queued
+ * transforms, running embeds, running transforms, transform pool bytes in
+ * use, engine ticks, transform pool bytes. */
+int fastembedr_hip_session_stats(fastembedr_hip_session* session, long long* stats);
+
+/* Queued requests fail, work on the device is waited for, everything is
+ * freed. */
+void fastembedr_hip_session_destroy(fastembedr_hip_session* session);
+
 /* ---- UMAP --------------------------------------------------------------- */
 
 /* Replaces fastembedr_cuda_umap_optimize_coo (embedding_cuda_kernels.cpp:
