@@ -875,14 +875,18 @@ static int launch_conv_any(TsnePlan& p, hipStream_t s) {
 }
 
 // attraction into diag[0, n), then the per-point gather/update (MODE 0) or
-// the repulsive force into diag[n, 2n) (MODE 1)
-static int launch_attract(TsnePlan& p, float exaggeration, hipStream_t s) {
+// the repulsive force into diag[n, 2n) (MODE 1).  The pair form leaves its
+// sums in partial slots; with to_array = false they stay there for the
+// optimiser update (launch_update<0>), which reads them itself.
+static int launch_attract(TsnePlan& p, float exaggeration, hipStream_t s, bool to_array = true) {
   const float2* yin = p.y[p.cur];
   if (p.attr_kind == kAttrPairs) {
     attract_pairs<<<p.pair_ranges, kPairThreads, kPairLds, s>>>(yin, p.pairs, p.st, p.n);
     FE_LAUNCH_CHECK("attract_pairs");
-    pairs_reduce<<<p.stat_blocks, kThreads, 0, s>>>(p.pairs, p.st, exaggeration, p.diag, p.n);
-    FE_LAUNCH_CHECK("pairs_reduce");
+    if (to_array) {
+      pairs_reduce<<<p.stat_blocks, kThreads, 0, s>>>(p.pairs, p.st, exaggeration, p.diag, p.n);
+      FE_LAUNCH_CHECK("pairs_reduce");
+    }
     return 0;
   }
   if (p.attr_kind == kAttrStaged) {
@@ -915,10 +919,17 @@ static int launch_attract(TsnePlan& p, float exaggeration, hipStream_t s) {
 
 template <int MODE>
 static int launch_update(TsnePlan& p, const StepParams& sp, hipStream_t s) {
-  update_kernel<MODE><<<p.stat_blocks, kThreads, 0, s>>>(p.y[p.cur], p.y[p.cur ^ 1], p.gains,
-                                                         p.upd, p.diag, p.field, p.st, p.lpart,
-                                                         p.n, p.g, sp, p.diag + p.n);
-  FE_LAUNCH_CHECK("update_kernel");
+  if (MODE == 0 && p.attr_kind == kAttrPairs) {
+    update_pairs_kernel<<<p.stat_blocks, kThreads, 0, s>>>(p.y[p.cur], p.y[p.cur ^ 1], p.gains,
+                                                           p.upd, p.pairs, p.field, p.st, p.lpart,
+                                                           p.n, p.g, sp);
+    FE_LAUNCH_CHECK("update_pairs_kernel");
+  } else {
+    update_kernel<MODE><<<p.stat_blocks, kThreads, 0, s>>>(p.y[p.cur], p.y[p.cur ^ 1], p.gains,
+                                                           p.upd, p.diag, p.field, p.st, p.lpart,
+                                                           p.n, p.g, sp, p.diag + p.n);
+    FE_LAUNCH_CHECK("update_kernel");
+  }
   if constexpr (MODE == 0) {
     layout_finalize<<<1, kThreads, 0, s>>>(p.lpart, p.stat_blocks, p.st, p.n, p.g, 1);
     FE_LAUNCH_CHECK("layout_finalize");
@@ -1000,7 +1011,7 @@ int TsnePlan::enqueue_iteration(const StepParams& sp) {
     hipEvent_t fork = forks.next(), join = forks.next();
     FE_CHECK(hipEventRecord(fork, s));
     FE_CHECK(hipStreamWaitEvent(a, fork, 0));
-    if (launch_attract(*this, sp.exaggeration, a)) return 1;
+    if (launch_attract(*this, sp.exaggeration, a, false)) return 1;
     FE_CHECK(hipEventRecord(join, a));
     if (launch_splat(*this, s)) return 1;
     if (launch_conv_any(*this, s)) return 1;
@@ -1008,7 +1019,7 @@ int TsnePlan::enqueue_iteration(const StepParams& sp) {
   } else {
     if (launch_splat(*this, s)) return 1;
     if (launch_conv_any(*this, s)) return 1;
-    if (launch_attract(*this, sp.exaggeration, s)) return 1;
+    if (launch_attract(*this, sp.exaggeration, s, false)) return 1;
   }
   if (launch_update<0>(*this, sp, s)) return 1;
   cur ^= 1;
@@ -1147,7 +1158,7 @@ int TsnePlan::profile(int iters, const StepParams& sp, double* ms) {
 #undef FE_PROF_CONV
     if (rc) return fail("unsupported FFT size");
     FE_CHECK(hipEventRecord(e[4], s));
-    if (launch_attract(*this, sp.exaggeration, s)) return 1;
+    if (launch_attract(*this, sp.exaggeration, s, false)) return 1;
     FE_CHECK(hipEventRecord(e[5], s));
     if (launch_update<0>(*this, sp, s)) return 1;
     FE_CHECK(hipEventRecord(e[6], s));
@@ -1299,8 +1310,7 @@ class TsneBatch {
   bool dirty = true;
   struct Launch {
     int kind;   // 0 splat_count, 11 splat_colscan, 6 splat_scatter, 7 splat_tiles,
-                // 1 rows_fwd, 2 cols, 3 rows_inv, 4 attract, 10 pair reduce,
-                // 5 update, 9 finalize
+                // 1 rows_fwd, 2 cols, 3 rows_inv, 4 attract, 5 update, 9 finalize
     int param;  // F or TPR
     int off;    // offset of slot[] in d_tab; first[] follows it
     int m;      // entries
@@ -1383,6 +1393,7 @@ class TsneBatch {
     d.lpart = p.lpart;
     d.zpart = p.zpart;
     d.attr = p.diag;
+    d.pair_attr = p.attr_kind == kAttrPairs ? 1 : 0;
     d.n = p.n;
     d.g = p.g;
     d.upd_blocks = p.stat_blocks;
@@ -1478,14 +1489,10 @@ class TsneBatch {
         if (slots[s] && slots[s]->attr_kind == kAttrLanes) ent.push_back({s, slots[s]->groups});
       add_launch(4, kLanesLaunch, ent);
     }
-    std::vector<std::pair<int, int>> pair_reduce;
     {
       std::vector<std::pair<int, int>> ent;
       for (int s = 0; s < max_jobs; ++s)
-        if (slots[s] && slots[s]->attr_kind == kAttrPairs) {
-          ent.push_back({s, slots[s]->pair_ranges});
-          pair_reduce.push_back({s, slots[s]->stat_blocks});
-        }
+        if (slots[s] && slots[s]->attr_kind == kAttrPairs) ent.push_back({s, slots[s]->pair_ranges});
       add_launch(4, kPairsLaunch, ent);
     }
     for (int tpr = 4; tpr <= 64; tpr *= 2) {
@@ -1495,7 +1502,6 @@ class TsneBatch {
           ent.push_back({s, slots[s]->upd_blocks});
       add_launch(4, tpr, ent);
     }
-    add_launch(10, 0, pair_reduce);
     std::vector<std::pair<int, int>> upd;
     for (int s = 0; s < max_jobs; ++s)
       if (slots[s]) upd.push_back({s, slots[s]->stat_blocks});
@@ -1575,9 +1581,6 @@ class TsneBatch {
           break;
         case 5:
           batch_update<<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
-          break;
-        case 10:
-          batch_pairs_reduce<<<L.blocks, kThreads, 0, s>>>(d_desc, sl, fi, L.m);
           break;
         default:
           return fail("unknown batch stage");

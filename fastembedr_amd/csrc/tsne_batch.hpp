@@ -42,6 +42,7 @@ struct JobDesc {
   LayoutPartial* lpart;
   double* zpart;
   float2* attr;
+  int pair_attr;  // the update reads the attraction from the pair tables' slots
   int n, g;
   int upd_blocks;  // blocks of the job's update launch (layout partials)
   int cur, iter, total_iter, early_iter, active;
@@ -150,8 +151,8 @@ __global__ __launch_bounds__(kStagedThreads) void batch_attract_blocked(JobDesc*
                        sy);
 }
 
-// pair-once attraction: one workgroup per chunk range of a job, then the reduce of
-// its partial slots (integer sums: the block order within a job is free)
+// pair-once attraction: one workgroup per chunk range of a job; the update
+// reads its partial slots (integer sums: the block order within a job is free)
 __global__ __launch_bounds__(kPairThreads) void batch_attract_pairs(JobDesc* jobs,
                                                                     const int* __restrict__ slot,
                                                                     const int* __restrict__ first,
@@ -162,18 +163,6 @@ __global__ __launch_bounds__(kPairThreads) void batch_attract_pairs(JobDesc* job
   if (!J->active) return;
   const int w = xcd_remap(static_cast<int>(blockIdx.x) - first[e], first[e + 1] - first[e]);
   attract_pairs_range(J->y[J->cur], J->st, J->pairs, J->n, w, lds);
-}
-
-__global__ __launch_bounds__(kThreads) void batch_pairs_reduce(JobDesc* jobs,
-                                                               const int* __restrict__ slot,
-                                                               const int* __restrict__ first,
-                                                               int m) {
-  const int e = find_entry(first, m);
-  JobDesc* J = jobs + slot[e];
-  if (!J->active) return;
-  const float exag = J->iter < J->early_iter ? J->early_exag : J->exag;
-  pairs_reduce_body(J->pairs, J->st, exag, J->attr, J->n, blockIdx.x - first[e],
-                    first[e + 1] - first[e]);
 }
 
 template <int TPR>
@@ -196,8 +185,14 @@ __global__ __launch_bounds__(kThreads) void batch_update(JobDesc* jobs, const in
   const bool early = it < J->early_iter;
   const StepParams sp{early ? J->early_exag : J->exag, early ? J->lr_early : J->lr_normal,
                       early ? J->mom0 : J->mom1, J->min_gain, J->max_step};
-  update_body<0>(J->y[cur], J->y[cur ^ 1], J->gains, J->upd, J->attr, J->field, J->st, J->lpart,
-                 J->n, J->g, sp, nullptr, blockIdx.x - first[e], first[e + 1] - first[e]);
+  const int blk = static_cast<int>(blockIdx.x) - first[e], nblk = first[e + 1] - first[e];
+  if (J->pair_attr)
+    update_body<0>(J->y[cur], J->y[cur ^ 1], J->gains, J->upd,
+                   attr_pairs(J->pairs, J->st, sp.exaggeration), J->field, J->st, J->lpart, J->n,
+                   J->g, sp, nullptr, blk, nblk);
+  else
+    update_body<0>(J->y[cur], J->y[cur ^ 1], J->gains, J->upd, AttrArray{J->attr}, J->field,
+                   J->st, J->lpart, J->n, J->g, sp, nullptr, blk, nblk);
 }
 
 // one block per job: the next iteration's geometry from the update's layout

@@ -1775,12 +1775,10 @@ __global__ __launch_bounds__(kThreads) void blk_pass_ranges(const int* __restric
 // force: gains, momentum, step clip (apply_open_tsne_update_f,
 // tsne_neighbors.cpp:1297-1336), the new position into y_out and into this
 // thread's layout statistics.
-__device__ __forceinline__ void opt_step(int i, float xi, float yi, float rx, float ry,
-                                         const float2* __restrict__ attr,
+__device__ __forceinline__ void opt_step(int i, float xi, float yi, float rx, float ry, float2 a,
                                          float2* __restrict__ gains, float2* __restrict__ upd,
                                          float2* __restrict__ y_out, const StepParams& sp,
                                          LayoutPartial& m) {
-  const float2 a = attr[i];
   const float gx = rx + a.x, gy = ry + a.y;
   float2 gn = gains[i];
   const float2 old = upd[i];
@@ -1810,11 +1808,18 @@ __device__ __forceinline__ void opt_step(int i, float xi, float yi, float rx, fl
   m.mxy = fmaxf(m.mxy, ny);
 }
 
-template <int MODE>
+// The attraction of point i as the update reads it: from the array the
+// attraction kernel wrote.
+struct AttrArray {
+  const float2* __restrict__ a;
+  __device__ __forceinline__ float2 operator()(int i) const { return a[i]; }
+};
+
+template <int MODE, class AttrAt>
 __device__ __forceinline__ void update_body(const float2* __restrict__ y_in,
                                             float2* __restrict__ y_out,
                                             float2* __restrict__ gains, float2* __restrict__ upd,
-                                            const float2* __restrict__ attr,
+                                            AttrAt attr,
                                             const float4* __restrict__ field, TsneStats* st,
                                             LayoutPartial* partials, int n, int g, StepParams sp,
                                             float2* __restrict__ diag_rep, int blk, int nblk) {
@@ -1843,7 +1848,7 @@ __device__ __forceinline__ void update_body(const float2* __restrict__ y_in,
     if constexpr (MODE == 1) {
       diag_rep[i] = make_float2(rx, ry);
     } else {
-      opt_step(i, xi, yi, rx, ry, attr, gains, upd, y_out, sp, m);
+      opt_step(i, xi, yi, rx, ry, attr(i), gains, upd, y_out, sp, m);
     }
   }
   if constexpr (MODE == 0) {
@@ -1858,8 +1863,8 @@ __global__ __launch_bounds__(kThreads) void update_kernel(
     float2* __restrict__ upd, const float2* __restrict__ attr, const float4* __restrict__ field,
     TsneStats* st, LayoutPartial* partials, int n, int g, StepParams sp,
     float2* __restrict__ diag_rep) {
-  update_body<MODE>(y_in, y_out, gains, upd, attr, field, st, partials, n, g, sp, diag_rep,
-                    blockIdx.x, gridDim.x);
+  update_body<MODE>(y_in, y_out, gains, upd, AttrArray{attr}, field, st, partials, n, g, sp,
+                    diag_rep, blockIdx.x, gridDim.x);
 }
 
 // Write the centred layout (final output): out_i = fl(y_i - mean).
@@ -1985,7 +1990,7 @@ __global__ __launch_bounds__(kThreads) void update_exact_kernel(
       diag_rep[i] = make_float2(rx, ry);
     } else {
       const float2 pi = y_in[i];
-      opt_step(i, pi.x - mxf, pi.y - myf, rx, ry, attr, gains, upd, y_out, sp, m);
+      opt_step(i, pi.x - mxf, pi.y - myf, rx, ry, attr[i], gains, upd, y_out, sp, m);
     }
   }
   if constexpr (MODE == 0) {

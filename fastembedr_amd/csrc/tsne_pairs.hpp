@@ -324,6 +324,49 @@ __device__ __forceinline__ void pairs_reduce_body(const PairTables& t,
   }
 }
 
+// The update reading the attraction straight from the partial slots (the
+// same sum and rounding as pairs_reduce_body, so the same bits), which saves
+// the reduce launch and the force array's write and read.
+struct AttrPairs {
+  PairTables t;
+  double inv_scale;
+  float exaggeration;
+  __device__ __forceinline__ float2 operator()(int p) const {
+    const int b = p / kPairBlock, l = p - b * kPairBlock;
+    const int s0 = t.slot_ptr[b], s1 = t.slot_ptr[b + 1];
+    // four slots' loads in flight at a time (integer sums: any order)
+    unsigned long long acc = 0ull;
+    int s = s0;
+    for (; s + 4 <= s1; s += 4) {
+      const int a0 = t.slot_list[s], a1 = t.slot_list[s + 1];
+      const int a2 = t.slot_list[s + 2], a3 = t.slot_list[s + 3];
+      const unsigned long long v0 = t.part[static_cast<size_t>(a0) * kPairBlock + l];
+      const unsigned long long v1 = t.part[static_cast<size_t>(a1) * kPairBlock + l];
+      const unsigned long long v2 = t.part[static_cast<size_t>(a2) * kPairBlock + l];
+      const unsigned long long v3 = t.part[static_cast<size_t>(a3) * kPairBlock + l];
+      acc += (v0 + v1) + (v2 + v3);
+    }
+    for (; s < s1; ++s) acc += t.part[static_cast<size_t>(t.slot_list[s]) * kPairBlock + l];
+    long long sx, sy;
+    unpack_fixed(acc, &sx, &sy);
+    return make_float2(exaggeration * static_cast<float>(static_cast<double>(sx) * inv_scale),
+                       exaggeration * static_cast<float>(static_cast<double>(sy) * inv_scale));
+  }
+};
+
+__device__ __forceinline__ AttrPairs attr_pairs(const PairTables& t, const TsneStats* st,
+                                                float exaggeration) {
+  return AttrPairs{t, ldexp(1.0, -pair_scale_exp(t, st)), exaggeration};
+}
+
+__global__ __launch_bounds__(kThreads) void update_pairs_kernel(
+    const float2* __restrict__ y_in, float2* __restrict__ y_out, float2* __restrict__ gains,
+    float2* __restrict__ upd, PairTables t, const float4* __restrict__ field, TsneStats* st,
+    LayoutPartial* partials, int n, int g, StepParams sp) {
+  update_body<0>(y_in, y_out, gains, upd, attr_pairs(t, st, sp.exaggeration), field, st,
+                 partials, n, g, sp, nullptr, blockIdx.x, gridDim.x);
+}
+
 __global__ __launch_bounds__(kThreads) void pairs_reduce(PairTables t,
                                                          const TsneStats* __restrict__ st,
                                                          float exaggeration,

@@ -218,7 +218,7 @@ def test_config5_served_results_equal_alone():
             qi, qd = workloads.knn_to_reference(xq, xr, 90)
             payload = dict(indices=qi, distances=qd)
         reqs.append(Request(rid, kind, 0.0, payload))
-    srv = Server(reference_layout=ref_y, max_jobs=8, tick=25)
+    srv = Server(reference_layout=ref_y, max_jobs=8)
     stats = srv.run(reqs, realtime=False)
     assert stats.completed == len(reqs)
     mixed = {r.rid: r.result for r in reqs}
