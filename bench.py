@@ -87,11 +87,14 @@ def log(msg):
 SCHEDULE = dict(early=250, normal=500, early_exaggeration=12.0, momentum=0.8)
 
 
-def run_plan_alone(fe, ki, kd, n, k, y0):
+def run_plan_alone(fe, ki, kd, n, k, y0, shape=None):
     """One job through the plain single-job path: P build, 250 + 500
-    iterations, layout read back."""
-    plan = fe.TsnePlan(ki, kd, perplexity=30.0, grid_size=0, Y_init=y0, device=True, n=n, k=k,
-                       index_offset=0)
+    iterations, layout read back (with an autotuned shape when given)."""
+    shape = shape or {}
+    plan = fe.TsnePlan(ki, kd, perplexity=30.0, grid_size=shape.get("grid_size", 0), Y_init=y0,
+                       device=True, n=n, k=k, index_offset=0,
+                       attraction=shape.get("attraction", 0),
+                       pair_workgroups=shape.get("pair_workgroups", 0))
     lr = n / SCHEDULE["early_exaggeration"]
     plan.run(SCHEDULE["early"], SCHEDULE["early_exaggeration"], SCHEDULE["momentum"], lr)
     plan.run(SCHEDULE["normal"], 1.0, SCHEDULE["momentum"], lr)
@@ -142,6 +145,7 @@ def bench_batched(fe, workloads, count, k=90):
     same = sum(int(np.array_equal(a, o)) for a, o in zip(alone, outs))
     log(f"[batched] batched: {t_batch:.2f}s (admission {t_admit:.2f}s); "
         f"{same}/{count} layouts bit-identical to the plain path")
+    tuned = bench_batched_tuned(fe, workloads, specs, jobs, alone, t_seq, k)
     return {"workload": f"{count} t-SNE jobs, n log-uniform in [2k, 100k] ({total_points} points), "
                         f"5-50-cluster mixtures in 30-50 dims, k={k}, perplexity 30, 750 iterations",
             "jobs": count, "points": total_points,
@@ -149,7 +153,72 @@ def bench_batched(fe, workloads, count, k=90):
             "sequential_jobs_per_s": round(count / t_seq, 3),
             "speedup_vs_sequential": round(t_seq / t_batch, 3),
             "batched_s": round(t_batch, 3), "sequential_s": round(t_seq, 3),
-            "identical_to_plain": same}
+            "identical_to_plain": same, "autotuned": tuned}
+
+
+def bench_batched_tuned(fe, workloads, specs, jobs, alone, t_seq, k, check_jobs=16):
+    """The same jobs with the autotuner's shape per size bucket (FFT grid kept
+    at the smallest size that holds the grid rule's layout quality, attraction
+    form and workgroups by timing): tuning time, batched jobs/s against the
+    untuned sequential plain path, tuned jobs alone = tuned jobs batched on a
+    sample, and the layout quality of tuned against plain jobs on that
+    sample (trustworthiness and kNN preservation, k = 15, exact ranks on 2000
+    rows per job)."""
+    from fastembedr_amd.batch import TsneBatch
+    from fastembedr_amd.quality import exact_structure_metrics
+    from fastembedr_amd.tune import Autotuner
+
+    count = len(jobs)
+    tuner = Autotuner(k=k, perplexity=30.0)
+    t0 = time.perf_counter()
+    tuner.tune(sizes=[n for _, _, n, _ in jobs])
+    t_tune = time.perf_counter() - t0
+    log(f"[batched] autotuned {len(tuner.table)} buckets in {t_tune:.1f}s: " + ", ".join(
+        f"{e['bucket']}: G={e['grid_size']} (rule {e['grid_rule']}), attraction "
+        f"{e['attraction']}, workgroups {e['pair_workgroups']}" for e in tuner.table.values()))
+    shapes = [tuner.shape_for(n) for _, _, n, _ in jobs]
+    t0 = time.perf_counter()
+    b = TsneBatch(max_jobs=count)
+    slots = [b.add(ki, kd, perplexity=30.0, Y_init=y0, device=True, n=n, k=k, index_offset=0,
+                   early_exaggeration_iter=SCHEDULE["early"], n_iter=SCHEDULE["normal"],
+                   early_exaggeration=SCHEDULE["early_exaggeration"],
+                   initial_momentum=SCHEDULE["momentum"], final_momentum=SCHEDULE["momentum"],
+                   **sh)
+             for (ki, kd, n, y0), sh in zip(jobs, shapes)]
+    b.run_to_completion(chunk=50)
+    outs = [b.layout(s) for s in slots]
+    t_batch = time.perf_counter() - t0
+    b.close()
+    pick = np.linspace(0, count - 1, min(check_jobs, count)).astype(int)
+    same = 0
+    dq = []
+    rng = np.random.default_rng(311)
+    for i in pick:
+        ki, kd, n, y0 = jobs[i]
+        same += int(np.array_equal(run_plan_alone(fe, ki, kd, n, k, y0, shapes[i]), outs[i]))
+        sp = specs[i]
+        x, _, _ = workloads.gaussian_mixture(sp["n"], sp["d"], sp["clusters"], seed=sp["seed"],
+                                             dirichlet=True)
+        rows = np.sort(rng.choice(n, size=min(2000, n), replace=False))
+        qt = exact_structure_metrics(x[rows], outs[i][rows], [15])[0]
+        qp = exact_structure_metrics(x[rows], alone[i][rows], [15])[0]
+        dq.append((qt[0] - qp[0], qt[2] - qp[2]))
+    dq = np.array(dq)
+    log(f"[batched] autotuned batch: {t_batch:.2f}s; {same}/{len(pick)} equal to tuned alone; "
+        f"quality delta trust min {dq[:, 0].min():+.4f}, kNN min {dq[:, 1].min():+.4f}")
+    return {"tune_s": round(t_tune, 2),
+            "table": [{k2: e[k2] for k2 in ("bucket", "n", "grid_size", "grid_rule", "attraction",
+                                             "pair_workgroups", "grid_trials", "shape_trials")}
+                      for e in tuner.table.values()],
+            "batched_jobs_per_s": round(count / t_batch, 3),
+            "batched_s": round(t_batch, 3),
+            "speedup_vs_sequential": round(t_seq / t_batch, 3),
+            "checked_jobs": int(len(pick)), "identical_to_tuned_alone": same,
+            "quality_delta_vs_plain": {
+                "trustworthiness_mean": round(float(dq[:, 0].mean()), 5),
+                "trustworthiness_min": round(float(dq[:, 0].min()), 5),
+                "knn_preservation_mean": round(float(dq[:, 1].mean()), 5),
+                "knn_preservation_min": round(float(dq[:, 1].min()), 5)}}
 
 
 def bench_served(fe, workloads, n_requests, k=90):

@@ -46,6 +46,9 @@ SIGNATURES = {
     "fastembedr_hip_tsne_plan_kl": (c_int, [_vp, _dp, _dp]),
     "fastembedr_hip_tsne_plan_create": (
         c_int, [_vp, _vp, c_int, c_int, c_int, c_int, c_float, c_int, _fp, c_uint, _vpp]),
+    "fastembedr_hip_tsne_plan_create_shaped": (
+        c_int, [_vp, _vp, c_int, c_int, c_int, c_int, c_float, c_int, c_int, c_int, _fp, c_uint,
+                _vpp]),
     "fastembedr_hip_tsne_plan_run": (
         c_int, [_vp, c_int, c_float, c_float, c_float, c_float, c_float]),
     "fastembedr_hip_tsne_plan_prepare": (

@@ -30,7 +30,8 @@ class JobStruct(ctypes.Structure):
                 ("early_exaggeration", ctypes.c_float), ("exaggeration", ctypes.c_float),
                 ("learning_rate_early", ctypes.c_float), ("learning_rate_normal", ctypes.c_float),
                 ("initial_momentum", ctypes.c_float), ("final_momentum", ctypes.c_float),
-                ("min_gain", ctypes.c_float), ("max_step_norm", ctypes.c_float)]
+                ("min_gain", ctypes.c_float), ("max_step_norm", ctypes.c_float),
+                ("attraction", ctypes.c_int), ("pair_workgroups", ctypes.c_int)]
 
 
 def job_bytes(n, k, grid_size=0, setup_peak=True):
@@ -53,12 +54,15 @@ class TsneBatch:
     def add(self, indices, distances, perplexity=30.0, Y_init=None, seed=4, grid_size=0,
             early_exaggeration_iter=250, n_iter=500, early_exaggeration=12.0, exaggeration=1.0,
             learning_rate=None, initial_momentum=0.8, final_momentum=0.8, min_gain=0.01,
-            max_step_norm=5.0, device=False, n=None, k=None, index_offset=None):
+            max_step_norm=5.0, device=False, n=None, k=None, index_offset=None, attraction=0,
+            pair_workgroups=0):
         """Add a job; returns its slot.  ``indices``/``distances`` are host
         n x k arrays or, with ``device=True``, objects exposing ``data_ptr()``
         to column-major device memory (a k x n torch tensor), as for
         ``TsnePlan``.  The learning rate defaults to n / early_exaggeration for
-        both phases (the reference's auto_config policy)."""
+        both phases (the reference's auto_config policy).  ``attraction`` and
+        ``pair_workgroups`` are the launch choices of ``TsnePlan`` (0: built
+        in; the autotuner's table gives them per size bucket)."""
         if device:
             n, k = int(n), int(k)
             off = 0 if index_offset is None else int(index_offset)
@@ -79,7 +83,7 @@ class TsneBatch:
                       seed & 0xFFFFFFFF, int(early_exaggeration_iter), int(n_iter),
                       float(early_exaggeration), float(exaggeration), lr, lr,
                       float(initial_momentum), float(final_momentum), float(min_gain),
-                      float(max_step_norm))
+                      float(max_step_norm), int(attraction), int(pair_workgroups))
         slot = ctypes.c_int(-1)
         rc = _lib.lib().fastembedr_hip_tsne_batch_add(self._h, ctypes.byref(j), int(bool(device)),
                                                       ctypes.byref(slot))

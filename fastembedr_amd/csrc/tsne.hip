@@ -238,6 +238,9 @@ class TsnePlan {
   long long seg_entries = 0;
   int groups = 0;
   int attr_kind = 0;  // AttractionKind
+  // the caller's choices: attraction form (kAttrRows, kAttrStaged or
+  // kAttrPairs; -1 built in) and workgroups of the pair form (0 built in)
+  int attr_request = -1, pair_wg_request = 0;
   // pair-once attraction: the staged and rest streams, chunks, slots
   Arena pair_store;
   PairTables pairs{};
@@ -442,7 +445,8 @@ int TsnePlan::setup_pairs(hipStream_t s, bool* built) {
   // unit (or one per chunk when there are fewer)
   const int nc = static_cast<int>(chunks.size());
   const long long all_entries = total + rtotal;
-  const int ranges = std::max(1, std::min(nc, 2 * compute_units()));
+  const int ranges =
+      std::max(1, std::min(nc, pair_wg_request > 0 ? pair_wg_request : 2 * compute_units()));
   std::vector<int> range_first(ranges + 1, nc);
   range_first[0] = 0;
   {
@@ -570,7 +574,7 @@ int TsnePlan::setup_pairs(hipStream_t s, bool* built) {
 int TsnePlan::setup_attraction(hipStream_t s) {
   const long long nnz = csr.nnz;
   attr_kind = kAttrRows;
-  int kind = attraction_kind();
+  int kind = attr_request >= 0 ? attr_request : attraction_kind();
   if (kind == kAttrPairs) {
     bool built = false;
     if (setup_pairs(s, &built)) return 1;
@@ -1211,7 +1215,22 @@ static int validate_tsne(const void* idx, const void* dist, int n, int k, int n_
   return 0;
 }
 
+// The C-ABI's attraction choice (0 built in, 1 row groups, 2 segment-blocked
+// LDS windows, 3 every pair once) as a plan request.
+static int attraction_request(int attraction, int* request) {
+  switch (attraction) {
+    case 0: *request = -1; return 0;
+    case 1: *request = kAttrRows; return 0;
+    case 2: *request = kAttrStaged; return 0;
+    case 3: *request = kAttrPairs; return 0;
+    default: return fail("attraction must be 0 (built in), 1 (rows), 2 (staged) or 3 (pairs)");
+  }
+}
+
 int tsne_job_validate(const fastembedr_hip_tsne_job& j) {
+  int request = -1;
+  if (attraction_request(j.attraction, &request)) return 1;
+  if (j.pair_workgroups < 0) return fail("pair_workgroups must be non-negative");
   if (j.grid_size < 0) return fail("batched jobs run the FFT-grid route (grid_size >= 0)");
   if (j.grid_size > 0 && (j.grid_size < 16 || j.grid_size > 1024 || (j.grid_size & (j.grid_size - 1))))
     return fail("grid_size must be a power of two between 16 and 1024");
@@ -1359,6 +1378,8 @@ class TsneBatch {
     }
     if (tsne_job_validate(j)) return 1;
     auto plan = std::make_unique<TsnePlan>();
+    if (attraction_request(j.attraction, &plan->attr_request)) return 1;
+    plan->pair_wg_request = j.pair_workgroups;
     if (plan->create<float>(j.indices, j.distances, device_inputs, j.n, j.k, j.index_offset,
                             j.perplexity, j.grid_size, j.init, j.seed, true))
       return 1;
@@ -1839,15 +1860,28 @@ int fastembedr_hip_tsne_plan_create(const int* indices, const float* distances,
                                     int device_inputs, int n, int k, int index_offset,
                                     float perplexity, int grid_size, const float* init,
                                     unsigned int seed, fastembedr_hip_tsne_plan** plan) {
+  return fastembedr_hip_tsne_plan_create_shaped(indices, distances, device_inputs, n, k,
+                                                index_offset, perplexity, grid_size, 0, 0, init,
+                                                seed, plan);
+}
+
+int fastembedr_hip_tsne_plan_create_shaped(const int* indices, const float* distances,
+                                           int device_inputs, int n, int k, int index_offset,
+                                           float perplexity, int grid_size, int attraction,
+                                           int pair_workgroups, const float* init,
+                                           unsigned int seed, fastembedr_hip_tsne_plan** plan) {
   fe::clear_error();
   if (!plan) return fail("null plan pointer");
   *plan = nullptr;
   if (fe::validate_tsne(indices, distances, n, k, 2, perplexity, 0, 1, 1.0f, 1.0f, 1.0f, 1, 0.0f,
                         0.0f, 1.0f, index_offset))
     return 1;
+  if (pair_workgroups < 0) return fail("pair_workgroups must be non-negative");
   auto* p = new (std::nothrow) fastembedr_hip_tsne_plan;
   if (!p) return fail("out of host memory");
-  if (p->impl.create<float>(indices, distances, device_inputs != 0, n, k, index_offset, perplexity,
+  p->impl.pair_wg_request = pair_workgroups;
+  if (fe::attraction_request(attraction, &p->impl.attr_request) ||
+      p->impl.create<float>(indices, distances, device_inputs != 0, n, k, index_offset, perplexity,
                             grid_size, init, seed, true)) {
     delete p;
     return 1;

@@ -62,7 +62,8 @@ class Session:
     def submit_embed(self, indices, distances, perplexity=30.0, Y_init=None, seed=4, grid_size=0,
                      early_exaggeration_iter=250, n_iter=500, early_exaggeration=12.0,
                      exaggeration=1.0, learning_rate=None, initial_momentum=0.8,
-                     final_momentum=0.8, min_gain=0.01, max_step_norm=5.0, index_offset=None):
+                     final_momentum=0.8, min_gain=0.01, max_step_norm=5.0, index_offset=None,
+                     attraction=0, pair_workgroups=0):
         """Queue a t-SNE job (host n x k KNN; a Fortran-ordered array is used
         as is); returns the request id.  The learning rate defaults to
         n / early_exaggeration for both phases (the reference's auto_config
@@ -80,7 +81,7 @@ class Session:
                       init.ctypes.data if init is not None else None, seed & 0xFFFFFFFF,
                       int(early_exaggeration_iter), int(n_iter), float(early_exaggeration),
                       float(exaggeration), lr, lr, float(initial_momentum), float(final_momentum),
-                      float(min_gain), float(max_step_norm))
+                      float(min_gain), float(max_step_norm), int(attraction), int(pair_workgroups))
         rid = ctypes.c_longlong(0)
         check(_lib.lib().fastembedr_hip_session_submit_embed(self._h, ctypes.byref(j),
                                                              ctypes.byref(rid)), "session submit")

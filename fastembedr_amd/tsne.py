@@ -288,10 +288,14 @@ class TsnePlan:
     """A resident t-SNE job: P, layout, optimizer state and FFT work arrays on
     the device.  ``indices``/``distances`` are host n x k arrays or, with
     ``device=True``, objects exposing ``data_ptr()`` to column-major device
-    memory (e.g. torch tensors created as ``t.T.contiguous()`` of a k x n)."""
+    memory (e.g. torch tensors created as ``t.T.contiguous()`` of a k x n).
+    ``attraction`` (0 built in, 1 row groups, 2 segment-blocked, 3 every pair
+    once) and ``pair_workgroups`` (0 built in) are the launch choices the
+    autotuner picks per size bucket."""
 
     def __init__(self, indices, distances, perplexity=30.0, grid_size=0, Y_init=None, seed=4,
-                 device=False, n=None, k=None, index_offset=None):
+                 device=False, n=None, k=None, index_offset=None, attraction=0,
+                 pair_workgroups=0):
         L = _lib.lib()
         self._h = _lib.ctypes.c_void_p()
         init = None
@@ -312,10 +316,10 @@ class TsnePlan:
             init = np.ascontiguousarray(Y_init, dtype=np.float32)
             if init.shape != (self.n, 2):
                 raise ValueError("`Y_init` has the wrong shape.")
-        rc = L.fastembedr_hip_tsne_plan_create(
+        rc = L.fastembedr_hip_tsne_plan_create_shaped(
             ip, dp, int(bool(device)), self.n, self.k, off, float(perplexity), int(grid_size),
-            ptr(init, _fp) if init is not None else None, seed & 0xFFFFFFFF,
-            _lib.ctypes.byref(self._h))
+            int(attraction), int(pair_workgroups), ptr(init, _fp) if init is not None else None,
+            seed & 0xFFFFFFFF, _lib.ctypes.byref(self._h))
         check(rc, "HIP t-SNE plan create")
         self._keep = None
 

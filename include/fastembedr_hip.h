@@ -124,6 +124,20 @@ int fastembedr_hip_tsne_plan_create(const int* indices, const float* distances,
                                     float perplexity, int grid_size, const float* init,
                                     unsigned int seed, fastembedr_hip_tsne_plan** plan);
 
+/* As _plan_create, with the launch choices the autotuner picks per size
+ * bucket (fastembedr_amd/tune.py): attraction 0 = built in (every pair once
+ * where most pairs fall in staged tiles, else segment-blocked), 1 = row
+ * groups, 2 = segment-blocked with LDS windows, 3 = every pair once;
+ * pair_workgroups = workgroups of the pair form (0 = two per compute unit,
+ * at most one per chunk).  The attraction form fixes the order of the
+ * attraction sums (forms agree to about 1e-5, not bit for bit); the
+ * workgroup count changes no bit. */
+int fastembedr_hip_tsne_plan_create_shaped(const int* indices, const float* distances,
+                                           int device_inputs, int n, int k, int index_offset,
+                                           float perplexity, int grid_size, int attraction,
+                                           int pair_workgroups, const float* init,
+                                           unsigned int seed, fastembedr_hip_tsne_plan** plan);
+
 /* Enqueue n_iter optimizer iterations with one phase's constants (chunks of
  * up to 50 iterations, each of 4 or more replayed from a captured hipGraph
  * unless FASTEMBEDR_HIP_GRAPH_CAPTURE=0).
@@ -201,6 +215,8 @@ typedef struct fastembedr_hip_tsne_job {
   int early_exaggeration_iter, n_iter;
   float early_exaggeration, exaggeration, learning_rate_early, learning_rate_normal;
   float initial_momentum, final_momentum, min_gain, max_step_norm;
+  int attraction;      /* as fastembedr_hip_tsne_plan_create_shaped (0 = built in) */
+  int pair_workgroups; /* as fastembedr_hip_tsne_plan_create_shaped (0 = built in) */
 } fastembedr_hip_tsne_job;
 
 typedef struct fastembedr_hip_tsne_batch fastembedr_hip_tsne_batch;
