@@ -72,6 +72,8 @@ def parse():
                          "(device metric, timed beside the CPU restatement); 0 skips it")
     ap.add_argument("--batched-jobs", type=int, default=256,
                     help="jobs of the batched workload (0: skip it)")
+    ap.add_argument("--umap-batched-jobs", type=int, default=64,
+                    help="jobs of the batched UMAP workload (0: skip it)")
     ap.add_argument("--served-requests", type=int, default=48,
                     help="requests of the served workload (0: skip it)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -219,6 +221,54 @@ def bench_batched_tuned(fe, workloads, specs, jobs, alone, t_seq, k, check_jobs=
                 "trustworthiness_min": round(float(dq[:, 0].min()), 5),
                 "knn_preservation_mean": round(float(dq[:, 1].mean()), 5),
                 "knn_preservation_min": round(float(dq[:, 1].min()), 5)}}
+
+
+def bench_umap_batched(workloads, count, k=15, epochs=200):
+    """Batched UMAP: `count` jobs of 2k-100k points (log-uniform), each a
+    5-50-cluster mixture in 30-50 dimensions, exact k = 15 KNN, the default
+    fuzzy graph, `epochs` epochs in optimizer_mode 1, uniform [-10, 10]
+    starts.  Times every job alone (graph build + epochs, one after another)
+    and all jobs in one UMAP batch, from host KNN to every layout on the
+    host; counts the layouts that are the same bits both ways."""
+    from fastembedr_amd.umap import FuzzyGraph, UmapBatch
+
+    specs = workloads.batch_job_specs(count, seed=331)
+    jobs = []
+    t0 = time.perf_counter()
+    for sp in specs:
+        ki, kd = workloads.job_knn_device(sp, k)
+        y0 = np.random.default_rng(sp["seed"] + 5).uniform(-10, 10, size=(sp["n"], 2)).astype(
+            np.float32)
+        jobs.append((ki.cpu().numpy().T, kd.cpu().numpy().T, y0, sp["seed"] & 0xFFFF))
+        del ki, kd
+    points = sum(len(j[0]) for j in jobs)
+    log(f"[umap batched] {count} jobs, {points} points, KNN in {time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
+    alone = []
+    for idx, dist, y0, seed in jobs:
+        g = FuzzyGraph(idx, dist)
+        alone.append(g.optimize(y0, epochs, seed=seed, optimizer_mode=1))
+        g.close()
+    t_seq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b = UmapBatch(max_jobs=count)
+    slots = [b.add(idx, dist, y0, n_epochs=epochs, seed=seed) for idx, dist, y0, seed in jobs]
+    b.run_to_completion(chunk=50)
+    outs = [b.layout(sl) for sl in slots]
+    t_batch = time.perf_counter() - t0
+    b.close()
+    same = sum(int(np.array_equal(a, o)) for a, o in zip(alone, outs))
+    log(f"[umap batched] sequential {t_seq:.2f}s, batched {t_batch:.2f}s, "
+        f"{same}/{count} identical")
+    return {"workload": f"{count} UMAP jobs, n log-uniform in [2k, 100k] ({points} points), "
+                        f"5-50-cluster mixtures in 30-50 dims, exact k={k} fuzzy graph, "
+                        f"{epochs} epochs, optimizer_mode 1",
+            "jobs": count, "points": points,
+            "batched_jobs_per_s": round(count / t_batch, 3),
+            "sequential_jobs_per_s": round(count / t_seq, 3),
+            "speedup_vs_sequential": round(t_seq / t_batch, 3),
+            "batched_s": round(t_batch, 3), "sequential_s": round(t_seq, 3),
+            "identical_to_alone": same}
 
 
 def bench_served(fe, workloads, n_requests, k=90):
@@ -766,6 +816,8 @@ def main():
         umap_gate = umap_quality_gate(fe, workloads, args.umap_quality_n)
     batched = bench_batched(fe, workloads, args.batched_jobs) if args.batched_jobs > 0 else None
     torch.cuda.empty_cache()
+    umap_batched = (bench_umap_batched(workloads, args.umap_batched_jobs)
+                    if args.umap_batched_jobs > 0 else None)
     served = bench_served(fe, workloads, args.served_requests) if args.served_requests > 0 else None
 
     out = {
@@ -809,6 +861,7 @@ def main():
     if umap_gate:
         out["umap_quality_gate"] = umap_gate
     out["batched"] = batched
+    out["umap_batched"] = umap_batched
     out["served"] = served
     print(json.dumps(out), flush=True)
 

@@ -89,6 +89,14 @@ SIGNATURES = {
                 c_float, c_uint, c_int, _vp]),
     "fastembedr_hip_umap_schedule": (
         c_int, [_ip, _ip, _fp, c_int, c_int, c_int, c_int, c_uint, c_int, _ip, _ip, _ip]),
+    "fastembedr_hip_umap_batch_create": (c_int, [c_int, c_ll, _vpp]),
+    "fastembedr_hip_umap_job_bytes": (c_ll, [c_int, c_int, c_int, c_int]),
+    "fastembedr_hip_umap_batch_add": (c_int, [_vp, _vp, c_int, _ip]),
+    "fastembedr_hip_umap_batch_step": (c_int, [_vp, c_int]),
+    "fastembedr_hip_umap_batch_status": (c_int, [_vp, _ip, _ip]),
+    "fastembedr_hip_umap_batch_layout": (c_int, [_vp, c_int, _fp]),
+    "fastembedr_hip_umap_batch_remove": (c_int, [_vp, c_int]),
+    "fastembedr_hip_umap_batch_destroy": (None, [_vp]),
     "fastembedr_hip_tsne_transform": (
         c_int, [_fp, c_int, _ip, _dp, c_int, c_int, c_int, _fp, c_int, c_double, c_int, c_int,
                 c_double, c_double, c_double, c_double, c_double, c_double, c_double, c_int,

@@ -17,8 +17,11 @@
 // keeps the epoch loop free of memsets and capturable in a single hipGraph.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -140,21 +143,22 @@ __device__ __forceinline__ unsigned long long shfl_up_u64(unsigned long long v, 
 // symmetric fuzzy graph); the reverse edge's positive update of t is exactly
 // this edge's tail update (the same d2, the difference negated, a symmetric
 // clip and rounding), so each edge adds twice its head update and no tail's.
+// (blk, nblk: this block's index among the nblk blocks that share the edges)
 template <bool SYM>
-__global__ __launch_bounds__(kUThreads) void umap_epoch_grouped(unsigned long long* delta,
-                                                                const float* __restrict__ layout,
-                                                                const int* __restrict__ heads,
-                                                                const int* __restrict__ tails,
-                                                                const float* __restrict__ period,
-                                                                int n_edges, UmapParams p,
-                                                                unsigned epoch) {
+__device__ __forceinline__ void epoch_grouped_body(unsigned long long* delta,
+                                                   const float* __restrict__ layout,
+                                                   const int* __restrict__ heads,
+                                                   const int* __restrict__ tails,
+                                                   const float* __restrict__ period, int n_edges,
+                                                   const UmapParams& p, unsigned epoch, int blk,
+                                                   int nblk) {
   const float alpha = p.lr * (1.0f - __fdiv_rn(static_cast<float>(epoch),
                                                fmaxf(1.0f, static_cast<float>(p.n_epochs))));
   constexpr float eps = 1.1920928955078125e-7f;
   const int lane = threadIdx.x & 63;
   const unsigned long long upto_lane = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-  for (int base = blockIdx.x * kUThreads + (threadIdx.x & ~63); base < n_edges;
-       base += gridDim.x * kUThreads) {
+  for (int base = blk * kUThreads + static_cast<int>(threadIdx.x & ~63); base < n_edges;
+       base += nblk * kUThreads) {
     const int e = base + lane;
     int h = -1;
     long long hx = 0, hy = 0;
@@ -225,6 +229,18 @@ __global__ __launch_bounds__(kUThreads) void umap_epoch_grouped(unsigned long lo
   }
 }
 
+template <bool SYM>
+__global__ __launch_bounds__(kUThreads) void umap_epoch_grouped(unsigned long long* delta,
+                                                                const float* __restrict__ layout,
+                                                                const int* __restrict__ heads,
+                                                                const int* __restrict__ tails,
+                                                                const float* __restrict__ period,
+                                                                int n_edges, UmapParams p,
+                                                                unsigned epoch) {
+  epoch_grouped_body<SYM>(delta, layout, heads, tails, period, n_edges, p, epoch,
+                          static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+}
+
 // optimizer_mode 0 (hogwild): every read sees the live layout and every
 // update is a float atomic on it, as in the reference's in-place kernel; an
 // edge's positive samples run one after another in registers (each seeing the
@@ -237,19 +253,18 @@ __global__ __launch_bounds__(kUThreads) void umap_epoch_grouped(unsigned long lo
 // own, and the tail none, so the pair's distance shrinks by the same amount
 // per sample and no tail atomics are issued.
 template <bool SYM>
-__global__ __launch_bounds__(kUThreads) void umap_epoch_hogwild(float* layout,
-                                                                const int* __restrict__ heads,
-                                                                const int* __restrict__ tails,
-                                                                const float* __restrict__ period,
-                                                                int n_edges, UmapParams p,
-                                                                unsigned epoch) {
+__device__ __forceinline__ void epoch_hogwild_body(float* layout, const int* __restrict__ heads,
+                                                   const int* __restrict__ tails,
+                                                   const float* __restrict__ period, int n_edges,
+                                                   const UmapParams& p, unsigned epoch, int blk,
+                                                   int nblk) {
   const float alpha = p.lr * (1.0f - __fdiv_rn(static_cast<float>(epoch),
                                                fmaxf(1.0f, static_cast<float>(p.n_epochs))));
   constexpr float eps = 1.1920928955078125e-7f;
   const int lane = threadIdx.x & 63;
   const unsigned long long upto_lane = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-  for (int base = blockIdx.x * kUThreads + (threadIdx.x & ~63); base < n_edges;
-       base += gridDim.x * kUThreads) {
+  for (int base = blk * kUThreads + static_cast<int>(threadIdx.x & ~63); base < n_edges;
+       base += nblk * kUThreads) {
     const int e = base + lane;
     int h = -1;
     float ux = 0.0f, uy = 0.0f;
@@ -331,12 +346,20 @@ __global__ __launch_bounds__(kUThreads) void umap_epoch_hogwild(float* layout,
   }
 }
 
+template <bool SYM>
+__global__ __launch_bounds__(kUThreads) void umap_epoch_hogwild(float* layout,
+                                                                const int* __restrict__ heads,
+                                                                const int* __restrict__ tails,
+                                                                const float* __restrict__ period,
+                                                                int n_edges, UmapParams p,
+                                                                unsigned epoch) {
+  epoch_hogwild_body<SYM>(layout, heads, tails, period, n_edges, p, epoch,
+                          static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+}
+
 // sanitise (:3227-3258); MODE 1 first adds and clears the fixed-point delta
 template <int MODE>
-__global__ __launch_bounds__(kUThreads) void umap_apply_kernel(float* layout,
-                                                               unsigned long long* delta, int m) {
-  const int i = blockIdx.x * kUThreads + threadIdx.x;
-  if (i >= m) return;
+__device__ __forceinline__ void apply_body(float* layout, unsigned long long* delta, int i) {
   float v = layout[i];
   if constexpr (MODE == 1) {
     const long long d = static_cast<long long>(delta[i]);
@@ -345,6 +368,13 @@ __global__ __launch_bounds__(kUThreads) void umap_apply_kernel(float* layout,
   }
   if (!isfinite(v)) v = 0.0f;
   layout[i] = fminf(16.0f, fmaxf(-16.0f, v));
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kUThreads) void umap_apply_kernel(float* layout,
+                                                               unsigned long long* delta, int m) {
+  const int i = blockIdx.x * kUThreads + threadIdx.x;
+  if (i < m) apply_body<MODE>(layout, delta, i);
 }
 
 __global__ void umap_schedule_kernel(const int* __restrict__ heads, const int* __restrict__ tails,
@@ -554,6 +584,361 @@ int umap_run_noalloc(const int* heads, const int* tails, const float* epochs_per
   return 0;
 }
 
+
+// ---------------------------------------------------------------------------
+// Batched UMAP jobs: many resident fuzzy graphs advance together through
+// job-table launches (per epoch: one epoch launch per optimizer mode, one
+// apply launch per mode, one advance launch), whatever the number of jobs.
+// A job's blocks find it by a binary search of the launch's prefix table and
+// run the epoch body with job-local block indices; each job reads its own
+// epoch counter from its descriptor, so a captured chunk of epochs stays
+// valid while jobs sit at different epochs, and a finished job's blocks
+// exit at once.  Mode 1 sums are integers, so a job gives the bits of its
+// resident graph optimised alone (fastembedr_hip_umap_graph_optimize).
+// ---------------------------------------------------------------------------
+namespace {
+
+struct UmapJobDesc {
+  const int* heads;
+  const int* tails;
+  const float* period;
+  float* layout;
+  unsigned long long* delta;
+  int n, n_edges, epoch, active, mode;
+  UmapParams p;
+};
+
+__device__ __forceinline__ int umap_find_entry(const int* __restrict__ first, int m) {
+  const int bid = static_cast<int>(blockIdx.x);
+  int lo = 0, hi = m - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first[mid] <= bid) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kUThreads) void umap_batch_epoch(UmapJobDesc* jobs,
+                                                              const int* __restrict__ slot,
+                                                              const int* __restrict__ first,
+                                                              int m) {
+  const int e = umap_find_entry(first, m);
+  const UmapJobDesc* J = jobs + slot[e];
+  if (!J->active) return;
+  const int blk = static_cast<int>(blockIdx.x) - first[e], nblk = first[e + 1] - first[e];
+  const unsigned epoch = static_cast<unsigned>(J->epoch);
+  if constexpr (MODE == 1)
+    epoch_grouped_body<true>(J->delta, J->layout, J->heads, J->tails, J->period, J->n_edges,
+                             J->p, epoch, blk, nblk);
+  else
+    epoch_hogwild_body<true>(J->layout, J->heads, J->tails, J->period, J->n_edges, J->p, epoch,
+                             blk, nblk);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kUThreads) void umap_batch_apply(UmapJobDesc* jobs,
+                                                              const int* __restrict__ slot,
+                                                              const int* __restrict__ first,
+                                                              int m) {
+  const int e = umap_find_entry(first, m);
+  const UmapJobDesc* J = jobs + slot[e];
+  if (!J->active) return;
+  const int i = (static_cast<int>(blockIdx.x) - first[e]) * kUThreads + static_cast<int>(threadIdx.x);
+  if (i < 2 * J->n) apply_body<MODE>(J->layout, J->delta, i);
+}
+
+// one thread per job: the epoch counter, then the job's activity
+__global__ void umap_batch_advance(UmapJobDesc* jobs, const int* __restrict__ slot, int m) {
+  const int e = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (e >= m) return;
+  UmapJobDesc* J = jobs + slot[e];
+  if (!J->active) return;
+  J->epoch += 1;
+  if (J->epoch >= J->p.n_epochs) J->active = 0;
+}
+
+}  // namespace
+
+// Device bytes a job of this shape holds while resident: the graph at its
+// worst case of 2 n k entries (COO heads, tails, weights, periods and the
+// CSR), the layout and mode 1's delta; setup adds the graph build's peak.
+static long long umap_job_bytes(int n, int k, int mode, bool setup) {
+  const long long nk2 = 2ll * n * k;
+  long long b = 20ll * nk2 + 4ll * (n + 1) + 8ll * n + (mode == 1 ? 16ll * n : 0) + 4096;
+  if (setup) b += 24ll * nk2 + 32ll * n;
+  return b;
+}
+
+class UmapBatch {
+ public:
+  struct Slot {
+    fastembedr_hip_umap_graph* graph = nullptr;
+    Arena mem;
+    long long bytes = 0;
+    int n = 0;
+  };
+  int max_jobs = 0;
+  long long budget = 0, used = 0;
+  Stream stream;
+  std::vector<std::unique_ptr<Slot>> slots;
+  std::vector<UmapJobDesc> desc;
+  Arena dev;
+  UmapJobDesc* d_desc = nullptr;
+  int* d_tab = nullptr;
+  int tab_cap = 0;
+  bool dirty = true;
+  struct Launch {
+    int kind;  // 0 epoch, 1 apply, 2 advance
+    int mode;
+    int off, m, blocks;
+  };
+  std::vector<Launch> launches;
+  std::unique_ptr<GraphExec> graph;
+
+  ~UmapBatch() {
+    if (stream.get()) (void)hipStreamSynchronize(stream.get());
+    for (auto& sl : slots)
+      if (sl && sl->graph) fastembedr_hip_umap_graph_destroy(sl->graph);
+  }
+
+  int init(int max_jobs_, long long budget_) {
+    if (max_jobs_ < 1) return fail("max_jobs must be positive");
+    max_jobs = max_jobs_;
+    budget = budget_;
+    if (stream.init()) return 1;
+    slots.resize(max_jobs);
+    desc.resize(max_jobs);
+    std::memset(desc.data(), 0, desc.size() * sizeof(UmapJobDesc));
+    tab_cap = 6 * (2 * max_jobs + 1) + 16;
+    ArenaPlan p;
+    p.add<UmapJobDesc>(max_jobs);
+    p.add<int>(tab_cap);
+    if (dev.init(p.bytes, "UMAP batch tables")) return 1;
+    d_desc = dev.take<UmapJobDesc>(max_jobs);
+    d_tab = dev.take<int>(tab_cap);
+    FE_CHECK(hipMemsetAsync(d_desc, 0, max_jobs * sizeof(UmapJobDesc), stream.get()));
+    return check(hipStreamSynchronize(stream.get()), "UMAP batch init");
+  }
+
+  // 0 admitted, 1 error, 2 not admitted now (budget or no free slot)
+  int add(const fastembedr_hip_umap_job& j, bool device_inputs, int* slot_out) {
+    if (!j.indices || !j.distances || !j.init) return fail("null UMAP job pointer");
+    if (j.n < 2 || j.k_total < 1 || j.n_neighbors < 1 || j.n_neighbors > j.k_total ||
+        j.n_epochs < 1 || j.negative_sample_rate < 0 || !(j.learning_rate > 0.0f) ||
+        !(j.repulsion_strength > 0.0f))
+      return fail("invalid HIP UMAP job parameters");
+    if (j.optimizer_mode != 0 && j.optimizer_mode != 1)
+      return fail("optimizer_mode must be 0 (atomic) or 1 (delta)");
+    for (size_t t = 0; t < 2 * static_cast<size_t>(j.n); ++t)
+      if (!std::isfinite(j.init[t])) return fail("UMAP init must contain only finite coordinates.");
+    int slot = -1;
+    for (int q = 0; q < max_jobs; ++q)
+      if (!slots[q]) {
+        slot = q;
+        break;
+      }
+    if (slot < 0) {
+      set_error("no free batch slot");
+      return 2;
+    }
+    if (budget > 0 &&
+        used + umap_job_bytes(j.n, j.n_neighbors, j.optimizer_mode, true) > budget) {
+      set_error("HBM budget does not admit this job yet");
+      return 2;
+    }
+    auto sl = std::make_unique<Slot>();
+    if (fastembedr_hip_umap_graph_create(j.indices, j.distances, j.n, j.k_total, 0, j.n_neighbors,
+                                         j.n_neighbors, j.index_offset, device_inputs ? 1 : 0,
+                                         &sl->graph))
+      return 1;
+    long long nnz = 0;
+    float max_w = 0.0f;
+    const int* heads = nullptr;
+    const int* tails = nullptr;
+    const float* weights = nullptr;
+    const float* period = nullptr;
+    if (fastembedr_hip_umap_graph_info(sl->graph, &nnz, &max_w) ||
+        fastembedr_hip_umap_graph_device(sl->graph, &heads, &tails, &weights, &period)) {
+      fastembedr_hip_umap_graph_destroy(sl->graph);
+      return 1;
+    }
+    if (nnz < 1 || nnz > 2147483647ll) {
+      fastembedr_hip_umap_graph_destroy(sl->graph);
+      return fail("the UMAP graph has no edges or too many for 32-bit edge ids");
+    }
+    const size_t m = 2 * static_cast<size_t>(j.n);
+    ArenaPlan plan;
+    plan.add<float>(m);
+    plan.add<unsigned long long>(j.optimizer_mode == 1 ? m : 1);
+    hipStream_t s = stream.get();
+    if (sl->mem.init(plan.bytes, "UMAP batch layout")) {
+      fastembedr_hip_umap_graph_destroy(sl->graph);
+      return 1;
+    }
+    float* layout = sl->mem.take<float>(m);
+    unsigned long long* delta = sl->mem.take<unsigned long long>(j.optimizer_mode == 1 ? m : 1);
+    const int mm = j.optimizer_mode == 1 ? static_cast<int>(m) : 1;
+    FE_CHECK(hipMemcpyAsync(layout, j.init, m * sizeof(float), hipMemcpyHostToDevice, s));
+    FE_CHECK(hipMemsetAsync(delta, 0, mm * sizeof(unsigned long long), s));
+    UmapJobDesc d;
+    std::memset(&d, 0, sizeof(d));
+    d.heads = heads;
+    d.tails = tails;
+    d.period = period;
+    d.layout = layout;
+    d.delta = delta;
+    d.n = j.n;
+    d.n_edges = static_cast<int>(nnz);
+    d.epoch = 0;
+    d.active = 1;
+    d.mode = j.optimizer_mode;
+    d.p = UmapParams{j.n, j.n_epochs, j.negative_sample_rate, j.seed, j.learning_rate, j.a, j.b,
+                     j.repulsion_strength};
+    desc[slot] = d;
+    FE_CHECK(hipMemcpyAsync(d_desc + slot, &desc[slot], sizeof(UmapJobDesc),
+                            hipMemcpyHostToDevice, s));
+    FE_CHECK(hipStreamSynchronize(s));
+    sl->n = j.n;
+    sl->bytes = 20ll * nnz + 4ll * (j.n + 1) + static_cast<long long>(sl->mem.capacity());
+    used += sl->bytes;
+    slots[slot] = std::move(sl);
+    dirty = true;
+    *slot_out = slot;
+    return 0;
+  }
+
+  int remove(int slot) {
+    if (slot < 0 || slot >= max_jobs || !slots[slot]) return fail("invalid batch slot");
+    FE_CHECK(hipStreamSynchronize(stream.get()));
+    UmapJobDesc z;
+    std::memset(&z, 0, sizeof(z));
+    desc[slot] = z;
+    FE_CHECK(hipMemcpy(d_desc + slot, &z, sizeof(UmapJobDesc), hipMemcpyHostToDevice));
+    used -= slots[slot]->bytes;
+    fastembedr_hip_umap_graph_destroy(slots[slot]->graph);
+    slots[slot].reset();
+    dirty = true;
+    return 0;
+  }
+
+  int rebuild() {
+    launches.clear();
+    std::vector<int> tab;
+    auto add_launch = [&](int kind, int mode, const std::vector<std::pair<int, int>>& ent) {
+      if (ent.empty()) return;
+      Launch L{kind, mode, static_cast<int>(tab.size()), static_cast<int>(ent.size()), 0};
+      for (auto& e : ent) tab.push_back(e.first);
+      int acc = 0;
+      for (auto& e : ent) {
+        tab.push_back(acc);
+        acc += e.second;
+      }
+      tab.push_back(acc);
+      L.blocks = acc;
+      launches.push_back(L);
+    };
+    for (int mode = 0; mode <= 1; ++mode) {
+      std::vector<std::pair<int, int>> ent;
+      for (int q = 0; q < max_jobs; ++q)
+        if (slots[q] && desc[q].mode == mode)
+          ent.push_back({q, static_cast<int>(std::min<long long>(
+                                (desc[q].n_edges + kUThreads - 1) / kUThreads, 8192))});
+      add_launch(0, mode, ent);
+    }
+    for (int mode = 0; mode <= 1; ++mode) {
+      std::vector<std::pair<int, int>> ent;
+      for (int q = 0; q < max_jobs; ++q)
+        if (slots[q] && desc[q].mode == mode)
+          ent.push_back({q, (2 * desc[q].n + kUThreads - 1) / kUThreads});
+      add_launch(1, mode, ent);
+    }
+    std::vector<std::pair<int, int>> all;
+    for (int q = 0; q < max_jobs; ++q)
+      if (slots[q]) all.push_back({q, 1});
+    add_launch(2, 0, all);
+    if (static_cast<int>(tab.size()) > tab_cap) return fail("UMAP batch table overflow");
+    if (!tab.empty())
+      FE_CHECK(hipMemcpy(d_tab, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice));
+    dirty = false;
+    return 0;
+  }
+
+  int enqueue_epoch() {
+    hipStream_t s = stream.get();
+    for (const Launch& L : launches) {
+      const int* sl = d_tab + L.off;
+      const int* fi = sl + L.m;
+      if (L.kind == 0) {
+        if (L.mode == 1) umap_batch_epoch<1><<<L.blocks, kUThreads, 0, s>>>(d_desc, sl, fi, L.m);
+        else umap_batch_epoch<0><<<L.blocks, kUThreads, 0, s>>>(d_desc, sl, fi, L.m);
+      } else if (L.kind == 1) {
+        if (L.mode == 1) umap_batch_apply<1><<<L.blocks, kUThreads, 0, s>>>(d_desc, sl, fi, L.m);
+        else umap_batch_apply<0><<<L.blocks, kUThreads, 0, s>>>(d_desc, sl, fi, L.m);
+      } else {
+        umap_batch_advance<<<(L.m + kUThreads - 1) / kUThreads, kUThreads, 0, s>>>(d_desc, sl, L.m);
+      }
+      FE_LAUNCH_CHECK("UMAP batch kernel");
+    }
+    return 0;
+  }
+
+  int step(int epochs) {
+    if (epochs < 0) return fail("epochs must be non-negative");
+    if (dirty) {
+      graph.reset();
+      if (rebuild()) return 1;
+    }
+    if (launches.empty()) return 0;
+    constexpr int kChunk = 25;
+    hipStream_t s = stream.get();
+    int left = epochs;
+    const char* env = std::getenv("FASTEMBEDR_HIP_GRAPH_CAPTURE");
+    if (!(env && env[0] == '0') && left >= kChunk) {
+      if (!graph) {
+        hipGraph_t g = nullptr;
+        bool ok = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        for (int e = 0; ok && e < kChunk; ++e) ok = enqueue_epoch() == 0;
+        const hipError_t end = hipStreamEndCapture(s, &g);
+        if (ok && end == hipSuccess && g) {
+          auto exec = std::make_unique<GraphExec>();
+          if (exec->instantiate(g) == 0) graph = std::move(exec);
+        } else if (g) {
+          (void)hipGraphDestroy(g);
+        }
+        (void)hipGetLastError();
+        clear_error();
+      }
+      if (graph)
+        for (; left >= kChunk; left -= kChunk) FE_CHECK(hipGraphLaunch(graph->get(), s));
+    }
+    for (; left > 0; --left)
+      if (enqueue_epoch()) return 1;
+    return 0;
+  }
+
+  int status(int* epochs, int* active) {
+    std::vector<UmapJobDesc> h(max_jobs);
+    FE_CHECK(hipMemcpyAsync(h.data(), d_desc, max_jobs * sizeof(UmapJobDesc),
+                            hipMemcpyDeviceToHost, stream.get()));
+    FE_CHECK(hipStreamSynchronize(stream.get()));
+    for (int q = 0; q < max_jobs; ++q) {
+      if (epochs) epochs[q] = slots[q] ? h[q].epoch : -1;
+      if (active) active[q] = slots[q] ? h[q].active : 0;
+    }
+    return 0;
+  }
+
+  int layout(int slot, float* out) {
+    if (slot < 0 || slot >= max_jobs || !slots[slot]) return fail("invalid batch slot");
+    if (!out) return fail("null output pointer");
+    FE_CHECK(hipMemcpyAsync(out, desc[slot].layout, 2 * static_cast<size_t>(slots[slot]->n) *
+                                                         sizeof(float),
+                            hipMemcpyDeviceToHost, stream.get()));
+    return check(hipStreamSynchronize(stream.get()), "UMAP batch layout");
+  }
+};
 }  // namespace fe
 
 extern "C" {
@@ -631,5 +1016,67 @@ int fastembedr_hip_umap_schedule(const int* heads, const int* tails,
                        hipMemcpyDeviceToHost));
   return 0;
 }
+
+}  // extern "C"
+
+struct fastembedr_hip_umap_batch {
+  fe::UmapBatch impl;
+};
+
+extern "C" {
+
+long long fastembedr_hip_umap_job_bytes(int n, int n_neighbors, int optimizer_mode,
+                                        int setup_peak) {
+  return fe::umap_job_bytes(n, n_neighbors, optimizer_mode, setup_peak != 0);
+}
+
+int fastembedr_hip_umap_batch_create(int max_jobs, long long hbm_budget_bytes,
+                                     fastembedr_hip_umap_batch** batch) {
+  fe::clear_error();
+  if (!batch) return fe::fail("null batch pointer");
+  *batch = nullptr;
+  auto* b = new (std::nothrow) fastembedr_hip_umap_batch;
+  if (!b) return fe::fail("out of host memory");
+  if (b->impl.init(max_jobs, hbm_budget_bytes)) {
+    delete b;
+    return 1;
+  }
+  *batch = b;
+  return 0;
+}
+
+int fastembedr_hip_umap_batch_add(fastembedr_hip_umap_batch* batch,
+                                  const fastembedr_hip_umap_job* job, int device_inputs,
+                                  int* slot) {
+  fe::clear_error();
+  if (!batch || !job || !slot) return fe::fail("null batch, job or slot pointer");
+  return batch->impl.add(*job, device_inputs != 0, slot);
+}
+
+int fastembedr_hip_umap_batch_step(fastembedr_hip_umap_batch* batch, int n_epochs) {
+  fe::clear_error();
+  if (!batch) return fe::fail("null batch pointer");
+  return batch->impl.step(n_epochs);
+}
+
+int fastembedr_hip_umap_batch_status(fastembedr_hip_umap_batch* batch, int* epochs, int* active) {
+  fe::clear_error();
+  if (!batch) return fe::fail("null batch pointer");
+  return batch->impl.status(epochs, active);
+}
+
+int fastembedr_hip_umap_batch_layout(fastembedr_hip_umap_batch* batch, int slot, float* out) {
+  fe::clear_error();
+  if (!batch) return fe::fail("null batch pointer");
+  return batch->impl.layout(slot, out);
+}
+
+int fastembedr_hip_umap_batch_remove(fastembedr_hip_umap_batch* batch, int slot) {
+  fe::clear_error();
+  if (!batch) return fe::fail("null batch pointer");
+  return batch->impl.remove(slot);
+}
+
+void fastembedr_hip_umap_batch_destroy(fastembedr_hip_umap_batch* batch) { delete batch; }
 
 }  // extern "C"

@@ -186,3 +186,108 @@ class FuzzyGraph:
             self.close()
         except Exception:
             pass
+
+
+class UmapJobStruct(ctypes.Structure):
+    """Mirror of fastembedr_hip_umap_job (include/fastembedr_hip.h)."""
+
+    _fields_ = [("indices", ctypes.c_void_p), ("distances", ctypes.c_void_p),
+                ("n", ctypes.c_int), ("k_total", ctypes.c_int), ("n_neighbors", ctypes.c_int),
+                ("index_offset", ctypes.c_int), ("init", ctypes.c_void_p),
+                ("n_epochs", ctypes.c_int), ("negative_sample_rate", ctypes.c_int),
+                ("learning_rate", ctypes.c_float), ("a", ctypes.c_float), ("b", ctypes.c_float),
+                ("repulsion_strength", ctypes.c_float), ("seed", ctypes.c_uint),
+                ("optimizer_mode", ctypes.c_int)]
+
+
+def umap_job_bytes(n, n_neighbors, optimizer_mode=1, setup_peak=True):
+    return int(_lib.lib().fastembedr_hip_umap_job_bytes(int(n), int(n_neighbors),
+                                                        int(optimizer_mode),
+                                                        int(bool(setup_peak))))
+
+
+class UmapNotAdmitted(Exception):
+    """The budget or the slot table cannot take the job now."""
+
+
+class UmapBatch:
+    """Many UMAP jobs advancing together (``fastembedr_hip_umap_batch_*``):
+    each job's fuzzy graph and layout stay resident, one set of job-table
+    launches per epoch covers every job, and a job in optimizer_mode 1 gives
+    the bits of ``FuzzyGraph.optimize`` on its own graph."""
+
+    def __init__(self, max_jobs=64, hbm_budget_bytes=0):
+        self.max_jobs = int(max_jobs)
+        self._h = ctypes.c_void_p()
+        check(_lib.lib().fastembedr_hip_umap_batch_create(self.max_jobs, int(hbm_budget_bytes),
+                                                          ctypes.byref(self._h)),
+              "UMAP batch create")
+        self.n_of = {}
+
+    def add(self, indices, distances, init, n_epochs, n_neighbors=None, negative_sample_rate=5,
+            learning_rate=1.0, a=A_MIN_DIST_001, b=B_MIN_DIST_001, repulsion_strength=1.0,
+            seed=0, optimizer_mode=1, index_offset=None):
+        """Add a job from a host n x k KNN (the graph uses its first
+        ``n_neighbors`` columns, all by default); returns its slot."""
+        idx = np.asarray(indices)
+        dist = np.asarray(distances)
+        if idx.ndim != 2 or idx.shape != dist.shape:
+            raise ValueError("indices and distances must have the same dimensions")
+        n, k_total = idx.shape
+        ic = np.asfortranarray(idx, dtype=np.int32)
+        dc = np.asfortranarray(np.asarray(dist, np.float64).astype(np.float32))
+        y = np.ascontiguousarray(init, dtype=np.float32)
+        if y.shape != (n, 2):
+            raise ValueError("`init` has the wrong shape.")
+        nn = k_total if n_neighbors is None else int(n_neighbors)
+        off = -1 if index_offset is None else int(index_offset)
+        j = UmapJobStruct(ic.ctypes.data, dc.ctypes.data, n, k_total, nn, off, y.ctypes.data,
+                          int(n_epochs), int(negative_sample_rate), float(learning_rate),
+                          float(a), float(b), float(repulsion_strength), int(seed) & 0xFFFFFFFF,
+                          int(optimizer_mode))
+        slot = ctypes.c_int(-1)
+        rc = _lib.lib().fastembedr_hip_umap_batch_add(self._h, ctypes.byref(j), 0,
+                                                      ctypes.byref(slot))
+        if rc == 2:
+            raise UmapNotAdmitted(_lib.last_error())
+        check(rc, "UMAP batch add")
+        self.n_of[slot.value] = n
+        return slot.value
+
+    def step(self, n_epochs):
+        check(_lib.lib().fastembedr_hip_umap_batch_step(self._h, int(n_epochs)), "UMAP batch step")
+
+    def status(self):
+        ep = np.zeros(self.max_jobs, np.int32)
+        act = np.zeros(self.max_jobs, np.int32)
+        check(_lib.lib().fastembedr_hip_umap_batch_status(self._h, ptr(ep, _ip), ptr(act, _ip)),
+              "UMAP batch status")
+        return ep, act
+
+    def layout(self, slot):
+        out = np.zeros((self.n_of[slot], 2), np.float32)
+        check(_lib.lib().fastembedr_hip_umap_batch_layout(self._h, int(slot), ptr(out, _fp)),
+              "UMAP batch layout")
+        return out
+
+    def remove(self, slot):
+        check(_lib.lib().fastembedr_hip_umap_batch_remove(self._h, int(slot)), "UMAP batch remove")
+        self.n_of.pop(slot, None)
+
+    def run_to_completion(self, chunk=50):
+        while True:
+            _, act = self.status()
+            if not act.any():
+                return
+            self.step(chunk)
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _lib.lib().fastembedr_hip_umap_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -412,6 +412,62 @@ int fastembedr_hip_umap_graph_optimize(fastembedr_hip_umap_graph* graph, const f
 
 void fastembedr_hip_umap_graph_destroy(fastembedr_hip_umap_graph* graph);
 
+/* ---- batched UMAP jobs ------------------------------------------------------
+ * No reference counterpart (the reference runs one job per call).  Many UMAP
+ * jobs of uneven size advance together: each job's fuzzy graph
+ * (fastembedr_hip_umap_graph_create at the default edge budget) and layout
+ * stay resident, and one epoch of every resident job costs two launches per
+ * optimizer mode present plus one, replayed in captured 25-epoch chunks.  A
+ * job in optimizer_mode 1 gives the bits fastembedr_hip_umap_graph_optimize
+ * gives on its graph alone; mode 0 is the reference's hogwild form, which is
+ * not reproducible alone or batched. */
+typedef struct fastembedr_hip_umap_job {
+  const int* indices;     /* column-major n x k_total */
+  const float* distances; /* column-major n x k_total, float32 */
+  int n, k_total;
+  int n_neighbors;        /* the graph uses columns 0 .. n_neighbors - 1 */
+  int index_offset;       /* 0, 1, or -1 to detect it as the reference does */
+  const float* init;      /* host row-major n x 2 */
+  int n_epochs, negative_sample_rate;
+  float learning_rate, a, b, repulsion_strength;
+  unsigned int seed;
+  int optimizer_mode;     /* 0 in-place float atomics, 1 fixed-point delta */
+} fastembedr_hip_umap_job;
+
+typedef struct fastembedr_hip_umap_batch fastembedr_hip_umap_batch;
+
+/* hbm_budget_bytes <= 0: no budget. */
+int fastembedr_hip_umap_batch_create(int max_jobs, long long hbm_budget_bytes,
+                                     fastembedr_hip_umap_batch** batch);
+
+/* Device bytes a UMAP job of this shape holds while resident (the graph at
+ * its worst case of 2 n n_neighbors entries), and at its peak during the
+ * graph build. */
+long long fastembedr_hip_umap_job_bytes(int n, int n_neighbors, int optimizer_mode,
+                                        int setup_peak);
+
+/* Builds the job's fuzzy graph and uploads its start layout.  Returns 0 and
+ * the slot, 2 when the budget or the slot table does not admit it now, 1 on
+ * error.  indices/distances are host (device_inputs = 0) or device (1)
+ * pointers. */
+int fastembedr_hip_umap_batch_add(fastembedr_hip_umap_batch* batch,
+                                  const fastembedr_hip_umap_job* job, int device_inputs,
+                                  int* slot);
+
+/* Enqueue n_epochs epochs of every resident, unfinished job (async). */
+int fastembedr_hip_umap_batch_step(fastembedr_hip_umap_batch* batch, int n_epochs);
+
+/* Waits for the batch and reports per slot (max_jobs entries) the epochs
+ * done (-1 = empty slot) and whether the job still runs. */
+int fastembedr_hip_umap_batch_status(fastembedr_hip_umap_batch* batch, int* epochs, int* active);
+
+/* Layout of a slot, host row-major n x 2. */
+int fastembedr_hip_umap_batch_layout(fastembedr_hip_umap_batch* batch, int slot, float* out);
+
+int fastembedr_hip_umap_batch_remove(fastembedr_hip_umap_batch* batch, int slot);
+
+void fastembedr_hip_umap_batch_destroy(fastembedr_hip_umap_batch* batch);
+
 /* ---- fixed-reference t-SNE transform ------------------------------------
  * The reference binds transform_tsne_cuda_cpp (R/transform_tsne.R:275) but
  * defines it nowhere; the argument list follows transform_tsne_metal_cpp
