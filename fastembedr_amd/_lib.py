@@ -75,6 +75,7 @@ SIGNATURES = {
         c_int, [_vp, _ip, _fp, c_int, c_int, c_int, _fp, c_int, c_double, c_int, c_int, c_double,
                 c_double, c_double, c_double, c_double, c_double, c_double, c_int, c_int, c_int,
                 _llp]),
+    "fastembedr_hip_session_submit_umap": (c_int, [_vp, _vp, _llp]),
     "fastembedr_hip_session_poll": (c_int, [_vp, c_ll, c_int, _ip, _fp, _dp]),
     "fastembedr_hip_session_stats": (c_int, [_vp, _llp]),
     "fastembedr_hip_session_destroy": (None, [_vp]),

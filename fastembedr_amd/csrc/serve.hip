@@ -10,6 +10,9 @@
 //    that arrives while others run starts at the next tick and shares their
 //    kernels.  A finished job is read back and removed, which frees its
 //    budget for waiting ones.
+//  * umap: a whole UMAP job (KNN in, layout out): its fuzzy graph is built
+//    when it joins the session's UMAP batch, which advances every resident
+//    job by 25 epochs per tick on its own stream, beside the t-SNE ticks.
 //  * transform: placement of query points into the fixed reference layout.
 //    Its device workspace and pinned host staging come from two pools carved
 //    out of the budget at creation; it is enqueued on one of the session's
@@ -19,9 +22,10 @@
 // from any thread (inputs are copied at submit, so the caller may free them)
 // and poll by request id.  The engine loop:
 //   1. retires transforms whose completion event has passed;
-//   2. when the tick in flight has passed: reads the batch status, retires
-//      finished jobs, admits waiting jobs in arrival order while the budget
-//      allows, and enqueues the next tick;
+//   2. for each batch (t-SNE, UMAP) whose tick in flight has passed: reads
+//      its status, retires finished jobs, admits waiting jobs in arrival
+//      order while the budget allows, and enqueues the next tick (the two
+//      batches share the budget left after the transform pool);
 //   3. admits waiting transforms in arrival order while a stream and pool
 //      space are free.
 // A job the budget cannot take with nothing resident, or a transform larger
@@ -106,16 +110,19 @@ class RangePool {
 
 enum RequestState { kQueued = 0, kRunning = 1, kDone = 2, kFailed = 3 };
 
+enum RequestKind { kEmbed = 0, kTransform = 1, kUmap = 2 };
+
 struct Request {
   long long id = 0;
-  bool embed = true;
+  int kind = kEmbed;
   int state = kQueued;
   std::string error;
   Clock::time_point submitted, finished;
   int rows = 0;
   std::vector<float> result;
-  // embed job: the job with pointers into the session's copies
+  // embed / UMAP job: the job with pointers into the session's copies
   fastembedr_hip_tsne_job job{};
+  fastembedr_hip_umap_job ujob{};
   std::vector<int> idx;
   std::vector<float> dist, init;
   // transform
@@ -152,8 +159,10 @@ class Session {
         FE_CHECK(hipEventCreateWithFlags(&tf_event_[i], hipEventDisableTiming));
       }
     }
-    const long long batch_budget = budget > 0 ? budget - static_cast<long long>(pool_bytes_) : 0;
-    if (fastembedr_hip_tsne_batch_create(max_jobs, batch_budget, &batch_)) return 1;
+    embed_budget_ = budget > 0 ? budget - static_cast<long long>(pool_bytes_) : 0;
+    if (fastembedr_hip_tsne_batch_create(max_jobs, embed_budget_, &batch_)) return 1;
+    if (fastembedr_hip_umap_batch_create(max_jobs, embed_budget_, &ubatch_)) return 1;
+    FE_CHECK(hipEventCreateWithFlags(&utick_event_, hipEventDisableTiming));
     max_jobs_ = max_jobs;
     FE_CHECK(hipEventCreateWithFlags(&tick_event_, hipEventDisableTiming));
     engine_ = std::thread([this] { engine(); });
@@ -163,7 +172,7 @@ class Session {
   int submit_embed(const fastembedr_hip_tsne_job& j, long long* id) {
     if (tsne_job_validate(j)) return 1;
     auto r = std::make_shared<Request>();
-    r->embed = true;
+    r->kind = kEmbed;
     r->rows = j.n;
     const std::size_t nk = static_cast<std::size_t>(j.n) * j.k;
     r->idx.assign(j.indices, j.indices + nk);
@@ -183,7 +192,7 @@ class Session {
                        double max_step, int n_negatives, int threshold, int seed, long long* id) {
     if (!ref_) return fail("the session has no reference layout for transforms");
     auto r = std::make_shared<Request>();
-    r->embed = false;
+    r->kind = kTransform;
     TransformTaskPtr t;
     if (transform_task_create(*transform_reference_of(ref_), indices, distances, n_query, k,
                               index_offset, init, initialization, perplexity, n_iter, early_iter,
@@ -201,6 +210,22 @@ class Session {
     const std::size_t total = static_cast<std::size_t>(n_query) * k;
     r->idx.assign(indices, indices + total);
     r->dist.assign(distances, distances + total);
+    return enqueue(std::move(r), id);
+  }
+
+  int submit_umap(const fastembedr_hip_umap_job& j, long long* id) {
+    if (umap_job_validate(j)) return 1;
+    auto r = std::make_shared<Request>();
+    r->kind = kUmap;
+    r->rows = j.n;
+    const std::size_t nk = static_cast<std::size_t>(j.n) * j.k_total;
+    r->idx.assign(j.indices, j.indices + nk);
+    r->dist.assign(j.distances, j.distances + nk);
+    r->init.assign(j.init, j.init + 2 * static_cast<std::size_t>(j.n));
+    r->ujob = j;
+    r->ujob.indices = r->idx.data();
+    r->ujob.distances = r->dist.data();
+    r->ujob.init = r->init.data();
     return enqueue(std::move(r), id);
   }
 
@@ -239,6 +264,8 @@ class Session {
     s[7] = static_cast<long long>(dev_ranges_.used());
     s[8] = ticks_;
     s[9] = static_cast<long long>(pool_bytes_);
+    s[10] = queued_umap_;
+    s[11] = running_umap_;
     return 0;
   }
 
@@ -251,6 +278,10 @@ class Session {
     if (engine_.joinable()) engine_.join();
     if (batch_) fastembedr_hip_tsne_batch_destroy(batch_);
     batch_ = nullptr;
+    if (ubatch_) fastembedr_hip_umap_batch_destroy(ubatch_);
+    ubatch_ = nullptr;
+    if (utick_event_) (void)hipEventDestroy(utick_event_);
+    utick_event_ = nullptr;
     for (int i = 0; i < kTransformStreams; ++i) {
       if (tf_stream_[i]) (void)hipStreamSynchronize(tf_stream_[i]);
       if (tf_event_[i]) (void)hipEventDestroy(tf_event_[i]);
@@ -277,23 +308,29 @@ class Session {
     requests_[r->id] = r;
     incoming_.push_back(r);
     ++submitted_;
-    if (r->embed) ++queued_embed_;
-    else ++queued_tf_;
+    ++*queued_of(r->kind);
     if (id) *id = r->id;
     work_cv_.notify_all();
     return 0;
   }
 
+  long long* queued_of(int kind) {
+    return kind == kEmbed ? &queued_embed_ : kind == kUmap ? &queued_umap_ : &queued_tf_;
+  }
+  long long* running_of(int kind) {
+    return kind == kEmbed ? &running_embed_ : kind == kUmap ? &running_umap_ : &running_tf_;
+  }
+
+  // the budget the batches share, less what the other batch holds
+  long long budget_left(long long other_used) const {
+    return std::max(1ll, embed_budget_ - other_used);
+  }
+
   // engine-side state changes (under the lock: poll reads them)
   void finish(const RequestPtr& r, bool ok, const std::string& error) {
     std::lock_guard<std::mutex> lk(mu_);
-    if (r->state == kRunning) {
-      if (r->embed) --running_embed_;
-      else --running_tf_;
-    } else if (r->state == kQueued) {
-      if (r->embed) --queued_embed_;
-      else --queued_tf_;
-    }
+    if (r->state == kRunning) --*running_of(r->kind);
+    else if (r->state == kQueued) --*queued_of(r->kind);
     r->state = ok ? kDone : kFailed;
     r->error = error;
     r->finished = Clock::now();
@@ -311,13 +348,8 @@ class Session {
   void started(const RequestPtr& r) {
     std::lock_guard<std::mutex> lk(mu_);
     r->state = kRunning;
-    if (r->embed) {
-      --queued_embed_;
-      ++running_embed_;
-    } else {
-      --queued_tf_;
-      ++running_tf_;
-    }
+    --*queued_of(r->kind);
+    ++*running_of(r->kind);
   }
 
   bool retire_transforms() {
@@ -415,8 +447,10 @@ class Session {
     while (!embed_waiting_.empty()) {
       RequestPtr r = embed_waiting_.front();
       int slot = -1;
+      if (embed_budget_ > 0) tsne_batch_set_budget(batch_, budget_left(umap_batch_used(ubatch_)));
       const int rc = fastembedr_hip_tsne_batch_add(batch_, &r->job, 0, &slot);
-      if (rc == 2 && !embed_running_.empty()) break;  // retried when a job leaves
+      // retried when a job of either batch leaves
+      if (rc == 2 && (!embed_running_.empty() || !umap_running_.empty())) break;
       embed_waiting_.pop_front();
       progressed = true;
       if (rc == 0) {
@@ -439,6 +473,66 @@ class Session {
     return progressed;
   }
 
+  // the same for the UMAP batch
+  bool advance_umap() {
+    bool progressed = false;
+    if (!umap_running_.empty()) {
+      std::vector<int> epochs(max_jobs_), active(max_jobs_);
+      if (fastembedr_hip_umap_batch_status(ubatch_, epochs.data(), active.data())) {
+        fail_running_umap(last_error());
+        return true;
+      }
+      for (auto it = umap_running_.begin(); it != umap_running_.end();) {
+        if (active[it->first]) {
+          ++it;
+          continue;
+        }
+        RequestPtr r = it->second;
+        r->result.resize(2 * static_cast<std::size_t>(r->rows));
+        const bool ok = fastembedr_hip_umap_batch_layout(ubatch_, it->first, r->result.data()) == 0;
+        const std::string error = ok ? std::string() : last_error();
+        (void)fastembedr_hip_umap_batch_remove(ubatch_, it->first);
+        finish(r, ok, error);
+        it = umap_running_.erase(it);
+        progressed = true;
+      }
+    }
+    while (!umap_waiting_.empty()) {
+      RequestPtr r = umap_waiting_.front();
+      int slot = -1;
+      if (embed_budget_ > 0) umap_batch_set_budget(ubatch_, budget_left(tsne_batch_used(batch_)));
+      const int rc = fastembedr_hip_umap_batch_add(ubatch_, &r->ujob, 0, &slot);
+      if (rc == 2 && (!embed_running_.empty() || !umap_running_.empty())) break;
+      umap_waiting_.pop_front();
+      progressed = true;
+      if (rc == 0) {
+        started(r);
+        umap_running_[slot] = r;
+        continue;
+      }
+      finish(r, false, rc == 2 ? "not admissible: " + last_error() : last_error());
+    }
+    if (!umap_running_.empty()) {
+      if (fastembedr_hip_umap_batch_step(ubatch_, kTick) ||
+          check(hipEventRecord(utick_event_, umap_batch_stream(ubatch_)), "hipEventRecord")) {
+        fail_running_umap(last_error());
+        return true;
+      }
+      utick_in_flight_ = true;
+      std::lock_guard<std::mutex> lk(mu_);
+      ++ticks_;
+    }
+    return progressed;
+  }
+
+  void fail_running_umap(const std::string& error) {
+    for (auto& e : umap_running_) {
+      (void)fastembedr_hip_umap_batch_remove(ubatch_, e.first);
+      finish(e.second, false, error);
+    }
+    umap_running_.clear();
+  }
+
   void fail_running_jobs(const std::string& error) {
     for (auto& e : embed_running_) {
       (void)fastembedr_hip_tsne_batch_remove(batch_, e.first);
@@ -453,24 +547,29 @@ class Session {
       bool stopping = false;
       {
         std::unique_lock<std::mutex> lk(mu_);
-        const bool busy = tick_in_flight_ || std::any_of(tf_running_, tf_running_ + kTransformStreams,
-                                                         [](const RequestPtr& r) { return !!r; });
-        const bool waiting = !embed_waiting_.empty() || !tf_waiting_.empty();
+        const bool busy = tick_in_flight_ || utick_in_flight_ ||
+                          std::any_of(tf_running_, tf_running_ + kTransformStreams,
+                                      [](const RequestPtr& r) { return !!r; });
+        const bool waiting = !embed_waiting_.empty() || !tf_waiting_.empty() ||
+                             !umap_waiting_.empty();
         if (stop_ && !busy) break;
         if (!busy && !waiting && incoming_.empty())
           work_cv_.wait(lk, [&] { return stop_ || !incoming_.empty(); });
         while (!incoming_.empty()) {
           RequestPtr r = incoming_.front();
           incoming_.pop_front();
-          (r->embed ? embed_waiting_ : tf_waiting_).push_back(r);
+          (r->kind == kEmbed ? embed_waiting_ : r->kind == kUmap ? umap_waiting_ : tf_waiting_)
+              .push_back(r);
         }
         stopping = stop_;
       }
       if (stopping) {
         // queued requests fail; work on the device is waited for
         for (auto& r : embed_waiting_) finish(r, false, "the session was destroyed");
+        for (auto& r : umap_waiting_) finish(r, false, "the session was destroyed");
         for (auto& r : tf_waiting_) finish(r, false, "the session was destroyed");
         embed_waiting_.clear();
+        umap_waiting_.clear();
         tf_waiting_.clear();
       }
       bool progressed = retire_transforms();
@@ -487,6 +586,21 @@ class Session {
           if (!embed_running_.empty()) fail_running_jobs("the session was destroyed");
         } else {
           progressed |= advance_batch();
+        }
+      }
+      if (utick_in_flight_) {
+        const hipError_t q = hipEventQuery(utick_event_);
+        if (q != hipErrorNotReady) {
+          utick_in_flight_ = false;
+          progressed = true;
+          if (q != hipSuccess) fail_running_umap(std::string("UMAP tick: ") + hipGetErrorString(q));
+        }
+      }
+      if (!utick_in_flight_) {
+        if (stopping) {
+          if (!umap_running_.empty()) fail_running_umap("the session was destroyed");
+        } else {
+          progressed |= advance_umap();
         }
       }
       if (!stopping) progressed |= admit_transforms();
@@ -512,8 +626,12 @@ class Session {
   RequestPtr tf_running_[kTransformStreams];
   hipEvent_t tick_event_ = nullptr;
   bool tick_in_flight_ = false;
-  std::deque<RequestPtr> embed_waiting_, tf_waiting_;
-  std::map<int, RequestPtr> embed_running_;
+  fastembedr_hip_umap_batch* ubatch_ = nullptr;
+  hipEvent_t utick_event_ = nullptr;
+  bool utick_in_flight_ = false;
+  long long embed_budget_ = 0;  // both batches together (<= 0: none)
+  std::deque<RequestPtr> embed_waiting_, tf_waiting_, umap_waiting_;
+  std::map<int, RequestPtr> embed_running_, umap_running_;
   std::thread engine_;
 
   std::mutex mu_;
@@ -524,6 +642,7 @@ class Session {
   long long next_id_ = 1;
   long long submitted_ = 0, completed_ = 0, failed_ = 0, ticks_ = 0;
   long long queued_embed_ = 0, queued_tf_ = 0, running_embed_ = 0, running_tf_ = 0;
+  long long queued_umap_ = 0, running_umap_ = 0;
 };
 
 }  // namespace fe
@@ -573,6 +692,13 @@ int fastembedr_hip_session_submit_transform(
       early_exaggeration_iter, learning_rate, early_exaggeration, exaggeration, initial_momentum,
       final_momentum, max_grad_norm, max_step_norm, n_negatives, exact_repulsion_threshold, seed,
       request);
+}
+
+int fastembedr_hip_session_submit_umap(fastembedr_hip_session* session,
+                                       const fastembedr_hip_umap_job* job, long long* request) {
+  fe::clear_error();
+  if (!session || !job) return fe::fail("null session or job");
+  return session->impl.submit_umap(*job, request);
 }
 
 int fastembedr_hip_session_poll(fastembedr_hip_session* session, long long request, int wait_ms,

@@ -56,6 +56,17 @@ int transform_task_result(const TransformTask& t, const void* pinned, const floa
 
 // The stream a batch enqueues its iterations on (fastembedr_hip_tsne_batch_step).
 hipStream_t tsne_batch_stream(const fastembedr_hip_tsne_batch* batch);
+// A batch's budget (<= 0: none) and the device bytes its jobs hold: the
+// session moves budget between its t-SNE and UMAP batches as jobs come and go.
+void tsne_batch_set_budget(fastembedr_hip_tsne_batch* batch, long long budget);
+long long tsne_batch_used(const fastembedr_hip_tsne_batch* batch);
+
+hipStream_t umap_batch_stream(const fastembedr_hip_umap_batch* batch);
+void umap_batch_set_budget(fastembedr_hip_umap_batch* batch, long long budget);
+long long umap_batch_used(const fastembedr_hip_umap_batch* batch);
+// Checks a UMAP job as fastembedr_hip_umap_batch_add does before it builds
+// anything.
+int umap_job_validate(const fastembedr_hip_umap_job& j);
 
 // Checks a t-SNE job as fastembedr_hip_tsne_batch_add does before it builds
 // anything (shape, schedule, learning rates).

@@ -1991,6 +1991,14 @@ hipStream_t fe::tsne_batch_stream(const fastembedr_hip_tsne_batch* batch) {
   return batch ? batch->impl.stream.get() : nullptr;
 }
 
+void fe::tsne_batch_set_budget(fastembedr_hip_tsne_batch* batch, long long budget) {
+  if (batch) batch->impl.budget = budget;
+}
+
+long long fe::tsne_batch_used(const fastembedr_hip_tsne_batch* batch) {
+  return batch ? batch->impl.used : 0;
+}
+
 extern "C" {
 
 int fastembedr_hip_tsne_batch_create(int max_jobs, long long hbm_budget_bytes,

@@ -28,6 +28,7 @@
 
 #include "../../include/fastembedr_hip.h"
 #include "fe_common.hpp"
+#include "serve_iface.hpp"
 
 namespace fe {
 
@@ -661,6 +662,19 @@ __global__ void umap_batch_advance(UmapJobDesc* jobs, const int* __restrict__ sl
 
 }  // namespace
 
+int umap_job_validate(const fastembedr_hip_umap_job& j) {
+  if (!j.indices || !j.distances || !j.init) return fail("null UMAP job pointer");
+  if (j.n < 2 || j.k_total < 1 || j.n_neighbors < 1 || j.n_neighbors > j.k_total ||
+      j.n_epochs < 1 || j.negative_sample_rate < 0 || !(j.learning_rate > 0.0f) ||
+      !(j.repulsion_strength > 0.0f))
+    return fail("invalid HIP UMAP job parameters");
+  if (j.optimizer_mode != 0 && j.optimizer_mode != 1)
+    return fail("optimizer_mode must be 0 (atomic) or 1 (delta)");
+  for (size_t t = 0; t < 2 * static_cast<size_t>(j.n); ++t)
+    if (!std::isfinite(j.init[t])) return fail("UMAP init must contain only finite coordinates.");
+  return 0;
+}
+
 // Device bytes a job of this shape holds while resident: the graph at its
 // worst case of 2 n k entries (COO heads, tails, weights, periods and the
 // CSR), the layout and mode 1's delta; setup adds the graph build's peak.
@@ -724,15 +738,7 @@ class UmapBatch {
 
   // 0 admitted, 1 error, 2 not admitted now (budget or no free slot)
   int add(const fastembedr_hip_umap_job& j, bool device_inputs, int* slot_out) {
-    if (!j.indices || !j.distances || !j.init) return fail("null UMAP job pointer");
-    if (j.n < 2 || j.k_total < 1 || j.n_neighbors < 1 || j.n_neighbors > j.k_total ||
-        j.n_epochs < 1 || j.negative_sample_rate < 0 || !(j.learning_rate > 0.0f) ||
-        !(j.repulsion_strength > 0.0f))
-      return fail("invalid HIP UMAP job parameters");
-    if (j.optimizer_mode != 0 && j.optimizer_mode != 1)
-      return fail("optimizer_mode must be 0 (atomic) or 1 (delta)");
-    for (size_t t = 0; t < 2 * static_cast<size_t>(j.n); ++t)
-      if (!std::isfinite(j.init[t])) return fail("UMAP init must contain only finite coordinates.");
+    if (umap_job_validate(j)) return 1;
     int slot = -1;
     for (int q = 0; q < max_jobs; ++q)
       if (!slots[q]) {
@@ -1022,6 +1028,18 @@ int fastembedr_hip_umap_schedule(const int* heads, const int* tails,
 struct fastembedr_hip_umap_batch {
   fe::UmapBatch impl;
 };
+
+hipStream_t fe::umap_batch_stream(const fastembedr_hip_umap_batch* batch) {
+  return batch ? batch->impl.stream.get() : nullptr;
+}
+
+void fe::umap_batch_set_budget(fastembedr_hip_umap_batch* batch, long long budget) {
+  if (batch) batch->impl.budget = budget;
+}
+
+long long fe::umap_batch_used(const fastembedr_hip_umap_batch* batch) {
+  return batch ? batch->impl.used : 0;
+}
 
 extern "C" {
 

@@ -10,6 +10,9 @@ requests:
   resident jobs by 25 iterations, so a job that arrives while others run
   starts at the next tick and its work shares their kernels.  Finished jobs
   are read back and removed, which frees their budget for waiting ones.
+* ``umap``: a full UMAP job; its fuzzy graph joins the session's UMAP batch,
+  which advances every resident job by 25 epochs per tick beside the t-SNE
+  ticks (the two batches share the budget).
 * ``transform``: placement of query points into the resident reference
   layout, enqueued on its own stream with workspace from the session's pool
   (its share of the budget), so it overlaps the tick on the device and never
@@ -114,6 +117,36 @@ class Session:
         self._rows[rid.value] = nq
         return rid.value
 
+    def submit_umap(self, indices, distances, init, n_epochs, n_neighbors=None,
+                    negative_sample_rate=5, learning_rate=1.0, a=None, b=None,
+                    repulsion_strength=1.0, seed=0, optimizer_mode=1, index_offset=None):
+        """Queue a UMAP job (host n x k KNN, its first ``n_neighbors``
+        columns; the arguments of ``UmapBatch.add``); returns the id."""
+        from .umap import A_MIN_DIST_001, B_MIN_DIST_001, UmapJobStruct
+
+        idx = np.asarray(indices)
+        dist = np.asarray(distances)
+        if idx.ndim != 2 or idx.shape != dist.shape:
+            raise ValueError("indices and distances must have the same dimensions")
+        n, k_total = idx.shape
+        ic = np.asfortranarray(idx, dtype=np.int32)
+        dc = np.asfortranarray(np.asarray(dist, np.float64).astype(np.float32))
+        y = np.ascontiguousarray(init, dtype=np.float32)
+        if y.shape != (n, 2):
+            raise ValueError("`init` has the wrong shape.")
+        j = UmapJobStruct(ic.ctypes.data, dc.ctypes.data, n, k_total,
+                          k_total if n_neighbors is None else int(n_neighbors),
+                          -1 if index_offset is None else int(index_offset), y.ctypes.data,
+                          int(n_epochs), int(negative_sample_rate), float(learning_rate),
+                          float(A_MIN_DIST_001 if a is None else a),
+                          float(B_MIN_DIST_001 if b is None else b), float(repulsion_strength),
+                          int(seed) & 0xFFFFFFFF, int(optimizer_mode))
+        rid = ctypes.c_longlong(0)
+        check(_lib.lib().fastembedr_hip_session_submit_umap(self._h, ctypes.byref(j),
+                                                            ctypes.byref(rid)), "session submit")
+        self._rows[rid.value] = n
+        return rid.value
+
     def poll(self, rid, wait_ms=0):
         """(state, layout or None, latency_ms, error or None).  A done or
         failed request is released by this call."""
@@ -145,11 +178,12 @@ class Session:
                 raise TimeoutError(f"request {rid} still in state {s}")
 
     def stats(self):
-        s = np.zeros(10, np.int64)
+        s = np.zeros(12, np.int64)
         check(_lib.lib().fastembedr_hip_session_stats(self._h, s.ctypes.data_as(_lib._llp)),
               "session stats")
         keys = ("submitted", "completed", "failed", "queued_embed", "queued_transform",
-                "running_embed", "running_transform", "pool_bytes_used", "ticks", "pool_bytes")
+                "running_embed", "running_transform", "pool_bytes_used", "ticks", "pool_bytes",
+                "queued_umap", "running_umap")
         return dict(zip(keys, (int(v) for v in s)))
 
     def close(self):
@@ -167,7 +201,7 @@ class Session:
 @dataclass
 class Request:
     rid: int
-    kind: str            # "embed" or "transform"
+    kind: str            # "embed", "umap" or "transform"
     arrival: float       # seconds on the loop's clock
     payload: dict
     started: float = -1.0
@@ -207,6 +241,8 @@ class Server:
     def _submit(self, r):
         if r.kind == "embed":
             return self.session.submit_embed(**r.payload)
+        if r.kind == "umap":
+            return self.session.submit_umap(**r.payload)
         return self.session.submit_transform(**r.payload)
 
     def run(self, requests, realtime=True):

@@ -279,6 +279,17 @@ int fastembedr_hip_session_create(const float* reference_layout, int n_reference
 int fastembedr_hip_session_submit_embed(fastembedr_hip_session* session,
                                         const fastembedr_hip_tsne_job* job, long long* request);
 
+/* Queues a UMAP job (host inputs, fastembedr_hip_umap_batch_add's rules;
+ * the job struct is declared with the UMAP batch below).  Its graph is built
+ * when it joins the session's UMAP batch, which advances every resident job
+ * by 25 epochs per tick beside the t-SNE ticks; the two batches share the
+ * budget left after the transform pool.  Returns 1 with the message when the
+ * job is invalid. */
+struct fastembedr_hip_umap_job;
+int fastembedr_hip_session_submit_umap(fastembedr_hip_session* session,
+                                       const struct fastembedr_hip_umap_job* job,
+                                       long long* request);
+
 /* Queues a transform against the session's reference: the arguments of
  * fastembedr_hip_transform_reference_run.  Returns 1 with the message when
  * the request is invalid or larger than the transform pool. */
@@ -299,9 +310,10 @@ int fastembedr_hip_session_submit_transform(
 int fastembedr_hip_session_poll(fastembedr_hip_session* session, long long request, int wait_ms,
                                 int* state, float* out, double* latency_ms);
 
-/* stats[0..9]: submitted, completed, failed, queued embeds, queued
+/* stats[0..11]: submitted, completed, failed, queued embeds, queued
  * transforms, running embeds, running transforms, transform pool bytes in
- * use, engine ticks, transform pool bytes. */
+ * use, engine ticks (t-SNE and UMAP), transform pool bytes, queued UMAP
+ * jobs, running UMAP jobs. */
 int fastembedr_hip_session_stats(fastembedr_hip_session* session, long long* stats);
 
 /* Queued requests fail, work on the device is waited for, everything is

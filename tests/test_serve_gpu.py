@@ -13,7 +13,15 @@ def _trace(rng, ref_x, n_req=10):
 
     reqs = []
     for rid in range(n_req):
-        if rid % 3 == 0:
+        if rid % 4 == 2:
+            n = int(rng.integers(600, 1800))
+            x, _ = gaussian_mixture(n, 8, 4, seed=200 + rid)
+            idx, dist = exact_knn(x, 15)
+            payload = dict(indices=idx, distances=dist.astype(np.float32),
+                           init=rng.uniform(-10, 10, size=(n, 2)).astype(np.float32),
+                           n_epochs=40 + 9 * rid, seed=rid)
+            reqs.append(Request(rid, "umap", 0.001 * rid, payload))
+        elif rid % 3 == 0:
             n = int(rng.integers(800, 2000))
             x, _ = gaussian_mixture(n, 8, 4, seed=100 + rid)
             idx, dist = exact_knn(x, 30)
@@ -37,11 +45,12 @@ def test_session_results_equal_alone_and_plain_path():
     import fastembedr_amd as fe
     from fastembedr_amd.serve import Request, Server
     from fastembedr_amd.transform import TransformReference
+    from fastembedr_amd.umap import FuzzyGraph
 
     rng = np.random.default_rng(0)
     ref_x, _ = gaussian_mixture(3000, 8, 5, seed=40)
     ref_y = rng.normal(size=(3000, 2)).astype(np.float32) * 5
-    reqs = _trace(rng, ref_x)
+    reqs = _trace(rng, ref_x, n_req=12)
     srv = Server(reference_layout=ref_y, max_jobs=4, transform_pool_bytes=64 << 20)
     print("session: mixed trace of", len(reqs), "requests", flush=True)
     stats = srv.run(reqs, realtime=False)
@@ -51,6 +60,7 @@ def test_session_results_equal_alone_and_plain_path():
     st = srv.session.stats()
     assert st["completed"] == len(reqs) and st["ticks"] > 0
     assert st["queued_embed"] == st["running_embed"] == 0 and st["pool_bytes_used"] == 0
+    assert st["queued_umap"] == st["running_umap"] == 0
     mixed = {r.rid: r.result for r in reqs}
     tref = TransformReference(ref_y)
     for r in reqs:
@@ -59,7 +69,11 @@ def test_session_results_equal_alone_and_plain_path():
         assert s2.completed == 1
         assert np.array_equal(alone.result, mixed[r.rid]), r.rid
         p = r.payload
-        if r.kind == "embed":
+        if r.kind == "umap":
+            g = FuzzyGraph(p["indices"], p["distances"])
+            plain = g.optimize(p["init"], p["n_epochs"], seed=p["seed"])
+            g.close()
+        elif r.kind == "embed":
             plain = fe.tsne_knn(p["indices"], p["distances"], perplexity=p["perplexity"],
                                 Y_init=p["Y_init"], grid_size=p["grid_size"],
                                 early_exaggeration_iter=p["early_exaggeration_iter"],
