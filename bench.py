@@ -223,16 +223,21 @@ def bench_batched_tuned(fe, workloads, specs, jobs, alone, t_seq, k, check_jobs=
                 "knn_preservation_min": round(float(dq[:, 1].min()), 5)}}
 
 
-def bench_umap_batched(workloads, count, k=15, epochs=200):
-    """Batched UMAP: `count` jobs of 2k-100k points (log-uniform), each a
+def bench_umap_batched(workloads, count, k=15, epochs=200, n_lo=2000, n_hi=100000,
+                       forms=("auto",)):
+    """Batched UMAP: `count` jobs of n_lo-n_hi points (log-uniform), each a
     5-50-cluster mixture in 30-50 dimensions, exact k = 15 KNN, the default
     fuzzy graph, `epochs` epochs in optimizer_mode 1, uniform [-10, 10]
     starts.  Times every job alone (graph build + epochs, one after another)
-    and all jobs in one UMAP batch, from host KNN to every layout on the
-    host; counts the layouts that are the same bits both ways."""
+    and all jobs in one UMAP batch per form (auto: the library's choice;
+    persistent / launches: small jobs in the persistent kernel or not), from
+    host KNN to every layout on the host; counts the layouts that are the
+    same bits both ways."""
+    import os
+
     from fastembedr_amd.umap import FuzzyGraph, UmapBatch
 
-    specs = workloads.batch_job_specs(count, seed=331)
+    specs = workloads.batch_job_specs(count, seed=331, n_lo=n_lo, n_hi=n_hi)
     jobs = []
     t0 = time.perf_counter()
     for sp in specs:
@@ -250,25 +255,37 @@ def bench_umap_batched(workloads, count, k=15, epochs=200):
         alone.append(g.optimize(y0, epochs, seed=seed, optimizer_mode=1))
         g.close()
     t_seq = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    b = UmapBatch(max_jobs=count)
-    slots = [b.add(idx, dist, y0, n_epochs=epochs, seed=seed) for idx, dist, y0, seed in jobs]
-    b.run_to_completion(chunk=50)
-    outs = [b.layout(sl) for sl in slots]
-    t_batch = time.perf_counter() - t0
-    b.close()
-    same = sum(int(np.array_equal(a, o)) for a, o in zip(alone, outs))
-    log(f"[umap batched] sequential {t_seq:.2f}s, batched {t_batch:.2f}s, "
-        f"{same}/{count} identical")
-    return {"workload": f"{count} UMAP jobs, n log-uniform in [2k, 100k] ({points} points), "
-                        f"5-50-cluster mixtures in 30-50 dims, exact k={k} fuzzy graph, "
-                        f"{epochs} epochs, optimizer_mode 1",
-            "jobs": count, "points": points,
-            "batched_jobs_per_s": round(count / t_batch, 3),
-            "sequential_jobs_per_s": round(count / t_seq, 3),
-            "speedup_vs_sequential": round(t_seq / t_batch, 3),
-            "batched_s": round(t_batch, 3), "sequential_s": round(t_seq, 3),
-            "identical_to_alone": same}
+    res = {"workload": f"{count} UMAP jobs, n log-uniform in [{n_lo}, {n_hi}] ({points} "
+                       f"points), 5-50-cluster mixtures in 30-50 dims, exact k={k} fuzzy graph, "
+                       f"{epochs} epochs, optimizer_mode 1",
+           "jobs": count, "points": points, "sequential_jobs_per_s": round(count / t_seq, 3),
+           "sequential_s": round(t_seq, 3)}
+    env_of = {"auto": None, "persistent": "1", "launches": "0"}
+    for form in forms:
+        old_env = os.environ.pop("FASTEMBEDR_HIP_UMAP_PERSISTENT", None)
+        if env_of[form] is not None:
+            os.environ["FASTEMBEDR_HIP_UMAP_PERSISTENT"] = env_of[form]
+        try:
+            t0 = time.perf_counter()
+            b = UmapBatch(max_jobs=count)
+            slots = [b.add(idx, dist, y0, n_epochs=epochs, seed=seed)
+                     for idx, dist, y0, seed in jobs]
+            b.run_to_completion(chunk=50)
+            outs = [b.layout(sl) for sl in slots]
+            t_batch = time.perf_counter() - t0
+            b.close()
+        finally:
+            os.environ.pop("FASTEMBEDR_HIP_UMAP_PERSISTENT", None)
+            if old_env is not None:
+                os.environ["FASTEMBEDR_HIP_UMAP_PERSISTENT"] = old_env
+        same = sum(int(np.array_equal(a, o)) for a, o in zip(alone, outs))
+        log(f"[umap batched] {count} jobs [{n_lo}, {n_hi}]: sequential {t_seq:.2f}s, batched "
+            f"({form}) {t_batch:.2f}s, {same}/{count} identical")
+        key = "" if form == "auto" else f"_{form}"
+        res.update({f"batched_jobs_per_s{key}": round(count / t_batch, 3),
+                    f"speedup_vs_sequential{key}": round(t_seq / t_batch, 3),
+                    f"batched_s{key}": round(t_batch, 3), f"identical_to_alone{key}": same})
+    return res
 
 
 def bench_served(fe, workloads, n_requests, k=90):
@@ -816,8 +833,13 @@ def main():
         umap_gate = umap_quality_gate(fe, workloads, args.umap_quality_n)
     batched = bench_batched(fe, workloads, args.batched_jobs) if args.batched_jobs > 0 else None
     torch.cuda.empty_cache()
-    umap_batched = (bench_umap_batched(workloads, args.umap_batched_jobs)
-                    if args.umap_batched_jobs > 0 else None)
+    umap_batched = None
+    if args.umap_batched_jobs > 0:
+        umap_batched = bench_umap_batched(workloads, args.umap_batched_jobs)
+        # many small jobs: the persistent kernel against the job-table launches
+        umap_batched["small_jobs"] = bench_umap_batched(
+            workloads, 8 * args.umap_batched_jobs, n_lo=1000, n_hi=6000,
+            forms=("persistent", "launches"))
     served = bench_served(fe, workloads, args.served_requests) if args.served_requests > 0 else None
 
     out = {

@@ -133,6 +133,86 @@ __device__ __forceinline__ unsigned long long shfl_up_u64(unsigned long long v, 
          static_cast<unsigned>(lo);
 }
 
+// Epoch-start coordinates of point i from the layout in memory or in LDS.
+struct GlobalPos {
+  const float* __restrict__ y;
+  __device__ __forceinline__ float2 operator()(size_t i) const {
+    return make_float2(y[2 * i], y[2 * i + 1]);
+  }
+};
+struct LdsPos {
+  const float* y;
+  __device__ __forceinline__ float2 operator()(size_t i) const {
+    return make_float2(y[2 * i], y[2 * i + 1]);
+  }
+};
+
+// The head's share of symmetric-graph edge (h, t) in a mode 1 epoch, as
+// integers of the fixed-point delta: twice its attraction (the reverse edge's
+// update of t is exactly this edge's tail update, see umap_epoch_grouped)
+// times the positive samples, plus its repulsions.  Nothing when the edge
+// takes no positive sample this epoch.
+template <class Pos>
+__device__ __forceinline__ void sym_edge_head(Pos pos, int h, int t, float per, const UmapParams& p,
+                                              unsigned epoch, float alpha, long long* hx,
+                                              long long* hy) {
+  constexpr float eps = 1.1920928955078125e-7f;
+  const int npos = n_positive(per, epoch);
+  if (npos <= 0) return;
+  const float2 hp = pos(static_cast<size_t>(h));
+  {
+    const float2 tp = pos(static_cast<size_t>(t));
+    const float dx = hp.x - tp.x;
+    const float dy = hp.y - tp.y;
+    const float d2 = fmaxf(eps, dx * dx + dy * dy);
+    const float c = attract(d2, p);
+    const float gx = clip4(c * dx) * alpha, gy = clip4(c * dy) * alpha;
+    *hx += 2 * (npos * fixed(gx));
+    *hy += 2 * (npos * fixed(gy));
+  }
+  const int nneg = n_negative(per, p.nsr, epoch);
+  for (int s = 0; s < nneg; ++s) {
+    const unsigned v = negative_vertex(static_cast<unsigned>(p.n), p.seed, epoch,
+                                       static_cast<unsigned>(h), static_cast<unsigned>(t),
+                                       static_cast<unsigned>(s));
+    if (static_cast<int>(v) == h || static_cast<int>(v) == t) continue;
+    const float2 vp = pos(static_cast<size_t>(v));
+    const float dx = hp.x - vp.x;
+    const float dy = hp.y - vp.y;
+    const float d2 = fmaxf(eps, dx * dx + dy * dy);
+    const float c = repel(d2, p);
+    *hx += fixed(clip4(c * dx) * alpha);
+    *hy += fixed(clip4(c * dy) * alpha);
+  }
+}
+
+// Inclusive sums over the runs of equal heads in a wave (COO heads come in
+// row order); `last` marks the lane that ends its run.
+__device__ __forceinline__ void head_run_sums(int h, long long hx, long long hy,
+                                              unsigned long long* ux, unsigned long long* uy,
+                                              bool* last) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long upto_lane = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const int hprev = __shfl_up(h, 1, 64);
+  const bool start = lane == 0 || hprev != h;
+  const unsigned long long starts = __ballot(start) & upto_lane;
+  const int lim = 63 - __clzll(starts);  // first lane of this lane's run
+  unsigned long long x = static_cast<unsigned long long>(hx);
+  unsigned long long y = static_cast<unsigned long long>(hy);
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const unsigned long long ox = shfl_up_u64(x, d), oy = shfl_up_u64(y, d);
+    if (lane - d >= lim) {
+      x += ox;
+      y += oy;
+    }
+  }
+  const int hnext = __shfl_down(h, 1, 64);
+  *last = lane == 63 || hnext != h;
+  *ux = x;
+  *uy = y;
+}
+
 // optimizer_mode 1 with the head updates summed before they reach memory.
 // The epoch reads its start layout, so an edge's positive samples are all the
 // same update (npos times the fixed-point value); the head's share of an edge
@@ -166,7 +246,9 @@ __device__ __forceinline__ void epoch_grouped_body(unsigned long long* delta,
     if (e < n_edges) {
       h = heads[e];
       const int t = tails[e];
-      if (h >= 0 && h < p.n && t >= 0 && t < p.n && h != t) {
+      if (SYM && h >= 0 && h < p.n && t >= 0 && t < p.n && h != t) {
+        sym_edge_head(GlobalPos{layout}, h, t, period[e], p, epoch, alpha, &hx, &hy);
+      } else if (h >= 0 && h < p.n && t >= 0 && t < p.n && h != t) {
         const float per = period[e];
         const int npos = n_positive(per, epoch);
         if (npos > 0) {
@@ -179,15 +261,10 @@ __device__ __forceinline__ void epoch_grouped_body(unsigned long long* delta,
             const float c = attract(d2, p);
             const float gx = clip4(c * dx) * alpha, gy = clip4(c * dy) * alpha;
             const long long fx = npos * fixed(gx), fy = npos * fixed(gy);
-            if constexpr (SYM) {
-              hx += 2 * fx;
-              hy += 2 * fy;
-            } else {
-              hx += fx;
-              hy += fy;
-              atomicAdd(delta + tb, static_cast<unsigned long long>(-fx));
-              atomicAdd(delta + tb + 1, static_cast<unsigned long long>(-fy));
-            }
+            hx += fx;
+            hy += fy;
+            atomicAdd(delta + tb, static_cast<unsigned long long>(-fx));
+            atomicAdd(delta + tb + 1, static_cast<unsigned long long>(-fy));
           }
           const int nneg = n_negative(per, p.nsr, epoch);
           for (int s = 0; s < nneg; ++s) {
@@ -650,6 +727,87 @@ __global__ __launch_bounds__(kUThreads) void umap_batch_apply(UmapJobDesc* jobs,
   if (i < 2 * J->n) apply_body<MODE>(J->layout, J->delta, i);
 }
 
+// Persistent form for small mode 1 jobs: a fixed grid of workgroups pulls
+// jobs from a queue (one atomic per job), and the workgroup that takes a job
+// runs `epochs` of its epochs start to end with the layout and the delta in
+// LDS: the negative samples' gathers and the head sums stay on the CU and no
+// launch separates the epochs.  The arithmetic is sym_edge_head's and
+// apply_body's, with integer sums, so a job gives the bits of the job-table
+// path and of its graph optimised alone.  Every workgroup exits once the
+// queue is empty.
+constexpr int kPersistThreads = 1024;
+constexpr int kPersistMaxPoints = 6144;  // 16 + 8 bytes per point: 144 KB of LDS
+constexpr size_t kPersistLds = 24ull * kPersistMaxPoints;
+
+__global__ __launch_bounds__(kPersistThreads) void umap_batch_persistent(
+    UmapJobDesc* jobs, const int* __restrict__ queue, int queue_len, unsigned* head, int epochs) {
+  extern __shared__ unsigned long long pers_lds[];
+  unsigned long long* d = pers_lds;                                          // 2 per point
+  float* y = reinterpret_cast<float*>(pers_lds + 2 * kPersistMaxPoints);    // 2 per point
+  __shared__ int s_q;
+  for (;;) {
+    __syncthreads();  // the previous job is written back and s_q read
+    if (threadIdx.x == 0) s_q = static_cast<int>(atomicAdd(head, 1u));
+    __syncthreads();
+    const int q = s_q;
+    if (q >= queue_len) return;
+    UmapJobDesc* J = jobs + queue[q];
+    if (!J->active) continue;
+    const int n = J->n, m = 2 * n, n_edges = J->n_edges;
+    const UmapParams p = J->p;
+    for (int i = static_cast<int>(threadIdx.x); i < m; i += kPersistThreads) {
+      y[i] = J->layout[i];
+      d[i] = 0ull;
+    }
+    const int e0 = J->epoch, e1 = min(p.n_epochs, e0 + epochs);
+    __syncthreads();
+    for (int ep = e0; ep < e1; ++ep) {
+      const unsigned epoch = static_cast<unsigned>(ep);
+      const float alpha = p.lr * (1.0f - __fdiv_rn(static_cast<float>(epoch),
+                                                   fmaxf(1.0f, static_cast<float>(p.n_epochs))));
+      for (int base = static_cast<int>(threadIdx.x & ~63); base < n_edges;
+           base += kPersistThreads) {
+        const int e = base + static_cast<int>(threadIdx.x & 63);
+        int h = -1;
+        long long hx = 0, hy = 0;
+        if (e < n_edges) {
+          h = J->heads[e];
+          const int t = J->tails[e];
+          if (h >= 0 && h < n && t >= 0 && t < n && h != t)
+            sym_edge_head(LdsPos{y}, h, t, J->period[e], p, epoch, alpha, &hx, &hy);
+        }
+        unsigned long long ux, uy;
+        bool last;
+        head_run_sums(h, hx, hy, &ux, &uy, &last);
+        if (last && h >= 0 && h < n && (ux != 0ull || uy != 0ull)) {
+          atomicAdd(d + 2 * h, ux);
+          atomicAdd(d + 2 * h + 1, uy);
+        }
+      }
+      __syncthreads();
+      for (int i = static_cast<int>(threadIdx.x); i < m; i += kPersistThreads) apply_body<1>(y, d, i);
+      __syncthreads();
+    }
+    for (int i = static_cast<int>(threadIdx.x); i < m; i += kPersistThreads) J->layout[i] = y[i];
+    if (threadIdx.x == 0) {
+      J->epoch = e1;
+      if (e1 >= p.n_epochs) J->active = 0;
+    }
+  }
+}
+
+__global__ void umap_queue_reset(unsigned* head) { *head = 0u; }
+
+// compute units of the current device (256 on MI355X)
+int compute_units() {
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+      prop.multiProcessorCount > 0)
+    return prop.multiProcessorCount;
+  return 256;
+}
+
 // one thread per job: the epoch counter, then the job's activity
 __global__ void umap_batch_advance(UmapJobDesc* jobs, const int* __restrict__ slot, int m) {
   const int e = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
@@ -710,9 +868,28 @@ class UmapBatch {
   };
   std::vector<Launch> launches;
   std::unique_ptr<GraphExec> graph;
+  // FASTEMBEDR_HIP_UMAP_PERSISTENT=1 runs the small mode 1 jobs in the
+  // persistent kernel, on `side` beside the launches of the other jobs.  Off
+  // by default: one workgroup per job is compute-bound on its CU, and on
+  // MI355X the captured job-table launches were faster even when the small
+  // jobs fill the GPU (512 jobs of 1k-6k points, 200 epochs: 0.50 s against
+  // 0.59 s, the same bits); =auto uses it when there are at least as many
+  // small jobs as compute units.
+  int persistent_mode = 0;  // 0 never, 1 always, -1 when the small jobs fill the GPU
+  bool use_persistent = false;
+  int pers_off = 0, pers_len = 0;
+  unsigned* d_head = nullptr;
+  Stream side;
+  hipEvent_t fork = nullptr, join = nullptr;
+
+  static bool small_job(const UmapJobDesc& d) { return d.mode == 1 && d.n <= kPersistMaxPoints; }
+  bool is_persistent(int q) const { return use_persistent && small_job(desc[q]); }
 
   ~UmapBatch() {
     if (stream.get()) (void)hipStreamSynchronize(stream.get());
+    if (side.get()) (void)hipStreamSynchronize(side.get());
+    if (fork) (void)hipEventDestroy(fork);
+    if (join) (void)hipEventDestroy(join);
     for (auto& sl : slots)
       if (sl && sl->graph) fastembedr_hip_umap_graph_destroy(sl->graph);
   }
@@ -725,13 +902,21 @@ class UmapBatch {
     slots.resize(max_jobs);
     desc.resize(max_jobs);
     std::memset(desc.data(), 0, desc.size() * sizeof(UmapJobDesc));
-    tab_cap = 6 * (2 * max_jobs + 1) + 16;
+    tab_cap = 7 * (2 * max_jobs + 1) + 16;
+    const char* env = std::getenv("FASTEMBEDR_HIP_UMAP_PERSISTENT");
+    if (env && env[0] == '1') persistent_mode = 1;
+    if (env && std::strcmp(env, "auto") == 0) persistent_mode = -1;
+    if (side.init()) return 1;
+    FE_CHECK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    FE_CHECK(hipEventCreateWithFlags(&join, hipEventDisableTiming));
     ArenaPlan p;
     p.add<UmapJobDesc>(max_jobs);
     p.add<int>(tab_cap);
+    p.add<unsigned>(1);
     if (dev.init(p.bytes, "UMAP batch tables")) return 1;
     d_desc = dev.take<UmapJobDesc>(max_jobs);
     d_tab = dev.take<int>(tab_cap);
+    d_head = dev.take<unsigned>(1);
     FE_CHECK(hipMemsetAsync(d_desc, 0, max_jobs * sizeof(UmapJobDesc), stream.get()));
     return check(hipStreamSynchronize(stream.get()), "UMAP batch init");
   }
@@ -832,6 +1017,10 @@ class UmapBatch {
   int rebuild() {
     launches.clear();
     std::vector<int> tab;
+    int small = 0;
+    for (int q = 0; q < max_jobs; ++q)
+      if (slots[q] && small_job(desc[q])) ++small;
+    use_persistent = persistent_mode == 1 || (persistent_mode < 0 && small >= compute_units());
     auto add_launch = [&](int kind, int mode, const std::vector<std::pair<int, int>>& ent) {
       if (ent.empty()) return;
       Launch L{kind, mode, static_cast<int>(tab.size()), static_cast<int>(ent.size()), 0};
@@ -848,7 +1037,7 @@ class UmapBatch {
     for (int mode = 0; mode <= 1; ++mode) {
       std::vector<std::pair<int, int>> ent;
       for (int q = 0; q < max_jobs; ++q)
-        if (slots[q] && desc[q].mode == mode)
+        if (slots[q] && desc[q].mode == mode && !is_persistent(q))
           ent.push_back({q, static_cast<int>(std::min<long long>(
                                 (desc[q].n_edges + kUThreads - 1) / kUThreads, 8192))});
       add_launch(0, mode, ent);
@@ -856,14 +1045,22 @@ class UmapBatch {
     for (int mode = 0; mode <= 1; ++mode) {
       std::vector<std::pair<int, int>> ent;
       for (int q = 0; q < max_jobs; ++q)
-        if (slots[q] && desc[q].mode == mode)
+        if (slots[q] && desc[q].mode == mode && !is_persistent(q))
           ent.push_back({q, (2 * desc[q].n + kUThreads - 1) / kUThreads});
       add_launch(1, mode, ent);
     }
     std::vector<std::pair<int, int>> all;
     for (int q = 0; q < max_jobs; ++q)
-      if (slots[q]) all.push_back({q, 1});
+      if (slots[q] && !is_persistent(q)) all.push_back({q, 1});
     add_launch(2, 0, all);
+    // the persistent kernel's queue: small jobs first in slot order
+    pers_off = static_cast<int>(tab.size());
+    pers_len = 0;
+    for (int q = 0; q < max_jobs; ++q)
+      if (slots[q] && is_persistent(q)) {
+        tab.push_back(q);
+        ++pers_len;
+      }
     if (static_cast<int>(tab.size()) > tab_cap) return fail("UMAP batch table overflow");
     if (!tab.empty())
       FE_CHECK(hipMemcpy(d_tab, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -896,9 +1093,35 @@ class UmapBatch {
       graph.reset();
       if (rebuild()) return 1;
     }
-    if (launches.empty()) return 0;
-    constexpr int kChunk = 25;
     hipStream_t s = stream.get();
+    if (pers_len > 0 && epochs > 0) {
+      static bool attr_done = false;
+      if (!attr_done) {
+        FE_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&umap_batch_persistent),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     static_cast<int>(kPersistLds)));
+        attr_done = true;
+      }
+      hipStream_t a = side.get();
+      FE_CHECK(hipEventRecord(fork, s));
+      FE_CHECK(hipStreamWaitEvent(a, fork, 0));
+      umap_queue_reset<<<1, 1, 0, a>>>(d_head);
+      FE_LAUNCH_CHECK("umap_queue_reset");
+      umap_batch_persistent<<<std::min(pers_len, compute_units()), kPersistThreads, kPersistLds,
+                              a>>>(d_desc, d_tab + pers_off, pers_len, d_head, epochs);
+      FE_LAUNCH_CHECK("umap_batch_persistent");
+      FE_CHECK(hipEventRecord(join, a));
+    }
+    if (!launches.empty() && enqueue_table(epochs)) return 1;
+    // the batch's stream passes only once the persistent jobs are done too
+    if (pers_len > 0 && epochs > 0) FE_CHECK(hipStreamWaitEvent(s, join, 0));
+    return 0;
+  }
+
+  // the job-table launches of `epochs` epochs: captured chunks, then the rest
+  int enqueue_table(int epochs) {
+    hipStream_t s = stream.get();
+    constexpr int kChunk = 25;
     int left = epochs;
     const char* env = std::getenv("FASTEMBEDR_HIP_GRAPH_CAPTURE");
     if (!(env && env[0] == '0') && left >= kChunk) {

@@ -25,10 +25,16 @@ def _alone(idx, dist, y0, epochs, seed, mode=1):
     return out
 
 
-def test_umap_batch_equals_alone_bitwise():
+@pytest.mark.parametrize("persistent", ["1", "0"])
+def test_umap_batch_equals_alone_bitwise(monkeypatch, persistent):
+    # with FASTEMBEDR_HIP_UMAP_PERSISTENT=1 jobs of up to 6144 points run in
+    # the persistent kernel (layout in LDS), larger ones and hogwild jobs in
+    # the job-table launches; with 0 (the default) every job takes the launches
     from fastembedr_amd.umap import UmapBatch
 
-    specs = [(1200, 1, 60), (3100, 2, 45), (700, 3, 80), (2400, 4, 33)]
+    monkeypatch.setenv("FASTEMBEDR_HIP_UMAP_PERSISTENT", persistent)
+    specs = [(1200, 1, 60), (3100, 2, 45), (700, 3, 80), (2400, 4, 33), (7000, 5, 52),
+             (6144, 6, 30)]
     jobs = [_job(n, s) for n, s, _ in specs]
     b = UmapBatch(max_jobs=8)
     slots = [b.add(i, d, y, n_epochs=e, seed=100 + s) for (n, s, e), (i, d, y) in zip(specs, jobs)]
