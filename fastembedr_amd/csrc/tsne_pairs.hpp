@@ -36,15 +36,26 @@
 namespace fe {
 namespace {
 
-constexpr int kPairBlock = 4096;                 // points per block
-constexpr int kPairWaves = 16;
+#ifndef FE_PAIR_BLOCK
+#define FE_PAIR_BLOCK 4096
+#endif
+// 8 waves (two per SIMD at 84 VGPRs) stream as fast as 16 and leave a CU's
+// registers to the kernels that run beside the attraction (the splat and the
+// FFT passes on the other stream)
+#ifndef FE_PAIR_WAVES
+#define FE_PAIR_WAVES 8
+#endif
+constexpr int kPairBlock = FE_PAIR_BLOCK;        // points per block
+constexpr int kPairWaves = FE_PAIR_WAVES;
+static_assert(kPairBlock % (64 * kPairWaves) == 0 && kPairBlock <= 65536,
+              "a block is staged in whole rounds of the workgroup, with 16-bit offsets");
 constexpr int kPairThreads = 64 * kPairWaves;
 // staged rows, staged columns, their packed accumulators: 128 KB
 constexpr size_t kPairLds = 2 * kPairBlock * sizeof(float2) + 2 * kPairBlock * sizeof(long long);
 constexpr int kPairChunkEntries = 128 * kLaneStep;  // 65536 entries per work item at most
 constexpr int kPairDenseMin = 8192;              // pairs that pay for staging an off-diagonal tile
 constexpr int kPairMaxBlocks = 4096;             // tile table nb^2 ints (n <= 16.7M)
-// the streams are read two lane-steps ahead (one workgroup per CU, 16 waves):
+// the streams are read two lane-steps ahead by every wave:
 // padding past their end for those reads
 constexpr int kPairPad = 5 * kLaneStep;
 
