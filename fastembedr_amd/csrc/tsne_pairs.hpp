@@ -47,8 +47,8 @@ namespace {
 #endif
 constexpr int kPairBlock = FE_PAIR_BLOCK;        // points per block
 constexpr int kPairWaves = FE_PAIR_WAVES;
-static_assert(kPairBlock % (64 * kPairWaves) == 0 && kPairBlock <= 65536,
-              "a block is staged in whole rounds of the workgroup, with 16-bit offsets");
+static_assert(kPairBlock % 64 == 0 && kPairBlock <= 65536,
+              "blocks of whole waves' worth of points, with 16-bit offsets");
 constexpr int kPairThreads = 64 * kPairWaves;
 // staged rows, staged columns, their packed accumulators: 128 KB
 constexpr size_t kPairLds = 2 * kPairBlock * sizeof(float2) + 2 * kPairBlock * sizeof(long long);
@@ -118,7 +118,7 @@ __device__ __forceinline__ int pad_lane(int len) {
 // every load of a thread is issued before its first LDS write
 __device__ __forceinline__ void pair_stage(const float2* __restrict__ y, int lo, int m, float mx,
                                            float my, float2* s) {
-  constexpr int L = kPairBlock / kPairThreads;
+  constexpr int L = (kPairBlock + kPairThreads - 1) / kPairThreads;
   float2 r[L];
 #pragma unroll
   for (int u = 0; u < L; ++u) {
