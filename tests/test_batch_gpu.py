@@ -81,3 +81,14 @@ def test_budget_admission():
     with pytest.raises(NotAdmitted):
         _add(b, a, 128, 5, 5)
     b.close()
+    # the admission estimate bounds every device byte the job holds (P, the
+    # attraction tables, state, grids), at the estimate's own grid and at the
+    # rule's
+    from fastembedr_amd import TsnePlan
+
+    idx, dist, y0 = a
+    for g in (128, 0):
+        plan = TsnePlan(idx, dist, perplexity=10, grid_size=g, Y_init=y0)
+        held = plan.info()["device_bytes"]
+        plan.close()
+        assert 0 < held <= job_bytes(2000, 30, g, setup_peak=False), (g, held)

@@ -127,3 +127,41 @@ def test_session_admission_and_errors():
     st = s.stats()
     assert st["failed"] == 1 and st["completed"] == 2
     s.close()
+
+
+def test_session_concurrent_submitters():
+    """Requests submitted and polled from several threads at once: every one
+    completes and equals its request served alone."""
+    import threading
+
+    from fastembedr_amd.serve import DONE, Session
+
+    rng = np.random.default_rng(2)
+    ref_x, _ = gaussian_mixture(2500, 8, 5, seed=44)
+    ref_y = rng.normal(size=(2500, 2)).astype(np.float32) * 3
+    reqs = []
+    for t in range(12):
+        m = 100 + 37 * t
+        q = ref_x[rng.choice(2500, m)] + rng.normal(scale=0.1, size=(m, 8)).astype(np.float32)
+        idx, dist = exact_knn(ref_x, 15, query=q)
+        reqs.append(dict(indices=idx, distances=dist.astype(np.float32), n_iter=40, seed=t))
+    s = Session(ref_y, max_jobs=2)
+    out = [None] * len(reqs)
+
+    def worker(ids):
+        for i in ids:
+            rid = s.submit_transform(**reqs[i])
+            out[i] = s.wait(rid)
+
+    threads = [threading.Thread(target=worker, args=(list(range(w, len(reqs), 4)),))
+               for w in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    for i, p in enumerate(reqs):
+        rid = s.submit_transform(**p)
+        state, alone, _, _ = s.poll(rid, wait_ms=60000)
+        assert state == DONE and np.array_equal(alone, out[i]), i
+    assert s.stats()["completed"] == 2 * len(reqs)
+    s.close()
