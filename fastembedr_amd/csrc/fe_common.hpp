@@ -10,6 +10,8 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <exception>
+#include <new>
 #include <string>
 #include <utility>
 #include <vector>
@@ -29,6 +31,21 @@ inline int check(hipError_t code, const char* where) {
   if (code == hipSuccess) return 0;
   set_error(std::string(where) + ": " + hipGetErrorString(code));
   return 1;
+}
+
+// Runs f() and turns an exception into the error convention above: nothing
+// throws across the C-ABI.
+template <class F>
+int guarded(const char* what, F&& f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    return fail(std::string(what) + ": out of host memory");
+  } catch (const std::exception& e) {
+    return fail(std::string(what) + ": " + e.what());
+  } catch (...) {
+    return fail(std::string(what) + ": unexpected error");
+  }
 }
 
 #define FE_CHECK(expr)                        \

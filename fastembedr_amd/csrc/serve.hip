@@ -303,6 +303,7 @@ class Session {
   int enqueue(RequestPtr r, long long* id) {
     std::lock_guard<std::mutex> lk(mu_);
     if (stop_) return fail("the session is shutting down");
+    if (broken_) return fail("the serving engine stopped after an error");
     r->id = next_id_++;
     r->submitted = Clock::now();
     requests_[r->id] = r;
@@ -541,8 +542,40 @@ class Session {
     embed_running_.clear();
   }
 
+  // The engine loop; an exception (host memory) fails every request and
+  // stops the engine, after which submits are refused.
   void engine() {
     (void)hipSetDevice(device_);
+    try {
+      engine_loop();
+    } catch (...) {
+      sync_all_streams();
+      std::lock_guard<std::mutex> lk(mu_);
+      broken_ = true;
+      for (auto& e : requests_) {
+        RequestPtr r = e.second;
+        if (r->state == kQueued || r->state == kRunning) {
+          r->state = kFailed;
+          r->error = "the serving engine stopped: out of host memory";
+          r->finished = Clock::now();
+          ++failed_;
+        }
+      }
+      queued_embed_ = queued_tf_ = queued_umap_ = 0;
+      running_embed_ = running_tf_ = running_umap_ = 0;
+      done_cv_.notify_all();
+    }
+  }
+
+  // waits for all device work of the session (the engine's error path)
+  void sync_all_streams() {
+    for (int i = 0; i < kTransformStreams; ++i)
+      if (tf_stream_[i]) (void)hipStreamSynchronize(tf_stream_[i]);
+    if (batch_) (void)hipStreamSynchronize(tsne_batch_stream(batch_));
+    if (ubatch_) (void)hipStreamSynchronize(umap_batch_stream(ubatch_));
+  }
+
+  void engine_loop() {
     for (;;) {
       bool stopping = false;
       {
@@ -637,6 +670,7 @@ class Session {
   std::mutex mu_;
   std::condition_variable work_cv_, done_cv_;
   bool stop_ = false;
+  bool broken_ = false;
   std::deque<RequestPtr> incoming_;
   std::unordered_map<long long, RequestPtr> requests_;
   long long next_id_ = 1;
@@ -661,8 +695,10 @@ int fastembedr_hip_session_create(const float* reference_layout, int n_reference
   *session = nullptr;
   auto* s = new (std::nothrow) fastembedr_hip_session;
   if (!s) return fe::fail("out of host memory");
-  if (s->impl.create(reference_layout, n_reference, max_jobs, hbm_budget_bytes,
-                     transform_pool_bytes)) {
+  if (fe::guarded("session create", [&] {
+        return s->impl.create(reference_layout, n_reference, max_jobs, hbm_budget_bytes,
+                              transform_pool_bytes);
+      })) {
     const std::string error = fe::last_error();
     delete s;
     return fe::fail(error);
@@ -675,7 +711,7 @@ int fastembedr_hip_session_submit_embed(fastembedr_hip_session* session,
                                         const fastembedr_hip_tsne_job* job, long long* request) {
   fe::clear_error();
   if (!session || !job) return fe::fail("null session or job");
-  return session->impl.submit_embed(*job, request);
+  return fe::guarded("session submit", [&] { return session->impl.submit_embed(*job, request); });
 }
 
 int fastembedr_hip_session_submit_transform(
@@ -687,25 +723,29 @@ int fastembedr_hip_session_submit_transform(
     long long* request) {
   fe::clear_error();
   if (!session) return fe::fail("null session");
-  return session->impl.submit_transform(
-      indices, distances, n_query, k, index_offset, init, initialization, perplexity, n_iter,
-      early_exaggeration_iter, learning_rate, early_exaggeration, exaggeration, initial_momentum,
-      final_momentum, max_grad_norm, max_step_norm, n_negatives, exact_repulsion_threshold, seed,
-      request);
+  return fe::guarded("session submit", [&] {
+    return session->impl.submit_transform(
+        indices, distances, n_query, k, index_offset, init, initialization, perplexity, n_iter,
+        early_exaggeration_iter, learning_rate, early_exaggeration, exaggeration,
+        initial_momentum, final_momentum, max_grad_norm, max_step_norm, n_negatives,
+        exact_repulsion_threshold, seed, request);
+  });
 }
 
 int fastembedr_hip_session_submit_umap(fastembedr_hip_session* session,
                                        const fastembedr_hip_umap_job* job, long long* request) {
   fe::clear_error();
   if (!session || !job) return fe::fail("null session or job");
-  return session->impl.submit_umap(*job, request);
+  return fe::guarded("session submit", [&] { return session->impl.submit_umap(*job, request); });
 }
 
 int fastembedr_hip_session_poll(fastembedr_hip_session* session, long long request, int wait_ms,
                                 int* state, float* out, double* latency_ms) {
   fe::clear_error();
   if (!session) return fe::fail("null session");
-  return session->impl.poll(request, wait_ms, state, out, latency_ms);
+  return fe::guarded("session poll", [&] {
+    return session->impl.poll(request, wait_ms, state, out, latency_ms);
+  });
 }
 
 int fastembedr_hip_session_stats(fastembedr_hip_session* session, long long* stats) {

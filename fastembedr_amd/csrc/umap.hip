@@ -1278,7 +1278,7 @@ int fastembedr_hip_umap_batch_create(int max_jobs, long long hbm_budget_bytes,
   *batch = nullptr;
   auto* b = new (std::nothrow) fastembedr_hip_umap_batch;
   if (!b) return fe::fail("out of host memory");
-  if (b->impl.init(max_jobs, hbm_budget_bytes)) {
+  if (fe::guarded("UMAP batch create", [&] { return b->impl.init(max_jobs, hbm_budget_bytes); })) {
     delete b;
     return 1;
   }
@@ -1291,19 +1291,20 @@ int fastembedr_hip_umap_batch_add(fastembedr_hip_umap_batch* batch,
                                   int* slot) {
   fe::clear_error();
   if (!batch || !job || !slot) return fe::fail("null batch, job or slot pointer");
-  return batch->impl.add(*job, device_inputs != 0, slot);
+  return fe::guarded("UMAP batch add",
+                     [&] { return batch->impl.add(*job, device_inputs != 0, slot); });
 }
 
 int fastembedr_hip_umap_batch_step(fastembedr_hip_umap_batch* batch, int n_epochs) {
   fe::clear_error();
   if (!batch) return fe::fail("null batch pointer");
-  return batch->impl.step(n_epochs);
+  return fe::guarded("UMAP batch step", [&] { return batch->impl.step(n_epochs); });
 }
 
 int fastembedr_hip_umap_batch_status(fastembedr_hip_umap_batch* batch, int* epochs, int* active) {
   fe::clear_error();
   if (!batch) return fe::fail("null batch pointer");
-  return batch->impl.status(epochs, active);
+  return fe::guarded("UMAP batch status", [&] { return batch->impl.status(epochs, active); });
 }
 
 int fastembedr_hip_umap_batch_layout(fastembedr_hip_umap_batch* batch, int slot, float* out) {
