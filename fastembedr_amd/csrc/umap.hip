@@ -41,7 +41,20 @@ struct UmapParams {
   int n, n_epochs, nsr;
   unsigned seed;
   float lr, a, b, gamma;
+  unsigned long long nmod;  // umap_nmod(n): x mod n by multiplication
 };
+
+// c = ceil(2^64 / n) for n >= 1: for every 32-bit x, x mod n is the high 64
+// bits of (c * x mod 2^64) * n (Lemire, Kaser and Kurz, "Faster remainder by
+// direct computation", 2019), the same value as the % operator.
+inline unsigned long long umap_nmod(int n) {
+  return ~0ull / static_cast<unsigned long long>(n > 0 ? n : 1) + 1ull;
+}
+
+__device__ __forceinline__ unsigned fast_mod(unsigned x, unsigned long long c, unsigned n) {
+  const unsigned long long low = c * x;
+  return static_cast<unsigned>(__umul64hi(low, static_cast<unsigned long long>(n)));
+}
 
 __device__ __forceinline__ unsigned mix_uint(unsigned x) {
   x ^= x >> 16;
@@ -53,14 +66,15 @@ __device__ __forceinline__ unsigned mix_uint(unsigned x) {
 }
 
 // deterministic_vertex (embedding_cuda_kernels.cpp:262-274)
-__device__ __forceinline__ unsigned negative_vertex(unsigned n, unsigned seed, unsigned epoch,
-                                                    unsigned head, unsigned tail, unsigned s) {
+__device__ __forceinline__ unsigned negative_vertex(unsigned n, unsigned long long nmod,
+                                                    unsigned seed, unsigned epoch, unsigned head,
+                                                    unsigned tail, unsigned s) {
   unsigned x = seed;
   x ^= epoch * 0x9e3779b9u;
   x ^= (head + 1u) * 0x85ebca6bu;
   x ^= (tail + 1u) * 0xc2b2ae35u;
   x ^= (s + 1u) * 0x27d4eb2du;
-  return mix_uint(x) % n;
+  return fast_mod(mix_uint(x), nmod, n);  // mix_uint(x) % n
 }
 
 __device__ __forceinline__ float clip4(float x) { return fminf(4.0f, fmaxf(-4.0f, x)); }
@@ -172,7 +186,7 @@ __device__ __forceinline__ void sym_edge_head(Pos pos, int h, int t, float per, 
   }
   const int nneg = n_negative(per, p.nsr, epoch);
   for (int s = 0; s < nneg; ++s) {
-    const unsigned v = negative_vertex(static_cast<unsigned>(p.n), p.seed, epoch,
+    const unsigned v = negative_vertex(static_cast<unsigned>(p.n), p.nmod, p.seed, epoch,
                                        static_cast<unsigned>(h), static_cast<unsigned>(t),
                                        static_cast<unsigned>(s));
     if (static_cast<int>(v) == h || static_cast<int>(v) == t) continue;
@@ -268,7 +282,7 @@ __device__ __forceinline__ void epoch_grouped_body(unsigned long long* delta,
           }
           const int nneg = n_negative(per, p.nsr, epoch);
           for (int s = 0; s < nneg; ++s) {
-            const unsigned v = negative_vertex(static_cast<unsigned>(p.n), p.seed, epoch,
+            const unsigned v = negative_vertex(static_cast<unsigned>(p.n), p.nmod, p.seed, epoch,
                                                static_cast<unsigned>(h), static_cast<unsigned>(t),
                                                static_cast<unsigned>(s));
             if (static_cast<int>(v) == h || static_cast<int>(v) == t) continue;
@@ -384,7 +398,7 @@ __device__ __forceinline__ void epoch_hogwild_body(float* layout, const int* __r
           uy = sy;
           const int nneg = n_negative(per, p.nsr, epoch);
           for (int s = 0; s < nneg; ++s) {
-            const unsigned v = negative_vertex(static_cast<unsigned>(p.n), p.seed, epoch,
+            const unsigned v = negative_vertex(static_cast<unsigned>(p.n), p.nmod, p.seed, epoch,
                                                static_cast<unsigned>(h), static_cast<unsigned>(t),
                                                static_cast<unsigned>(s));
             if (static_cast<int>(v) == h || static_cast<int>(v) == t) continue;
@@ -457,8 +471,9 @@ __global__ __launch_bounds__(kUThreads) void umap_apply_kernel(float* layout,
 
 __global__ void umap_schedule_kernel(const int* __restrict__ heads, const int* __restrict__ tails,
                                      const float* __restrict__ period, int n_edges, int n,
-                                     unsigned epoch, int nsr, unsigned seed, int max_neg,
-                                     int* n_pos, int* n_neg, int* neg_ids) {
+                                     unsigned epoch, int nsr, unsigned seed,
+                                     unsigned long long nmod, int max_neg, int* n_pos,
+                                     int* n_neg, int* neg_ids) {
   const int e = blockIdx.x * kUThreads + threadIdx.x;
   if (e >= n_edges) return;
   const int h = heads[e], t = tails[e];
@@ -469,7 +484,7 @@ __global__ void umap_schedule_kernel(const int* __restrict__ heads, const int* _
   for (int s = 0; s < max_neg; ++s) {
     int v = -1;
     if (s < nn) {
-      v = static_cast<int>(negative_vertex(static_cast<unsigned>(n), seed, epoch,
+      v = static_cast<int>(negative_vertex(static_cast<unsigned>(n), nmod, seed, epoch,
                                            static_cast<unsigned>(h), static_cast<unsigned>(t),
                                            static_cast<unsigned>(s)));
       if (v == h || v == t) v = -1;
@@ -524,7 +539,7 @@ int umap_run(const int* heads, const int* tails, const float* weights,
   unsigned long long* delta = arena.take<unsigned long long>(mode == 1 ? m : 1);
   FE_CHECK(hipMemcpyAsync(layout, init, m * sizeof(float), hipMemcpyHostToDevice, s));
   FE_CHECK(hipMemsetAsync(delta, 0, (mode == 1 ? m : 1) * sizeof(unsigned long long), s));
-  const UmapParams p{n, n_epochs, nsr, seed, lr, a, b, gamma};
+  const UmapParams p{n, n_epochs, nsr, seed, lr, a, b, gamma, umap_nmod(n)};
   const int eblocks = static_cast<int>(std::min<long long>((n_edges + kUThreads - 1) / kUThreads, 8192));
   const int rblocks = static_cast<int>((m + kUThreads - 1) / kUThreads);
   auto enqueue = [&](int epoch) -> int {
@@ -641,7 +656,7 @@ int umap_run_noalloc(const int* heads, const int* tails, const float* epochs_per
     clear_u64_kernel<<<rblocks, kUThreads, 0, s>>>(delta, m);
     FE_LAUNCH_CHECK("clear_u64_kernel");
   }
-  const UmapParams p{n, n_epochs, nsr, seed, lr, a, b, gamma};
+  const UmapParams p{n, n_epochs, nsr, seed, lr, a, b, gamma, umap_nmod(n)};
   const int eblocks = static_cast<int>(std::min<long long>((n_edges + kUThreads - 1) / kUThreads, 8192));
   for (int e = 0; e < n_epochs; ++e) {
     if (mode == 0) {
@@ -986,7 +1001,7 @@ class UmapBatch {
     d.active = 1;
     d.mode = j.optimizer_mode;
     d.p = UmapParams{j.n, j.n_epochs, j.negative_sample_rate, j.seed, j.learning_rate, j.a, j.b,
-                     j.repulsion_strength};
+                     j.repulsion_strength, umap_nmod(j.n)};
     desc[slot] = d;
     FE_CHECK(hipMemcpyAsync(d_desc + slot, &desc[slot], sizeof(UmapJobDesc),
                             hipMemcpyHostToDevice, s));
@@ -1234,8 +1249,8 @@ int fastembedr_hip_umap_schedule(const int* heads, const int* tails,
   FE_CHECK(hipMemcpy(p, epochs_per_sample, n_edges * sizeof(float), hipMemcpyHostToDevice));
   fe::umap_schedule_kernel<<<(n_edges + 255) / 256, 256>>>(h, t, p, n_edges, n,
                                                            static_cast<unsigned>(epoch),
-                                                           negative_sample_rate, seed, max_neg,
-                                                           dp, dn, di);
+                                                           negative_sample_rate, seed,
+                                                           fe::umap_nmod(n), max_neg, dp, dn, di);
   FE_LAUNCH_CHECK("umap_schedule_kernel");
   FE_CHECK(hipDeviceSynchronize());
   FE_CHECK(hipMemcpy(n_pos, dp, n_edges * sizeof(int), hipMemcpyDeviceToHost));

@@ -30,6 +30,25 @@ def test_umap_schedule_is_bit_exact():
             assert np.array_equal(a, b)
 
 
+def test_umap_schedule_negative_ids_at_large_n():
+    # the negative vertex is the hash modulo n, taken on the device by a
+    # multiplication (an exact remainder for every 32-bit hash): the ids must
+    # equal the oracle's % for vertex counts up to 2^31 - 1
+    from fastembedr_amd import umap
+
+    rng = np.random.default_rng(15)
+    for n in (7, 65537, 1_000_003, 2_097_152, 2**31 - 1):
+        m = 2000
+        heads = rng.integers(0, min(n, 2**31 - 1), size=m).astype(np.int32)
+        tails = ((heads.astype(np.int64) + 1) % n).astype(np.int32)
+        per = rng.uniform(1.0, 3.0, size=m).astype(np.float32)
+        for epoch in (1, 3, 77):
+            g = umap.schedule(heads, tails, per, n, epoch, 5, seed=99, max_neg=12)
+            o = oracle.umap_schedule(heads, tails, per, n, epoch, 5, 99, 12)
+            for a, b in zip(g, o):
+                assert np.array_equal(a, b), (n, epoch)
+
+
 def test_umap_delta_mode_matches_oracle_and_is_deterministic():
     from fastembedr_amd import umap
 
