@@ -74,7 +74,9 @@ def parse():
                     help="jobs of the batched workload (0: skip it)")
     ap.add_argument("--umap-batched-jobs", type=int, default=64,
                     help="jobs of the batched UMAP workload (0: skip it)")
-    ap.add_argument("--served-requests", type=int, default=48,
+    ap.add_argument("--config1", type=int, default=1,
+                    help="run the 2000-point case (exact and G=128 routes; 0: skip it)")
+    ap.add_argument("--served-requests", type=int, default=96,
                     help="requests of the served workload (0: skip it)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run a plain device copy of this many MiB (3 times) so hardware byte "
@@ -288,6 +290,59 @@ def bench_umap_batched(workloads, count, k=15, epochs=200, n_lo=2000, n_hi=10000
     return res
 
 
+def bench_config1(fe, workloads, cpu):
+    """The reference's CPU-runnable case: 2000 points, 10 clusters of 200 in
+    50 dims, exact k = 90 KNN, perplexity 30, 250 early (12) + 750 normal
+    iterations from a centred N(0, 1e-4^2) start (seed 102).  The reference
+    routes n <= 3000 to exact pair repulsion; both that route and the FFT grid
+    at G = 128 run on the device (wall time from host KNN to host layout),
+    beside the reference CPU algorithm on the host (the CPU port) when `cpu`;
+    quality is the exact metrics at k = 15 on all 2000 points."""
+    from fastembedr_amd.quality import exact_structure_metrics
+
+    x, _, _ = workloads.gaussian_mixture(2000, 50, 10, seed=101)
+    ki, kd = workloads.exact_knn_tensors(x, 90)
+    idx = np.ascontiguousarray(ki.T.cpu().numpy())
+    dist = np.ascontiguousarray(kd.T.cpu().numpy())
+    y0 = np.random.default_rng(102).normal(scale=1e-4, size=(2000, 2)).astype(np.float32)
+    y0 -= y0.mean(0)
+    out = {"workload": "2000 points, 10 clusters x 200, 50-d, exact k=90 KNN, perplexity 30, "
+                       "250 early (12) + 750 normal, centred N(0, 1e-4^2) start"}
+    for route, grid in (("exact", -1), ("fft_g128", 128)):
+        method = dict(negative_gradient_method="exact") if grid < 0 else dict(grid_size=grid)
+        fe.tsne_knn(idx, dist, perplexity=30.0, Y_init=y0, early_exaggeration_iter=5, n_iter=5,
+                    **method)  # warm-up (kernels loaded)
+        t0 = time.perf_counter()
+        y = fe.tsne_knn(idx, dist, perplexity=30.0, Y_init=y0, early_exaggeration_iter=250,
+                        n_iter=750, **method)
+        dev_s = time.perf_counter() - t0
+        m = exact_structure_metrics(x, y, [15])[0]
+        r = {"device_s": round(dev_s, 4), "device_iters_per_s": round(1000 / dev_s, 1),
+             "trustworthiness": round(float(m[0]), 5), "knn_preservation": round(float(m[2]), 5)}
+        if cpu:
+            import oracle
+
+            threads = os.cpu_count() or 1
+            t0 = time.perf_counter()
+            c, aff_s, opt_s = oracle.port_tsne_run(idx, dist, y0, perplexity=30.0, early_iter=250,
+                                                   n_iter=750, early_exag=12.0, exag=1.0,
+                                                   learning_rate=2000 / 12.0, mom0=0.8, mom1=0.8,
+                                                   max_step=5.0, grid=grid, n_threads=threads)
+            cpu_s = time.perf_counter() - t0
+            mc = exact_structure_metrics(x, c, [15])[0]
+            r["cpu"] = {"kind": "port (oracle/cpu_port.c, the reference CPU algorithm)",
+                        "threads": threads, "seconds": round(cpu_s, 3),
+                        "affinity_s": round(aff_s, 3), "optimise_s": round(opt_s, 3),
+                        "trustworthiness": round(float(mc[0]), 5),
+                        "knn_preservation": round(float(mc[2]), 5)}
+            r["speedup_vs_cpu"] = round(cpu_s / dev_s, 1)
+            r["delta_trustworthiness"] = round(float(m[0] - mc[0]), 5)
+            r["delta_knn_preservation"] = round(float(m[2] - mc[2]), 5)
+        out[route] = r
+        log(f"[config1] {route}: {r}")
+    return out
+
+
 def bench_served(fe, workloads, n_requests, k=90):
     """Served stream: a 500k-point reference embedding resident on the device;
     Poisson arrivals of transform queries (1k-50k points, 50-d, k = 90 to the
@@ -307,10 +362,12 @@ def bench_served(fe, workloads, n_requests, k=90):
     torch.cuda.empty_cache()
     reqs = []
     t = 0.0
-    rate = 8.0  # requests per second
+    # 80% transforms, 20% t-SNE jobs; about 24 requests/s keeps the device
+    # busy most of the time (a job of the mean size takes ~0.15 s alone)
+    rate = 24.0  # requests per second
     for rid in range(n_requests):
         t += rng.exponential(1.0 / rate)
-        if rng.random() < 0.25:
+        if rng.random() < 0.2:
             sp = dict(n=int(workloads.log_uniform_sizes(1, 10_000, 200_000, rng)[0]),
                       d=int(rng.integers(30, 51)), clusters=int(rng.integers(5, 51)),
                       seed=int(rng.integers(1 << 30)))
@@ -833,6 +890,7 @@ def main():
         umap_gate = umap_quality_gate(fe, workloads, args.umap_quality_n)
     batched = bench_batched(fe, workloads, args.batched_jobs) if args.batched_jobs > 0 else None
     torch.cuda.empty_cache()
+    config1 = bench_config1(fe, workloads, not args.no_cpu_baseline) if args.config1 else None
     umap_batched = None
     if args.umap_batched_jobs > 0:
         umap_batched = bench_umap_batched(workloads, args.umap_batched_jobs)
@@ -883,6 +941,7 @@ def main():
     if umap_gate:
         out["umap_quality_gate"] = umap_gate
     out["batched"] = batched
+    out["config1"] = config1
     out["umap_batched"] = umap_batched
     out["served"] = served
     print(json.dumps(out), flush=True)
