@@ -446,11 +446,11 @@ int TsnePlan::setup_pairs(hipStream_t s, bool* built) {
   const int nc = static_cast<int>(chunks.size());
   const long long all_entries = total + rtotal;
   // workgroups: two per CU when the attraction has the GPU to itself; beside
-  // the splat and the FFT passes (the overlapped iteration) three for every
-  // four CUs, so those kernels keep CUs of their own (1M points: 3700 it/s
-  // against 3450 with 512 workgroups)
+  // the splat and the FFT passes (the overlapped iteration) five for every
+  // eight CUs, so those kernels keep CUs of their own (1M points: 3830 it/s
+  // against 3785 with 3/4 of the CUs and 3370 with one per CU)
   const int cus = compute_units();
-  const int want = pair_wg_request > 0 ? pair_wg_request : (overlap ? 3 * cus / 4 : 2 * cus);
+  const int want = pair_wg_request > 0 ? pair_wg_request : (overlap ? 5 * cus / 8 : 2 * cus);
   const int ranges = std::max(1, std::min(nc, want));
   std::vector<int> range_first(ranges + 1, nc);
   range_first[0] = 0;

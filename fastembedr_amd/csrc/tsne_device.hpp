@@ -705,24 +705,34 @@ __device__ __forceinline__ void splat_tiles_body(const TsneStats* __restrict__ s
   }
 }
 
+// The splat and FFT kernels run beside the attraction in the overlapped
+// iteration and are its critical path: their waves take a higher issue
+// priority than the attraction's where both share a SIMD (1M points: 3780
+// against 3700 iterations/s).
+__device__ __forceinline__ void repulsion_wave_priority() { __builtin_amdgcn_s_setprio(1); }
+
 __global__ __launch_bounds__(kSplatThreads) void splat_count(const float2* __restrict__ y,
                                                              const TsneStats* __restrict__ st,
                                                              SplatBuffers sb, int n, int g) {
+  repulsion_wave_priority();
   splat_count_body(y, st, sb, n, g, blockIdx.x, gridDim.x);
 }
 
 __global__ __launch_bounds__(kSplatThreads) void splat_colscan(SplatBuffers sb, int g, int nrows) {
+  repulsion_wave_priority();
   splat_colscan_body(sb, g, nrows, blockIdx.x);
 }
 
 __global__ __launch_bounds__(kSplatThreads) void splat_scatter(const float2* __restrict__ y,
                                                                const TsneStats* __restrict__ st,
                                                                SplatBuffers sb, int n, int g) {
+  repulsion_wave_priority();
   splat_scatter_body(y, st, sb, n, g, blockIdx.x, gridDim.x);
 }
 
 __global__ __launch_bounds__(kThreads) void splat_tiles(const TsneStats* __restrict__ st,
                                                         SplatBuffers sb, int g) {
+  repulsion_wave_priority();
   splat_tiles_body(st, sb, g, blockIdx.x, gridDim.x);
 }
 
@@ -838,6 +848,7 @@ __global__ __launch_bounds__(kFftThreads) void conv_rows_fwd(SplatBuffers sb,
                                                           float2* __restrict__ z1,
                                                           float2* __restrict__ z2,
                                                           float2* __restrict__ kt) {
+  repulsion_wave_priority();
   conv_rows_fwd_body<F>(sb, st, tw, z1, z2, kt, blockIdx.x);
 }
 
@@ -976,6 +987,7 @@ __global__ __launch_bounds__(kFftThreads) void conv_cols(const float2* __restric
                                                       float2* __restrict__ o1,
                                                       float2* __restrict__ o2,
                                                       double* __restrict__ partials) {
+  repulsion_wave_priority();
   extern __shared__ float2 lds[];
   constexpr int G = F / 2;
   const int c = xcd_column(blockIdx.x, G);
@@ -1050,6 +1062,7 @@ __global__ __launch_bounds__(kFftThreads) void conv_rows_inv(const float2* __res
                                                           const double* __restrict__ zpart,
                                                           TsneStats* st,
                                                           float4* __restrict__ field) {
+  repulsion_wave_priority();
   conv_rows_inv_body<F>(o1, o2, tw, zpart, st, field, blockIdx.x);
 }
 

@@ -15,10 +15,10 @@
 // of the other tiles go to the "rest" stream, stored in both directions with
 // global columns, and add to their row only (gathers from the layout).
 //
-// Sums are integers: every contribution is scaled by a power of two and
-// rounded (the row side once per lane of 8 entries, the column side per
-// entry), packed as the 64-bit integer y * 2^32 + x and added with one LDS
-// 64-bit atomic into one accumulator per staged point; a work item writes its
+// Sums are integers: every pair's contribution is scaled by a power of two,
+// truncated toward zero and packed as the 64-bit integer y * 2^32 + x; its
+// column adds it with one LDS 64-bit atomic into one accumulator per staged
+// point, and its row adds a lane's 8 of them at once; a work item writes its
 // accumulators to its partial slots, and the reduce adds a point's slots.
 // A sum of packed values is the packed sum Y * 2^32 + X exactly, and while
 // |X| < 2^31 and |Y| < 2^30 the low 32 bits read as a signed integer are X
@@ -98,10 +98,20 @@ __device__ __forceinline__ int pair_scale_exp(const PairTables& t, const TsneSta
   return min(120, max(-120, 29 - ilogb(b)));
 }
 
-// (x, y) scaled and rounded, packed as y * 2^32 + x
-__device__ __forceinline__ unsigned long long pack_fixed(float x, float y, float scale) {
-  const long long ix = __float2int_rn(x * scale), iy = __float2int_rn(y * scale);
+// (x, y), already scaled, truncated toward zero and packed as y * 2^32 + x:
+// one conversion instruction a component, and the pack of -(x, y) is the
+// negated pack exactly
+__device__ __forceinline__ unsigned long long pack_fixed_rz(float x, float y) {
+  const long long ix = static_cast<int>(x), iy = static_cast<int>(y);
   return static_cast<unsigned long long>(iy * 4294967296ll + ix);
+}
+
+// The LDS holds, per staged block, the points' positions followed by their
+// accumulators, kPairBlock float2 (32 KB) apart: the accumulator of a staged
+// point sits at a fixed offset from its position, so one address serves the
+// gather and the atomic.
+__device__ __forceinline__ unsigned long long* pair_acc(const float2* p) {
+  return reinterpret_cast<unsigned long long*>(const_cast<float2*>(p + kPairBlock));
 }
 
 __device__ __forceinline__ void unpack_fixed(unsigned long long v, long long* x, long long* y) {
@@ -192,28 +202,29 @@ __device__ __forceinline__ PairLane pair_lane_load(const PairTables& t, int eb) 
 template <bool REST>
 __device__ __forceinline__ void pair_step(const float2* __restrict__ y, const PairLane& L,
                                           bool valid, const float2* yI, const float2* yJ,
-                                          unsigned long long* accI, unsigned long long* accJ,
-                                          float mx, float my, float scale) {
+                                          float mx, float my, float nscale) {
   float2 pj[kLaneEntries];
 #pragma unroll
   for (int u = 0; u < kLaneEntries; ++u) pj[u] = REST ? y[L.j[u]] : yJ[L.j[u]];
   const float2 pi = yI[L.row];
   if (!valid) return;
-  float sx = 0.0f, sy = 0.0f;
+  // nt = the pack of -t (t the entry's term for its row, scaled): the column
+  // adds it, the row subtracts the lane's sum of them once
+  unsigned long long nrow = 0ull;
 #pragma unroll
   for (int u = 0; u < kLaneEntries; ++u) {
     const float2 q2 = REST ? make_float2(pj[u].x - mx, pj[u].y - my) : pj[u];
     const float dx = pi.x - q2.x, dy = pi.y - q2.y;
     const float q = __builtin_amdgcn_rcpf(1.0f + dx * dx + dy * dy);
-    const float c = L.v[u] * q;
-    const float fx = c * dx, fy = c * dy;
-    sx += fx;
-    sy += fy;
+    const float c = (L.v[u] * nscale) * q;
+    const unsigned long long nt = pack_fixed_rz(c * dx, c * dy);
+    nrow += nt;
     if constexpr (!REST) {
-      if (L.v[u] != 0.0f) atomicAdd(&accJ[L.j[u]], pack_fixed(-fx, -fy, scale));
+      // padding entries (value 0) add 0: cheaper than a branch per entry
+      atomicAdd(pair_acc(yJ + L.j[u]), nt);
     }
   }
-  atomicAdd(&accI[L.row], pack_fixed(sx, sy, scale));
+  atomicAdd(pair_acc(yI + L.row), 0ull - nrow);
 }
 
 // Entries [e0, e1) of one wave (multiples of kLaneEntries): lane l of a step
@@ -226,19 +237,18 @@ __device__ __forceinline__ void pair_step(const float2* __restrict__ y, const Pa
 // are dropped.
 template <bool REST>
 __device__ __forceinline__ void pair_range(const float2* __restrict__ y, const PairTables& t,
-                                           const float2* yI, const float2* yJ,
-                                           unsigned long long* accI, unsigned long long* accJ,
-                                           int e0, int e1, float mx, float my, float scale) {
+                                           const float2* yI, const float2* yJ, int e0, int e1,
+                                           float mx, float my, float nscale) {
   const int lane = threadIdx.x & 63;
   int base = e0 & ~(kLaneStep - 1);
   PairLane a = pair_lane_load<REST>(t, base + kLaneEntries * lane);
   PairLane b = pair_lane_load<REST>(t, base + kLaneStep + kLaneEntries * lane);
   for (; base < e1; base += 2 * kLaneStep) {
     const int ea = base + kLaneEntries * lane, eb = ea + kLaneStep;
-    pair_step<REST>(y, a, ea >= e0 && ea < e1, yI, yJ, accI, accJ, mx, my, scale);
+    pair_step<REST>(y, a, ea >= e0 && ea < e1, yI, yJ, mx, my, nscale);
     a = pair_lane_load<REST>(t, ea + 2 * kLaneStep);
     if (base + kLaneStep >= e1) break;
-    pair_step<REST>(y, b, eb >= e0 && eb < e1, yI, yJ, accI, accJ, mx, my, scale);
+    pair_step<REST>(y, b, eb >= e0 && eb < e1, yI, yJ, mx, my, nscale);
     b = pair_lane_load<REST>(t, eb + 2 * kLaneStep);
   }
 }
@@ -252,10 +262,11 @@ __device__ __forceinline__ void attract_pairs_range(const float2* __restrict__ y
                                                     float2* lds) {
   const float mx = st->mean_xf, my = st->mean_yf;
   const float scale = ldexpf(1.0f, pair_scale_exp(t, st));
+  // [row block positions | their accumulators | column block positions | theirs]
   float2* yI = lds;
-  float2* yJ = lds + kPairBlock;
-  unsigned long long* accI = reinterpret_cast<unsigned long long*>(lds + 2 * kPairBlock);
-  unsigned long long* accJ = accI + kPairBlock;
+  float2* yJ = lds + 2 * kPairBlock;
+  unsigned long long* accI = pair_acc(yI);
+  unsigned long long* accJ = pair_acc(yJ);
   for (int q = static_cast<int>(threadIdx.x); q < kPairBlock; q += kPairThreads) {
     accI[q] = 0ull;
     accJ[q] = 0ull;
@@ -280,10 +291,9 @@ __device__ __forceinline__ void attract_pairs_range(const float2* __restrict__ y
     const int e1 = min(ch.e1, (s0 + (wave + 1) * steps / kPairWaves) * kLaneStep);
     if (e0 < e1) {
       if (rest)
-        pair_range<true>(y, t, yI, nullptr, accI, nullptr, e0, e1, mx, my, scale);
+        pair_range<true>(y, t, yI, nullptr, e0, e1, mx, my, -scale);
       else
-        pair_range<false>(y, t, yI, diag ? yI : yJ, accI, diag ? accI : accJ, e0, e1, mx, my,
-                          scale);
+        pair_range<false>(y, t, yI, diag ? yI : yJ, e0, e1, mx, my, -scale);
     }
     __syncthreads();
     if (ch.flush & 1) {
@@ -440,7 +450,9 @@ __global__ __launch_bounds__(kThreads) void pair_rest_count_kernel(
 
 // Per row: its runs of staged pairs (j > i, one run per tile, placed by the
 // tile's cursor) and its rest entries (placed by the row block's cursor).
-// Padding entries carry value 0 and in-range columns.  Run placement depends
+// Padding entries carry value 0; a staged run's padding repeats the run's
+// last column, so its column atomics (which add 0) do not pile the padding
+// lanes of a wave onto one accumulator.  Run placement depends
 // on timing; no result does (integer sums).
 __global__ __launch_bounds__(kThreads) void pair_fill_kernel(
     const int* __restrict__ row_ptr, const int* __restrict__ col, const float* __restrict__ val,
@@ -461,7 +473,7 @@ __global__ __launch_bounds__(kThreads) void pair_fill_kernel(
       const int len = f - e, pl = pad_lane(len);
       const unsigned pos = atomicAdd(&tile_cursor[t], static_cast<unsigned>(pl));
       for (int q = 0; q < pl; ++q) {
-        col16[pos + q] = static_cast<unsigned short>(q < len ? col[e + q] - J * kPairBlock : 0);
+        col16[pos + q] = static_cast<unsigned short>(col[e + min(q, len - 1)] - J * kPairBlock);
         pval[split_index(static_cast<int>(pos) + q)] = q < len ? val[e + q] : 0.0f;
       }
       for (int r = 0; r < pl / kLaneEntries; ++r)
