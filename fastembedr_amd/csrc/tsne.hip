@@ -441,10 +441,9 @@ int TsnePlan::setup_pairs(hipStream_t s, bool* built) {
     }
     chunks.swap(ordered);
   }
-  // workgroups: contiguous ranges of about equal entries, two per compute
-  // unit (or one per chunk when there are fewer)
+  // workgroups: contiguous ranges of chunks (one per chunk when there are
+  // fewer chunks than workgroups)
   const int nc = static_cast<int>(chunks.size());
-  const long long all_entries = total + rtotal;
   // workgroups: two per CU when the attraction has the GPU to itself; beside
   // the splat and the FFT passes (the overlapped iteration) five for every
   // eight CUs, so those kernels keep CUs of their own (1M points: 3830 it/s
@@ -455,11 +454,29 @@ int TsnePlan::setup_pairs(hipStream_t s, bool* built) {
   std::vector<int> range_first(ranges + 1, nc);
   range_first[0] = 0;
   {
+    // ranges of about equal cost: a rest entry (gathers from the layout in
+    // memory) counts 1.3 staged entries, and staging a block of the layout
+    // (a new row block, a new off-diagonal tile) 8192 entries (1M points:
+    // the attraction alone 188 -> 177 us, 3860 -> 3915 it/s overlapped,
+    // against ranges of equal entries)
+    constexpr long long kStageCost = 8192, kRestPct = 130;
+    std::vector<long long> cost(nc);
+    long long all_cost = 0;
+    for (int c = 0; c < nc; ++c) {
+      const PairChunk& ch = chunks[c];
+      long long w = static_cast<long long>(ch.e1 - ch.e0) * (ch.rest ? kRestPct : 100) / 100;
+      if (c == 0 || chunks[c - 1].I != ch.I) w += kStageCost;
+      if (!ch.rest && ch.J != ch.I &&
+          (c == 0 || chunks[c - 1].rest || chunks[c - 1].I != ch.I || chunks[c - 1].J != ch.J))
+        w += kStageCost;
+      cost[c] = w;
+      all_cost += w;
+    }
     long long acc = 0;
     int r = 1;
     for (int c = 0; c < nc && r < ranges; ++c) {
-      acc += chunks[c].e1 - chunks[c].e0;
-      while (r < ranges && acc * ranges >= all_entries * r) range_first[r++] = c + 1;
+      acc += cost[c];
+      while (r < ranges && acc * ranges >= all_cost * r) range_first[r++] = c + 1;
     }
     for (; r < ranges; ++r) range_first[r] = nc;
   }
