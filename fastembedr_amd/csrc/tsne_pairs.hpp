@@ -52,7 +52,11 @@ static_assert(kPairBlock % 64 == 0 && kPairBlock <= 65536,
 constexpr int kPairThreads = 64 * kPairWaves;
 // staged rows, staged columns, their packed accumulators: 128 KB
 constexpr size_t kPairLds = 2 * kPairBlock * sizeof(float2) + 2 * kPairBlock * sizeof(long long);
-constexpr int kPairChunkEntries = 128 * kLaneStep;  // 65536 entries per work item at most
+// 131072 entries per work item at most: every chunk starts its stream reads
+// cold, and at 1M points 64K-entry chunks ran the overlapped iteration at 3890
+// it/s against 3965 (32K: 3750, 192K: 3910, 256K: 3945, where the ranges
+// balance less evenly)
+constexpr int kPairChunkEntries = 256 * kLaneStep;
 constexpr int kPairDenseMin = 8192;              // pairs that pay for staging an off-diagonal tile
 constexpr int kPairMaxBlocks = 4096;             // tile table nb^2 ints (n <= 16.7M)
 // the streams are read two lane-steps ahead by every wave:
