@@ -441,45 +441,74 @@ int TsnePlan::setup_pairs(hipStream_t s, bool* built) {
     }
     chunks.swap(ordered);
   }
-  // workgroups: contiguous ranges of chunks (one per chunk when there are
-  // fewer chunks than workgroups)
-  const int nc = static_cast<int>(chunks.size());
   // workgroups: two per CU when the attraction has the GPU to itself; beside
   // the splat and the FFT passes (the overlapped iteration) five for every
   // eight CUs, so those kernels keep CUs of their own (1M points: 3830 it/s
-  // against 3785 with 3/4 of the CUs and 3370 with one per CU)
+  // against 3785 with 3/4 of the CUs and 3370 with one per CU); never more
+  // than there are chunks
   const int cus = compute_units();
   const int want = pair_wg_request > 0 ? pair_wg_request : (overlap ? 5 * cus / 8 : 2 * cus);
-  const int ranges = std::max(1, std::min(nc, want));
-  std::vector<int> range_first(ranges + 1, nc);
-  range_first[0] = 0;
+  const int ranges = std::max(1, std::min(static_cast<int>(chunks.size()), want));
+  // Workgroups take contiguous ranges of the chunk list of about equal cost:
+  // a rest entry (gathers from the layout in memory) counts 1.3 staged
+  // entries, and staging a block of the layout (a new row block, a new
+  // off-diagonal tile) 8192 entries.  A chunk that a range boundary falls
+  // inside is cut there (at a multiple of kLaneEntries), so the ranges meet
+  // their shares whatever the chunk size (1M points: the attraction alone
+  // 188 us with ranges of equal entries in whole chunks, 170 us so).
+  constexpr long long kStageCost = 8192, kRestPct = 130;
+  std::vector<int> range_first(ranges + 1, 0);
   {
-    // ranges of about equal cost: a rest entry (gathers from the layout in
-    // memory) counts 1.3 staged entries, and staging a block of the layout
-    // (a new row block, a new off-diagonal tile) 8192 entries (1M points:
-    // the attraction alone 188 -> 177 us, 3860 -> 3915 it/s overlapped,
-    // against ranges of equal entries)
-    constexpr long long kStageCost = 8192, kRestPct = 130;
-    std::vector<long long> cost(nc);
-    long long all_cost = 0;
-    for (int c = 0; c < nc; ++c) {
-      const PairChunk& ch = chunks[c];
-      long long w = static_cast<long long>(ch.e1 - ch.e0) * (ch.rest ? kRestPct : 100) / 100;
-      if (c == 0 || chunks[c - 1].I != ch.I) w += kStageCost;
+    const int nc0 = static_cast<int>(chunks.size());
+    auto stage_cost = [&](const PairChunk& ch, const PairChunk* prev) {
+      long long w = 0;
+      if (!prev || prev->I != ch.I) w += kStageCost;
       if (!ch.rest && ch.J != ch.I &&
-          (c == 0 || chunks[c - 1].rest || chunks[c - 1].I != ch.I || chunks[c - 1].J != ch.J))
+          (!prev || prev->rest || prev->I != ch.I || prev->J != ch.J))
         w += kStageCost;
-      cost[c] = w;
-      all_cost += w;
+      return w;
+    };
+    long long all_cost = 0;
+    for (int c = 0; c < nc0; ++c) {
+      const PairChunk& ch = chunks[c];
+      all_cost += stage_cost(ch, c ? &chunks[c - 1] : nullptr) +
+                  static_cast<long long>(ch.e1 - ch.e0) * (ch.rest ? kRestPct : 100) / 100;
     }
-    long long acc = 0;
+    // a cut adds a staging to the range after it
+    all_cost += static_cast<long long>(ranges - 1) * kStageCost;
+    std::vector<PairChunk> cutc;
+    cutc.reserve(nc0 + ranges);
+    long long acc = 0;  // cost of the chunks (and pieces) before the current one
     int r = 1;
-    for (int c = 0; c < nc && r < ranges; ++c) {
-      acc += cost[c];
-      while (r < ranges && acc * ranges >= all_cost * r) range_first[r++] = c + 1;
+    for (int c = 0; c < nc0; ++c) {
+      PairChunk ch = chunks[c];
+      const long long pct = ch.rest ? kRestPct : 100;
+      acc += stage_cost(ch, c ? &chunks[c - 1] : nullptr);
+      // boundaries inside this chunk: the entry where the running cost
+      // reaches r * all_cost / ranges
+      while (r < ranges) {
+        const long long target = (all_cost * r + ranges - 1) / ranges;
+        const long long len = ch.e1 - ch.e0;
+        if (acc + len * pct / 100 < target) break;
+        const long long x = std::max(0ll, (target - acc) * 100 / pct) / kLaneEntries * kLaneEntries;
+        if (x >= len) break;  // at the chunk's end: the next chunk starts range r
+        if (x > 0) {
+          PairChunk head = ch;
+          head.e1 = static_cast<int>(ch.e0 + x);
+          cutc.push_back(head);
+          ch.e0 = head.e1;
+          acc += x * pct / 100 + kStageCost;  // the next range stages again
+        }
+        range_first[r++] = static_cast<int>(cutc.size());
+      }
+      acc += static_cast<long long>(ch.e1 - ch.e0) * pct / 100;
+      cutc.push_back(ch);
     }
-    for (; r < ranges; ++r) range_first[r] = nc;
+    for (; r < ranges; ++r) range_first[r] = static_cast<int>(cutc.size());
+    range_first[ranges] = static_cast<int>(cutc.size());
+    chunks.swap(cutc);
   }
+  const int nc = static_cast<int>(chunks.size());
   // slots: one per run of chunks of a range sharing a row block (row side)
   // and per run sharing a staged off-diagonal tile (column side)
   std::vector<std::vector<int>> block_slots(nb);
@@ -1317,7 +1346,8 @@ static long long job_resident_bytes(int n, int k, int g) {
   const long long tiles = nb * (nb + 1) / 2;
   const long long staged = nk + 1ll * kLaneEntries * (n + tiles) + kPairPad;
   const long long rest = nk + 1ll * kLaneEntries * n + kPairPad;
-  const long long chunks = staged / kPairChunkEntries + rest / kPairChunkEntries + tiles + nb;
+  // (the cuts at workgroup range boundaries at most double the chunks)
+  const long long chunks = 2 * (staged / kPairChunkEntries + rest / kPairChunkEntries + tiles + nb);
   const long long pairs = 6ll * staged + staged / 4 + 8ll * rest + rest / 4 +
                           static_cast<long long>(sizeof(PairChunk)) * chunks + 8ll * chunks +
                           4ll * (nb + 1) + 2ll * chunks * kPairBlock * 8;

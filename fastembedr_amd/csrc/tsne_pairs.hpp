@@ -52,11 +52,12 @@ static_assert(kPairBlock % 64 == 0 && kPairBlock <= 65536,
 constexpr int kPairThreads = 64 * kPairWaves;
 // staged rows, staged columns, their packed accumulators: 128 KB
 constexpr size_t kPairLds = 2 * kPairBlock * sizeof(float2) + 2 * kPairBlock * sizeof(long long);
-// 131072 entries per work item at most: every chunk starts its stream reads
-// cold, and at 1M points 64K-entry chunks ran the overlapped iteration at 3890
-// it/s against 3965 (32K: 3750, 192K: 3910, 256K: 3945, where the ranges
-// balance less evenly)
-constexpr int kPairChunkEntries = 256 * kLaneStep;
+// Regions are cut into chunks of at most 524288 entries, and again where a
+// workgroup's range ends: every chunk starts its stream reads cold, so few
+// and long chunks pay (1M points: 64K-entry chunks ran the overlapped
+// iteration at 3890 it/s, 128K at 3965, whole regions cut at the range
+// boundaries at 3996)
+constexpr int kPairChunkEntries = 1024 * kLaneStep;
 constexpr int kPairDenseMin = 8192;              // pairs that pay for staging an off-diagonal tile
 constexpr int kPairMaxBlocks = 4096;             // tile table nb^2 ints (n <= 16.7M)
 // the streams are read two lane-steps ahead by every wave:
