@@ -441,13 +441,13 @@ int TsnePlan::setup_pairs(hipStream_t s, bool* built) {
     }
     chunks.swap(ordered);
   }
-  // workgroups: two per CU when the attraction has the GPU to itself; beside
-  // the splat and the FFT passes (the overlapped iteration) five for every
-  // eight CUs, so those kernels keep CUs of their own (1M points: 3830 it/s
-  // against 3785 with 3/4 of the CUs and 3370 with one per CU); never more
-  // than there are chunks
+  // workgroups: one per CU when the attraction has the GPU to itself (1M
+  // points: 145 us alone against 151 with two per CU); beside the splat and
+  // the FFT passes (the overlapped iteration) five for every eight CUs, so
+  // those kernels keep CUs of their own (3830 it/s against 3785 with 3/4 of
+  // the CUs and 3370 with one per CU); never more than there are chunks
   const int cus = compute_units();
-  const int want = pair_wg_request > 0 ? pair_wg_request : (overlap ? 5 * cus / 8 : 2 * cus);
+  const int want = pair_wg_request > 0 ? pair_wg_request : (overlap ? 5 * cus / 8 : cus);
   const int ranges = std::max(1, std::min(static_cast<int>(chunks.size()), want));
   // Workgroups take contiguous ranges of the chunk list of about equal cost:
   // a rest entry (gathers from the layout in memory) counts 1.3 staged

@@ -198,7 +198,9 @@ class Autotuner:
         from .batch import TsneBatch
 
         cus = _compute_units()
-        shapes = [(ATTRACTION_PAIRS, 0), (ATTRACTION_PAIRS, cus), (ATTRACTION_PAIRS, 4 * cus),
+        # the pair form at its rule (one workgroup per CU), two and four per
+        # CU, and the segment-blocked form
+        shapes = [(ATTRACTION_PAIRS, 0), (ATTRACTION_PAIRS, 2 * cus), (ATTRACTION_PAIRS, 4 * cus),
                   (ATTRACTION_STAGED, 0)]
         timings = []
         for att, wg in shapes:
