@@ -777,7 +777,7 @@ def main():
     traffic = None
     if args.traffic_json is None and n == 1_000_000:
         # rocprofv3 FETCH_SIZE + WRITE_SIZE of this workload, calibrated
-        # against a plain copy (profiles/attract_blocked_1M_counters.md)
+        # against a plain copy (profiles/attract_pairs_1M_counters.md)
         args.traffic_json = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                                          "traffic_1M.json")
     if args.traffic_json and os.path.exists(args.traffic_json):
