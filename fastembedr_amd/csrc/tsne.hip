@@ -276,6 +276,7 @@ class TsnePlan {
   // `stream` (both read only the layout and its statistics); the update waits
   // for both.  FASTEMBEDR_HIP_OVERLAP=0 keeps every launch on `stream`.
   bool overlap = false;
+  bool allow_overlap = true;  // false for a batch's jobs: the batch forks for all of them
   Stream side;
   EventRing forks;
   struct CachedGraph {
@@ -754,7 +755,7 @@ int TsnePlan::create(const int* idx, const DistT* dist, bool device_inputs, int 
   if (g < 16 || g > 1024 || (g & (g - 1)) != 0)
     return fail("grid_size must be a power of two between 16 and 1024");
   if (stream.init()) return 1;
-  overlap = !exact && overlap_enabled();
+  overlap = !exact && allow_overlap && overlap_enabled();
   if (overlap && (side.init() || forks.init(kForkEvents))) return 1;
   hipStream_t s = stream.get();
   const auto t0 = std::chrono::steady_clock::now();
@@ -1394,11 +1395,19 @@ class TsneBatch {
   };
   std::vector<Launch> launches;
   std::unique_ptr<GraphExec> graph;  // captured iterations of the current table
+  // the attraction launches run on `side`, beside the splat and FFT launches
+  // of every job on `stream`, as in a single plan; FASTEMBEDR_HIP_OVERLAP=0
+  // keeps every launch on `stream`
+  bool overlap = false;
+  Stream side;
+  EventRing forks;
 
   int init(int max_jobs_, long long budget_) {
     max_jobs = max_jobs_;
     budget = budget_;
     if (stream.init()) return 1;
+    overlap = overlap_enabled();
+    if (overlap && (side.init() || forks.init(kForkEvents))) return 1;
     slots.resize(max_jobs);
     slot_bytes.assign(max_jobs, 0);
     desc.resize(max_jobs);
@@ -1437,6 +1446,7 @@ class TsneBatch {
     auto plan = std::make_unique<TsnePlan>();
     if (attraction_request(j.attraction, &plan->attr_request)) return 1;
     plan->pair_wg_request = j.pair_workgroups;
+    plan->allow_overlap = false;
     if (plan->create<float>(j.indices, j.distances, device_inputs, j.n, j.k, j.index_offset,
                             j.perplexity, j.grid_size, j.init, j.seed, true))
       return 1;
@@ -1592,9 +1602,40 @@ class TsneBatch {
     return 0;
   }
 
+  // One iteration of every job: the splat launches, then (forked onto the
+  // side stream) the attraction launches, the FFT launches, and after the
+  // join the update and finalize launches.  The splat is enqueued before the
+  // fork, as in a single plan, so a replayed graph keeps the splat and FFT
+  // chain on the first stream.
   int enqueue_iteration() {
     hipStream_t s = stream.get();
-    for (const Launch& L : launches) {
+    auto stage = [](int kind) {
+      return kind == 0 || kind == 11 || kind == 6 || kind == 7 ? 0 : (kind == 4 ? 1 : (kind <= 3 ? 2 : 3));
+    };
+    if (!overlap) {
+      for (const Launch& L : launches)
+        if (launch(L, s)) return 1;
+      return 0;
+    }
+    hipStream_t a = side.get();
+    hipEvent_t fork = forks.next(), join = forks.next();
+    FE_CHECK(hipEventRecord(fork, s));
+    for (const Launch& L : launches)
+      if (stage(L.kind) == 0 && launch(L, s)) return 1;
+    FE_CHECK(hipStreamWaitEvent(a, fork, 0));
+    for (const Launch& L : launches)
+      if (stage(L.kind) == 1 && launch(L, a)) return 1;
+    FE_CHECK(hipEventRecord(join, a));
+    for (const Launch& L : launches)
+      if (stage(L.kind) == 2 && launch(L, s)) return 1;
+    FE_CHECK(hipStreamWaitEvent(s, join, 0));
+    for (const Launch& L : launches)
+      if (stage(L.kind) == 3 && launch(L, s)) return 1;
+    return 0;
+  }
+
+  int launch(const Launch& L, hipStream_t s) {
+    {
       const int* sl = d_tab + L.off;
       const int* fi = sl + L.m;
       switch (L.kind) {
@@ -1691,6 +1732,7 @@ class TsneBatch {
     // a job that finishes inside a chunk goes inactive on the device and its
     // blocks exit, so the graph stays valid until a job joins or leaves.
     constexpr int kChunk = 25;
+    static_assert(2 * kChunk <= kForkEvents, "a captured chunk reuses no fork event");
     hipStream_t s = stream.get();
     int left = iters;
     if (graph_capture_enabled() && left >= kChunk) {
