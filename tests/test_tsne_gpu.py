@@ -300,6 +300,39 @@ def test_pair_attraction_staged_and_rest_streams(monkeypatch, dense_min):
     assert np.array_equal(a.attractive_force, b.attractive_force)
 
 
+@pytest.mark.parametrize("dense_min", [None, "1000000000"])
+def test_pair_attraction_same_bits_for_any_workgroup_count(monkeypatch, dense_min):
+    # The streams are dealt to workgroups as contiguous ranges of equal
+    # estimated cost, and a chunk is cut where a range ends; every pair must
+    # be summed exactly once whatever the cuts.  The sums are integers, so one
+    # workgroup, a few (chunks cut at the range ends) and the default (as
+    # many workgroups as chunks at this size) give the same bits.
+    n = 30000
+    x, _ = gaussian_mixture(n, 12, 8, seed=721)
+    idx, dist = exact_knn_clusters(x, 30)
+    y0 = np.random.default_rng(742).normal(scale=1e-4, size=(n, 2)).astype(np.float32)
+    if dense_min:
+        monkeypatch.setenv("FASTEMBEDR_HIP_PAIR_DENSE_MIN", dense_min)
+    outs, shapes = [], []
+    for wg in (1, 3, 7, 0):
+        plan = fe().TsnePlan(idx, dist, perplexity=10, grid_size=256, Y_init=y0, attraction=3,
+                             pair_workgroups=wg)
+        info = plan.attraction_info()
+        assert info["kind"] == "pairs"
+        assert (info["rest_entries"] > 0) == bool(dense_min)
+        plan.run(30, 12.0, 0.5, n / 12.0)
+        plan.run(20, 1.0, 0.8, n / 12.0)
+        outs.append(plan.layout())
+        shapes.append((info["workgroups"], info["chunks"]))
+        plan.close()
+    assert [w for w, _ in shapes[:3]] == [1, 3, 7], shapes
+    # the cuts add chunks
+    assert shapes[2][1] > shapes[0][1], shapes
+    assert np.all(np.isfinite(outs[0]))
+    for o, s in zip(outs[1:], shapes[1:]):
+        assert np.array_equal(o, outs[0]), s
+
+
 def _hub_knn(n=30000, k=15):
     """A KNN-shaped input with hub points: row i's neighbours are i+1 .. i+k
     (mod n) at increasing distances; the last neighbour of 20000 rows is
