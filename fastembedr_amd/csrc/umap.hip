@@ -247,73 +247,101 @@ __device__ __forceinline__ void epoch_grouped_body(unsigned long long* delta,
                                                    const float* __restrict__ period, int n_edges,
                                                    const UmapParams& p, unsigned epoch, int blk,
                                                    int nblk) {
+  // per wave: the sums of the negative samples of each lane's edge
+  __shared__ unsigned long long neg_sum[kUThreads / 64][2][64];
   const float alpha = p.lr * (1.0f - __fdiv_rn(static_cast<float>(epoch),
                                                fmaxf(1.0f, static_cast<float>(p.n_epochs))));
   constexpr float eps = 1.1920928955078125e-7f;
   const int lane = threadIdx.x & 63;
-  const unsigned long long upto_lane = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  unsigned long long* nsx = neg_sum[threadIdx.x >> 6][0];
+  unsigned long long* nsy = neg_sum[threadIdx.x >> 6][1];
+  const GlobalPos pos{layout};
   for (int base = blk * kUThreads + static_cast<int>(threadIdx.x & ~63); base < n_edges;
        base += nblk * kUThreads) {
     const int e = base + lane;
-    int h = -1;
+    int h = -1, t = -1, nneg = 0;
+    float2 hp = make_float2(0.0f, 0.0f);
     long long hx = 0, hy = 0;
     if (e < n_edges) {
       h = heads[e];
-      const int t = tails[e];
-      if (SYM && h >= 0 && h < p.n && t >= 0 && t < p.n && h != t) {
-        sym_edge_head(GlobalPos{layout}, h, t, period[e], p, epoch, alpha, &hx, &hy);
-      } else if (h >= 0 && h < p.n && t >= 0 && t < p.n && h != t) {
+      t = tails[e];
+      if (h >= 0 && h < p.n && t >= 0 && t < p.n && h != t) {
         const float per = period[e];
         const int npos = n_positive(per, epoch);
         if (npos > 0) {
-          const size_t hb = static_cast<size_t>(h) * 2, tb = static_cast<size_t>(t) * 2;
-          const float hxv = layout[hb], hyv = layout[hb + 1];
-          {
-            const float dx = hxv - layout[tb];
-            const float dy = hyv - layout[tb + 1];
-            const float d2 = fmaxf(eps, dx * dx + dy * dy);
-            const float c = attract(d2, p);
-            const float gx = clip4(c * dx) * alpha, gy = clip4(c * dy) * alpha;
-            const long long fx = npos * fixed(gx), fy = npos * fixed(gy);
+          hp = pos(static_cast<size_t>(h));
+          const float2 tp = pos(static_cast<size_t>(t));
+          const float dx = hp.x - tp.x;
+          const float dy = hp.y - tp.y;
+          const float d2 = fmaxf(eps, dx * dx + dy * dy);
+          const float c = attract(d2, p);
+          const float gx = clip4(c * dx) * alpha, gy = clip4(c * dy) * alpha;
+          const long long fx = npos * fixed(gx), fy = npos * fixed(gy);
+          if constexpr (SYM) {
+            hx += 2 * fx;
+            hy += 2 * fy;
+          } else {
             hx += fx;
             hy += fy;
+            const size_t tb = static_cast<size_t>(t) * 2;
             atomicAdd(delta + tb, static_cast<unsigned long long>(-fx));
             atomicAdd(delta + tb + 1, static_cast<unsigned long long>(-fy));
           }
-          const int nneg = n_negative(per, p.nsr, epoch);
-          for (int s = 0; s < nneg; ++s) {
-            const unsigned v = negative_vertex(static_cast<unsigned>(p.n), p.nmod, p.seed, epoch,
-                                               static_cast<unsigned>(h), static_cast<unsigned>(t),
-                                               static_cast<unsigned>(s));
-            if (static_cast<int>(v) == h || static_cast<int>(v) == t) continue;
-            const size_t vb = static_cast<size_t>(v) * 2;
-            const float dx = hxv - layout[vb];
-            const float dy = hyv - layout[vb + 1];
-            const float d2 = fmaxf(eps, dx * dx + dy * dy);
-            const float c = repel(d2, p);
-            hx += fixed(clip4(c * dx) * alpha);
-            hy += fixed(clip4(c * dy) * alpha);
-          }
+          nneg = n_negative(per, p.nsr, epoch);
         }
       }
     }
-    // segmented inclusive scan over the runs of equal heads
-    const int hprev = __shfl_up(h, 1, 64);
-    const bool start = lane == 0 || hprev != h;
-    const unsigned long long starts = __ballot(start) & upto_lane;
-    const int lim = 63 - __clzll(starts);  // first lane of this lane's run
-    unsigned long long ux = static_cast<unsigned long long>(hx);
-    unsigned long long uy = static_cast<unsigned long long>(hy);
+    // The wave's negative samples, flattened: sample i of the wave belongs to
+    // the lane o whose prefix range [incl(o) - nneg(o), incl(o)) holds it, and
+    // the wave's lanes take the samples 64 at a time, whatever lane owns them
+    // (only the sampled edges of an epoch have any, so a loop over each
+    // lane's own samples would leave most lanes idle).  A sample adds into
+    // its owner's LDS sums; integer sums, so the same delta in any order.
+    nsx[lane] = 0ull;
+    nsy[lane] = 0ull;
+    int incl = nneg;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
-      const unsigned long long ox = shfl_up_u64(ux, d), oy = shfl_up_u64(uy, d);
-      if (lane - d >= lim) {
-        ux += ox;
-        uy += oy;
+      const int o = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += o;
+    }
+    // the wave's sample count, in a scalar register: the rounds below are
+    // wave-uniform, so every shuffle in them reads lanes that are active
+    const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
+    for (int r = 0; r < total; r += 64) {
+      const int i = r + lane;
+      // owner: the number of lanes whose inclusive count is <= i
+      int o = 0;
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1)
+        if (__shfl(incl, o + step - 1, 64) <= i) o += step;
+      o = min(o, 63);
+      const int oh = __shfl(h, o, 64), ot = __shfl(t, o, 64);
+      const int first = __shfl(incl, o, 64) - __shfl(nneg, o, 64);
+      const float ox = __shfl(hp.x, o, 64), oy = __shfl(hp.y, o, 64);
+      if (i < total) {
+        const unsigned v = negative_vertex(static_cast<unsigned>(p.n), p.nmod, p.seed, epoch,
+                                           static_cast<unsigned>(oh), static_cast<unsigned>(ot),
+                                           static_cast<unsigned>(i - first));
+        if (static_cast<int>(v) != oh && static_cast<int>(v) != ot) {
+          const float2 vp = pos(static_cast<size_t>(v));
+          const float dx = ox - vp.x;
+          const float dy = oy - vp.y;
+          const float d2 = fmaxf(eps, dx * dx + dy * dy);
+          const float c = repel(d2, p);
+          atomicAdd(nsx + o, static_cast<unsigned long long>(fixed(clip4(c * dx) * alpha)));
+          atomicAdd(nsy + o, static_cast<unsigned long long>(fixed(clip4(c * dy) * alpha)));
+        }
       }
     }
-    const int hnext = __shfl_down(h, 1, 64);
-    const bool last = lane == 63 || hnext != h;
+    // the wave's LDS operations complete in order: the sums are whole here
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    hx += static_cast<long long>(nsx[lane]);
+    hy += static_cast<long long>(nsy[lane]);
+    unsigned long long ux, uy;
+    bool last;
+    head_run_sums(h, hx, hy, &ux, &uy, &last);
     if (last && h >= 0 && h < p.n && (ux != 0ull || uy != 0ull)) {
       atomicAdd(delta + static_cast<size_t>(h) * 2, ux);
       atomicAdd(delta + static_cast<size_t>(h) * 2 + 1, uy);
