@@ -19,7 +19,7 @@
 // truncated toward zero and packed as the 64-bit integer y * 2^32 + x; its
 // column adds it with one LDS 64-bit atomic into one accumulator per staged
 // point, and its row adds a lane's 8 of them at once; a work item writes its
-// accumulators to its partial slots, and the reduce adds a point's slots.
+// accumulators to its partial slots, and the update adds a point's slots.
 // A sum of packed values is the packed sum Y * 2^32 + X exactly, and while
 // |X| < 2^31 and |Y| < 2^30 the low 32 bits read as a signed integer are X
 // and the rest is Y.  The result is therefore
